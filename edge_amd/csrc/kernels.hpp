@@ -1,0 +1,42 @@
+// Device-side data of a solver and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace edge {
+
+// Reference-element matrices on the device in the solver's precision.
+template <typename T>
+struct DgDev {
+  const T* stiffTn;  // [3][N][N]  -(K_d^T M^-1), negated as src/dg/TimePred.hpp:86 stores it
+  const T* stiffV;   // [3][N][N]  K_d M^-1
+  const T* fluxLN;   // [16][N][NF] local (0..3) then neighbouring (4 + v*4 + f)
+  const T* fluxT;    // [4][NF][N]
+};
+
+// Per element-face neighbour data, precomputed from the mesh.
+struct NeighDev {
+  const int32_t* ne;   // [nEl][4]: neighbour element (self on free surfaces), -1: skip (outflow)
+  const uint8_t* fmat; // [nEl][4]: index into fluxLN for the neighbour's contribution
+};
+
+template <typename T>
+struct SolverDev {
+  int order, n_runs;
+  int64_t n_el;
+  DgDev<T> dg;
+  NeighDev nb;
+  const T* star;  // [nEl][3][81]
+  const T* fs;    // [2][nEl][4][81]
+};
+
+// Generic kernels: one workgroup per element, all tiles in LDS.
+template <typename T>
+int launch_local_generic(const SolverDev<T>& s, int64_t first, int64_t n, T dt, T* dofs, T* t_int, hipStream_t st);
+template <typename T>
+int launch_neigh_generic(const SolverDev<T>& s, int64_t first, int64_t n, const T* t_int, T* dofs, hipStream_t st);
+bool generic_supported(int order, int n_runs, int prec_bytes);
+
+}  // namespace edge

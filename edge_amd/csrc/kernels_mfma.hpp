@@ -1,0 +1,14 @@
+// MFMA-tiled element kernels (see kernels_mfma.hip).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace edge {
+
+bool mfma_supported(int order, int n_runs, int precision);
+template <typename T>
+int launch_local_mfma(const SolverDev<T>& s, int64_t first, int64_t n, T dt, T* dofs, T* t_int, hipStream_t st);
+template <typename T>
+int launch_neigh_mfma(const SolverDev<T>& s, int64_t first, int64_t n, const T* t_int, T* dofs, hipStream_t st);
+
+}  // namespace edge

@@ -1,0 +1,279 @@
+// Structured tet4 meshes of a box: every hex is cut into the 6 tets of the
+// Kuhn (Freudenthal) split, which is conforming across hexes and across
+// periodic sides.
+//
+// The output follows the conventions the element update depends on:
+//  * positively oriented vertex order (tools/edge_v/src/geom/Tet4.cpp:80-112),
+//  * local faces 0: 0-2-1, 1: 0-1-3, 2: 0-3-2, 3: 1-2-3 (src/constants.hpp:527-553),
+//  * fIdElFaEl = local id of the shared face in the neighbour and
+//    vIdElFaEl = position of this face's vertex 0 within the neighbour's face
+//    (tools/edge_v/src/geom/Tet4.cpp:114-228),
+//  * per-face outward normal of the first adjacent element and tangents
+//    (tools/edge_v/src/geom/Tria3.cpp:48-115, mesh/Mesh.cpp:763-841),
+//  * in-diameter 12 V / sum |2 A_face| (tools/edge_v/src/geom/Tet4.cpp:49-78).
+// Periodic faces are stored once and shared by both elements, so a periodic
+// face looks like an interior face to the solver; its type is PERIODIC.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+// local face -> local vertices
+const int kFaVe[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};
+
+struct Rng {  // splitmix64, documented so a caller can reproduce the jitter
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+void cross(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+double dot(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+double norm(const double a[3]) { return std::sqrt(dot(a, a)); }
+
+}  // namespace
+
+extern "C" int edge_mesh_box_sizes(const edge_box_params* p, int64_t* n_ve, int64_t* n_el, int64_t* n_fa) {
+  if (!p || !n_ve || !n_el || !n_fa) return edge::fail(EDGE_ERR_ARG, "null argument");
+  const int64_t n[3] = {p->n[0], p->n[1], p->n[2]};
+  for (int d = 0; d < 3; d++) {
+    if (n[d] < 1) return edge::fail(EDGE_ERR_ARG, "box needs at least one hex per axis");
+    if (p->bc[d] == EDGE_FA_PERIODIC && n[d] < 3)
+      return edge::fail(EDGE_ERR_ARG, "periodic axes need at least three hexes");
+  }
+  const int64_t nh = n[0] * n[1] * n[2];
+  *n_ve = (n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+  *n_el = 6 * nh;
+  int64_t nb = 0;  // boundary (non-periodic) triangles
+  for (int d = 0; d < 3; d++)
+    if (p->bc[d] != EDGE_FA_PERIODIC) nb += 2 * 2 * (nh / n[d]);
+  *n_fa = (4 * (*n_el) - nb) / 2 + nb;
+  return EDGE_OK;
+}
+
+extern "C" int edge_mesh_box(const edge_box_params* p, edge_mesh_out* m) {
+  int64_t nve, nel, nfa;
+  int st = edge_mesh_box_sizes(p, &nve, &nel, &nfa);
+  if (st != EDGE_OK) return st;
+  if (!m || !m->ve_crds || !m->el_ve || !m->el_fa || !m->fa_el || !m->el_fa_el || !m->f_id || !m->v_id ||
+      !m->fa_type || !m->normals || !m->tangents || !m->areas || !m->in_dia || !m->volume)
+    return edge::fail(EDGE_ERR_ARG, "mesh output arrays missing");
+  for (int d = 0; d < 3; d++) {
+    const int t = p->bc[d];
+    if (t != EDGE_FA_PERIODIC && t != EDGE_FA_FREE_SURFACE && t != EDGE_FA_OUTFLOW)
+      return edge::fail(EDGE_ERR_ARG, "boundary type must be periodic, free surface or outflow");
+  }
+  const int64_t n[3] = {p->n[0], p->n[1], p->n[2]};
+  const double h[3] = {p->extent[0] / n[0], p->extent[1] / n[1], p->extent[2] / n[2]};
+  const int64_t nv[3] = {n[0] + 1, n[1] + 1, n[2] + 1};
+  auto vid = [&](int64_t i, int64_t j, int64_t k) { return (k * nv[1] + j) * nv[0] + i; };
+
+  // vertices, interior ones jittered
+  Rng rng{p->seed};
+  for (int64_t k = 0; k < nv[2]; k++)
+    for (int64_t j = 0; j < nv[1]; j++)
+      for (int64_t i = 0; i < nv[0]; i++) {
+        const int64_t ijk[3] = {i, j, k};
+        bool interior = true;
+        for (int d = 0; d < 3; d++) interior = interior && ijk[d] > 0 && ijk[d] < n[d];
+        double* x = m->ve_crds + 3 * vid(i, j, k);
+        for (int d = 0; d < 3; d++) {
+          // every vertex draws three numbers so the sequence does not depend on the box shape
+          double u = rng.uniform();
+          x[d] = p->origin[d] + ijk[d] * h[d] + (interior ? (2.0 * u - 1.0) * p->jitter * h[d] : 0.0);
+        }
+      }
+
+  // canonical (periodic) vertex id
+  auto canon = [&](int64_t v) -> int64_t {
+    int64_t i = v % nv[0], j = (v / nv[0]) % nv[1], k = v / (nv[0] * nv[1]);
+    if (p->bc[0] == EDGE_FA_PERIODIC) i %= n[0];
+    if (p->bc[1] == EDGE_FA_PERIODIC) j %= n[1];
+    if (p->bc[2] == EDGE_FA_PERIODIC) k %= n[2];
+    return vid(i, j, k);
+  };
+
+  // hex order: blocks of b^3 hexes, lexicographic inside and across blocks
+  const int64_t b = p->block > 0 ? p->block : 1;
+  std::vector<int64_t> hexes;
+  hexes.reserve(n[0] * n[1] * n[2]);
+  for (int64_t bk = 0; bk < n[2]; bk += b)
+    for (int64_t bj = 0; bj < n[1]; bj += b)
+      for (int64_t bi = 0; bi < n[0]; bi += b)
+        for (int64_t k = bk; k < std::min(bk + b, n[2]); k++)
+          for (int64_t j = bj; j < std::min(bj + b, n[1]); j++)
+            for (int64_t i = bi; i < std::min(bi + b, n[0]); i++) hexes.push_back((k * n[1] + j) * n[0] + i);
+
+  // elements: Kuhn split, vertex order v0 -> +e_p0 -> +e_p1 -> (1,1,1)
+  const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  int64_t el = 0;
+  for (int64_t hx : hexes) {
+    const int64_t i0 = hx % n[0], j0 = (hx / n[0]) % n[1], k0 = hx / (n[0] * n[1]);
+    for (int t = 0; t < 6; t++) {
+      int64_t c[3] = {0, 0, 0};
+      int64_t ve[4];
+      ve[0] = vid(i0, j0, k0);
+      for (int s = 0; s < 2; s++) {
+        c[perm[t][s]] = 1;
+        ve[s + 1] = vid(i0 + c[0], j0 + c[1], k0 + c[2]);
+      }
+      ve[3] = vid(i0 + 1, j0 + 1, k0 + 1);
+      // positive orientation: swap vertices 2 and 3 on a negative determinant
+      double a[3][3];
+      for (int r = 0; r < 3; r++)
+        for (int d = 0; d < 3; d++) a[r][d] = m->ve_crds[3 * ve[r + 1] + d] - m->ve_crds[3 * ve[0] + d];
+      double cr[3];
+      cross(a[1], a[2], cr);
+      const double det = dot(a[0], cr);
+      if (std::fabs(det) < 1e-14 * h[0] * h[1] * h[2])
+        return edge::fail(EDGE_ERR_MESH, "degenerate tet (jitter too large?)");
+      if (det < 0) std::swap(ve[2], ve[3]);
+      for (int v = 0; v < 4; v++) m->el_ve[el * 4 + v] = ve[v];
+      m->volume[el] = std::fabs(det) / 6.0;
+      el++;
+    }
+  }
+
+  // faces, matched through canonical vertex triples
+  std::unordered_map<uint64_t, int64_t> faces;
+  faces.reserve((size_t)nfa * 2);
+  int64_t fa_count = 0;
+  for (int64_t e = 0; e < nel; e++) {
+    for (int lf = 0; lf < 4; lf++) {
+      int64_t cv[3];
+      for (int s = 0; s < 3; s++) cv[s] = canon(m->el_ve[e * 4 + kFaVe[lf][s]]);
+      if (cv[0] > cv[1]) std::swap(cv[0], cv[1]);
+      if (cv[1] > cv[2]) std::swap(cv[1], cv[2]);
+      if (cv[0] > cv[1]) std::swap(cv[0], cv[1]);
+      const uint64_t key = ((uint64_t)cv[0] << 42) | ((uint64_t)cv[1] << 21) | (uint64_t)cv[2];
+      auto it = faces.find(key);
+      if (it == faces.end()) {
+        if (fa_count >= nfa) return edge::fail(EDGE_ERR_MESH, "face count mismatch");
+        faces.emplace(key, fa_count);
+        m->fa_el[fa_count * 2] = e;
+        m->fa_el[fa_count * 2 + 1] = -1;
+        m->el_fa[e * 4 + lf] = fa_count;
+        fa_count++;
+      } else {
+        const int64_t f = it->second;
+        if (m->fa_el[f * 2 + 1] != -1) return edge::fail(EDGE_ERR_MESH, "face shared by more than two tets");
+        m->fa_el[f * 2 + 1] = e;
+        m->el_fa[e * 4 + lf] = f;
+      }
+    }
+  }
+  if (fa_count != nfa) return edge::fail(EDGE_ERR_MESH, "face count mismatch");
+  if ((uint64_t)nve >= (1ull << 21)) return edge::fail(EDGE_ERR_MESH, "box too large for face keys");
+
+  // face types, neighbours, geometry
+  for (int64_t f = 0; f < nfa; f++) {
+    const int64_t eL = m->fa_el[f * 2], eR = m->fa_el[f * 2 + 1];
+    int lf = -1;
+    for (int s = 0; s < 4; s++)
+      if (m->el_fa[eL * 4 + s] == f) lf = s;
+    // which box side (if any) the face lies on, from the left element's vertices
+    int side = -1;
+    for (int d = 0; d < 3 && side < 0; d++) {
+      for (int hi = 0; hi < 2 && side < 0; hi++) {
+        bool on = true;
+        for (int s = 0; s < 3; s++) {
+          const int64_t v = m->el_ve[eL * 4 + kFaVe[lf][s]];
+          const int64_t ijk[3] = {v % nv[0], (v / nv[0]) % nv[1], v / (nv[0] * nv[1])};
+          on = on && ijk[d] == (hi ? n[d] : 0);
+        }
+        if (on) side = 2 * d + hi;
+      }
+    }
+    if (eR < 0) {
+      if (side < 0) return edge::fail(EDGE_ERR_MESH, "unmatched interior face");
+      m->fa_type[f] = p->bc[side / 2];
+    } else {
+      m->fa_type[f] = (side >= 0) ? EDGE_FA_PERIODIC : 0;
+    }
+
+    // geometry of the left element's face
+    double x[3][3], np[3];
+    for (int s = 0; s < 3; s++)
+      for (int d = 0; d < 3; d++) x[s][d] = m->ve_crds[3 * m->el_ve[eL * 4 + kFaVe[lf][s]] + d];
+    {
+      int opp = 0;
+      for (int v = 0; v < 4; v++) {
+        bool in = false;
+        for (int s = 0; s < 3; s++) in = in || kFaVe[lf][s] == v;
+        if (!in) opp = v;
+      }
+      for (int d = 0; d < 3; d++) np[d] = m->ve_crds[3 * m->el_ve[eL * 4 + opp] + d];
+    }
+    double d0[3], d1[3], c[3];
+    for (int d = 0; d < 3; d++) {
+      d0[d] = x[1][d] - x[0][d];
+      d1[d] = x[2][d] - x[0][d];
+    }
+    cross(d0, d1, c);
+    m->areas[f] = 0.5 * norm(c);
+    // tangents: d0 and its in-plane normal n0; swapped if d0 x n0 points inwards
+    const double alpha = -dot(d0, d1) / dot(d0, d0);
+    double n0[3];
+    for (int d = 0; d < 3; d++) n0[d] = d1[d] + alpha * d0[d];
+    const double l0 = norm(d0), l1 = norm(n0);
+    for (int d = 0; d < 3; d++) {
+      d0[d] /= l0;
+      n0[d] /= l1;
+    }
+    double n1[3], dn[3];
+    cross(d0, n0, n1);
+    for (int d = 0; d < 3; d++) dn[d] = np[d] - x[0][d];
+    if (dot(n1, dn) > 0)
+      for (int d = 0; d < 3; d++) std::swap(d0[d], n0[d]);
+    double nrm[3];
+    cross(d0, n0, nrm);
+    for (int d = 0; d < 3; d++) {
+      m->tangents[f * 6 + d] = d0[d];
+      m->tangents[f * 6 + 3 + d] = n0[d];
+      m->normals[f * 3 + d] = nrm[d];
+    }
+  }
+
+  // element-face-element relations
+  for (int64_t e = 0; e < nel; e++) {
+    double asum = 0;
+    for (int lf = 0; lf < 4; lf++) {
+      const int64_t f = m->el_fa[e * 4 + lf];
+      asum += 2.0 * m->areas[f];
+      const int64_t ne = (m->fa_el[f * 2] == e) ? m->fa_el[f * 2 + 1] : m->fa_el[f * 2];
+      m->el_fa_el[e * 4 + lf] = ne;
+      if (ne < 0) {
+        m->f_id[e * 4 + lf] = EDGE_NO_ID16;
+        m->v_id[e * 4 + lf] = EDGE_NO_ID16;
+        continue;
+      }
+      int nf = -1;
+      for (int s = 0; s < 4; s++)
+        if (m->el_fa[ne * 4 + s] == f) nf = s;
+      if (nf < 0) return edge::fail(EDGE_ERR_MESH, "neighbour misses the shared face");
+      const int64_t c0 = canon(m->el_ve[e * 4 + (lf <= 2 ? 0 : 1)]);
+      int pos = -1;
+      for (int s = 0; s < 3; s++)
+        if (canon(m->el_ve[ne * 4 + kFaVe[nf][s]]) == c0) pos = s;
+      if (pos < 0) return edge::fail(EDGE_ERR_MESH, "shared face vertex not found in the neighbour");
+      m->f_id[e * 4 + lf] = (uint16_t)nf;
+      m->v_id[e * 4 + lf] = (uint16_t)pos;
+    }
+    m->in_dia[e] = 12.0 * m->volume[e] / asum;
+  }
+  return EDGE_OK;
+}

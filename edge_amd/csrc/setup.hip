@@ -1,0 +1,276 @@
+// FP64 set-up on the GPU: element Jacobians and star matrices, and the
+// per-face Godunov flux solvers of both adjacent elements.  Results are
+// written in the solver's storage precision T.
+//
+// Physics follows the reference's setup:
+//  * star matrices S_d = sum_k A_k (J^-1)[k][d]
+//    (src/impl/seismic/solvers/AderDgInit.hpp:129-174,
+//     src/impl/seismic/setups/Elasticity.cpp:167-252);
+//  * flux solvers fsL = T FlMid T^-1, fsR = T FrMid [mirror] T^-1 in the
+//    face frame (n, s, t), the right element with (-n, s, t), both scaled by
+//    -2 area / |J| (AderDgInit.hpp:394-665, Elasticity.cpp:372-441).
+// FlMid / FrMid are derived here from the welded-interface Riemann problem.
+#include <hip/hip_runtime.h>
+
+#include "setup.hpp"
+
+namespace edge {
+namespace {
+
+__device__ void jac_inv(const double* ve, double* det, double jinv[9]) {
+  double a[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) a[i * 3 + j] = ve[(i + 1) * 3 + j] - ve[j];
+  const double d = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                   a[2] * (a[3] * a[7] - a[4] * a[6]);
+  *det = d;
+  const double id = 1.0 / d;
+  jinv[0] = (a[4] * a[8] - a[5] * a[7]) * id;
+  jinv[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  jinv[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  jinv[3] = (a[5] * a[6] - a[3] * a[8]) * id;
+  jinv[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  jinv[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  jinv[6] = (a[3] * a[7] - a[4] * a[6]) * id;
+  jinv[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  jinv[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+__device__ void load_ve(const double* crds, const int64_t* el_ve, int64_t el, double ve[12]) {
+  for (int v = 0; v < 4; v++) {
+    const int64_t id = el_ve[el * 4 + v];
+    for (int d = 0; d < 3; d++) ve[v * 3 + d] = crds[id * 3 + d];
+  }
+}
+
+template <typename T>
+__global__ void k_star(int64_t n_el, const double* crds, const int64_t* el_ve, const double* rho,
+                       const double* lam, const double* mu, T* star, double* jdet, int* bad) {
+  const int64_t el = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= n_el) return;
+  double ve[12], det, ji[9];
+  load_ve(crds, el_ve, el, ve);
+  jac_inv(ve, &det, ji);
+  jdet[el] = det;
+  if (!(det > 0)) atomicOr(bad, 1);
+  const double lp = -lam[el] - 2 * mu[el], l = -lam[el], m = -mu[el], ir = -1.0 / rho[el];
+  double s[243];
+  for (int i = 0; i < 243; i++) s[i] = 0;
+  // nonzeros (row, col) of the Jacobians A (x), B (y), C (z) and their values
+  const int rc[3][8][2] = {{{0, 6}, {1, 6}, {2, 6}, {3, 7}, {5, 8}, {6, 0}, {7, 3}, {8, 5}},
+                           {{0, 7}, {1, 7}, {2, 7}, {3, 6}, {4, 8}, {6, 3}, {7, 1}, {8, 4}},
+                           {{0, 8}, {1, 8}, {2, 8}, {4, 7}, {5, 6}, {6, 5}, {7, 4}, {8, 2}}};
+  const double va[3][8] = {{lp, l, l, m, m, ir, ir, ir}, {l, lp, l, m, m, ir, ir, ir}, {l, l, lp, m, m, ir, ir, ir}};
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 3; k++)
+      for (int e = 0; e < 8; e++) s[d * 81 + rc[k][e][0] * 9 + rc[k][e][1]] += va[k][e] * ji[k * 3 + d];
+  for (int i = 0; i < 243; i++) star[el * 243 + i] = (T)s[i];
+}
+
+// quantities face-aligned (n, s, t) -> xyz and back, Voigt order xx yy zz xy yz xz
+__device__ void trafo(const double n[3], const double s[3], const double t[3], double T[81], double Ti[81]) {
+  const double* f[3] = {n, s, t};
+  const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
+  for (int i = 0; i < 81; i++) T[i] = Ti[i] = 0;
+  for (int r = 0; r < 6; r++)
+    for (int c = 0; c < 6; c++) {
+      const int i = vi[r][0], j = vi[r][1], a = vi[c][0], b = vi[c][1];
+      double v = f[a][i] * f[b][j];
+      if (a != b) v += f[b][i] * f[a][j];
+      T[r * 9 + c] = v;
+      // inverse: face-aligned (a', b') = vi[r] from xyz (i', j') = vi[c]
+      const int ap = vi[r][0], bp = vi[r][1], ip = vi[c][0], jp = vi[c][1];
+      double w = f[ap][ip] * f[bp][jp];
+      if (ip != jp) w += f[ap][jp] * f[bp][ip];
+      Ti[r * 9 + c] = w;
+    }
+  for (int i = 0; i < 3; i++)
+    for (int a = 0; a < 3; a++) {
+      T[(6 + i) * 9 + 6 + a] = f[a][i];
+      Ti[(6 + a) * 9 + 6 + i] = f[a][i];
+    }
+}
+
+// derivative of the left element's flux A_L q* w.r.t. the left (fl) and the
+// right (fr) face-aligned states; q* is the welded-interface middle state
+__device__ void fs_mid(double rhoL, double rhoR, double lamL, double lamR, double muL, double muR,
+                       double fl[81], double fr[81]) {
+  for (int i = 0; i < 81; i++) fl[i] = fr[i] = 0;
+  const double zpL = sqrt((lamL + 2 * muL) * rhoL), zpR = sqrt((lamR + 2 * muR) * rhoR);
+  const double zsL = sqrt(muL * rhoL), zsR = sqrt(muR * rhoR);
+  const double ip = 1.0 / (zpL + zpR);
+  const double cu[3] = {-(lamL + 2 * muL), -lamL, -lamL};
+  for (int r = 0; r < 3; r++) {
+    fl[r * 9 + 0] = -cu[r] * ip;
+    fl[r * 9 + 6] = cu[r] * zpL * ip;
+    fr[r * 9 + 0] = cu[r] * ip;
+    fr[r * 9 + 6] = cu[r] * zpR * ip;
+  }
+  fl[6 * 9 + 0] = -zpR * ip / rhoL;
+  fl[6 * 9 + 6] = zpL * zpR * ip / rhoL;
+  fr[6 * 9 + 0] = -zpL * ip / rhoL;
+  fr[6 * 9 + 6] = -zpL * zpR * ip / rhoL;
+  const double is = 1.0 / (zsL + zsR);
+  const int pr[2][2] = {{3, 7}, {5, 8}};
+  for (int p = 0; p < 2; p++) {
+    const int si = pr[p][0], vv = pr[p][1];
+    fl[si * 9 + si] = muL * is;
+    fl[si * 9 + vv] = -muL * zsL * is;
+    fr[si * 9 + si] = -muL * is;
+    fr[si * 9 + vv] = -muL * zsR * is;
+    fl[vv * 9 + si] = -zsR * is / rhoL;
+    fl[vv * 9 + vv] = zsL * zsR * is / rhoL;
+    fr[vv * 9 + si] = -zsL * is / rhoL;
+    fr[vv * 9 + vv] = -zsL * zsR * is / rhoL;
+  }
+}
+
+__device__ void mm9(const double* a, const double* b, double* c) {
+  for (int i = 0; i < 9; i++)
+    for (int j = 0; j < 9; j++) {
+      double s = 0;
+      for (int k = 0; k < 9; k++) s += a[i * 9 + k] * b[k * 9 + j];
+      c[i * 9 + j] = s;
+    }
+}
+
+__device__ void face_solvers(double rL, double rR, double lL, double lR, double mL, double mR, const double n[3],
+                             const double s[3], const double t[3], bool free_surface, double fsL[81],
+                             double fsR[81]) {
+  double T[81], Ti[81], fl[81], fr[81], tmp[81];
+  trafo(n, s, t, T, Ti);
+  fs_mid(rL, rR, lL, lR, mL, mR, fl, fr);
+  mm9(T, fl, tmp);
+  mm9(tmp, Ti, fsL);
+  mm9(T, fr, tmp);
+  if (free_surface)  // mirror the face-aligned normal tractions (columns 0, 3, 5)
+    for (int r = 0; r < 9; r++) {
+      tmp[r * 9 + 0] = -tmp[r * 9 + 0];
+      tmp[r * 9 + 3] = -tmp[r * 9 + 3];
+      tmp[r * 9 + 5] = -tmp[r * 9 + 5];
+    }
+  mm9(tmp, Ti, fsR);
+}
+
+template <typename T>
+__global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el, const int64_t* el_fa,
+                               const int32_t* fa_type, const double* normals, const double* tangents,
+                               const double* areas, const double* jdet, const double* rho, const double* lam,
+                               const double* mu, T* fs, int* bad) {
+  const int64_t fa = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (fa >= n_fa) return;
+  const int64_t eL = fa_el[fa * 2], eR = fa_el[fa * 2 + 1];
+  if (eL < 0 || eL >= n_el || eR >= n_el) {
+    atomicOr(bad, 2);
+    return;
+  }
+  int fL = -1, fR = -1;
+  for (int f = 0; f < 4; f++) {
+    if (el_fa[eL * 4 + f] == fa) fL = f;
+    if (eR >= 0 && el_fa[eR * 4 + f] == fa) fR = f;
+  }
+  if (fL < 0 || (eR >= 0 && fR < 0)) {
+    atomicOr(bad, 2);
+    return;
+  }
+  if (!(areas[fa] > 0)) atomicOr(bad, 4);
+  const bool fsurf = (fa_type[fa] & EDGE_FA_FREE_SURFACE) == EDGE_FA_FREE_SURFACE;
+  const double rL = rho[eL], lL = lam[eL], mL = mu[eL];
+  const double rR = eR >= 0 ? rho[eR] : rL, lR = eR >= 0 ? lam[eR] : lL, mR = eR >= 0 ? mu[eR] : mL;
+  const double* n = normals + fa * 3;
+  const double* s = tangents + fa * 6;
+  const double* t = tangents + fa * 6 + 3;
+  double a[81], b[81];
+  face_solvers(rL, rR, lL, lR, mL, mR, n, s, t, fsurf, a, b);
+  double sc = -2.0 * areas[fa] / jdet[eL];
+  T* o0 = fs + (eL * 4 + fL) * 81;
+  T* o1 = fs + ((n_el + eL) * 4 + fL) * 81;
+  for (int i = 0; i < 81; i++) {
+    o0[i] = (T)(a[i] * sc);
+    o1[i] = (T)(b[i] * sc);
+  }
+  if (eR >= 0) {
+    const double nm[3] = {-n[0], -n[1], -n[2]};
+    face_solvers(rR, rL, lR, lL, mR, mL, nm, s, t, false, a, b);
+    sc = -2.0 * areas[fa] / jdet[eR];
+    o0 = fs + (eR * 4 + fR) * 81;
+    o1 = fs + ((n_el + eR) * 4 + fR) * 81;
+    for (int i = 0; i < 81; i++) {
+      o0[i] = (T)(a[i] * sc);
+      o1[i] = (T)(b[i] * sc);
+    }
+  }
+}
+
+}  // namespace
+
+template <typename T>
+int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, hipStream_t st) {
+  double *d_crds = nullptr, *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr, *d_nrm = nullptr,
+         *d_tan = nullptr, *d_area = nullptr;
+  int64_t *d_elve = nullptr, *d_fael = nullptr, *d_elfa = nullptr;
+  int32_t* d_type = nullptr;
+  int* d_bad = nullptr;
+  int rc = EDGE_OK;
+  const int64_t ne = in.n_el, nf = in.n_fa;
+  auto cleanup = [&]() {
+    hipFree(d_crds); hipFree(d_rho); hipFree(d_lam); hipFree(d_mu); hipFree(d_nrm); hipFree(d_tan);
+    hipFree(d_area); hipFree(d_elve); hipFree(d_fael); hipFree(d_elfa); hipFree(d_type);
+    hipFree(d_bad);
+  };
+#define EDGE_SETUP_TRY(x)                                                                         \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) {                                                                       \
+      cleanup();                                                                                  \
+      return fail(EDGE_ERR_HIP, std::string("setup: ") + #x + ": " + hipGetErrorString(e_));      \
+    }                                                                                             \
+  } while (0)
+  EDGE_SETUP_TRY(hipMalloc(&d_crds, sizeof(double) * 3 * in.n_ve));
+  EDGE_SETUP_TRY(hipMalloc(&d_elve, sizeof(int64_t) * 4 * ne));
+  EDGE_SETUP_TRY(hipMalloc(&d_rho, sizeof(double) * ne));
+  EDGE_SETUP_TRY(hipMalloc(&d_lam, sizeof(double) * ne));
+  EDGE_SETUP_TRY(hipMalloc(&d_mu, sizeof(double) * ne));
+  EDGE_SETUP_TRY(hipMalloc(&d_fael, sizeof(int64_t) * 2 * nf));
+  EDGE_SETUP_TRY(hipMalloc(&d_elfa, sizeof(int64_t) * 4 * ne));
+  EDGE_SETUP_TRY(hipMalloc(&d_type, sizeof(int32_t) * nf));
+  EDGE_SETUP_TRY(hipMalloc(&d_nrm, sizeof(double) * 3 * nf));
+  EDGE_SETUP_TRY(hipMalloc(&d_tan, sizeof(double) * 6 * nf));
+  EDGE_SETUP_TRY(hipMalloc(&d_area, sizeof(double) * nf));
+  EDGE_SETUP_TRY(hipMalloc(&d_bad, sizeof(int)));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_crds, in.ve_crds, sizeof(double) * 3 * in.n_ve, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_elve, in.el_ve, sizeof(int64_t) * 4 * ne, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_rho, in.rho, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_lam, in.lam, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_mu, in.mu, sizeof(double) * ne, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_fael, in.fa_el, sizeof(int64_t) * 2 * nf, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_elfa, in.el_fa, sizeof(int64_t) * 4 * ne, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_type, in.fa_type, sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_nrm, in.normals, sizeof(double) * 3 * nf, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_tan, in.tangents, sizeof(double) * 6 * nf, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemcpyAsync(d_area, in.areas, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+  EDGE_SETUP_TRY(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+  // sentinel values: every (element, face) slot must be written by its face
+  EDGE_SETUP_TRY(hipMemsetAsync(d_fs, 0xFF, sizeof(T) * 2 * 4 * 81 * ne, st));
+  const int bs = 128;
+  hipLaunchKernelGGL(k_star<T>, dim3((unsigned)((ne + bs - 1) / bs)), dim3(bs), 0, st, ne, d_crds, d_elve, d_rho,
+                     d_lam, d_mu, d_star, d_jdet, d_bad);
+  EDGE_SETUP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_flux_solvers<T>, dim3((unsigned)((nf + bs - 1) / bs)), dim3(bs), 0, st, nf, ne, d_fael,
+                     d_elfa, d_type, d_nrm, d_tan, d_area, d_jdet, d_rho, d_lam, d_mu, d_fs, d_bad);
+  EDGE_SETUP_TRY(hipGetLastError());
+  int bad = 0;
+  EDGE_SETUP_TRY(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  EDGE_SETUP_TRY(hipStreamSynchronize(st));
+  if (bad & 1) rc = fail(EDGE_ERR_MESH, "element with non-positive Jacobian determinant");
+  else if (bad & 2) rc = fail(EDGE_ERR_MESH, "inconsistent face-element relations");
+  else if (bad & 4) rc = fail(EDGE_ERR_MESH, "face with non-positive area");
+  cleanup();
+#undef EDGE_SETUP_TRY
+  return rc;
+}
+
+template int build_setup<double>(const SetupInput&, double*, double*, double*, hipStream_t);
+template int build_setup<float>(const SetupInput&, float*, float*, double*, hipStream_t);
+
+}  // namespace edge

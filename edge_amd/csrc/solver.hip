@@ -1,0 +1,452 @@
+// The element-update solver behind the C ABI: device copies of the
+// reference-element matrices and of the neighbour relations, the set-up, and
+// the local / neighbour steps.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "kernels.hpp"
+#include "kernels_mfma.hpp"
+#include "setup.hpp"
+
+struct edge_el_solver {
+  edge_el_config cfg;
+  int64_t n_el = 0;
+  int n_modes = 0, n_face_modes = 0;
+  int kernel = 1;  // 1 generic, 2 MFMA-tiled
+  size_t real_bytes = 8;
+  // device allocations
+  void* mats = nullptr;  // stiffTn | stiffV | fluxLN | fluxT  (precision T)
+  void* star = nullptr;  // [nEl][3][81]
+  void* fs = nullptr;    // [2][nEl][4][81]
+  int32_t* ne = nullptr;
+  uint8_t* fmat = nullptr;
+  double* jdet = nullptr;  // [nEl] Jacobian determinants (geometry-built solvers only)
+  double* mass = nullptr;  // [N] diagonal reference mass matrix
+  // graph path
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  void* graph_dofs = nullptr;
+  void* graph_tint = nullptr;
+  double graph_dt = 0;
+  hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+template <typename T>
+edge::SolverDev<T> dev_view(const edge_el_solver* s) {
+  edge::SolverDev<T> d;
+  d.order = s->cfg.order;
+  d.n_runs = s->cfg.n_runs;
+  d.n_el = s->n_el;
+  const size_t N = s->n_modes, NF = s->n_face_modes;
+  const T* m = static_cast<const T*>(s->mats);
+  d.dg.stiffTn = m;
+  d.dg.stiffV = m + 3 * N * N;
+  d.dg.fluxLN = m + 6 * N * N;
+  d.dg.fluxT = m + 6 * N * N + 16 * N * NF;
+  d.nb.ne = s->ne;
+  d.nb.fmat = s->fmat;
+  d.star = static_cast<const T*>(s->star);
+  d.fs = static_cast<const T*>(s->fs);
+  return d;
+}
+
+template <typename T>
+int upload_mats(edge_el_solver* s, const edge::DgTables& t) {
+  const size_t N = t.n, NF = t.nf;
+  const size_t total = 6 * N * N + 16 * N * NF + 4 * NF * N;
+  std::vector<T> h(total);
+  size_t o = 0;
+  for (size_t i = 0; i < 3 * N * N; i++) h[o++] = (T)(-t.stiffT[i]);
+  for (size_t i = 0; i < 3 * N * N; i++) h[o++] = (T)t.stiffV[i];
+  for (size_t i = 0; i < 4 * N * NF; i++) h[o++] = (T)t.fluxL[i];
+  for (size_t i = 0; i < 12 * N * NF; i++) h[o++] = (T)t.fluxN[i];
+  for (size_t i = 0; i < 4 * NF * N; i++) h[o++] = (T)t.fluxT[i];
+  EDGE_HIP_TRY(hipMalloc(&s->mats, sizeof(T) * total));
+  EDGE_HIP_TRY(hipMemcpy(s->mats, h.data(), sizeof(T) * total, hipMemcpyHostToDevice));
+  return EDGE_OK;
+}
+
+int check_cfg(const edge_el_config* c) {
+  if (!c) return edge::fail(EDGE_ERR_ARG, "config missing");
+  if (c->order < edge::kMinOrder || c->order > edge::kMaxOrder)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "order must be in 2..5");
+  if (c->n_runs != 1 && c->n_runs != 16) return edge::fail(EDGE_ERR_UNSUPPORTED, "n_runs must be 1 or 16");
+  if (c->precision != 64 && c->precision != 32) return edge::fail(EDGE_ERR_UNSUPPORTED, "precision must be 32 or 64");
+  if (c->kernel < 0 || c->kernel > 2) return edge::fail(EDGE_ERR_ARG, "kernel selector");
+  return EDGE_OK;
+}
+
+int check_mesh(const edge_mesh_in* m) {
+  if (!m || m->n_el <= 0 || m->n_fa <= 0 || !m->el_fa || !m->fa_el || !m->el_fa_el || !m->f_id || !m->v_id ||
+      !m->fa_type)
+    return edge::fail(EDGE_ERR_ARG, "mesh connectivity missing");
+  if (m->n_el >= (int64_t)1 << 31) return edge::fail(EDGE_ERR_UNSUPPORTED, "more than 2^31 elements");
+  return EDGE_OK;
+}
+
+// neighbour element and flux-matrix id per element face
+// (src/impl/seismic/solvers/AderDg.hpp:586-600, :686-698; kernels/SurfInt.hpp:127-130)
+int build_neigh(edge_el_solver* s, const edge_mesh_in* m) {
+  const int64_t ne = m->n_el;
+  std::vector<int32_t> nb(4 * ne);
+  std::vector<uint8_t> fm(4 * ne);
+  for (int64_t el = 0; el < ne; el++)
+    for (int fa = 0; fa < 4; fa++) {
+      const int64_t f = m->el_fa[el * 4 + fa];
+      if (f < 0 || f >= m->n_fa) return edge::fail(EDGE_ERR_MESH, "element face id out of range");
+      const int32_t ty = m->fa_type[f];
+      if ((ty & EDGE_FA_OUTFLOW) == EDGE_FA_OUTFLOW) {
+        nb[el * 4 + fa] = -1;
+        fm[el * 4 + fa] = 0;
+      } else if ((ty & EDGE_FA_FREE_SURFACE) == EDGE_FA_FREE_SURFACE) {
+        nb[el * 4 + fa] = (int32_t)el;
+        fm[el * 4 + fa] = (uint8_t)fa;
+      } else {
+        const int64_t n = m->el_fa_el[el * 4 + fa];
+        const unsigned v = m->v_id[el * 4 + fa], fi = m->f_id[el * 4 + fa];
+        if (n < 0 || n >= ne || v > 2 || fi > 3)
+          return edge::fail(EDGE_ERR_MESH, "interior face without a valid neighbour");
+        nb[el * 4 + fa] = (int32_t)n;
+        fm[el * 4 + fa] = (uint8_t)(4 + v * 4 + fi);
+      }
+    }
+  EDGE_HIP_TRY(hipMalloc(&s->ne, sizeof(int32_t) * 4 * ne));
+  EDGE_HIP_TRY(hipMalloc(&s->fmat, 4 * ne));
+  EDGE_HIP_TRY(hipMemcpy(s->ne, nb.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice));
+  EDGE_HIP_TRY(hipMemcpy(s->fmat, fm.data(), 4 * ne, hipMemcpyHostToDevice));
+  return EDGE_OK;
+}
+
+int pick_kernel(edge_el_solver* s) {
+  const bool mfma_ok = edge::mfma_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision);
+  const bool gen_ok = edge::generic_supported(s->cfg.order, s->cfg.n_runs, (int)s->real_bytes);
+  if (s->cfg.kernel == 2) {
+    if (!mfma_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no MFMA-tiled kernel for this configuration");
+    s->kernel = 2;
+  } else if (s->cfg.kernel == 1) {
+    if (!gen_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "generic kernel does not fit this configuration");
+    s->kernel = 1;
+  } else {
+    s->kernel = mfma_ok ? 2 : 1;
+    if (!mfma_ok && !gen_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no kernel for this configuration");
+  }
+  return EDGE_OK;
+}
+
+int create_common(const edge_el_config* cfg, const edge_mesh_in* mesh, edge_el_solver** out,
+                  edge_el_solver** made) {
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  if ((rc = check_mesh(mesh))) return rc;
+  if (!out) return edge::fail(EDGE_ERR_ARG, "output handle missing");
+  edge::DgTables t;
+  if (!edge::dg_tables(cfg->order, &t)) return edge::fail(EDGE_ERR_UNSUPPORTED, "order not generated");
+  edge_el_solver* s = new (std::nothrow) edge_el_solver();
+  if (!s) return edge::fail(EDGE_ERR_NOMEM, "solver");
+  *made = s;
+  s->cfg = *cfg;
+  s->n_el = mesh->n_el;
+  s->n_modes = t.n;
+  s->n_face_modes = t.nf;
+  s->real_bytes = cfg->precision == 64 ? 8 : 4;
+  if ((rc = pick_kernel(s))) return rc;
+  rc = cfg->precision == 64 ? upload_mats<double>(s, t) : upload_mats<float>(s, t);
+  if (rc) return rc;
+  if ((rc = build_neigh(s, mesh))) return rc;
+  EDGE_HIP_TRY(hipMalloc(&s->star, s->real_bytes * 243 * s->n_el));
+  EDGE_HIP_TRY(hipMalloc(&s->fs, s->real_bytes * 2 * 324 * s->n_el));
+  EDGE_HIP_TRY(hipMalloc(&s->mass, sizeof(double) * t.n));
+  EDGE_HIP_TRY(hipMemcpy(s->mass, t.mass, sizeof(double) * t.n, hipMemcpyHostToDevice));
+  return EDGE_OK;
+}
+
+template <typename T>
+int local_t(edge_el_solver* s, int64_t first, int64_t n, double dt, void* dofs, void* t_int, hipStream_t st) {
+  const auto d = dev_view<T>(s);
+  if (s->kernel == 2) return edge::launch_local_mfma<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
+  return edge::launch_local_generic<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
+}
+
+template <typename T>
+int neigh_t(edge_el_solver* s, int64_t first, int64_t n, const void* t_int, void* dofs, hipStream_t st) {
+  const auto d = dev_view<T>(s);
+  if (s->kernel == 2) return edge::launch_neigh_mfma<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
+  return edge::launch_neigh_generic<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
+}
+
+int check_range(const edge_el_solver* s, int64_t first, int64_t n) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (first < 0 || n < 0 || first + n > s->n_el) return edge::fail(EDGE_ERR_ARG, "element range out of bounds");
+  return EDGE_OK;
+}
+
+int one_step(edge_el_solver* s, double dt, void* dofs, void* t_int, hipStream_t st) {
+  int rc = s->cfg.precision == 64 ? local_t<double>(s, 0, s->n_el, dt, dofs, t_int, st)
+                                  : local_t<float>(s, 0, s->n_el, dt, dofs, t_int, st);
+  if (rc) return rc;
+  return s->cfg.precision == 64 ? neigh_t<double>(s, 0, s->n_el, t_int, dofs, st)
+                                : neigh_t<float>(s, 0, s->n_el, t_int, dofs, st);
+}
+
+void drop_graph(edge_el_solver* s) {
+  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  s->graph_exec = nullptr;
+  s->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mesh, const double* rho,
+                              const double* lam, const double* mu, edge_el_solver** out) {
+  if (out) *out = nullptr;
+  if (!mesh || !mesh->ve_crds || !mesh->el_ve || !mesh->normals || !mesh->tangents || !mesh->areas || !rho ||
+      !lam || !mu)
+    return edge::fail(EDGE_ERR_ARG, "mesh geometry or material missing");
+  for (int64_t el = 0; mesh && el < mesh->n_el; el++)
+    if (!(rho[el] > 0) || !(mu[el] > 0) || !(lam[el] + 2 * mu[el] > 0))
+      return edge::fail(EDGE_ERR_ARG, "non-physical material parameters");
+  edge_el_solver* s = nullptr;
+  int rc = create_common(cfg, mesh, out, &s);
+  if (rc == EDGE_OK) {
+    int64_t nve = 0;
+    for (int64_t i = 0; i < 4 * mesh->n_el; i++) nve = std::max(nve, mesh->el_ve[i] + 1);
+    edge::SetupInput in{mesh->n_el, mesh->n_fa, nve,         mesh->ve_crds, mesh->el_ve, mesh->el_fa,
+                        mesh->fa_el, mesh->fa_type, mesh->normals, mesh->tangents, mesh->areas, rho,
+                        lam,         mu};
+    if (hipMalloc(&s->jdet, sizeof(double) * s->n_el) != hipSuccess)
+      rc = edge::fail(EDGE_ERR_HIP, "allocation of Jacobian determinants");
+    else
+      rc = cfg->precision == 64 ? edge::build_setup<double>(in, (double*)s->star, (double*)s->fs, s->jdet, nullptr)
+                                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, nullptr);
+  }
+  if (rc != EDGE_OK) {
+    if (s) edge_el_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_create_raw(const edge_el_config* cfg, const edge_mesh_in* mesh, const double* star,
+                                  const double* fs, edge_el_solver** out) {
+  if (out) *out = nullptr;
+  if (!star || !fs) return edge::fail(EDGE_ERR_ARG, "star matrices or flux solvers missing");
+  edge_el_solver* s = nullptr;
+  int rc = create_common(cfg, mesh, out, &s);
+  if (rc == EDGE_OK) {
+    const size_t ns = 243 * (size_t)s->n_el, nf = 648 * (size_t)s->n_el;
+    if (cfg->precision == 64) {
+      if (hipMemcpy(s->star, star, 8 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(s->fs, fs, 8 * nf, hipMemcpyHostToDevice) != hipSuccess)
+        rc = edge::fail(EDGE_ERR_HIP, "upload of star matrices / flux solvers");
+    } else {
+      std::vector<float> a(star, star + ns), b(fs, fs + nf);
+      if (hipMemcpy(s->star, a.data(), 4 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(s->fs, b.data(), 4 * nf, hipMemcpyHostToDevice) != hipSuccess)
+        rc = edge::fail(EDGE_ERR_HIP, "upload of star matrices / flux solvers");
+    }
+  }
+  if (rc != EDGE_OK) {
+    if (s) edge_el_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_destroy(edge_el_solver* s) {
+  if (!s) return EDGE_OK;
+  drop_graph(s);
+  hipFree(s->mats);
+  hipFree(s->star);
+  hipFree(s->fs);
+  hipFree(s->ne);
+  hipFree(s->fmat);
+  hipFree(s->jdet);
+  hipFree(s->mass);
+  delete s;
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_get_setup(const edge_el_solver* s, double* star, double* fs) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  const size_t ns = 243 * (size_t)s->n_el, nf = 648 * (size_t)s->n_el;
+  if (s->cfg.precision == 64) {
+    if (star) EDGE_HIP_TRY(hipMemcpy(star, s->star, 8 * ns, hipMemcpyDeviceToHost));
+    if (fs) EDGE_HIP_TRY(hipMemcpy(fs, s->fs, 8 * nf, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> a;
+    if (star) {
+      a.resize(ns);
+      EDGE_HIP_TRY(hipMemcpy(a.data(), s->star, 4 * ns, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < ns; i++) star[i] = a[i];
+    }
+    if (fs) {
+      a.resize(nf);
+      EDGE_HIP_TRY(hipMemcpy(a.data(), s->fs, 4 * nf, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < nf; i++) fs[i] = a[i];
+    }
+  }
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_local(edge_el_solver* s, int64_t first, int64_t n_els, double dt, void* dofs, void* t_int,
+                             void* stream) {
+  int rc = check_range(s, first, n_els);
+  if (rc) return rc;
+  if (!dofs || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  hipStream_t st = (hipStream_t)stream;
+  return s->cfg.precision == 64 ? local_t<double>(s, first, n_els, dt, dofs, t_int, st)
+                                : local_t<float>(s, first, n_els, dt, dofs, t_int, st);
+}
+
+extern "C" int edge_el_neigh(edge_el_solver* s, int64_t first, int64_t n_els, const void* t_int, void* dofs,
+                             void* stream) {
+  int rc = check_range(s, first, n_els);
+  if (rc) return rc;
+  if (!dofs || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  hipStream_t st = (hipStream_t)stream;
+  return s->cfg.precision == 64 ? neigh_t<double>(s, first, n_els, t_int, dofs, st)
+                                : neigh_t<float>(s, first, n_els, t_int, dofs, st);
+}
+
+extern "C" int edge_el_step(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* t_int, int use_graph,
+                            void* stream) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (!dofs || !t_int || n_steps < 0) return edge::fail(EDGE_ERR_ARG, "step arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) {
+    for (int64_t i = 0; i < n_steps; i++) {
+      int rc = one_step(s, dt, dofs, t_int, st);
+      if (rc) return rc;
+    }
+    return EDGE_OK;
+  }
+  if (s->graph_exec && (s->graph_dofs != dofs || s->graph_tint != t_int || s->graph_dt != dt ||
+                        s->graph_stream != st))
+    drop_graph(s);
+  if (!s->graph_exec) {
+    // capture needs a non-default stream
+    hipStream_t cap = st;
+    bool own = false;
+    if (!cap) {
+      EDGE_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      own = true;
+    }
+    EDGE_HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    int rc = one_step(s, dt, dofs, t_int, cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &g);
+    if (rc == EDGE_OK && e != hipSuccess) rc = edge::fail(EDGE_ERR_HIP, hipGetErrorString(e));
+    if (rc == EDGE_OK) {
+      e = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = edge::fail(EDGE_ERR_HIP, hipGetErrorString(e));
+    }
+    s->graph = g;
+    if (own) hipStreamDestroy(cap);
+    if (rc) {
+      drop_graph(s);
+      return rc;
+    }
+    s->graph_dofs = dofs;
+    s->graph_tint = t_int;
+    s->graph_dt = dt;
+    s->graph_stream = st;
+  }
+  for (int64_t i = 0; i < n_steps; i++) EDGE_HIP_TRY(hipGraphLaunch(s->graph_exec, st));
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_info(const edge_el_solver* s, int64_t* n_el, int32_t* n_modes, int32_t* n_runs,
+                            int64_t* dof_bytes) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (n_el) *n_el = s->n_el;
+  if (n_modes) *n_modes = s->n_modes;
+  if (n_runs) *n_runs = s->cfg.n_runs;
+  if (dof_bytes) *dof_bytes = (int64_t)s->real_bytes * 9 * s->n_modes * s->cfg.n_runs * s->n_el;
+  return EDGE_OK;
+}
+
+namespace {
+
+// Per-block partial sums of |J| M_n (a - b)^2 and |J| M_n b^2 per (quantity, run).
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_l2(int64_t n_el, int n_md, int n_crs, const double* __restrict__ jdet,
+                                            const double* __restrict__ mass, const TA* __restrict__ a,
+                                            const TB* __restrict__ b, double* __restrict__ part) {
+  // one block per chunk of elements; thread t accumulates entry t of [9][R]
+  const int nqr = 9 * n_crs;
+  const int64_t per = (n_el + gridDim.x - 1) / gridDim.x;
+  const int64_t e0 = blockIdx.x * per, e1 = min(n_el, e0 + per);
+  for (int qr = threadIdx.x; qr < nqr; qr += blockDim.x) {
+    const int q = qr / n_crs, r = qr % n_crs;
+    double d2 = 0, r2 = 0;
+    for (int64_t el = e0; el < e1; el++) {
+      const double j = fabs(jdet[el]);
+      const size_t base = ((size_t)el * 9 + q) * n_md * n_crs + r;
+      for (int md = 0; md < n_md; md++) {
+        const double av = (double)a[base + (size_t)md * n_crs];
+        const double bv = b ? (double)b[base + (size_t)md * n_crs] : av;
+        const double w = j * mass[md];
+        d2 += w * (av - bv) * (av - bv);
+        r2 += w * bv * bv;
+      }
+    }
+    part[(size_t)blockIdx.x * 2 * nqr + qr] = d2;
+    part[(size_t)blockIdx.x * 2 * nqr + nqr + qr] = r2;
+  }
+}
+
+template <typename TA, typename TB>
+int l2_launch(edge_el_solver* s, const void* a, const void* b, double* d_part, unsigned nblk, hipStream_t st) {
+  hipLaunchKernelGGL((k_l2<TA, TB>), dim3(nblk), dim3(256), 0, st, s->n_el, s->n_modes, s->cfg.n_runs, s->jdet,
+                     s->mass, (const TA*)a, (const TB*)b, d_part);
+  EDGE_HIP_TRY(hipGetLastError());
+  return EDGE_OK;
+}
+
+}  // namespace
+
+extern "C" int edge_el_l2_norms(edge_el_solver* s, const void* a, int prec_a, const void* b, int prec_b,
+                                double* diff_sq, double* ref_sq, void* stream) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (!s->jdet) return edge::fail(EDGE_ERR_UNSUPPORTED, "norms need a solver built from geometry");
+  if (!a || (prec_a != 32 && prec_a != 64) || (b && prec_b != 32 && prec_b != 64))
+    return edge::fail(EDGE_ERR_ARG, "norm arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int nqr = 9 * s->cfg.n_runs;
+  const unsigned nblk = 1024;
+  double* d_part = nullptr;
+  EDGE_HIP_TRY(hipMalloc(&d_part, sizeof(double) * 2 * nqr * nblk));
+  int rc;
+  const int pb = b ? prec_b : prec_a;
+  if (prec_a == 64 && pb == 64) rc = l2_launch<double, double>(s, a, b, d_part, nblk, st);
+  else if (prec_a == 64) rc = l2_launch<double, float>(s, a, b, d_part, nblk, st);
+  else if (pb == 64) rc = l2_launch<float, double>(s, a, b, d_part, nblk, st);
+  else rc = l2_launch<float, float>(s, a, b, d_part, nblk, st);
+  std::vector<double> h(2 * nqr * nblk);
+  if (rc == EDGE_OK &&
+      hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st) == hipSuccess &&
+      hipStreamSynchronize(st) == hipSuccess) {
+    for (int i = 0; i < nqr; i++) {
+      double d2 = 0, r2 = 0;
+      for (unsigned k = 0; k < nblk; k++) {
+        d2 += h[(size_t)k * 2 * nqr + i];
+        r2 += h[(size_t)k * 2 * nqr + nqr + i];
+      }
+      if (diff_sq) diff_sq[i] = d2;
+      if (ref_sq) ref_sq[i] = r2;
+    }
+  } else if (rc == EDGE_OK) {
+    rc = edge::fail(EDGE_ERR_HIP, "norm reduction");
+  }
+  hipFree(d_part);
+  return rc;
+}

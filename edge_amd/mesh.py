@@ -1,0 +1,65 @@
+"""Box meshes of tet4 elements (6 Kuhn tets per hex) through the library's
+C++ generator, returned as numpy arrays in the reference's entity layout."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC  # noqa: F401
+
+
+class Mesh:
+    """Entity arrays of a tet4 mesh (see include/edge_amd.h, edge_mesh_out)."""
+
+    def __init__(self, **arrays):
+        self.__dict__.update(arrays)
+        self.n_el = self.el_ve.shape[0]
+        self.n_fa = self.fa_el.shape[0]
+        self.n_ve = self.ve_crds.shape[0]
+
+    def mesh_in(self):
+        """C struct edge_mesh_in pointing at this mesh's arrays."""
+        m = _lib.MeshIn()
+        m.n_el, m.n_fa = self.n_el, self.n_fa
+        for name in ["ve_crds", "el_ve", "el_fa", "fa_el", "el_fa_el", "f_id", "v_id", "fa_type", "normals",
+                     "tangents", "areas"]:
+            setattr(m, name, self.__dict__[name].ctypes.data)
+        return m
+
+    def el_crds(self):
+        """Vertex coordinates per element [nEl][4][3]."""
+        return self.ve_crds[self.el_ve]
+
+
+def box(n, extent=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), bc=(FA_PERIODIC,) * 3, jitter=0.0, seed=3343,
+        block=1):
+    """Box of n[0] x n[1] x n[2] hexes, each cut into 6 tets.
+
+    bc per axis: FA_PERIODIC, FA_FREE_SURFACE or FA_OUTFLOW.  jitter moves the
+    interior vertices by U[-jitter h, jitter h] per axis.  block orders the
+    elements in blocks of block^3 hexes."""
+    if np.isscalar(n):
+        n = (n, n, n)
+    p = _lib.BoxParams()
+    p.n[:] = [int(x) for x in n]
+    p.origin[:] = [float(x) for x in origin]
+    p.extent[:] = [float(x) for x in extent]
+    p.bc[:] = [int(x) for x in bc]
+    p.jitter = float(jitter)
+    p.seed = int(seed)
+    p.block = int(block)
+    L = _lib.lib()
+    nve, nel, nfa = C.c_int64(), C.c_int64(), C.c_int64()
+    _lib.check(L.edge_mesh_box_sizes(C.byref(p), C.byref(nve), C.byref(nel), C.byref(nfa)))
+    nve, nel, nfa = nve.value, nel.value, nfa.value
+    a = dict(
+        ve_crds=np.zeros((nve, 3)), el_ve=np.zeros((nel, 4), np.int64), el_fa=np.zeros((nel, 4), np.int64),
+        fa_el=np.zeros((nfa, 2), np.int64), el_fa_el=np.zeros((nel, 4), np.int64),
+        f_id=np.zeros((nel, 4), np.uint16), v_id=np.zeros((nel, 4), np.uint16), fa_type=np.zeros(nfa, np.int32),
+        normals=np.zeros((nfa, 3)), tangents=np.zeros((nfa, 2, 3)), areas=np.zeros(nfa), in_dia=np.zeros(nel),
+        volume=np.zeros(nel))
+    m = _lib.MeshOut()
+    for k, v in a.items():
+        setattr(m, k, v.ctypes.data)
+    _lib.check(L.edge_mesh_box(C.byref(p), C.byref(m)))
+    return Mesh(**a)

@@ -1,0 +1,123 @@
+"""Host-side mirror of edge::seismic::solvers::AderDg for the element update
+(src/impl/seismic/solvers/AderDg.hpp:225-722): a solver handle over
+libedge_amd's C ABI.  DOF and time-integrated DOF arrays are caller-owned
+device tensors in the reference layout [el][9][mode][run].
+
+torch is used only for device memory and streams.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .setups import n_modes
+
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA = 0, 1, 2
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+
+
+class ElasticSolver:
+    """ADER-DG element update (elastic, tet4) on one GPU.
+
+    Built either from mesh geometry and materials (star matrices and flux
+    solvers are then set up in FP64 on the GPU), or from caller-given star
+    matrices [nEl][3][81] and flux solvers [2][nEl][4][81] (``raw``)."""
+
+    def __init__(self, mesh, order, n_runs=1, precision=64, rho=None, lam=None, mu=None, star=None, fs=None,
+                 kernel=KERNEL_AUTO):
+        self.mesh = mesh
+        self.order = order
+        self.n_runs = n_runs
+        self.precision = precision
+        self.n_modes = n_modes(order)
+        cfg = _lib.ElConfig(order, n_runs, precision, kernel)
+        mi = mesh.mesh_in()
+        h = C.c_void_p()
+        L = _lib.lib()
+        if star is not None:
+            star = np.ascontiguousarray(star, dtype=np.float64)
+            fs = np.ascontiguousarray(fs, dtype=np.float64)
+            _lib.check(L.edge_el_create_raw(C.byref(cfg), C.byref(mi), star.ctypes.data, fs.ctypes.data,
+                                            C.byref(h)))
+        else:
+            rho, lam, mu = (np.ascontiguousarray(a, dtype=np.float64) for a in (rho, lam, mu))
+            _lib.check(L.edge_el_create(C.byref(cfg), C.byref(mi), rho.ctypes.data, lam.ctypes.data,
+                                        mu.ctypes.data, C.byref(h)))
+        self._h = h
+
+    # ------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().edge_el_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def dtype(self):
+        import torch
+        return torch.float64 if self.precision == 64 else torch.float32
+
+    def dof_shape(self):
+        return (self.mesh.n_el, 9, self.n_modes, self.n_runs)
+
+    def alloc(self, device="cuda"):
+        import torch
+        return torch.zeros(self.dof_shape(), dtype=self.dtype, device=device)
+
+    def setup(self):
+        """(star [nEl][3][81], fs [2][nEl][4][81]) as FP64 host arrays."""
+        ne = self.mesh.n_el
+        star = np.zeros((ne, 3, 81))
+        fs = np.zeros((2, ne, 4, 81))
+        _lib.check(_lib.lib().edge_el_get_setup(self._h, star.ctypes.data, fs.ctypes.data))
+        return star, fs
+
+    def _check_arr(self, a):
+        if tuple(a.shape) != self.dof_shape() or a.dtype != self.dtype or not a.is_cuda or not a.is_contiguous():
+            raise ValueError(f"expected a contiguous {self.dtype} device tensor of shape {self.dof_shape()}")
+
+    # ------------------------------------------------------------------
+    def local(self, dofs, t_int, dt, first=0, n=None, stream=None):
+        """AderDg::local over elements [first, first+n)."""
+        self._check_arr(dofs)
+        self._check_arr(t_int)
+        n = self.mesh.n_el - first if n is None else n
+        _lib.check(_lib.lib().edge_el_local(self._h, first, n, float(dt), dofs.data_ptr(), t_int.data_ptr(),
+                                            _stream_ptr(stream)))
+
+    def neigh(self, dofs, t_int, first=0, n=None, stream=None):
+        """AderDg::neigh over elements [first, first+n)."""
+        self._check_arr(dofs)
+        self._check_arr(t_int)
+        n = self.mesh.n_el - first if n is None else n
+        _lib.check(_lib.lib().edge_el_neigh(self._h, first, n, t_int.data_ptr(), dofs.data_ptr(),
+                                            _stream_ptr(stream)))
+
+    def step(self, dofs, t_int, dt, n_steps=1, graph=False, stream=None):
+        """n_steps global time steps (local, then neigh)."""
+        self._check_arr(dofs)
+        self._check_arr(t_int)
+        _lib.check(_lib.lib().edge_el_step(self._h, int(n_steps), float(dt), dofs.data_ptr(), t_int.data_ptr(),
+                                           1 if graph else 0, _stream_ptr(stream)))
+
+    def l2_norms(self, a, b=None, stream=None):
+        """FP64 (||a-b||^2, ||b||^2) per [9][R] through the mass matrix."""
+        import torch
+        r = self.n_runs
+        d = np.zeros((9, r))
+        n = np.zeros((9, r))
+        pa = 64 if a.dtype == torch.float64 else 32
+        pb = 0 if b is None else (64 if b.dtype == torch.float64 else 32)
+        _lib.check(_lib.lib().edge_el_l2_norms(self._h, a.data_ptr(), pa, None if b is None else b.data_ptr(),
+                                               pb, d.ctypes.data, n.ctypes.data, _stream_ptr(stream)))
+        return d, n

@@ -1,0 +1,193 @@
+/*
+ * edge_amd: MI355X-native ADER-DG element update for elastic tet4 meshes.
+ *
+ * C ABI of libedge_amd.so.  Plain pointers and sizes only.  Device pointers
+ * are HIP device allocations owned by the caller; host pointers are ordinary
+ * host memory.  Every entry point returns EDGE_OK (0) or an error code;
+ * edge_last_error() returns a message for the calling thread's last failure.
+ *
+ * Layouts at the boundary are the reference's:
+ *   elastic DOFs / time-integrated DOFs  [el][9][mode][run]   (run innermost)
+ *   star matrices                        [el][3][9*9]
+ *   flux solvers                         [2][el][4][9*9]      (0: local, 1: neighbour)
+ * with mode counts N = O(O+1)(O+2)/6 and face modes NF = O(O+1)/2 for order O.
+ */
+#ifndef EDGE_AMD_H
+#define EDGE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define EDGE_OK 0
+#define EDGE_ERR_ARG 1
+#define EDGE_ERR_HIP 2
+#define EDGE_ERR_MESH 3
+#define EDGE_ERR_UNSUPPORTED 4
+#define EDGE_ERR_NOMEM 5
+
+/* ---- sentinels and face types (values of the reference's seismic spType,
+ *      src/impl/seismic/const.inc:143-148; "no id" as in
+ *      src/impl/seismic/solvers/AderDg.hpp:692-698) ---- */
+#define EDGE_NO_ID16 0xFFFFu
+#define EDGE_FA_INTERIOR 0
+#define EDGE_FA_FREE_SURFACE 101
+#define EDGE_FA_OUTFLOW 105
+#define EDGE_FA_PERIODIC 106
+
+/* Message of the calling thread's last error ("" if none). */
+const char *edge_last_error(void);
+
+/* Library version string. */
+const char *edge_version(void);
+
+/* ====================================================================== */
+/* Reference-element matrices (tet4), replaces edge::dg::Basis /           */
+/* dg::{TimePred,VolInt,SurfInt}::store*Dense fed from edge::pre::dg::g_*  */
+/* (src/dg/Basis.h:34-62, :344-401; src/dg/TimePred.hpp:69-127,            */
+/* src/dg/VolInt.hpp:63-118, src/dg/SurfInt.hpp:76-170).                   */
+/* ====================================================================== */
+
+/* Mode counts for order O (= polynomial degree + 1). */
+int edge_dg_sizes(int order, int *n_modes, int *n_face_modes);
+
+/* Copies the matrices of order O into host arrays (any pointer may be NULL):
+ *   mass[N] (diagonal), stiffT[3][N][N] = K_d^T M^-1 (not negated),
+ *   stiffV[3][N][N] = K_d M^-1, fluxL[4][N][NF], fluxN[12][N][NF],
+ *   fluxT[4][NF][N]. */
+int edge_dg_matrices(int order, double *mass, double *stiff_t, double *stiff_v, double *flux_l,
+                     double *flux_n, double *flux_t);
+
+/* Evaluates the N modal basis functions of order O at reference points
+ * pts[n][3] (xi in the unit tet): val[n][N].  (src/dg/Basis.cpp:136-161) */
+int edge_dg_eval_basis(int order, int64_t n_pts, const double *pts, double *val);
+
+/* Collapsed Gauss-Legendre rule with n1d points per direction on the unit
+ * tet (src/dg/QuadraturePoints.h:424-530): pts[n1d^3][3], w[n1d^3]. */
+int edge_dg_quad_tet(int n1d, double *pts, double *w);
+
+/* ====================================================================== */
+/* Structured box meshes (the conventions of tools/edge_v, see mesh.cpp)   */
+/* ====================================================================== */
+typedef struct {
+  int64_t n[3];      /* hexes per axis */
+  double origin[3];  /* lower corner */
+  double extent[3];  /* edge lengths */
+  int32_t bc[3];     /* per axis: EDGE_FA_PERIODIC, _FREE_SURFACE or _OUTFLOW */
+  double jitter;     /* interior vertices moved by U[-jitter h, jitter h] per axis */
+  uint64_t seed;     /* jitter seed */
+  int64_t block;     /* element order: hex blocks of block^3 (1 = lexicographic) */
+} edge_box_params;
+
+typedef struct { /* caller-allocated outputs, sizes from edge_mesh_box_sizes */
+  double *ve_crds;   /* [nVe][3] */
+  int64_t *el_ve;    /* [nEl][4] */
+  int64_t *el_fa;    /* [nEl][4] */
+  int64_t *fa_el;    /* [nFa][2], -1 = none */
+  int64_t *el_fa_el; /* [nEl][4], -1 = none */
+  uint16_t *f_id;    /* [nEl][4] fIdElFaEl */
+  uint16_t *v_id;    /* [nEl][4] vIdElFaEl */
+  int32_t *fa_type;  /* [nFa] */
+  double *normals;   /* [nFa][3] outward w.r.t. fa_el[.][0] */
+  double *tangents;  /* [nFa][2][3] */
+  double *areas;     /* [nFa] */
+  double *in_dia;    /* [nEl] */
+  double *volume;    /* [nEl] */
+} edge_mesh_out;
+
+int edge_mesh_box_sizes(const edge_box_params *p, int64_t *n_ve, int64_t *n_el, int64_t *n_fa);
+int edge_mesh_box(const edge_box_params *p, edge_mesh_out *m);
+
+/* ====================================================================== */
+/* Element-update solver: replaces                                          */
+/* edge::seismic::solvers::AderDg<real, 0, TET4, O, O, R>                   */
+/* (src/impl/seismic/solvers/AderDg.hpp:225-312 constructor, :338-534       */
+/* local, :553-722 neigh) with its kernels::Kernels members                 */
+/* (src/impl/seismic/kernels/Kernels.hpp:71-142).                           */
+/* ====================================================================== */
+typedef struct edge_el_solver edge_el_solver;
+
+typedef struct {
+  int32_t order;     /* O: 2..5 */
+  int32_t n_runs;    /* fused runs R (innermost DOF axis): 1 or 16 */
+  int32_t precision; /* 64 or 32 (arithmetic and storage type of DOFs) */
+  int32_t kernel;    /* 0 = pick automatically, 1 = generic, 2 = MFMA-tiled */
+} edge_el_config;
+
+typedef struct { /* host arrays in the reference's entity layout */
+  int64_t n_el, n_fa;
+  const double *ve_crds;    /* [nVe][3] */
+  const int64_t *el_ve;     /* [nEl][4] */
+  const int64_t *el_fa;     /* [nEl][4] */
+  const int64_t *fa_el;     /* [nFa][2] */
+  const int64_t *el_fa_el;  /* [nEl][4] */
+  const uint16_t *f_id;     /* [nEl][4] */
+  const uint16_t *v_id;     /* [nEl][4] */
+  const int32_t *fa_type;   /* [nFa] */
+  const double *normals;    /* [nFa][3] */
+  const double *tangents;   /* [nFa][2][3] */
+  const double *areas;      /* [nFa] */
+} edge_mesh_in;
+
+/* Builds the solver: copies mesh connectivity to the device and builds the
+ * star matrices (AderDgInit.hpp:129-174) and the per-face Godunov flux
+ * solvers (AderDgInit.hpp:394-665) in FP64 on the GPU from per-element
+ * density / Lame parameters (host arrays [nEl]). */
+int edge_el_create(const edge_el_config *cfg, const edge_mesh_in *mesh, const double *rho,
+                   const double *lam, const double *mu, edge_el_solver **out);
+
+/* Same, with caller-provided star matrices [nEl][3][81] and flux solvers
+ * [2][nEl][4][81] (host, FP64); the mesh only supplies connectivity and face
+ * types (normals, tangents, areas and vertices may be NULL). */
+int edge_el_create_raw(const edge_el_config *cfg, const edge_mesh_in *mesh, const double *star,
+                       const double *fs, edge_el_solver **out);
+
+int edge_el_destroy(edge_el_solver *s);
+
+/* Copies the solver's star matrices [nEl][3][81] / flux solvers
+ * [2][nEl][4][81] to host FP64 arrays (either may be NULL). */
+int edge_el_get_setup(const edge_el_solver *s, double *star, double *fs);
+
+/* Local step for elements [first, first+n): CK time prediction into tInt,
+ * volume integral and local surface integral into dofs.  dofs and tInt are
+ * device pointers for the whole mesh in the reference layout and precision.
+ * stream: hipStream_t (NULL = default stream).  Mirrors AderDg::local with
+ * global time stepping (AderDg.hpp:338-534). */
+int edge_el_local(edge_el_solver *s, int64_t first, int64_t n_els, double dt, void *dofs, void *t_int,
+                  void *stream);
+
+/* Neighbour step for elements [first, first+n): surface integrals of the
+ * face-adjacent elements' tInt (free surface: own tInt and local flux
+ * matrix; outflow: skipped).  Mirrors AderDg::neigh (AderDg.hpp:553-722). */
+int edge_el_neigh(edge_el_solver *s, int64_t first, int64_t n_els, const void *t_int, void *dofs,
+                  void *stream);
+
+/* n_steps full time steps (local then neigh over all elements) with time
+ * step dt.  use_graph != 0 captures one step into a HIP graph on first use
+ * and replays it. */
+int edge_el_step(edge_el_solver *s, int64_t n_steps, double dt, void *dofs, void *t_int, int use_graph,
+                 void *stream);
+
+/* Number of elements / modes / runs / bytes of one DOF array of a solver. */
+int edge_el_info(const edge_el_solver *s, int64_t *n_el, int32_t *n_modes, int32_t *n_runs,
+                 int64_t *dof_bytes);
+
+/* FP64 L2 norms per quantity and run, exact through the diagonal mass
+ * matrix: for device DOF arrays a (element precision prec_a: 32 or 64) and
+ * b (prec_b; NULL for none), sets diff_sq[9][R] = ||a - b||^2 and
+ * ref_sq[9][R] = ||b||^2 (or ||a||^2 without b), each sum_el |J| sum_n M_n x_n^2.
+ * The relative L2 error of a against b is sqrt(diff_sq / ref_sq).  Needs a
+ * solver built by edge_el_create (element Jacobians).  The error-norm
+ * reduction of src/setups/InitialDofs.hpp:277-370 in modal form. */
+int edge_el_l2_norms(edge_el_solver *s, const void *a, int prec_a, const void *b, int prec_b, double *diff_sq,
+                     double *ref_sq, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDGE_AMD_H */

@@ -1,0 +1,503 @@
+/*
+ * Oracle for the ADER-DG elastic / viscoelastic tet4 element update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the CPU checker: a plain-C, FP64
+ * restatement of the reference's vanilla path.  Only the test suite, the
+ * smoke check and the CPU-baseline leg of the benchmark load it; the product
+ * (the HIP library) never calls into it.
+ *
+ * Each function names the reference lines it restates.  Layouts follow the
+ * reference exactly: DOFs [el][q][mode][run] (run innermost), anelastic DOFs
+ * [el][rm][6][mode][run], star matrices [el][3][81] (dense 9x9),
+ * flux solvers [2][el][4][81], anelastic star [el][3][18] (6x3 dense),
+ * anelastic source [el][rm][36] (6x6 dense), anelastic flux solvers
+ * [2][el][4][54] (6x9 dense).
+ *
+ * Pinning: see tests/test_oracle_*.py.  Kernels are pinned by the reference's
+ * golden vectors (TimePred/VolInt/SurfInt .test.inc), the setup by the
+ * reference's own Elasticity.cpp compiled into oracle/_ref, and the scheme by
+ * the plane-wave convergence order.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NQ 9   /* elastic quantities */
+#define NQA 6  /* quantities per relaxation mechanism */
+#define NO_ID16 0xFFFFu
+
+/* face types, values of the reference's seismic spType enum
+ * (src/impl/seismic/const.inc:143-148) */
+#define FA_FREE_SURFACE 101
+#define FA_OUTFLOW 105
+#define FA_PERIODIC 106
+
+static int nmodes(int order) { return order * (order + 1) * (order + 2) / 6; }
+static int nmodes_face(int order) { return order * (order + 1) / 2; }
+
+/* -------------------------------------------------------------------------
+ * Small fused GEMMs, src/linalg/Matrix.h:296-380 (matMulFusedAC / BC).
+ * Row-major, run axis innermost; alpha = 1, beta in {0, 1}.
+ * ---------------------------------------------------------------------- */
+static void mm_ac(int r, int m, int n, int k, int lda, int ldb, int ldc, int beta,
+                  const double *a, const double *b, double *c) {
+  if (!beta)
+    for (int im = 0; im < m; im++)
+      for (int in = 0; in < n; in++)
+        for (int ir = 0; ir < r; ir++) c[(im * ldc + in) * r + ir] = 0;
+  for (int ik = 0; ik < k; ik++)
+    for (int im = 0; im < m; im++)
+      for (int in = 0; in < n; in++) {
+        double bv = b[ik * ldb + in];
+        for (int ir = 0; ir < r; ir++) c[(im * ldc + in) * r + ir] += a[(im * lda + ik) * r + ir] * bv;
+      }
+}
+
+static void mm_bc(int r, int m, int n, int k, int lda, int ldb, int ldc, int beta,
+                  const double *a, const double *b, double *c) {
+  if (!beta)
+    for (int im = 0; im < m; im++)
+      for (int in = 0; in < n; in++)
+        for (int ir = 0; ir < r; ir++) c[(im * ldc + in) * r + ir] = 0;
+  for (int ik = 0; ik < k; ik++)
+    for (int im = 0; im < m; im++) {
+      double av = a[im * lda + ik];
+      for (int in = 0; in < n; in++)
+        for (int ir = 0; ir < r; ir++) c[(im * ldc + in) * r + ir] += av * b[(ik * ldb + in) * r + ir];
+    }
+}
+
+/* -------------------------------------------------------------------------
+ * Reference-element matrices as handed to the kernels.
+ * stiffT is stored un-negated; the CK negates it the way
+ * src/dg/TimePred.hpp:86 does when it stores the matrices.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int order;            /* O = degree + 1 */
+  int n_crs;            /* fused runs R */
+  int n_rms;            /* relaxation mechanisms */
+  const double *stiffT; /* [3][N][N]  K_d^T M^-1 */
+  const double *stiffV; /* [3][N][N]  K_d M^-1   */
+  const double *fluxL;  /* [4][N][NF] */
+  const double *fluxN;  /* [12][N][NF] */
+  const double *fluxT;  /* [4][NF][N] */
+  const double *rfs;    /* [n_rms] relaxation frequencies */
+} orc_dg;
+
+/* -------------------------------------------------------------------------
+ * Cauchy-Kowalevski time prediction,
+ * src/impl/seismic/kernels/TimePredVanilla.hpp:200-306 (kernels :103-158).
+ * derE: [O][9][N][R], tIntE: [9][N][R], derA: [rm][O][6][N][R], tIntA: [rm][6][N][R]
+ * ---------------------------------------------------------------------- */
+void orc_ck(const orc_dg *dg, double dt, const double *starE, const double *starA,
+            const double *srcA, const double *dofsE, const double *dofsA, double *derE,
+            double *derA, double *tIntE, double *tIntA) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs, NR = dg->n_rms;
+  const int sz = NQ * N * R, szA = NQA * N * R;
+  double *scratch = (double *)calloc((size_t)sz, sizeof(double));
+  double *scrA = (double *)calloc((size_t)szA, sizeof(double));
+  double *negT = (double *)malloc(sizeof(double) * 3 * N * N);
+  for (int i = 0; i < 3 * N * N; i++) negT[i] = -dg->stiffT[i];
+
+  double sca = dt;
+  for (int i = 0; i < sz; i++) {
+    derE[i] = dofsE[i];
+    tIntE[i] = sca * dofsE[i];
+  }
+  for (int rm = 0; rm < NR; rm++)
+    for (int i = 0; i < szA; i++) {
+      derA[(rm * O + 0) * szA + i] = dofsA[rm * szA + i];
+      tIntA[rm * szA + i] = sca * dofsA[rm * szA + i];
+    }
+
+  for (int de = 1; de < O; de++) {
+    /* elastic runs truncate to the hierarchical modes still alive; the
+     * viscoelastic branch keeps all modes (TimePredVanilla.hpp:232, :296) */
+    const int re = (NR == 0) ? de : 1;
+    const int nk = (NR == 0) ? nmodes(O - re + 1) : N; /* CE_N_ELEMENT_MODES_CK(l_re-1) */
+    const int nn = (NR == 0) ? nmodes(O - re) : nmodes(O - 1);
+    double *dcur = derE + (size_t)de * sz;
+    const double *dprev = derE + (size_t)(de - 1) * sz;
+    for (int i = 0; i < sz; i++) dcur[i] = 0;
+    for (int i = 0; i < szA; i++) scrA[i] = 0;
+    for (int d = 0; d < 3; d++) {
+      mm_ac(R, NQ, nn, nk, N, N, N, 0, dprev, negT + (size_t)d * N * N, scratch);
+      mm_bc(R, NQ, nn, NQ, NQ, N, N, 1, starE + d * 81, scratch, dcur);
+      if (NR > 0) mm_bc(R, NQA, nn, 3, 3, N, N, 1, starA + d * 18, scratch + NQA * N * R, scrA);
+    }
+    sca *= dt / (de + 1);
+    for (int rm = 0; rm < NR; rm++) {
+      mm_bc(R, NQA, N, NQA, NQA, N, N, 1, srcA + rm * 36, derA + ((size_t)rm * O + de - 1) * szA, dcur);
+      double *dA = derA + ((size_t)rm * O + de) * szA;
+      const double *dAp = derA + ((size_t)rm * O + de - 1) * szA;
+      for (int i = 0; i < szA; i++) {
+        dA[i] = dg->rfs[rm] * (scrA[i] + dAp[i]);
+        tIntA[rm * szA + i] += sca * dA[i];
+      }
+    }
+    const int ncp = (NR == 0) ? nmodes(O - de) : N;
+    for (int q = 0; q < NQ; q++)
+      for (int md = 0; md < ncp; md++)
+        for (int r = 0; r < R; r++) tIntE[(q * N + md) * R + r] += sca * dcur[(q * N + md) * R + r];
+  }
+  free(scratch);
+  free(scrA);
+  free(negT);
+}
+
+/* -------------------------------------------------------------------------
+ * Taylor integration over [0, dt] of stored derivatives,
+ * src/impl/seismic/kernels/TimePred.hpp:165-202; and evaluation at a time,
+ * TimePred.hpp:130-156.
+ * ---------------------------------------------------------------------- */
+void orc_integrate(const orc_dg *dg, double dt, const double *derE, double *tIntE) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs, sz = NQ * N * R;
+  double sca = dt;
+  for (int i = 0; i < sz; i++) tIntE[i] = 0;
+  for (int de = 0; de < O; de++) {
+    for (int i = 0; i < sz; i++) tIntE[i] += sca * derE[(size_t)de * sz + i];
+    sca *= dt / (de + 2);
+  }
+}
+
+void orc_eval_time_pred(const orc_dg *dg, double t, const double *derE, double *out) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs, sz = NQ * N * R;
+  double sca = 1.0;
+  for (int i = 0; i < sz; i++) out[i] = 0;
+  for (int de = 0; de < O; de++) {
+    for (int i = 0; i < sz; i++) out[i] += sca * derE[(size_t)de * sz + i];
+    sca *= t / (de + 1);
+  }
+}
+
+/* -------------------------------------------------------------------------
+ * Volume integral, src/impl/seismic/kernels/VolIntVanilla.hpp:189-241.
+ * ---------------------------------------------------------------------- */
+void orc_vol(const orc_dg *dg, const double *starE, const double *starA, const double *srcA,
+             const double *tIntE, const double *tIntA, double *dofsE, double *dofsA) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs, NR = dg->n_rms;
+  const int szA = NQA * N * R;
+  double *scratch = (double *)calloc((size_t)NQ * N * R, sizeof(double));
+  double *scrA = (double *)calloc((size_t)szA, sizeof(double));
+  const int nk = nmodes(O - 1);
+  for (int d = 0; d < 3; d++) {
+    mm_ac(R, NQ, N, nk, N, N, N, 0, tIntE, dg->stiffV + (size_t)d * N * N, scratch);
+    mm_bc(R, NQ, N, NQ, NQ, N, N, 1, starE + d * 81, scratch, dofsE);
+    if (NR > 0) mm_bc(R, NQA, N, 3, 3, N, N, 1, starA + d * 18, scratch + NQA * N * R, scrA);
+  }
+  for (int rm = 0; rm < NR; rm++) {
+    mm_bc(R, NQA, N, NQA, NQA, N, N, 1, srcA + rm * 36, tIntA + rm * szA, dofsE);
+    for (int i = 0; i < szA; i++) dofsA[rm * szA + i] += dg->rfs[rm] * (scrA[i] - tIntA[rm * szA + i]);
+  }
+  free(scratch);
+  free(scrA);
+}
+
+/* -------------------------------------------------------------------------
+ * Surface integrals, src/impl/seismic/kernels/SurfIntVanilla.hpp:212-357 and
+ * SurfInt.hpp:127-156.
+ * ---------------------------------------------------------------------- */
+void orc_scatter_update_a(const orc_dg *dg, const double *upd, double *dofsA) {
+  const int N = nmodes(dg->order), R = dg->n_crs, szA = NQA * N * R;
+  for (int rm = 0; rm < dg->n_rms; rm++)
+    for (int i = 0; i < szA; i++) dofsA[rm * szA + i] += dg->rfs[rm] * upd[i];
+}
+
+void orc_surf_local(const orc_dg *dg, const double *fsE /*[4][81]*/, const double *fsA /*[4][54]*/,
+                    const double *tIntE, double *dofsE, double *dofsA) {
+  const int O = dg->order, N = nmodes(O), NF = nmodes_face(O), R = dg->n_crs;
+  double *s0 = (double *)calloc((size_t)NQ * NF * R, sizeof(double));
+  double *s1 = (double *)calloc((size_t)NQ * NF * R, sizeof(double));
+  double *upA = (double *)calloc((size_t)NQA * N * R, sizeof(double));
+  for (int fa = 0; fa < 4; fa++) {
+    mm_ac(R, NQ, NF, N, N, NF, NF, 0, tIntE, dg->fluxL + (size_t)fa * N * NF, s0);
+    mm_bc(R, NQ, NF, NQ, NQ, NF, NF, 0, fsE + fa * 81, s0, s1);
+    mm_ac(R, NQ, N, NF, NF, N, N, 1, s1, dg->fluxT + (size_t)fa * NF * N, dofsE);
+    if (dg->n_rms > 0) {
+      mm_bc(R, NQA, NF, NQ, NQ, NF, NF, 0, fsA + fa * 54, s0, s1);
+      mm_ac(R, NQA, N, NF, NF, N, N, 1, s1, dg->fluxT + (size_t)fa * NF * N, upA);
+    }
+  }
+  if (dg->n_rms > 0) orc_scatter_update_a(dg, upA, dofsA);
+  free(s0);
+  free(s1);
+  free(upA);
+}
+
+void orc_neigh_flux_int(const orc_dg *dg, int fa, unsigned vId, unsigned fId, const double *tIntE,
+                        double *out /*[9][NF][R]*/) {
+  const int O = dg->order, N = nmodes(O), NF = nmodes_face(O), R = dg->n_crs;
+  const double *fm = (vId != NO_ID16) ? dg->fluxN + (size_t)(vId * 4 + fId) * N * NF
+                                      : dg->fluxL + (size_t)fa * N * NF;
+  mm_ac(R, NQ, NF, N, N, NF, NF, 0, tIntE, fm, out);
+}
+
+/* dofsA receives the UNSCALED anelastic update (SurfIntVanilla.hpp:343-355). */
+void orc_surf_neigh(const orc_dg *dg, int fa, unsigned vId, unsigned fId, const double *fsE /*[81]*/,
+                    const double *fsA /*[54]*/, const double *tIntE, double *dofsE, double *upA) {
+  const int O = dg->order, N = nmodes(O), NF = nmodes_face(O), R = dg->n_crs;
+  double *s0 = (double *)calloc((size_t)NQ * NF * R, sizeof(double));
+  double *s1 = (double *)calloc((size_t)NQ * NF * R, sizeof(double));
+  orc_neigh_flux_int(dg, fa, vId, fId, tIntE, s0);
+  mm_bc(R, NQ, NF, NQ, NQ, NF, NF, 0, fsE, s0, s1);
+  mm_ac(R, NQ, N, NF, NF, N, N, 1, s1, dg->fluxT + (size_t)fa * NF * N, dofsE);
+  if (dg->n_rms > 0) {
+    mm_bc(R, NQA, NF, NQ, NQ, NF, NF, 0, fsA, s0, s1);
+    mm_ac(R, NQA, N, NF, NF, N, N, 1, s1, dg->fluxT + (size_t)fa * NF * N, upA);
+  }
+  free(s0);
+  free(s1);
+}
+
+/* -------------------------------------------------------------------------
+ * Solver loops, src/impl/seismic/solvers/AderDg.hpp:338-534 (local) and
+ * :553-722 (neigh); global time stepping (tDofs[0] only).
+ * ---------------------------------------------------------------------- */
+void orc_local(const orc_dg *dg, int64_t first, int64_t n_els, double dt, const double *starE,
+               const double *starA, const double *srcA, const double *fsE0 /*[el][4][81]*/,
+               const double *fsA0 /*[el][4][54]*/, double *dofsE, double *dofsA, double *tIntE) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs, NR = dg->n_rms;
+  const size_t sz = (size_t)NQ * N * R, szA = (size_t)NQA * N * R;
+#pragma omp parallel
+  {
+    double *der = (double *)malloc(sizeof(double) * O * sz);
+    double *derA = (double *)malloc(sizeof(double) * (NR > 0 ? NR : 1) * O * szA);
+    double *tA = (double *)malloc(sizeof(double) * (NR > 0 ? NR : 1) * szA);
+#pragma omp for schedule(static)
+    for (int64_t el = first; el < first + n_els; el++) {
+      double *dA = (NR > 0) ? dofsA + (size_t)el * NR * szA : NULL;
+      orc_ck(dg, dt, starE + el * 243, NR ? starA + el * 54 : NULL, NR ? srcA + el * NR * 36 : NULL,
+             dofsE + el * sz, dA, der, derA, tIntE + el * sz, tA);
+      orc_vol(dg, starE + el * 243, NR ? starA + el * 54 : NULL, NR ? srcA + el * NR * 36 : NULL,
+              tIntE + el * sz, tA, dofsE + el * sz, dA);
+      orc_surf_local(dg, fsE0 + el * 324, NR ? fsA0 + el * 216 : NULL, tIntE + el * sz, dofsE + el * sz,
+                     dA);
+    }
+    free(der);
+    free(derA);
+    free(tA);
+  }
+}
+
+void orc_neigh(const orc_dg *dg, int64_t first, int64_t n_els, const int32_t *fa_type /*[nFa]*/,
+               const int64_t *el_fa /*[el][4]*/, const int64_t *el_fa_el /*[el][4]*/,
+               const uint16_t *f_id /*[el][4]*/, const uint16_t *v_id /*[el][4]*/,
+               const double *fsE1 /*[el][4][81]*/, const double *fsA1 /*[el][4][54]*/,
+               const double *tIntE, double *dofsE, double *dofsA) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs, NR = dg->n_rms;
+  const size_t sz = (size_t)NQ * N * R, szA = (size_t)NQA * N * R;
+#pragma omp parallel
+  {
+    double *upA = (double *)malloc(sizeof(double) * szA);
+#pragma omp for schedule(static)
+    for (int64_t el = first; el < first + n_els; el++) {
+      for (size_t i = 0; i < szA; i++) upA[i] = 0;
+      for (int fa = 0; fa < 4; fa++) {
+        const int64_t fid = el_fa[el * 4 + fa];
+        const int32_t ty = fa_type[fid];
+        if ((ty & FA_OUTFLOW) == FA_OUTFLOW) continue;
+        const int fs = (ty & FA_FREE_SURFACE) == FA_FREE_SURFACE;
+        const int64_t ne = fs ? el : el_fa_el[el * 4 + fa];
+        unsigned vId = NO_ID16, fId = NO_ID16;
+        if (!fs) {
+          vId = v_id[el * 4 + fa];
+          fId = f_id[el * 4 + fa];
+        }
+        orc_surf_neigh(dg, fa, vId, fId, fsE1 + (el * 4 + fa) * 81, NR ? fsA1 + (el * 4 + fa) * 54 : NULL,
+                       tIntE + ne * sz, dofsE + el * sz, upA);
+      }
+      if (NR > 0) orc_scatter_update_a(dg, upA, dofsA + (size_t)el * NR * szA);
+    }
+    free(upA);
+  }
+}
+
+/* -------------------------------------------------------------------------
+ * Geometry: Jacobian, src/linalg/Mappings.hpp:189-230 with the tet4 shape
+ * derivatives (jac[i][j] = dx_j / dxi_i); determinant and inverse.
+ * ve: [4][3] vertex coordinates.
+ * ---------------------------------------------------------------------- */
+void orc_jac(const double *ve, double jac[9], double *det, double jinv[9]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) jac[i * 3 + j] = ve[(i + 1) * 3 + j] - ve[j];
+  const double *a = jac;
+  double d = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+             a[2] * (a[3] * a[7] - a[4] * a[6]);
+  *det = d;
+  jinv[0] = (a[4] * a[8] - a[5] * a[7]) / d;
+  jinv[1] = (a[2] * a[7] - a[1] * a[8]) / d;
+  jinv[2] = (a[1] * a[5] - a[2] * a[4]) / d;
+  jinv[3] = (a[5] * a[6] - a[3] * a[8]) / d;
+  jinv[4] = (a[0] * a[8] - a[2] * a[6]) / d;
+  jinv[5] = (a[2] * a[3] - a[0] * a[5]) / d;
+  jinv[6] = (a[3] * a[7] - a[4] * a[6]) / d;
+  jinv[7] = (a[1] * a[6] - a[0] * a[7]) / d;
+  jinv[8] = (a[0] * a[4] - a[1] * a[3]) / d;
+}
+
+/* Elastic star matrices S_d = sum_k A_k jinv[k][d],
+ * src/impl/seismic/setups/Elasticity.cpp:167-252 (Jacobians A, B, C of
+ * q = (sxx, syy, szz, sxy, syz, sxz, u, v, w), q_t + A q_x + B q_y + C q_z = 0). */
+void orc_star(double rho, double lam, double mu, const double jinv[9], double star[243]) {
+  memset(star, 0, sizeof(double) * 243);
+  const double lp = -lam - 2 * mu, l = -lam, m = -mu, ir = -1.0 / rho;
+  /* (row, col, value) of A_k, k = 0..2 */
+  const int rc[3][8][2] = {
+      {{0, 6}, {1, 6}, {2, 6}, {3, 7}, {5, 8}, {6, 0}, {7, 3}, {8, 5}},
+      {{0, 7}, {1, 7}, {2, 7}, {3, 6}, {4, 8}, {6, 3}, {7, 1}, {8, 4}},
+      {{0, 8}, {1, 8}, {2, 8}, {4, 7}, {5, 6}, {6, 5}, {7, 4}, {8, 2}}};
+  const double val[3][8] = {{lp, l, l, m, m, ir, ir, ir}, {l, lp, l, m, m, ir, ir, ir}, {l, l, lp, m, m, ir, ir, ir}};
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 3; k++)
+      for (int e = 0; e < 8; e++) star[d * 81 + rc[k][e][0] * 9 + rc[k][e][1]] += val[k][e] * jinv[k * 3 + d];
+}
+
+/* Rotation of the quantities from face-aligned (n, s, t) to xyz
+ * (src/impl/seismic/generated/Trafo3D.inc semantics) and its inverse,
+ * computed here from the orthonormal frame. */
+void orc_trafo(const double n[3], const double s[3], const double t[3], double T[81], double Ti[81]) {
+  const double *f[3] = {n, s, t};
+  /* stress index pairs in Voigt order (xx, yy, zz, xy, yz, xz) */
+  const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
+  memset(T, 0, sizeof(double) * 81);
+  memset(Ti, 0, sizeof(double) * 81);
+  for (int r = 0; r < 6; r++) {
+    const int i = vi[r][0], j = vi[r][1];
+    for (int c = 0; c < 6; c++) {
+      const int a = vi[c][0], b = vi[c][1];
+      /* sigma_ij = sum_ab R_ia R_jb sigma'_ab, R = [n s t] as columns */
+      double v = f[a][i] * f[b][j];
+      if (a != b) v += f[b][i] * f[a][j];
+      T[r * 9 + c] = v;
+      /* sigma'_ab = sum_ij R_ia R_jb sigma_ij */
+      double w = f[vi[r][0]][vi[c][0]] * f[vi[r][1]][vi[c][1]];
+      if (vi[c][0] != vi[c][1]) w += f[vi[r][0]][vi[c][1]] * f[vi[r][1]][vi[c][0]];
+      Ti[r * 9 + c] = w;
+    }
+  }
+  for (int i = 0; i < 3; i++)
+    for (int a = 0; a < 3; a++) {
+      T[(6 + i) * 9 + 6 + a] = f[a][i];
+      Ti[(6 + a) * 9 + 6 + i] = f[a][i];
+    }
+}
+
+/* Left / right contributions to the exact Riemann solution in face-aligned
+ * coordinates (the role of generated/FlMid3D.inc and FrMid3D.inc): the flux
+ * of the left element A_L q*, with q* the middle state of the welded-interface
+ * Riemann problem, differentiated w.r.t. the left and right face states. */
+void orc_fs_mid(double rhoL, double rhoR, double lamL, double lamR, double muL, double muR,
+                double flMid[81], double frMid[81]) {
+  memset(flMid, 0, sizeof(double) * 81);
+  memset(frMid, 0, sizeof(double) * 81);
+  const double zpL = sqrt((lamL + 2 * muL) * rhoL), zpR = sqrt((lamR + 2 * muR) * rhoR);
+  const double zsL = sqrt(muL * rhoL), zsR = sqrt(muR * rhoR);
+  /* normal: u* = (sR - sL + zpR uR + zpL uL)/(zpL+zpR),
+   *         s* = (zpR sL + zpL sR + zpL zpR (uR - uL))/(zpL+zpR) */
+  const double ip = 1.0 / (zpL + zpR);
+  const double du_dsL = -ip, du_dsR = ip, du_duL = zpL * ip, du_duR = zpR * ip;
+  const double ds_dsL = zpR * ip, ds_dsR = zpL * ip, ds_duL = -zpL * zpR * ip, ds_duR = zpL * zpR * ip;
+  /* flux rows: F[snn] = -(lamL+2muL) u*, F[sss] = F[stt] = -lamL u*, F[un] = -(s*) / rhoL */
+  const double cu[3] = {-(lamL + 2 * muL), -lamL, -lamL};
+  for (int r = 0; r < 3; r++) {
+    flMid[r * 9 + 0] = cu[r] * du_dsL;
+    flMid[r * 9 + 6] = cu[r] * du_duL;
+    frMid[r * 9 + 0] = cu[r] * du_dsR;
+    frMid[r * 9 + 6] = cu[r] * du_duR;
+  }
+  flMid[6 * 9 + 0] = -ds_dsL / rhoL;
+  flMid[6 * 9 + 6] = -ds_duL / rhoL;
+  frMid[6 * 9 + 0] = -ds_dsR / rhoL;
+  frMid[6 * 9 + 6] = -ds_duR / rhoL;
+  /* shear pairs (sns, us) = (3, 7) and (snt, ut) = (5, 8) */
+  const double is = 1.0 / (zsL + zsR);
+  const int pr[2][2] = {{3, 7}, {5, 8}};
+  for (int p = 0; p < 2; p++) {
+    const int si = pr[p][0], vi = pr[p][1];
+    flMid[si * 9 + si] = -muL * (-is);
+    flMid[si * 9 + vi] = -muL * (zsL * is);
+    frMid[si * 9 + si] = -muL * (is);
+    frMid[si * 9 + vi] = -muL * (zsR * is);
+    flMid[vi * 9 + si] = -(zsR * is) / rhoL;
+    flMid[vi * 9 + vi] = -(-zsL * zsR * is) / rhoL;
+    frMid[vi * 9 + si] = -(zsL * is) / rhoL;
+    frMid[vi * 9 + vi] = -(zsL * zsR * is) / rhoL;
+  }
+}
+
+static void mm9(const double *a, const double *b, double *c) {
+  for (int i = 0; i < 9; i++)
+    for (int j = 0; j < 9; j++) {
+      double s = 0;
+      for (int k = 0; k < 9; k++) s += a[i * 9 + k] * b[k * 9 + j];
+      c[i * 9 + j] = s;
+    }
+}
+
+/* Elastic flux solvers of one face side, src/impl/seismic/setups/Elasticity.cpp:372-441
+ * (fsL = T FlMid T^-1; fsR = T FrMid [mirror] T^-1; the free surface mirrors
+ * the face-aligned normal tractions, columns 0, 3, 5). */
+void orc_fs(double rhoL, double rhoR, double lamL, double lamR, double muL, double muR, const double n[3],
+            const double s[3], const double t[3], int free_surface, double fsL[81], double fsR[81]) {
+  double T[81], Ti[81], fl[81], fr[81], tmp[81];
+  orc_trafo(n, s, t, T, Ti);
+  orc_fs_mid(rhoL, rhoR, lamL, lamR, muL, muR, fl, fr);
+  mm9(T, fl, tmp);
+  mm9(tmp, Ti, fsL);
+  mm9(T, fr, tmp);
+  if (free_surface)
+    for (int r = 0; r < 9; r++) {
+      tmp[r * 9 + 0] *= -1;
+      tmp[r * 9 + 3] *= -1;
+      tmp[r * 9 + 5] *= -1;
+    }
+  mm9(tmp, Ti, fsR);
+}
+
+/* Per-face set up of the flux solvers of both adjacent elements,
+ * src/impl/seismic/solvers/AderDgInit.hpp:412-665 (GTS, single partition):
+ * the left element uses the outward normal n, the right one -n with the
+ * same tangents; both are scaled by -2 area / |J|. */
+void orc_init_fs(int64_t n_fas, const int64_t *fa_el /*[fa][2]*/, const int64_t *el_fa /*[el][4]*/,
+                 const int32_t *fa_type, const double *normals /*[fa][3]*/,
+                 const double *tangents /*[fa][2][3]*/, const double *areas, const double *jdet /*[el]*/,
+                 const double *rho, const double *lam, const double *mu, double *fsE0, double *fsE1) {
+  for (int64_t fa = 0; fa < n_fas; fa++) {
+    const int64_t eL = fa_el[fa * 2], eR = fa_el[fa * 2 + 1];
+    const int exR = eR >= 0;
+    int fL = -1, fR = -1;
+    for (int f = 0; f < 4; f++) {
+      if (el_fa[eL * 4 + f] == fa) fL = f;
+      if (exR && el_fa[eR * 4 + f] == fa) fR = f;
+    }
+    const int fs = (fa_type[fa] & FA_FREE_SURFACE) == FA_FREE_SURFACE;
+    const double rL = rho[eL], lL = lam[eL], mL = mu[eL];
+    const double rR = exR ? rho[eR] : rL, lR = exR ? lam[eR] : lL, mR = exR ? mu[eR] : mL;
+    const double *n = normals + fa * 3, *s = tangents + fa * 6, *t = tangents + fa * 6 + 3;
+    double a[81], b[81];
+    if (fL >= 0) {
+      orc_fs(rL, rR, lL, lR, mL, mR, n, s, t, fs, a, b);
+      const double sc = -2.0 * areas[fa] / jdet[eL];
+      for (int i = 0; i < 81; i++) {
+        fsE0[(eL * 4 + fL) * 81 + i] = a[i] * sc;
+        fsE1[(eL * 4 + fL) * 81 + i] = b[i] * sc;
+      }
+    }
+    if (exR && fR >= 0) {
+      const double nm[3] = {-n[0], -n[1], -n[2]};
+      orc_fs(rR, rL, lR, lL, mR, mL, nm, s, t, 0, a, b);
+      const double sc = -2.0 * areas[fa] / jdet[eR];
+      for (int i = 0; i < 81; i++) {
+        fsE0[(eR * 4 + fR) * 81 + i] = a[i] * sc;
+        fsE1[(eR * 4 + fR) * 81 + i] = b[i] * sc;
+      }
+    }
+  }
+}
+
+/* Element time step by CFL, src/impl/seismic/common.hpp:84-98, with the
+ * reference's scale 1/2 (const.inc:135-137). */
+double orc_dt_cfl(double rho, double lam, double mu, double in_dia, int order) {
+  return in_dia * 0.5 / sqrt((lam + 2 * mu) / rho) / (2.0 * order - 1);
+}

@@ -1,0 +1,206 @@
+"""Python side of the oracle.  TEST INFRASTRUCTURE ONLY.
+
+ctypes wrappers around liboracle.so (the C restatement in edge_oracle.c)
+and around oracle/_ref/libedge_ref.so (the reference's own Elasticity.cpp,
+built from the reference tree when it is present).  Imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg only.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libedge_ref.so")
+
+NO_ID16 = 0xFFFF
+_dp = C.c_void_p
+
+
+class OrcDg(C.Structure):
+    _fields_ = [("order", C.c_int), ("n_crs", C.c_int), ("n_rms", C.c_int), ("stiffT", _dp), ("stiffV", _dp),
+                ("fluxL", _dp), ("fluxN", _dp), ("fluxT", _dp), ("rfs", _dp)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-C", HERE, "oracle"], check=True, capture_output=True)
+        _lib = C.CDLL(LIB)
+        _lib.orc_dt_cfl.restype = C.c_double
+        _lib.orc_dt_cfl.argtypes = [C.c_double] * 5 + [C.c_int]
+        for f in ["orc_ck", "orc_vol", "orc_surf_local", "orc_surf_neigh", "orc_neigh_flux_int", "orc_local",
+                  "orc_neigh", "orc_star", "orc_fs", "orc_jac", "orc_trafo", "orc_fs_mid", "orc_init_fs",
+                  "orc_integrate", "orc_eval_time_pred", "orc_scatter_update_a"]:
+            getattr(_lib, f).restype = None
+    return _lib
+
+
+def ref_lib():
+    """The reference's Elasticity.cpp (or None when it was not built)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB):
+        _ref = C.CDLL(REF_LIB)
+        for f in ["ref_star", "ref_trafo", "ref_fs"]:
+            getattr(_ref, f).restype = None
+    return _ref
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Dg:
+    """Holds the matrices (FP64) and the C struct handed to the kernels."""
+
+    def __init__(self, order, n_crs=1, mats=None, rfs=None):
+        if mats is None:
+            from edge_amd.setups import dg_matrices
+            mats = dg_matrices(order)
+        self.keep = {k: _c(v) for k, v in mats.items()}
+        self.rfs = _c(rfs) if rfs is not None else None
+        n_rms = 0 if rfs is None else len(rfs)
+        self.s = OrcDg(order, n_crs, n_rms, *[_p(self.keep.get(k)) for k in
+                                              ["stiffT", "stiffV", "fluxL", "fluxN", "fluxT"]], _p(self.rfs))
+        self.order, self.n_crs, self.n_rms = order, n_crs, n_rms
+        self.n = order * (order + 1) * (order + 2) // 6
+        self.nf = order * (order + 1) // 2
+
+
+# ---------------------------------------------------------------- kernels
+def ck(dg, dt, star, dofs, star_a=None, src_a=None, dofs_a=None):
+    """Returns (der [O][9][N][R], tInt [9][N][R], derA, tIntA)."""
+    O, N, R, NR = dg.order, dg.n, dg.n_crs, dg.n_rms
+    der = np.zeros((O, 9, N, R))
+    tint = np.zeros((9, N, R))
+    derA = np.zeros((max(NR, 1), O, 6, N, R))
+    tintA = np.zeros((max(NR, 1), 6, N, R))
+    star, dofs, star_a, src_a, dofs_a = map(_c, (star, dofs, star_a, src_a, dofs_a))
+    lib().orc_ck(C.byref(dg.s), C.c_double(dt), _p(star), _p(star_a), _p(src_a), _p(dofs), _p(dofs_a), _p(der),
+                 _p(derA), _p(tint), _p(tintA))
+    return der, tint, derA[:NR], tintA[:NR]
+
+
+def vol(dg, star, tint, dofs, star_a=None, src_a=None, tint_a=None, dofs_a=None):
+    dofs = _c(dofs).copy()
+    dofs_a = None if dofs_a is None else _c(dofs_a).copy()
+    star, tint, star_a, src_a, tint_a = map(_c, (star, tint, star_a, src_a, tint_a))
+    lib().orc_vol(C.byref(dg.s), _p(star), _p(star_a), _p(src_a), _p(tint), _p(tint_a), _p(dofs), _p(dofs_a))
+    return dofs, dofs_a
+
+
+def surf_local(dg, fs, tint, dofs, fs_a=None, dofs_a=None):
+    dofs = _c(dofs).copy()
+    dofs_a = None if dofs_a is None else _c(dofs_a).copy()
+    fs, tint, fs_a = map(_c, (fs, tint, fs_a))
+    lib().orc_surf_local(C.byref(dg.s), _p(fs), _p(fs_a), _p(tint), _p(dofs), _p(dofs_a))
+    return dofs, dofs_a
+
+
+def surf_neigh(dg, fa, v_id, f_id, fs, tint, dofs, fs_a=None):
+    """Returns (dofs updated, unscaled anelastic update)."""
+    dofs = _c(dofs).copy()
+    up = np.zeros((6, dg.n, dg.n_crs))
+    fs, tint, fs_a = map(_c, (fs, tint, fs_a))
+    lib().orc_surf_neigh(C.byref(dg.s), C.c_int(fa), C.c_uint(v_id), C.c_uint(f_id), _p(fs), _p(fs_a), _p(tint),
+                         _p(dofs), _p(up))
+    return dofs, up
+
+
+# ---------------------------------------------------------------- set-up
+def jac(ve):
+    """ve [4][3] -> (jac [3][3], det, jinv [3][3])."""
+    j = np.zeros(9)
+    ji = np.zeros(9)
+    d = C.c_double()
+    lib().orc_jac(_p(_c(ve)), _p(j), C.byref(d), _p(ji))
+    return j.reshape(3, 3), d.value, ji.reshape(3, 3)
+
+
+def star(rho, lam, mu, jinv):
+    out = np.zeros(243)
+    lib().orc_star(C.c_double(rho), C.c_double(lam), C.c_double(mu), _p(_c(jinv)), _p(out))
+    return out.reshape(3, 81)
+
+
+def trafo(n, s, t):
+    T = np.zeros(81)
+    Ti = np.zeros(81)
+    lib().orc_trafo(_p(_c(n)), _p(_c(s)), _p(_c(t)), _p(T), _p(Ti))
+    return T.reshape(9, 9), Ti.reshape(9, 9)
+
+
+def fs_face(rL, rR, lL, lR, mL, mR, n, s, t, free_surface=False):
+    a = np.zeros(81)
+    b = np.zeros(81)
+    d = C.c_double
+    lib().orc_fs(d(rL), d(rR), d(lL), d(lR), d(mL), d(mR), _p(_c(n)), _p(_c(s)), _p(_c(t)),
+                 C.c_int(1 if free_surface else 0), _p(a), _p(b))
+    return a.reshape(9, 9), b.reshape(9, 9)
+
+
+def setup_mesh(mesh, rho, lam, mu):
+    """(star [nEl][3][81], fs [2][nEl][4][81], jdet [nEl]) built on the CPU."""
+    ne = mesh.n_el
+    ve = mesh.el_crds()
+    d = ve[:, 1:, :] - ve[:, :1, :]
+    jdet = np.linalg.det(d)
+    st = np.zeros((ne, 3, 81))
+    for e in range(ne):
+        _, _, ji = jac(ve[e])
+        st[e] = star(rho[e], lam[e], mu[e], ji)
+    fs = np.full((2, ne, 4, 81), np.nan)
+    lib().orc_init_fs(C.c_int64(mesh.n_fa), _p(np.ascontiguousarray(mesh.fa_el)),
+                      _p(np.ascontiguousarray(mesh.el_fa)), _p(np.ascontiguousarray(mesh.fa_type)),
+                      _p(_c(mesh.normals)), _p(_c(mesh.tangents)), _p(_c(mesh.areas)), _p(_c(jdet)),
+                      _p(_c(rho)), _p(_c(lam)), _p(_c(mu)), _p(fs[0]), _p(fs[1]))
+    return st, fs, jdet
+
+
+def dt_cfl(rho, lam, mu, in_dia, order):
+    return lib().orc_dt_cfl(rho, lam, mu, in_dia, order)
+
+
+# ---------------------------------------------------------------- solver loops
+class MeshOracle:
+    """FP64 CPU time stepping of a whole mesh (AderDg::local + neigh, GTS)."""
+
+    def __init__(self, mesh, order, n_crs=1, star=None, fs=None, rho=None, lam=None, mu=None):
+        self.mesh = mesh
+        self.dg = Dg(order, n_crs)
+        if star is None:
+            star, fs, _ = setup_mesh(mesh, rho, lam, mu)
+        self.star = _c(star)
+        self.fs = _c(fs)
+        self.el_fa = np.ascontiguousarray(mesh.el_fa, dtype=np.int64)
+        self.el_fa_el = np.ascontiguousarray(mesh.el_fa_el, dtype=np.int64)
+        self.f_id = np.ascontiguousarray(mesh.f_id, dtype=np.uint16)
+        self.v_id = np.ascontiguousarray(mesh.v_id, dtype=np.uint16)
+        self.fa_type = np.ascontiguousarray(mesh.fa_type, dtype=np.int32)
+
+    def local(self, dofs, tint, dt, first=0, n=None):
+        n = self.mesh.n_el - first if n is None else n
+        lib().orc_local(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), C.c_double(dt), _p(self.star), None,
+                        None, _p(self.fs[0]), None, _p(dofs), None, _p(tint))
+
+    def neigh(self, dofs, tint, first=0, n=None):
+        n = self.mesh.n_el - first if n is None else n
+        lib().orc_neigh(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), _p(self.fa_type), _p(self.el_fa),
+                        _p(self.el_fa_el), _p(self.f_id), _p(self.v_id), _p(self.fs[1]), None, _p(tint), _p(dofs),
+                        None)
+
+    def step(self, dofs, tint, dt, n_steps=1):
+        for _ in range(n_steps):
+            self.local(dofs, tint, dt)
+            self.neigh(dofs, tint)

@@ -1,0 +1,188 @@
+"""Extracts the reference's own golden vectors into small JSON fixtures.
+
+Run in a container that has the reference tree mounted (it reads
+``/root/reference`` and writes next to this script).  Only numbers are
+extracted -- inputs and expected outputs of the reference's unit tests -- no
+source text is kept.
+
+Sources (all read as text):
+
+* src/impl/seismic/kernels/TimePred.test.inc  -- degree-3 transposed stiffness
+  (python block, lines 29-90) and the C tables of the CK test (lines 245-400).
+* src/impl/seismic/kernels/VolInt.test.inc    -- degree-3 volume stiffness and
+  the volume-integral test tables.
+* src/impl/seismic/kernels/SurfInt.test.inc   -- degree-2 fluxL / fluxN[6] /
+  fluxT and the surface-integral test tables.
+* tools/edge_pre/edge_pre/int/test_Matrices.py -- exact degree-2 stiffness
+  matrices (test_stiffTet) and degree-3 flux products fm_fa / fp_{f1 f2 v}
+  (test_fluxTet).
+* src/dg/generated/basis_tet.inc, basis_tria.inc -- the reference's generated
+  basis polynomials, evaluated here at seeded points (values only are stored).
+"""
+
+from fractions import Fraction
+import json
+import math
+import os
+import re
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+KER = os.path.join(REF, "src/impl/seismic/kernels")
+
+
+def _read(path):
+    with open(path) as f:
+        return f.read()
+
+
+def py_list(text, name):
+    """Values of a python list literal ``name = [ ... ]`` inside a comment."""
+    m = re.search(r"^%s\s*=\s*\[(.*?)\]" % re.escape(name), text, re.S | re.M)
+    if m is None:
+        raise KeyError(name)
+    return [float(t) for t in re.findall(r"[-+]?[0-9]*\.?[0-9]+(?:[eE][-+]?[0-9]+)?", m.group(1))]
+
+
+def c_array(text, name):
+    """Values and shape of ``float name[a][b].. = { ... };``."""
+    m = re.search(r"float\s+%s((?:\[\d+\])+)\s*=\s*\{(.*?)\};" % re.escape(name), text, re.S)
+    if m is None:
+        raise KeyError(name)
+    shape = [int(x) for x in re.findall(r"\[(\d+)\]", m.group(1))]
+    vals = [float(t) for t in re.findall(r"[-+]?[0-9]*\.?[0-9]+(?:[eE][-+]?[0-9]+)?", m.group(2))]
+    assert len(vals) == int(np.prod(shape)), (name, len(vals), shape)
+    return np.array(vals).reshape(shape)
+
+
+def _frac_tok(tok):
+    tok = tok.strip()
+    if "/" in tok:
+        a, b = tok.split("/")
+        return Fraction(int(a), int(b))
+    return Fraction(tok)
+
+
+def sympy_matrix_block(text, key):
+    """Rational entries of ``'key': sympy.Matrix([ [..], .. ])``."""
+    m = re.search(r"'%s'\s*:\s*sympy\.Matrix\(\s*\[(.*?)\]\s*\)" % re.escape(key), text, re.S)
+    if m is None:
+        raise KeyError(key)
+    rows = re.findall(r"\[([^\[\]]*)\]", m.group(1))
+    return [[_frac_tok(t) for t in r.split(",") if t.strip()] for r in rows]
+
+
+def stiff_ref_deg2(text):
+    """The three exact 10x10 stiffness matrices of test_stiffTet."""
+    start = text.index("def test_stiffTet")
+    block = text[start:text.index("# 3D coords", start)]
+    mats = re.findall(r"sympy\.Matrix\(\[(.*?)\]\)", block, re.S)
+    out = []
+    for mt in mats:
+        rows = re.findall(r"\[([^\[\]]*)\]", mt)
+        out.append([[float(_frac_tok(t)) for t in r.split(",") if t.strip()] for r in rows])
+    assert len(out) == 3
+    return out
+
+
+def c_expr_basis(path, var_name, nvar, max_fn, pts):
+    """Evaluates the reference's generated basis (C expressions of the form
+    ``if( b == i ) { val = <expr> ; }``) at the given points."""
+    txt = _read(path)
+    exprs = {}
+    for m in re.finditer(r"if\(\s*b\s*==\s*(\d+)\s*\)\s*\{\s*%s\s*=\s*(.*?);\s*\}" % var_name, txt, re.S):
+        b = int(m.group(1))
+        if b < max_fn:
+            exprs[b] = re.sub(r"(\d)L\b", r"\1", m.group(2).replace("std::pow", "pow").replace("std::sqrt", "sqrt"))
+    assert len(exprs) == max_fn, (path, len(exprs))
+    out = np.zeros((max_fn, len(pts)))
+    for b in range(max_fn):
+        code = compile(exprs[b], "<basis>", "eval")
+        for p, pt in enumerate(pts):
+            env = {"pow": pow, "sqrt": math.sqrt}
+            for d in range(nvar):
+                env["xi_%d" % (d + 1)] = float(pt[d])
+            out[b, p] = eval(code, {"__builtins__": {}}, env)
+    return out
+
+
+def main():
+    fx = {}
+
+    # ---------------- CK (TimePred) ----------------
+    t = _read(os.path.join(KER, "TimePred.test.inc"))
+    fx["timepred"] = {
+        "stiffT": [py_list(t, "l_stiffT%d" % d) for d in range(3)],
+        **{k: c_array(t, "l_" + k).tolist() for k in
+           ["rfs", "starE", "starA", "srcA", "dofsE", "dofsA", "refEtDofs", "refVtDofsE", "refVtDofsA"]},
+        "dt": 0.017,
+    }
+
+    # ---------------- volume integral ----------------
+    t = _read(os.path.join(KER, "VolInt.test.inc"))
+    fx["volint"] = {
+        "stiffV": [py_list(t, "l_stiff%d" % d) for d in range(3)],
+        **{k: c_array(t, "l_" + k).tolist() for k in
+           ["rfs", "starE", "starA", "srcA", "dofsE", "dofsA", "tDofsE", "tDofsA",
+            "refEdofs", "refVdofsE", "refVdofsA"]},
+    }
+
+    # ---------------- surface integral ----------------
+    t = _read(os.path.join(KER, "SurfInt.test.inc"))
+    fx["surfint"] = {
+        "fluxL": py_list(t, "l_fIntL"),
+        "fluxN6": py_list(t, "l_fIntN6"),
+        "fluxT": py_list(t, "l_fIntT"),
+        **{k: c_array(t, "l_" + k).tolist() for k in
+           ["rfs", "fSolvE", "fSolvA", "dofsE", "dofsA", "tDofsE", "refEdofs", "refEneighDofs",
+            "refEneighFdofs", "refVdofsA", "refVneighDofsA", "refVneighFdofsA"]},
+    }
+    with open(os.path.join(HERE, "ref_kernel_fixtures.json"), "w") as f:
+        json.dump(fx, f)
+
+    # ---------------- exact matrices from the generator's tests ----------------
+    t = _read(os.path.join(REF, "tools/edge_pre/edge_pre/int/test_Matrices.py"))
+    start = t.index("def test_fluxTet")
+    tt = t[start:]
+    mats = {"stiff_deg2": stiff_ref_deg2(t)}
+    mats["fm_deg3"] = [[[float(v) for v in r] for r in sympy_matrix_block(tt, "fm%d" % fa)] for fa in range(4)]
+    fp = {}
+    for f1 in range(4):
+        for f2 in range(4):
+            for v in range(3):
+                key = "fp%d%d%d" % (f1, f2, v)
+                fp[key] = [[float(x) for x in r] for r in sympy_matrix_block(tt, key)]
+    mats["fp_deg3"] = fp
+
+    # ---------------- generated basis polynomials ----------------
+    rng = np.random.default_rng(3343)
+    pts3 = []
+    while len(pts3) < 24:
+        p = rng.random(3)
+        if p.sum() < 1.0:
+            pts3.append(p)
+    pts2 = []
+    while len(pts2) < 16:
+        p = rng.random(2)
+        if p.sum() < 1.0:
+            pts2.append(p)
+    gen = os.path.join(REF, "src/dg/generated")
+    nb3 = 56  # degree <= 5
+    nb2 = 21
+    mats["basis_tet_pts"] = [list(map(float, p)) for p in pts3]
+    mats["basis_tet_val"] = c_expr_basis(os.path.join(gen, "basis_tet.inc"), "val", 3, nb3, pts3).tolist()
+    for d in range(3):
+        mats["basis_tet_der%d" % d] = c_expr_basis(os.path.join(gen, "basis_tet_der_xi%d.inc" % (d + 1)),
+                                                   "valDxi%d" % (d + 1), 3, nb3, pts3).tolist()
+    mats["basis_tria_pts"] = [list(map(float, p)) for p in pts2]
+    mats["basis_tria_val"] = c_expr_basis(os.path.join(gen, "basis_tria.inc"), "val", 2, nb2, pts2).tolist()
+    with open(os.path.join(HERE, "ref_matrix_fixtures.json"), "w") as f:
+        json.dump(mats, f)
+    print("wrote fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
