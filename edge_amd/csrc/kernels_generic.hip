@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kThreads) void k_neigh(SolverDev<T> s, int64_t firs
 
 template <typename T, int O, int R>
 constexpr bool fits() {
-  return Sizes<T, O, R>::kLocalLds <= 64 * 1024;
+  return Sizes<T, O, R>::kLocalLds <= 160 * 1024;
 }
 
 template <typename T, int O, int R>
@@ -200,7 +200,7 @@ int dispatch_neigh(int order, const SolverDev<T>& s, int64_t first, int64_t n, c
 bool generic_supported(int order, int n_runs, int prec_bytes) {
   if (order < 2 || order > 5 || (n_runs != 1 && n_runs != 16)) return false;
   const size_t sz = 9 * n_modes(order) * n_runs, szf = 9 * n_face_modes(order) * n_runs;
-  return prec_bytes * (5 * sz + 2 * szf + 7 * 81) <= 64 * 1024;
+  return prec_bytes * (5 * sz + 2 * szf + 7 * 81) <= 160 * 1024;
 }
 
 template <typename T>

@@ -72,6 +72,8 @@ extern "C" int edge_mesh_box(const edge_box_params* p, edge_mesh_out* m) {
   if (!m || !m->ve_crds || !m->el_ve || !m->el_fa || !m->fa_el || !m->el_fa_el || !m->f_id || !m->v_id ||
       !m->fa_type || !m->normals || !m->tangents || !m->areas || !m->in_dia || !m->volume)
     return edge::fail(EDGE_ERR_ARG, "mesh output arrays missing");
+  if (!(p->jitter >= 0.0 && p->jitter < 0.5))
+    return edge::fail(EDGE_ERR_ARG, "jitter must be in [0, 0.5) of the hex edge (vertices may not cross)");
   for (int d = 0; d < 3; d++) {
     const int t = p->bc[d];
     if (t != EDGE_FA_PERIODIC && t != EDGE_FA_FREE_SURFACE && t != EDGE_FA_OUTFLOW)

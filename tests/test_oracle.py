@@ -1,0 +1,203 @@
+"""Pins the oracle (oracle/edge_oracle.c) before it is trusted as the checker:
+kernels against the reference's golden vectors, set-up against the
+reference's own Elasticity.cpp (oracle/_ref) and against closed-form
+properties of the Godunov flux, and the whole scheme against the analytic
+plane wave (convergence order p+1)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from edge_amd import mesh as M
+from edge_amd import setups as S
+
+
+def _a(x, *shape):
+    a = np.array(x, dtype=np.float64)
+    return a.reshape(shape) if shape else a
+
+
+# --------------------------------------------------------------- kernels
+def test_ck_elastic_matches_timepred_fixture(kernel_fx):
+    """TimePredVanilla.test.cpp:31-128 with the fixture's degree-3 inputs."""
+    tp = kernel_fx["timepred"]
+    dg = orc.Dg(4, 1, mats={"stiffT": _a(tp["stiffT"], 3, 20, 20)})
+    _, tint, _, _ = orc.ck(dg, tp["dt"], _a(tp["starE"], 3, 81), _a(tp["dofsE"])[..., None])
+    np.testing.assert_allclose(tint[..., 0], _a(tp["refEtDofs"]), rtol=1e-13, atol=1e-13)
+
+
+def test_ck_viscoelastic_matches_timepred_fixture(kernel_fx):
+    tp = kernel_fx["timepred"]
+    dg = orc.Dg(4, 1, mats={"stiffT": _a(tp["stiffT"], 3, 20, 20)}, rfs=_a(tp["rfs"]))
+    _, tint, _, tint_a = orc.ck(dg, tp["dt"], _a(tp["starE"], 3, 81), _a(tp["dofsE"])[..., None],
+                                _a(tp["starA"], 3, 18), _a(tp["srcA"], 2, 36), _a(tp["dofsA"])[..., None])
+    np.testing.assert_allclose(tint[..., 0], _a(tp["refVtDofsE"]), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(tint_a[..., 0], _a(tp["refVtDofsA"]), rtol=1e-13, atol=1e-15)
+
+
+def test_volume_matches_volint_fixture(kernel_fx):
+    vi = kernel_fx["volint"]
+    dg = orc.Dg(4, 1, mats={"stiffV": _a(vi["stiffV"], 3, 20, 20)})
+    out, _ = orc.vol(dg, _a(vi["starE"], 3, 81), _a(vi["tDofsE"])[..., None], _a(vi["dofsE"])[..., None])
+    np.testing.assert_allclose(out[..., 0], _a(vi["refEdofs"]), rtol=1e-13, atol=1e-12)
+    dg = orc.Dg(4, 1, mats={"stiffV": _a(vi["stiffV"], 3, 20, 20)}, rfs=_a(vi["rfs"]))
+    out, out_a = orc.vol(dg, _a(vi["starE"], 3, 81), _a(vi["tDofsE"])[..., None], _a(vi["dofsE"])[..., None],
+                         _a(vi["starA"], 3, 18), _a(vi["srcA"], 2, 36), _a(vi["tDofsA"])[..., None],
+                         _a(vi["dofsA"])[..., None])
+    np.testing.assert_allclose(out[..., 0], _a(vi["refVdofsE"]), rtol=1e-13, atol=1e-12)
+    np.testing.assert_allclose(out_a[..., 0], _a(vi["refVdofsA"]), rtol=1e-13, atol=1e-12)
+
+
+def _surf_dg(si, rfs=None):
+    fn = np.zeros((12, 10, 6))
+    fn[6] = _a(si["fluxN6"], 10, 6)
+    return orc.Dg(3, 1, mats={"fluxL": _a(si["fluxL"], 4, 10, 6), "fluxN": fn, "fluxT": _a(si["fluxT"], 4, 6, 10)},
+                  rfs=rfs)
+
+
+def test_surface_matches_surfint_fixture(kernel_fx):
+    """SurfIntVanilla.test.cpp:29-211: local, neighbour (fa 3, vId 1, fId 2) and free surface (fa 2)."""
+    si = kernel_fx["surfint"]
+    dg = _surf_dg(si)
+    fs = _a(si["fSolvE"], 4, 81)
+    t, d = _a(si["tDofsE"])[..., None], _a(si["dofsE"])[..., None]
+    out, _ = orc.surf_local(dg, fs, t, d)
+    np.testing.assert_allclose(out[..., 0], _a(si["refEdofs"]), rtol=1e-13, atol=1e-12)
+    out, _ = orc.surf_neigh(dg, 3, 1, 2, fs[0], t, d)
+    np.testing.assert_allclose(out[..., 0], _a(si["refEneighDofs"]), rtol=1e-13, atol=1e-12)
+    out, _ = orc.surf_neigh(dg, 2, orc.NO_ID16, orc.NO_ID16, fs[0], t, d)
+    np.testing.assert_allclose(out[..., 0], _a(si["refEneighFdofs"]), rtol=1e-13, atol=1e-12)
+
+
+def test_surface_viscoelastic_matches_surfint_fixture(kernel_fx):
+    si = kernel_fx["surfint"]
+    dg = _surf_dg(si, rfs=_a(si["rfs"]))
+    fs, fsa = _a(si["fSolvE"], 4, 81), _a(si["fSolvA"], 4, 54)
+    t, d = _a(si["tDofsE"])[..., None], _a(si["dofsE"])[..., None]
+    _, out_a = orc.surf_local(dg, fs, t, d, fsa, _a(si["dofsA"])[..., None])
+    np.testing.assert_allclose(out_a[..., 0], _a(si["refVdofsA"]), rtol=1e-13, atol=1e-12)
+    _, up = orc.surf_neigh(dg, 3, 1, 2, fs[0], t, d, fsa[0])
+    np.testing.assert_allclose(up[..., 0], _a(si["refVneighDofsA"]), rtol=1e-13, atol=1e-12)
+    _, up = orc.surf_neigh(dg, 2, orc.NO_ID16, orc.NO_ID16, fs[0], t, d, fsa[0])
+    np.testing.assert_allclose(up[..., 0], _a(si["refVneighFdofsA"]), rtol=1e-13, atol=1e-12)
+
+
+def test_fused_runs_are_independent():
+    """R fused runs equal R single runs; permuting runs permutes outputs (the
+    reference's fused tests duplicate one run and cannot see this)."""
+    rng = np.random.default_rng(7)
+    order, R = 4, 5
+    st = rng.normal(size=(3, 81))
+    dofs = rng.normal(size=(9, 20, R))
+    dg1, dgR = orc.Dg(order, 1), orc.Dg(order, R)
+    _, tR, _, _ = orc.ck(dgR, 0.01, st, dofs)
+    for r in range(R):
+        _, t1, _, _ = orc.ck(dg1, 0.01, st, dofs[..., r:r + 1])
+        np.testing.assert_allclose(tR[..., r], t1[..., 0], rtol=1e-14, atol=1e-14)
+    perm = rng.permutation(R)
+    _, tp, _, _ = orc.ck(dgR, 0.01, st, dofs[..., perm])
+    np.testing.assert_allclose(tp, tR[..., perm], rtol=1e-14, atol=1e-14)
+
+
+# --------------------------------------------------------------- set-up
+def _frame(rng):
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    s = np.cross(n, rng.normal(size=3))
+    s /= np.linalg.norm(s)
+    return n, s, np.cross(n, s)
+
+
+def _ref():
+    r = orc.ref_lib()
+    if r is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    return r
+
+
+def _P(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def test_star_matches_reference_elasticity():
+    r = _ref()
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        ji = rng.normal(size=(3, 3))
+        rho, lam, mu = rng.uniform(1, 3), rng.uniform(0.5, 3), rng.uniform(0.5, 2)
+        out = np.zeros(243)
+        r.ref_star(C.c_double(rho), C.c_double(lam), C.c_double(mu), _P(ji), _P(out))
+        np.testing.assert_array_equal(orc.star(rho, lam, mu, ji), out.reshape(3, 81))
+
+
+@pytest.mark.parametrize("free_surface", [False, True])
+def test_flux_solvers_match_reference_elasticity(free_surface):
+    """oracle fsL / fsR vs Elasticity::fs with the reference's own trafos."""
+    r = _ref()
+    rng = np.random.default_rng(12 + free_surface)
+    for _ in range(50):
+        n, s, t = _frame(rng)
+        if rng.random() < 0.5:  # the right element's frame (-n, s, t)
+            n = -n
+        mats = [rng.uniform(1, 3), rng.uniform(1, 3), rng.uniform(0.5, 3), rng.uniform(0.5, 3),
+                rng.uniform(0.3, 2), rng.uniform(0.3, 2)]
+        a, b = np.zeros(81), np.zeros(81)
+        r.ref_fs(*[C.c_double(x) for x in mats], _P(n), _P(s), _P(t), C.c_int(int(free_surface)), _P(a), _P(b))
+        fl, fr = orc.fs_face(*mats, n, s, t, free_surface)
+        np.testing.assert_allclose(fl, a.reshape(9, 9), rtol=1e-13, atol=1e-14)
+        np.testing.assert_allclose(fr, b.reshape(9, 9), rtol=1e-13, atol=1e-14)
+
+
+def _jacobian_n(rho, lam, mu, n):
+    """A_n = sum_k n_k A_k: with jinv = diag(n) in the first column, S_0 = A_n."""
+    ji = np.zeros((3, 3))
+    ji[:, 0] = n
+    return orc.star(rho, lam, mu, ji)[0].reshape(9, 9)
+
+
+def test_flux_solver_closed_form_properties():
+    """Equal materials: fsL + fsR = A_n (consistency) and fsL / fsR are the
+    positive / negative parts of A_n (eigenvalues c_p, c_s, c_s, 0 / their
+    negatives).  Free surface: the right state is the traction mirror."""
+    rng = np.random.default_rng(13)
+    for _ in range(20):
+        n, s, t = _frame(rng)
+        rho, lam, mu = rng.uniform(1, 3), rng.uniform(0.5, 3), rng.uniform(0.3, 2)
+        fl, fr = orc.fs_face(rho, rho, lam, lam, mu, mu, n, s, t)
+        an = _jacobian_n(rho, lam, mu, n)
+        np.testing.assert_allclose(fl + fr, an, rtol=1e-13, atol=1e-13)
+        cp, cs = np.sqrt((lam + 2 * mu) / rho), np.sqrt(mu / rho)
+        ev = np.sort(np.linalg.eigvals(fl).real)
+        np.testing.assert_allclose(ev, np.sort([cp, cs, cs, 0, 0, 0, 0, 0, 0]), atol=1e-12)
+        ev = np.sort(np.linalg.eigvals(fr).real)
+        np.testing.assert_allclose(ev, np.sort([-cp, -cs, -cs, 0, 0, 0, 0, 0, 0]), atol=1e-12)
+        # free surface: fsR_free = fsR_equal * T D T^-1 with D the traction mirror
+        _, ffs = orc.fs_face(rho, rho, lam, lam, mu, mu, n, s, t, True)
+        T, Ti = orc.trafo(n, s, t)
+        D = np.diag([-1, 1, 1, -1, 1, -1, 1, 1, 1.0])
+        np.testing.assert_allclose(ffs, fr @ T @ D @ Ti, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(T @ Ti, np.eye(9), atol=1e-14)
+
+
+# --------------------------------------------------------------- the scheme
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_oracle_plane_wave_convergence(order):
+    """Errors of the P plane wave shrink like h^(p+1) (p = order - 1)."""
+    pw = S.PlaneWave(1.0, 2.0, 1.0, k=(2 * np.pi,) * 3)
+    errs = []
+    for n in (4, 8):
+        m = M.box(n)
+        ne = m.n_el
+        rho, lam, mu = np.ones(ne), 2 * np.ones(ne), np.ones(ne)
+        mo = orc.MeshOracle(m, order, rho=rho, lam=lam, mu=mu)
+        dofs = np.ascontiguousarray(S.project(m, order, pw))
+        tint = np.zeros_like(dofs)
+        T = 0.1
+        dt = S.dt_cfl(m, rho, lam, mu, order)
+        ns = int(np.ceil(T / dt))
+        mo.step(dofs, tint, T / ns, ns)
+        _, l2, _ = S.error_norms(m, order, dofs, lambda x: pw(x, T), n1d=order + 2)
+        errs.append(np.sqrt(l2.sum()))
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > order - 0.3, (errs, rate)
