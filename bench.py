@@ -1,0 +1,188 @@
+"""Benchmark of the ADER-DG elastic tet4 element update on one MI355X.
+
+Workload (the headline): 32^3 hexes x 6 = 196,608 tets, interior vertices
+jittered by +-10% (seeded), per-element rho ~ U[2,3], vp ~ U[4,6],
+vs = vp / sqrt(3), degree 4 (order 5, 35 modes), 16 fused runs in FP32,
+global time stepping at the CFL time step.  Beside it the plain FP64
+single-run update on the same mesh is timed (the drop-in), and the FP64
+oracle (the vanilla-semantics CPU restatement) on a bounded sample of the
+same workload on the host cores.
+
+A step is one local + one neighbour pass over all elements.  Inputs are
+resident in HBM before the timed region.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DOF-updates/s, tet4 elastic O5 FP32 fused-16 vs FP64 plain; % roofline"
+PEAK = {"fp32_mfma": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
+
+# Dense algorithmic FLOPs per element, step and run (vanilla GEMM shapes,
+# src/impl/seismic/kernels/*Vanilla.hpp): local = CK (+axpy) + volume + local
+# surface, neigh = neighbouring surface.
+FLOPS = {4: {"local": 20466 + 630 + 20520 + 35280, "neigh": 35280},
+         5: {"local": 67986 + 1260 + 54810 + 85320, "neigh": 85320}}
+
+
+def alg_bytes(order, runs, s, s_c):
+    """Compulsory bytes per element-step: DOFs read+write, tInt write+read,
+    dense star + two flux-solver sets, connectivity."""
+    n = order * (order + 1) * (order + 2) // 6
+    return s * 9 * n * runs * 4 + s_c * (3 * 81 + 2 * 4 * 81) + 80
+
+
+def build_problem(n_hex, order, seed=3343):
+    from edge_amd import mesh as M
+    from edge_amd import setups as S
+    m = M.box(n_hex, jitter=0.1, seed=seed, block=4)
+    rho, lam, mu = S.random_materials(m.n_el, seed=seed)
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    q0 = S.project(m, order, S.PlaneWave(k=(2 * np.pi,) * 3), chunk=4096)[..., 0]  # [el][9][N]
+    return m, rho, lam, mu, dt, q0
+
+
+def time_solver(solver, dofs, tint, dt, steps, warmup):
+    """Times exactly `steps` steps between synchronisations; per-kernel
+    durations from HIP events recorded around each launch on the stream
+    the kernels run on."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        solver.step(dofs, tint, dt, 1, stream=stream)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i][0].record(stream)
+        solver.local(dofs, tint, dt, stream=stream)
+        ev[i][1].record(stream)
+        solver.neigh(dofs, tint, stream=stream)
+        ev[i][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    loc = np.mean([e[0].elapsed_time(e[1]) for e in ev]) * 1e-3
+    nei = np.mean([e[1].elapsed_time(e[2]) for e in ev]) * 1e-3
+    return wall, loc, nei
+
+
+def cpu_baseline(m, order, runs, rho, lam, mu, dt, q0, budget_s=15.0):
+    """FP64 oracle (vanilla-semantics CPU restatement; LIBXSMM is not
+    available offline) on a bounded element sample, all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    star, fs, sample = orc.setup_mesh_sample(m, rho, lam, mu, n_sample=4096)
+    n_s = star.shape[0]
+    mo = orc.SampleOracle(order, runs, star, fs, sample)
+    dofs = np.ascontiguousarray(np.repeat(q0[:n_s, ..., None], runs, axis=-1))
+    tint = np.zeros_like(dofs)
+    t0 = time.perf_counter()
+    n_el = 0
+    reps = 0
+    while time.perf_counter() - t0 < budget_s and reps < 64:
+        chunk = 256
+        for e0 in range(0, n_s, chunk):
+            mo.local(dofs, tint, dt, e0, min(chunk, n_s - e0))
+            mo.neigh(dofs, tint, e0, min(chunk, n_s - e0))
+            n_el += min(chunk, n_s - e0)
+            if time.perf_counter() - t0 > budget_s:
+                break
+        reps += 1
+    wall = time.perf_counter() - t0
+    n = order * (order + 1) * (order + 2) // 6
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n_el * 9 * n * runs / wall, "unit": "DOF-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{n_el} element-steps (local+neigh) of the {m.n_el}-tet O{order} R={runs} FP64 workload, "
+                      f"{wall:.1f} s, OpenMP over elements; vanilla-semantics restatement, LIBXSMM unavailable"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--hex", type=int, default=32, help="hexes per axis (32 -> 196,608 tets)")
+    ap.add_argument("--order", type=int, default=5)
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 MFMA-tiled")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the element update's multi-GPU sharding is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from edge_amd.solver import ElasticSolver
+
+    order, runs = args.order, 16
+    print(f"[bench] building the {6 * args.hex ** 3}-tet mesh (order {order})", file=sys.stderr, flush=True)
+    m, rho, lam, mu, dt, q0 = build_problem(args.hex, order)
+    n = q0.shape[2]
+    scale = torch.linspace(0.25, 1.0, runs, dtype=torch.float64)  # distinct runs
+    res = {}
+
+    # ---- FP32 fused-16 (headline)
+    s32 = ElasticSolver(m, order, runs, 32, rho, lam, mu, kernel=args.kernel)
+    d32 = (torch.from_numpy(q0)[..., None] * scale).to(torch.float32).cuda().contiguous()
+    t32 = torch.zeros_like(d32)
+    print("[bench] fp32 fused-16 timing", file=sys.stderr, flush=True)
+    wall, loc, nei = time_solver(s32, d32, t32, dt, args.steps, args.warmup)
+    ok = bool(torch.isfinite(d32).all())
+    res["fp32"] = dict(wall=wall, local=loc, neigh=nei, finite=ok)
+    del d32, t32
+    s32.close()
+
+    # ---- FP64 plain single run (the drop-in)
+    if not args.no_fp64:
+        s64 = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=args.kernel)
+        d64 = torch.from_numpy(q0[..., None].copy()).cuda().contiguous()
+        t64 = torch.zeros_like(d64)
+        print("[bench] fp64 plain timing", file=sys.stderr, flush=True)
+        w64, l64, n64 = time_solver(s64, d64, t64, dt, max(1, args.steps // 2), args.warmup)
+        res["fp64"] = dict(wall=w64, local=l64, neigh=n64, steps=max(1, args.steps // 2))
+        del d64, t64
+        s64.close()
+
+    dofs_per_step = m.n_el * 9 * n
+    value = dofs_per_step * runs * args.steps / res["fp32"]["wall"]
+    fl = FLOPS.get(order)
+    roof = None
+    if fl:
+        achieved = fl["local"] * runs * m.n_el / res["fp32"]["local"]
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32_mfma"] / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK["fp32_mfma"], "traffic": None, "kernel": "local (CK + volume + local surface)",
+                "neigh_tflops": fl["neigh"] * runs * m.n_el / res["fp32"]["neigh"] / 1e12}
+    out = {
+        "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * res["fp32"]["wall"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"elastic tet4 O{order} FP32 fused-{runs}, {m.n_el} tets (32^3 hexes x 6, "
+                               f"+-10% jitter, random rho/vp), plane-wave initial DOFs scaled per run",
+                   "elements": m.n_el, "order": order, "runs": runs, "dt": dt, "finite": res["fp32"]["finite"]},
+        "roofline": roof,
+    }
+    if "fp64" in res:
+        v64 = dofs_per_step * res["fp64"]["steps"] / res["fp64"]["wall"]
+        out["fp64_plain"] = {"value": v64, "unit": "DOF-updates/s", "ms_per_step": 1e3 * res["fp64"]["wall"] /
+                             res["fp64"]["steps"], "local_ms": 1e3 * res["fp64"]["local"],
+                             "neigh_ms": 1e3 * res["fp64"]["neigh"]}
+        if fl:
+            a64 = (fl["local"] + fl["neigh"]) * m.n_el / (res["fp64"]["local"] + res["fp64"]["neigh"])
+            out["fp64_plain"]["roofline_frac"] = a64 / PEAK["fp64"]
+        out["speedup_fp32_fused16_vs_fp64_plain"] = value / v64
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(m, order, runs, rho, lam, mu, dt, q0, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -204,3 +204,47 @@ class MeshOracle:
         for _ in range(n_steps):
             self.local(dofs, tint, dt)
             self.neigh(dofs, tint)
+
+
+# ---------------------------------------------------------------- CPU baseline sample
+def setup_mesh_sample(mesh, rho, lam, mu, n_sample):
+    """Star matrices and flux solvers of the first n_sample elements of a
+    mesh (flux solvers built over all faces, then sliced)."""
+    n_s = min(n_sample, mesh.n_el)
+    ve = mesh.el_crds()[:n_s]
+    st = np.zeros((n_s, 3, 81))
+    for e in range(n_s):
+        _, _, ji = jac(ve[e])
+        st[e] = star(rho[e], lam[e], mu[e], ji)
+    d = mesh.el_crds()
+    jdet = np.linalg.det(d[:, 1:, :] - d[:, :1, :])
+    fs = np.zeros((2, mesh.n_el, 4, 81))
+    lib().orc_init_fs(C.c_int64(mesh.n_fa), _p(np.ascontiguousarray(mesh.fa_el)),
+                      _p(np.ascontiguousarray(mesh.el_fa)), _p(np.ascontiguousarray(mesh.fa_type)),
+                      _p(_c(mesh.normals)), _p(_c(mesh.tangents)), _p(_c(mesh.areas)), _p(_c(jdet)),
+                      _p(_c(rho)), _p(_c(lam)), _p(_c(mu)), _p(fs[0]), _p(fs[1]))
+    sample = {"el_fa_el": np.where(mesh.el_fa_el[:n_s] >= 0, mesh.el_fa_el[:n_s] % n_s, -1),
+              "fa_type": mesh.fa_type[mesh.el_fa[:n_s]].ravel().astype(np.int32),
+              "f_id": mesh.f_id[:n_s], "v_id": mesh.v_id[:n_s]}
+    fs_s = np.ascontiguousarray(fs[:, :n_s])
+    return st, fs_s, sample
+
+
+class SampleOracle(MeshOracle):
+    """The oracle's loops over a sample of a mesh; neighbours outside the
+    sample are folded back into it (same work per element)."""
+
+    def __init__(self, order, n_crs, star, fs, sample):
+        self.dg = Dg(order, n_crs)
+        self.star = _c(star)
+        self.fs = _c(fs)
+        n_s = star.shape[0]
+
+        class _M:
+            n_el = n_s
+        self.mesh = _M()
+        self.el_fa = np.arange(4 * n_s, dtype=np.int64).reshape(n_s, 4)
+        self.el_fa_el = np.ascontiguousarray(sample["el_fa_el"], dtype=np.int64)
+        self.f_id = np.ascontiguousarray(sample["f_id"], dtype=np.uint16)
+        self.v_id = np.ascontiguousarray(sample["v_id"], dtype=np.uint16)
+        self.fa_type = np.ascontiguousarray(sample["fa_type"], dtype=np.int32)
