@@ -205,3 +205,31 @@ def test_gpu_plane_wave_convergence(gpu, order):
         errs.append(np.sqrt(l2.sum()))
     rate = np.log2(errs[0] / errs[1])
     assert rate > order - 0.3, (errs, rate)
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+def test_mfma_random_matrices_match_oracle(gpu, order):
+    """The MFMA-tiled FP32 fused-16 kernels with dense random star matrices
+    and flux solvers (free-surface, outflow and periodic faces), one local and
+    one neighbour pass, then three steps, against the FP64 oracle."""
+    m = M.box(3, bc=(FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC), jitter=0.1, seed=4)
+    rng = np.random.default_rng(order)
+    star = rng.normal(size=(m.n_el, 3, 81)) * 0.2
+    fs = rng.normal(size=(2, m.n_el, 4, 81)) * 0.2
+    s = ElasticSolver(m, order, 16, 32, star=star, fs=fs, kernel=KERNEL_MFMA)
+    mo = orc.MeshOracle(m, order, 16, star=star, fs=fs)
+    dofs0 = rng.normal(size=(m.n_el, 9, S.n_modes(order), 16))
+    d, t = dofs0.copy(), np.zeros_like(dofs0)
+    mo.local(d, t, 0.05)
+    gd, gt = _to_dev(dofs0, 32), _to_dev(np.zeros_like(dofs0), 32)
+    s.local(gd, gt, 0.05)
+    assert rel_l2(gt.cpu().numpy(), t) <= FP32_TOL
+    assert rel_l2(gd.cpu().numpy(), d) <= FP32_TOL
+    mo.neigh(d, t)
+    s.neigh(gd, gt)
+    assert rel_l2(gd.cpu().numpy(), d) <= FP32_TOL
+    mo.step(d, t, 0.05, 3)
+    s.step(gd, gt, 0.05, 3)
+    got = gd.cpu().numpy()
+    for r in range(16):
+        assert rel_l2(got[..., r], d[..., r]) <= FP32_TOL
