@@ -30,7 +30,15 @@ struct SolverDev {
   NeighDev nb;
   const T* star;  // [nEl][3][81]
   const T* fs;    // [2][nEl][4][81]
+  bool sparse_star;  // every star matrix has the elastic 24-entry pattern
 };
+
+// Non-zero pattern of the elastic star matrices: the union of the patterns
+// of the Jacobians A, B, C (src/impl/seismic/setups/Elasticity.cpp:172-250),
+// 24 of 81 entries; row q lists its columns.
+constexpr int kStarNnzRow[9] = {3, 3, 3, 2, 2, 2, 3, 3, 3};
+constexpr int kStarCol[9][3] = {{6, 7, 8}, {6, 7, 8}, {6, 7, 8}, {6, 7, 0}, {7, 8, 0},
+                                {6, 8, 0}, {0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
 
 // Generic kernels: one workgroup per element, all tiles in LDS.
 template <typename T>

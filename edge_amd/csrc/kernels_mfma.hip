@@ -10,10 +10,15 @@
 // with g = lane / 16, c = lane % 16.  B operands come either from memory in
 // row order (k = 4 s + g) or straight from a previous D tile with the
 // k-order permuted (k = 16 u + 4 g + v), so GEMM chains stay in registers.
-// One wave handles one element at a time: the 9 quantities of an (n, r)
-// position then live in the same lane, and the per-element 9 x 9 star and
-// flux-solver products are lane-local VALU FMAs with wave-uniform
-// coefficients (scalar loads).
+// One wave handles one element: the 9 quantities of an (n, r) position then
+// live in the same lane, so the per-element 9 x 9 star and flux-solver
+// products are lane-local VALU FMAs with wave-uniform coefficients (scalar
+// loads).  The star matrices are applied to the B operands before the
+// stiffness products (9 mixed values per k-step instead of whole tiles);
+// with the elastic pattern only its 24 non-zeros are touched.
+//
+// Occupancy: two waves per SIMD (<= 256 registers, <= 20 KiB LDS per wave)
+// so that one wave's VALU work runs while the other's MFMAs do.
 //
 // Operations (src/impl/seismic/solvers/AderDg.hpp:362-533, :553-722):
 //   local: CK time prediction (TimePredVanilla.hpp:200-306), volume
@@ -21,7 +26,7 @@
 //          (SurfIntVanilla.hpp:212-262);
 //   neigh: neighbouring surface integral (SurfIntVanilla.hpp:274-357).
 // Summation order differs from the reference's loops (tile order, the star
-// applied after the stiffness product); results agree to rounding.
+// applied before the stiffness product); results agree to rounding.
 #include <hip/hip_runtime.h>
 
 #include "kernels_mfma.hpp"
@@ -45,6 +50,8 @@ struct Dims {
   static constexpr int NF = n_face_modes(O);
   static constexpr int NT = tiles(N);
   static constexpr int SZ = 9 * N * R;
+  static constexpr int NVEC = SZ / 4;             // 16-byte vectors per element tile
+  static constexpr int CHUNKS = (NVEC + 63) / 64;  // KiB-sized async copies per tile
 };
 
 // blockIdx -> element offset: each of the 8 XCDs (blocks dispatched round
@@ -63,6 +70,25 @@ __device__ __forceinline__ float ldg(const float* __restrict__ p, int idx, int c
   const float v = p[ok ? idx : clamp_idx];
   return ok ? v : 0.f;
 }
+
+typedef __attribute__((address_space(3))) void* lds_ptr;
+
+// Asynchronous copy of an element tile (n_vec 16-byte vectors, contiguous)
+// from global memory into LDS (global_load_lds_dwordx4: 1 KiB per wave and
+// instruction).  The LDS buffer holds CHUNKS KiB; lanes past the tile re-read
+// the tile's first vector into the buffer's padding, so every access stays in
+// bounds.  Completion is awaited with vmcnt.
+template <int CHUNKS>
+__device__ __forceinline__ void tile_to_lds(const float* __restrict__ src, float* dst, int n_vec, int lane) {
+#pragma unroll 2
+  for (int i = 0; i < CHUNKS; i++) {
+    const int idx = i * 64 + lane;
+    const float* gp = src + 4 * (idx < n_vec ? idx : 0);
+    __builtin_amdgcn_global_load_lds((const void*)gp, (lds_ptr)(dst + i * 256), 16, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // D-layout load of rows [16 t + 4 g + v] (< rows) of X[k][r] (row stride R)
 __device__ __forceinline__ v4f load_d(const float* __restrict__ x, int rows, int t, int g, int c) {
@@ -83,19 +109,33 @@ __device__ __forceinline__ void store_d(float* __restrict__ x, int rows, int t, 
   }
 }
 
+// Row q of a star matrix times b: z = sum_p S[q][p] b[p] (dense, or the 24
+// elastic non-zeros).  q is a compile-time constant after unrolling.
+template <bool SP>
+__device__ __forceinline__ float star_row(const float* __restrict__ sd, int q, const float (&b)[9]) {
+  if constexpr (SP) {
+    constexpr int cols[9][3] = {{6, 7, 8}, {6, 7, 8}, {6, 7, 8}, {6, 7, 0}, {7, 8, 0},
+                                {6, 8, 0}, {0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
+    float z = sd[q * 9 + cols[q][0]] * b[cols[q][0]] + sd[q * 9 + cols[q][1]] * b[cols[q][1]];
+    if (q < 3 || q > 5) z += sd[q * 9 + cols[q][2]] * b[cols[q][2]];
+    return z;
+  } else {
+    float z = 0.f;
+#pragma unroll
+    for (int p = 0; p < 9; p++) z += sd[q * 9 + p] * b[p];
+    return z;
+  }
+}
+
 // ---------------------------------------------------------------- local
-template <int O, int DE>
-__device__ __forceinline__ void ck_stage(const float* __restrict__ stiffTn, const float* __restrict__ star,
-                                         const float* __restrict__ q0, float* __restrict__ l_der,
-                                         float* __restrict__ l_tint, float dt, float& sca, int g, int c) {
+// CK stage 1: der_1[q] = sum_d (-K_d^T M^-1)^T (sum_p S_d[q][p] Q_p), B operands
+// read from the DOFs in global memory in row order.
+template <int O, bool SP>
+__device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, const float* __restrict__ star,
+                                         const float* __restrict__ q0, v4f (&der)[9][tiles(n_modes(O - 1))], int g,
+                                         int c) {
   constexpr int N = Dims<O>::N;
-  constexpr int NK = n_modes(O - DE + 1);  // rows of the previous derivative
-  constexpr int NN = n_modes(O - DE);      // rows of this derivative
-  constexpr int TN = tiles(NN);
-  constexpr int KS = ksteps(NK);
-  // der_q = sum_d sum_k (-K_d^T M^-1)[k][n] (sum_p S_d[q][p] der_prev_p[k][r]): the
-  // star matrix is applied to the B operands (row order k = 4 s + g)
-  v4f der[9][TN];
+  constexpr int NK = N, NN = n_modes(O - 1), TN = tiles(NN), KS = ksteps(NK);
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
@@ -104,69 +144,156 @@ __device__ __forceinline__ void ck_stage(const float* __restrict__ stiffTn, cons
   for (int d = 0; d < 3; d++) {
     const float* m = stiffTn + d * N * N;
     const float* sd = star + d * 81;
-#pragma unroll 1
+#pragma unroll
     for (int s = 0; s < KS; s++) {
+      __builtin_amdgcn_sched_barrier(0);
       const int k = 4 * s + g;
       float a[TN], b[9];
 #pragma unroll
       for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
 #pragma unroll
       for (int p = 0; p < 9; p++) {
-        if constexpr (DE == 1)
-          b[p] = ldg(q0, (p * N + k) * R + c, c, k < NK);
+        if (4 * s + 3 < NK)  // constant after unrolling
+          b[p] = q0[(p * N + k) * R + c];
         else
-          b[p] = ldg(l_der, (p * NK + k) * R + c, c, k < NK);
+          b[p] = ldg(q0, (p * N + k) * R + c, c, k < NK);
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        float z = 0.f;
-#pragma unroll
-        for (int p = 0; p < 9; p++) z += sd[q * 9 + p] * b[p];
+        const float z = star_row<SP>(sd, q, b);
 #pragma unroll
         for (int t = 0; t < TN; t++) der[q][t] = mfma(a[t], z, der[q][t]);
       }
     }
   }
-  sca *= dt / float(DE + 1);
-  // tInt += dt^{DE+1}/(DE+1)! der_DE (each lane updates the entries it holds)
+}
+
+// CK stage DE >= 2 from the previous derivative held in registers
+// (permuted k: row 16 u + 4 g + v is register v of tile u).
+template <int O, int DE, bool SP>
+__device__ __forceinline__ void ck_next(const float* __restrict__ stiffTn, const float* __restrict__ star,
+                                        const v4f (&prev)[9][tiles(n_modes(O - DE + 1))],
+                                        v4f (&cur)[9][tiles(n_modes(O - DE))], int g, int c) {
+  constexpr int N = Dims<O>::N;
+  constexpr int NK = n_modes(O - DE + 1), NN = n_modes(O - DE), TP = tiles(NK), TN = tiles(NN);
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
-    for (int t = 0; t < TN; t++) {
+    for (int t = 0; t < TN; t++) cur[q][t] = zero4();
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const float* m = stiffTn + d * N * N;
+    const float* sd = star + d * 81;
+#pragma unroll
+    for (int u = 0; u < TP; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        if (16 * u + v < NK) {  // constant after unrolling
+          __builtin_amdgcn_sched_barrier(0);
+          const int k = 16 * u + 4 * g + v;
+          float a[TN], b[9];
+#pragma unroll
+          for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+#pragma unroll
+          for (int p = 0; p < 9; p++) b[p] = prev[p][u][v];
+#pragma unroll
+          for (int q = 0; q < 9; q++) {
+            const float z = star_row<SP>(sd, q, b);
+#pragma unroll
+            for (int t = 0; t < TN; t++) cur[q][t] = mfma(a[t], z, cur[q][t]);
+          }
+        }
+      }
+  }
+}
+
+// tInt (LDS) += sca * der, each lane updating the entries it holds
+template <int O, int TN>
+__device__ __forceinline__ void add_tint(float* __restrict__ l_ti, int rows, float sca, const v4f (&der)[9][TN],
+                                         int g, int c) {
+  constexpr int N = Dims<O>::N;
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+#pragma unroll
+    for (int t = 0; t < TN; t++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int k = 16 * t + 4 * g + v;
-        if (k < NN) l_tint[(q * N + k) * R + c] += sca * der[q][t][v];
+        if (k < rows) l_ti[(q * N + k) * R + c] += sca * der[q][t][v];
       }
-    }
-  if constexpr (DE + 1 < O) {
-    __syncthreads();  // previous readers of l_der are done
-#pragma unroll
-    for (int q = 0; q < 9; q++)
-#pragma unroll
-      for (int t = 0; t < TN; t++) store_d(l_der + q * NN * R, NN, t, g, c, der[q][t]);
-    __syncthreads();
-  }
 }
 
-template <int O, int DE>
-__device__ __forceinline__ void ck_all(const float* stiffTn, const float* star, const float* q0, float* l_der,
-                                       float* l_tint, float dt, float& sca, int g, int c) {
+template <int O, int DE, bool SP>
+__device__ __forceinline__ void ck_rest(const float* stiffTn, const float* star,
+                                        const v4f (&prev)[9][tiles(n_modes(O - DE + 1))], float* l_ti, float dt,
+                                        float sca, int g, int c) {
   if constexpr (DE < O) {
-    ck_stage<O, DE>(stiffTn, star, q0, l_der, l_tint, dt, sca, g, c);
-    ck_all<O, DE + 1>(stiffTn, star, q0, l_der, l_tint, dt, sca, g, c);
+    constexpr int NN = n_modes(O - DE);
+    v4f cur[9][tiles(NN)];
+    ck_next<O, DE, SP>(stiffTn, star, prev, cur, g, c);
+    sca *= dt / float(DE + 1);
+    add_tint<O, tiles(NN)>(l_ti, NN, sca, cur, g, c);
+    ck_rest<O, DE + 1, SP>(stiffTn, star, cur, l_ti, dt, sca, g, c);
   }
 }
 
+// acc_q += F^T_fa^T (fs (F_fa^T X_q)) for one face: the projection with F
+// (fluxL or fluxN, [N][NF]) of the tile X in LDS ([q][N][R]), the 9 x 9 flux
+// solver fs (wave-uniform), and the lift with F^T ([NF][N]) from registers.
 template <int O>
-__global__ __launch_bounds__(64) void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt,
-                                              float* __restrict__ dofs, float* __restrict__ t_int) {
+__device__ __forceinline__ void face_update(const float* __restrict__ fl, const float* __restrict__ ft,
+                                            const float* __restrict__ f, const float* __restrict__ x,
+                                            v4f (&acc)[9][Dims<O>::NT], int g, int c) {
+  constexpr int N = Dims<O>::N, NF = Dims<O>::NF, NT = Dims<O>::NT;
+  constexpr int KS = ksteps(N);
+  v4f s0[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) s0[q] = zero4();
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+    const int k = 4 * ks + g;
+    const float al = ldg(fl, k * NF + c, 0, k < N && c < NF);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      float b;
+      if (4 * ks + 3 < N)  // constant after unrolling
+        b = x[(q * N + k) * R + c];
+      else
+        b = ldg(x, (q * N + k) * R + c, c, k < N);
+      s0[q] = mfma(al, b, s0[q]);
+    }
+  }
+  // per register column v (face modes 4 g + v): s1 = fs s0, then the lift
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    __builtin_amdgcn_sched_barrier(0);
+    float s1[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      float z = 0.f;
+#pragma unroll
+      for (int p = 0; p < 9; p++) z += f[q * 9 + p] * s0[p][v];
+      s1[q] = z;
+    }
+    const int fr = 4 * g + v;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const float at = ldg(ft, fr * N + 16 * t + c, 0, fr < NF && 16 * t + c < N);
+#pragma unroll
+      for (int q = 0; q < 9; q++) acc[q][t] = mfma(at, s1[q], acc[q][t]);
+    }
+  }
+}
+
+template <int O, bool SP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __restrict__ dofs,
+             float* __restrict__ t_int) {
   using D = Dims<O>;
   constexpr int N = D::N, NF = D::NF, NT = D::NT;
   constexpr int NV = n_modes(O - 1);  // rows of tInt the volume integral reads
-  // time-integrated DOFs and the derivative handed from one CK stage to the next
-  __shared__ float l_buf[9 * N * R];
-  __shared__ float l_der[9 * (O > 1 ? n_modes(O - 1) : 1) * R];
+  __shared__ float l_ti[D::SZ];       // time-integrated DOFs
   const int64_t off = xcd_element(n);
   if (off >= n) return;
   const int64_t el = first + off;
@@ -175,15 +302,17 @@ __global__ __launch_bounds__(64) void k_local(SolverDev<float> s, int64_t first,
   const float* star = s.star + el * 243;
 
   // ---- CK: tInt = dt Q + sum_de dt^{de+1}/(de+1)! der_de, accumulated in LDS
-#pragma unroll
-  for (int q = 0; q < 9; q++)
-#pragma unroll
-    for (int t = 0; t < NT; t++) store_d(l_buf + q * N * R, N, t, g, c, dt * load_d(q0 + q * N * R, N, t, g, c));
-  float sca = dt;
-  ck_all<O, 1>(s.dg.stiffTn, star, q0, l_der, l_buf, dt, sca, g, c);
+  for (int i = lane; i < D::SZ; i += 64) l_ti[i] = dt * q0[i];
+  {
+    v4f der[9][tiles(NV)];
+    ck_first<O, SP>(s.dg.stiffTn, star, q0, der, g, c);
+    const float sca = dt * dt / 2.f;
+    add_tint<O, tiles(NV)>(l_ti, NV, sca, der, g, c);
+    ck_rest<O, 2, SP>(s.dg.stiffTn, star, der, l_ti, dt, sca, g, c);
+  }
   __syncthreads();
   float* gti = t_int + el * D::SZ;
-  for (int i = lane; i < 9 * N * R; i += 64) gti[i] = l_buf[i];
+  for (int i = lane; i < D::SZ; i += 64) gti[i] = l_ti[i];
 
   // ---- volume: acc_q += sum_d stiffV_d^T (sum_p S_d[q][p] tInt_p)
   v4f acc[9][NT];
@@ -196,19 +325,23 @@ __global__ __launch_bounds__(64) void k_local(SolverDev<float> s, int64_t first,
   for (int d = 0; d < 3; d++) {
     const float* m = s.dg.stiffV + d * N * N;
     const float* sd = star + d * 81;
-#pragma unroll 1
+#pragma unroll
     for (int ks = 0; ks < KV; ks++) {
+      __builtin_amdgcn_sched_barrier(0);
       const int k = 4 * ks + g;
       float a[NT], b[9];
 #pragma unroll
       for (int t = 0; t < NT; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NV && 16 * t + c < N);
 #pragma unroll
-      for (int p = 0; p < 9; p++) b[p] = ldg(l_buf, (p * N + k) * R + c, c, k < NV);
+      for (int p = 0; p < 9; p++) {
+        if (4 * ks + 3 < NV)  // constant after unrolling
+          b[p] = l_ti[(p * N + k) * R + c];
+        else
+          b[p] = ldg(l_ti, (p * N + k) * R + c, c, k < NV);
+      }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        float z = 0.f;
-#pragma unroll
-        for (int p = 0; p < 9; p++) z += sd[q * 9 + p] * b[p];
+        const float z = star_row<SP>(sd, q, b);
 #pragma unroll
         for (int t = 0; t < NT; t++) acc[q][t] = mfma(a[t], z, acc[q][t]);
       }
@@ -217,43 +350,10 @@ __global__ __launch_bounds__(64) void k_local(SolverDev<float> s, int64_t first,
 
   // ---- local surface integral
   const float* fs0 = s.fs + el * 324;
-  constexpr int KS = ksteps(N);
 #pragma unroll 1
-  for (int fa = 0; fa < 4; fa++) {
-    const float* fl = s.dg.fluxLN + fa * N * NF;
-    v4f s0[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) s0[q] = zero4();
-#pragma unroll 1
-    for (int ks = 0; ks < KS; ks++) {
-      const int k = 4 * ks + g;
-      const float a = ldg(fl, k * NF + c, 0, k < N && c < NF);
-#pragma unroll
-      for (int q = 0; q < 9; q++) {
-        const float b = ldg(l_buf, (q * N + k) * R + c, c, k < N);
-        s0[q] = mfma(a, b, s0[q]);
-      }
-    }
-    const float* f = fs0 + fa * 81;
-    v4f s1[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-      s1[q] = zero4();
-#pragma unroll
-      for (int p = 0; p < 9; p++) s1[q] += f[q * 9 + p] * s0[p];
-    }
-    const float* ft = s.dg.fluxT + fa * NF * N;
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int fr = 4 * g + v;  // permuted k: face mode held in register v
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const float a = ldg(ft, fr * N + 16 * t + c, 0, fr < NF && 16 * t + c < N);
-#pragma unroll
-        for (int q = 0; q < 9; q++) acc[q][t] = mfma(a, s1[q][v], acc[q][t]);
-      }
-    }
-  }
+  for (int fa = 0; fa < 4; fa++)
+    face_update<O>(s.dg.fluxLN + fa * N * NF, s.dg.fluxT + fa * NF * N, fs0 + fa * 81, l_ti, acc, g, c);
+
   float* out = dofs + el * D::SZ;
 #pragma unroll
   for (int q = 0; q < 9; q++)
@@ -263,10 +363,12 @@ __global__ __launch_bounds__(64) void k_local(SolverDev<float> s, int64_t first,
 
 // ---------------------------------------------------------------- neigh
 template <int O>
-__global__ __launch_bounds__(64) void k_neigh(SolverDev<float> s, int64_t first, int64_t n,
-                                              const float* __restrict__ t_int, float* __restrict__ dofs) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_neigh(SolverDev<float> s, int64_t first, int64_t n, const float* __restrict__ t_int,
+             float* __restrict__ dofs) {
   using D = Dims<O>;
   constexpr int N = D::N, NF = D::NF, NT = D::NT;
+  __shared__ float l_nb[D::CHUNKS * 256];  // the neighbour's tInt tile
   const int64_t off = xcd_element(n);
   if (off >= n) return;
   const int64_t el = first + off;
@@ -277,46 +379,17 @@ __global__ __launch_bounds__(64) void k_neigh(SolverDev<float> s, int64_t first,
   for (int q = 0; q < 9; q++)
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[q][t] = load_d(q0 + q * N * R, N, t, g, c);
-  constexpr int KS = ksteps(N);
 #pragma unroll 1
   for (int fa = 0; fa < 4; fa++) {
     const int32_t ne = s.nb.ne[el * 4 + fa];
     if (ne < 0) continue;  // outflow (uniform across the wave)
+    __syncthreads();       // the previous face's reads of l_nb are done
+    tile_to_lds<D::CHUNKS>(t_int + (int64_t)ne * D::SZ, l_nb, D::NVEC, lane);
+    wait_vm_all();
+    __syncthreads();
     const int fm = s.nb.fmat[el * 4 + fa];
-    const float* fl = s.dg.fluxLN + fm * N * NF;
-    const float* tn = t_int + (int64_t)ne * D::SZ;
-    v4f s0[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) s0[q] = zero4();
-#pragma unroll 1
-    for (int ks = 0; ks < KS; ks++) {
-      const int k = 4 * ks + g;
-      const float a = ldg(fl, k * NF + c, 0, k < N && c < NF);
-#pragma unroll
-      for (int q = 0; q < 9; q++) {
-        const float b = ldg(tn, (q * N + k) * R + c, c, k < N);
-        s0[q] = mfma(a, b, s0[q]);
-      }
-    }
-    const float* f = s.fs + ((s.n_el + el) * 4 + fa) * 81;
-    v4f s1[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-      s1[q] = zero4();
-#pragma unroll
-      for (int p = 0; p < 9; p++) s1[q] += f[q * 9 + p] * s0[p];
-    }
-    const float* ft = s.dg.fluxT + fa * NF * N;
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int fr = 4 * g + v;
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const float a = ldg(ft, fr * N + 16 * t + c, 0, fr < NF && 16 * t + c < N);
-#pragma unroll
-        for (int q = 0; q < 9; q++) acc[q][t] = mfma(a, s1[q][v], acc[q][t]);
-      }
-    }
+    face_update<O>(s.dg.fluxLN + fm * N * NF, s.dg.fluxT + fa * NF * N, s.fs + ((s.n_el + el) * 4 + fa) * 81,
+                   l_nb, acc, g, c);
   }
 #pragma unroll
   for (int q = 0; q < 9; q++)
@@ -325,6 +398,16 @@ __global__ __launch_bounds__(64) void k_neigh(SolverDev<float> s, int64_t first,
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)(8 * ((n + 7) / 8)); }
+
+template <int O>
+void launch_local_o(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs, float* t_int,
+                    hipStream_t st) {
+  const dim3 grid(grid_of(n)), block(64);
+  if (s.sparse_star)
+    hipLaunchKernelGGL((k_local<O, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+  else
+    hipLaunchKernelGGL((k_local<O, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+}
 
 }  // namespace
 
@@ -336,12 +419,11 @@ template <>
 int launch_local_mfma<float>(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs,
                              float* t_int, hipStream_t st) {
   if (n <= 0) return EDGE_OK;
-  const dim3 grid(grid_of(n)), block(64);
   switch (s.order) {
-    case 2: hipLaunchKernelGGL(k_local<2>, grid, block, 0, st, s, first, n, dt, dofs, t_int); break;
-    case 3: hipLaunchKernelGGL(k_local<3>, grid, block, 0, st, s, first, n, dt, dofs, t_int); break;
-    case 4: hipLaunchKernelGGL(k_local<4>, grid, block, 0, st, s, first, n, dt, dofs, t_int); break;
-    case 5: hipLaunchKernelGGL(k_local<5>, grid, block, 0, st, s, first, n, dt, dofs, t_int); break;
+    case 2: launch_local_o<2>(s, first, n, dt, dofs, t_int, st); break;
+    case 3: launch_local_o<3>(s, first, n, dt, dofs, t_int, st); break;
+    case 4: launch_local_o<4>(s, first, n, dt, dofs, t_int, st); break;
+    case 5: launch_local_o<5>(s, first, n, dt, dofs, t_int, st); break;
     default: return fail(EDGE_ERR_UNSUPPORTED, "order");
   }
   EDGE_HIP_TRY(hipGetLastError());

@@ -25,6 +25,7 @@ struct edge_el_solver {
   int32_t* ne = nullptr;
   uint8_t* fmat = nullptr;
   double* jdet = nullptr;  // [nEl] Jacobian determinants (geometry-built solvers only)
+  bool sparse_star = false;  // all star matrices have the elastic 24-entry pattern
   double* mass = nullptr;  // [N] diagonal reference mass matrix
   // graph path
   hipGraph_t graph = nullptr;
@@ -53,6 +54,7 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.nb.fmat = s->fmat;
   d.star = static_cast<const T*>(s->star);
   d.fs = static_cast<const T*>(s->fs);
+  d.sparse_star = s->sparse_star;
   return d;
 }
 
@@ -220,6 +222,7 @@ extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mes
     edge::SetupInput in{mesh->n_el, mesh->n_fa, nve,         mesh->ve_crds, mesh->el_ve, mesh->el_fa,
                         mesh->fa_el, mesh->fa_type, mesh->normals, mesh->tangents, mesh->areas, rho,
                         lam,         mu};
+    s->sparse_star = true;  // built from the Jacobians' pattern only
     if (hipMalloc(&s->jdet, sizeof(double) * s->n_el) != hipSuccess)
       rc = edge::fail(EDGE_ERR_HIP, "allocation of Jacobian determinants");
     else
@@ -242,6 +245,13 @@ extern "C" int edge_el_create_raw(const edge_el_config* cfg, const edge_mesh_in*
   int rc = create_common(cfg, mesh, out, &s);
   if (rc == EDGE_OK) {
     const size_t ns = 243 * (size_t)s->n_el, nf = 648 * (size_t)s->n_el;
+    // the sparse-star kernels apply only when every entry off the pattern is zero
+    bool pat[81] = {};
+    for (int q = 0; q < 9; q++)
+      for (int j = 0; j < edge::kStarNnzRow[q]; j++) pat[q * 9 + edge::kStarCol[q][j]] = true;
+    bool sparse = true;
+    for (size_t i = 0; i < ns && sparse; i++) sparse = pat[i % 81] || star[i] == 0.0;
+    s->sparse_star = sparse;
     if (cfg->precision == 64) {
       if (hipMemcpy(s->star, star, 8 * ns, hipMemcpyHostToDevice) != hipSuccess ||
           hipMemcpy(s->fs, fs, 8 * nf, hipMemcpyHostToDevice) != hipSuccess)
