@@ -140,24 +140,37 @@ __device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, cons
   for (int q = 0; q < 9; q++)
 #pragma unroll
     for (int t = 0; t < TN; t++) der[q][t] = zero4();
+  // operands of step (d, s) are loaded one step ahead of their use
+  auto load = [&](int d, int s, float (&a)[TN], float (&b)[9]) {
+    const float* m = stiffTn + d * N * N;
+    const int k = 4 * s + g;
+#pragma unroll
+    for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+      if (4 * s + 3 < NK)  // constant after unrolling
+        b[p] = q0[(p * N + k) * R + c];
+      else
+        b[p] = ldg(q0, (p * N + k) * R + c, c, k < NK);
+    }
+  };
+  float an[TN], bn[9];
+  load(0, 0, an, bn);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const float* m = stiffTn + d * N * N;
     const float* sd = star + d * 81;
 #pragma unroll
     for (int s = 0; s < KS; s++) {
-      __builtin_amdgcn_sched_barrier(0);
-      const int k = 4 * s + g;
       float a[TN], b[9];
 #pragma unroll
-      for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+      for (int t = 0; t < TN; t++) a[t] = an[t];
 #pragma unroll
-      for (int p = 0; p < 9; p++) {
-        if (4 * s + 3 < NK)  // constant after unrolling
-          b[p] = q0[(p * N + k) * R + c];
-        else
-          b[p] = ldg(q0, (p * N + k) * R + c, c, k < NK);
-      }
+      for (int p = 0; p < 9; p++) b[p] = bn[p];
+      if (s + 1 < KS)
+        load(d, s + 1, an, bn);
+      else if (d + 1 < 3)
+        load(d + 1, 0, an, bn);
+      __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
 #pragma unroll
       for (int q = 0; q < 9; q++) {
         const float z = star_row<SP>(sd, q, b);
@@ -176,34 +189,48 @@ __device__ __forceinline__ void ck_next(const float* __restrict__ stiffTn, const
                                         v4f (&cur)[9][tiles(n_modes(O - DE))], int g, int c) {
   constexpr int N = Dims<O>::N;
   constexpr int NK = n_modes(O - DE + 1), NN = n_modes(O - DE), TP = tiles(NK), TN = tiles(NN);
+  // k-steps (d, u, v) with any row below NK, in order
+  constexpr int NS = 3 * ((TP - 1) * 4 + (NK - 16 * (TP - 1) < 4 ? NK - 16 * (TP - 1) : 4));
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
     for (int t = 0; t < TN; t++) cur[q][t] = zero4();
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
+  auto step_of = [](int i, int& d, int& u, int& v) {
+    constexpr int per_d = NS / 3;
+    d = i / per_d;
+    const int j = i % per_d;
+    u = j / 4;
+    v = j % 4;
+  };
+  auto load = [&](int i, float (&a)[TN]) {
+    int d, u, v;
+    step_of(i, d, u, v);
     const float* m = stiffTn + d * N * N;
+    const int k = 16 * u + 4 * g + v;
+#pragma unroll
+    for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+  };
+  float an[TN];
+  load(0, an);
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    int d, u, v;
+    step_of(i, d, u, v);
+    float a[TN];
+#pragma unroll
+    for (int t = 0; t < TN; t++) a[t] = an[t];
+    if (i + 1 < NS) load(i + 1, an);
+    __builtin_amdgcn_sched_barrier(0);
     const float* sd = star + d * 81;
+    float b[9];
 #pragma unroll
-    for (int u = 0; u < TP; u++)
+    for (int p = 0; p < 9; p++) b[p] = prev[p][u][v];
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        if (16 * u + v < NK) {  // constant after unrolling
-          __builtin_amdgcn_sched_barrier(0);
-          const int k = 16 * u + 4 * g + v;
-          float a[TN], b[9];
+    for (int q = 0; q < 9; q++) {
+      const float z = star_row<SP>(sd, q, b);
 #pragma unroll
-          for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
-#pragma unroll
-          for (int p = 0; p < 9; p++) b[p] = prev[p][u][v];
-#pragma unroll
-          for (int q = 0; q < 9; q++) {
-            const float z = star_row<SP>(sd, q, b);
-#pragma unroll
-            for (int t = 0; t < TN; t++) cur[q][t] = mfma(a[t], z, cur[q][t]);
-          }
-        }
-      }
+      for (int t = 0; t < TN; t++) cur[q][t] = mfma(a[t], z, cur[q][t]);
+    }
   }
 }
 
@@ -246,23 +273,40 @@ __device__ __forceinline__ void face_update(const float* __restrict__ fl, const 
                                             v4f (&acc)[9][Dims<O>::NT], int g, int c) {
   constexpr int N = Dims<O>::N, NF = Dims<O>::NF, NT = Dims<O>::NT;
   constexpr int KS = ksteps(N);
+  float at[4][NT];
+#pragma unroll
+  for (int v = 0; v < 4; v++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int fr = 4 * g + v;  // permuted k: face mode held in register v
+      at[v][t] = ldg(ft, fr * N + 16 * t + c, 0, fr < NF && 16 * t + c < N);
+    }
+  auto load = [&](int ks, float& al, float (&b)[9]) {
+    const int k = 4 * ks + g;
+    al = ldg(fl, k * NF + c, 0, k < N && c < NF);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      if (4 * ks + 3 < N)  // constant after unrolling
+        b[q] = x[(q * N + k) * R + c];
+      else
+        b[q] = ldg(x, (q * N + k) * R + c, c, k < N);
+    }
+  };
   v4f s0[9];
 #pragma unroll
   for (int q = 0; q < 9; q++) s0[q] = zero4();
+  float aln, bn[9];
+  load(0, aln, bn);
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
-    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
-    const int k = 4 * ks + g;
-    const float al = ldg(fl, k * NF + c, 0, k < N && c < NF);
+    const float al = aln;
+    float b[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) {
-      float b;
-      if (4 * ks + 3 < N)  // constant after unrolling
-        b = x[(q * N + k) * R + c];
-      else
-        b = ldg(x, (q * N + k) * R + c, c, k < N);
-      s0[q] = mfma(al, b, s0[q]);
-    }
+    for (int q = 0; q < 9; q++) b[q] = bn[q];
+    if (ks + 1 < KS) load(ks + 1, aln, bn);
+    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+#pragma unroll
+    for (int q = 0; q < 9; q++) s0[q] = mfma(al, b[q], s0[q]);
   }
   // per register column v (face modes 4 g + v): s1 = fs s0, then the lift
 #pragma unroll
@@ -276,13 +320,10 @@ __device__ __forceinline__ void face_update(const float* __restrict__ fl, const 
       for (int p = 0; p < 9; p++) z += f[q * 9 + p] * s0[p][v];
       s1[q] = z;
     }
-    const int fr = 4 * g + v;
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const float at = ldg(ft, fr * N + 16 * t + c, 0, fr < NF && 16 * t + c < N);
+    for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int q = 0; q < 9; q++) acc[q][t] = mfma(at, s1[q], acc[q][t]);
-    }
+      for (int q = 0; q < 9; q++) acc[q][t] = mfma(at[v][t], s1[q], acc[q][t]);
   }
 }
 
@@ -321,15 +362,10 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[q][t] = load_d(q0 + q * N * R, N, t, g, c);
   constexpr int KV = ksteps(NV);
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const float* m = s.dg.stiffV + d * N * N;
-    const float* sd = star + d * 81;
-#pragma unroll
-    for (int ks = 0; ks < KV; ks++) {
-      __builtin_amdgcn_sched_barrier(0);
+  {
+    auto load = [&](int d, int ks, float (&a)[NT], float (&b)[9]) {
+      const float* m = s.dg.stiffV + d * N * N;
       const int k = 4 * ks + g;
-      float a[NT], b[9];
 #pragma unroll
       for (int t = 0; t < NT; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NV && 16 * t + c < N);
 #pragma unroll
@@ -339,11 +375,30 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
         else
           b[p] = ldg(l_ti, (p * N + k) * R + c, c, k < NV);
       }
+    };
+    float an[NT], bn[9];
+    load(0, 0, an, bn);
 #pragma unroll
-      for (int q = 0; q < 9; q++) {
-        const float z = star_row<SP>(sd, q, b);
+    for (int d = 0; d < 3; d++) {
+      const float* sd = star + d * 81;
 #pragma unroll
-        for (int t = 0; t < NT; t++) acc[q][t] = mfma(a[t], z, acc[q][t]);
+      for (int ks = 0; ks < KV; ks++) {
+        float a[NT], b[9];
+#pragma unroll
+        for (int t = 0; t < NT; t++) a[t] = an[t];
+#pragma unroll
+        for (int p = 0; p < 9; p++) b[p] = bn[p];
+        if (ks + 1 < KV)
+          load(d, ks + 1, an, bn);
+        else if (d + 1 < 3)
+          load(d + 1, 0, an, bn);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+          const float z = star_row<SP>(sd, q, b);
+#pragma unroll
+          for (int t = 0; t < NT; t++) acc[q][t] = mfma(a[t], z, acc[q][t]);
+        }
       }
     }
   }
