@@ -109,23 +109,40 @@ __device__ __forceinline__ void store_d(float* __restrict__ x, int rows, int t, 
   }
 }
 
-// Row q of a star matrix times b: z = sum_p S[q][p] b[p] (dense, or the 24
-// elastic non-zeros).  q is a compile-time constant after unrolling.
+// Coefficients of one star matrix kept in (scalar) registers across a k-loop:
+// the 24 elastic non-zeros in row order, or all 81 entries.
 template <bool SP>
-__device__ __forceinline__ float star_row(const float* __restrict__ sd, int q, const float (&b)[9]) {
-  if constexpr (SP) {
-    constexpr int cols[9][3] = {{6, 7, 8}, {6, 7, 8}, {6, 7, 8}, {6, 7, 0}, {7, 8, 0},
-                                {6, 8, 0}, {0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
-    float z = sd[q * 9 + cols[q][0]] * b[cols[q][0]] + sd[q * 9 + cols[q][1]] * b[cols[q][1]];
-    if (q < 3 || q > 5) z += sd[q * 9 + cols[q][2]] * b[cols[q][2]];
-    return z;
-  } else {
-    float z = 0.f;
+struct StarCoef {
+  static constexpr int NC = SP ? 24 : 81;
+  float v[NC];
+  __device__ __forceinline__ void load(const float* __restrict__ sd) {
+    if constexpr (SP) {
+      constexpr int cols[24] = {6, 7, 8, 6, 7, 8, 6, 7, 8, 6, 7, 7, 8, 6, 8, 0, 3, 5, 1, 3, 4, 2, 4, 5};
+      constexpr int rows[24] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 6, 7, 7, 7, 8, 8, 8};
 #pragma unroll
-    for (int p = 0; p < 9; p++) z += sd[q * 9 + p] * b[p];
-    return z;
+      for (int i = 0; i < 24; i++) v[i] = sd[rows[i] * 9 + cols[i]];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 81; i++) v[i] = sd[i];
+    }
   }
-}
+  // z = sum_p S[q][p] b[p]; q is a compile-time constant after unrolling
+  __device__ __forceinline__ float row(int q, const float (&b)[9]) const {
+    if constexpr (SP) {
+      constexpr int cols[24] = {6, 7, 8, 6, 7, 8, 6, 7, 8, 6, 7, 7, 8, 6, 8, 0, 3, 5, 1, 3, 4, 2, 4, 5};
+      constexpr int first[10] = {0, 3, 6, 9, 11, 13, 15, 18, 21, 24};
+      float z = 0.f;
+#pragma unroll
+      for (int i = first[q]; i < first[q + 1]; i++) z += v[i] * b[cols[i]];
+      return z;
+    } else {
+      float z = 0.f;
+#pragma unroll
+      for (int p = 0; p < 9; p++) z += v[q * 9 + p] * b[p];
+      return z;
+    }
+  }
+};
 
 // ---------------------------------------------------------------- local
 // CK stage 1: der_1[q] = sum_d (-K_d^T M^-1)^T (sum_p S_d[q][p] Q_p), B operands
@@ -158,7 +175,8 @@ __device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, cons
   load(0, 0, an, bn);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const float* sd = star + d * 81;
+    StarCoef<SP> sc;
+    sc.load(star + d * 81);
 #pragma unroll
     for (int s = 0; s < KS; s++) {
       float a[TN], b[9];
@@ -173,7 +191,7 @@ __device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, cons
       __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        const float z = star_row<SP>(sd, q, b);
+        const float z = sc.row(q, b);
 #pragma unroll
         for (int t = 0; t < TN; t++) der[q][t] = mfma(a[t], z, der[q][t]);
       }
@@ -212,22 +230,23 @@ __device__ __forceinline__ void ck_next(const float* __restrict__ stiffTn, const
   };
   float an[TN];
   load(0, an);
+  StarCoef<SP> sc;
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     int d, u, v;
     step_of(i, d, u, v);
+    if (i % (NS / 3) == 0) sc.load(star + d * 81);  // constant after unrolling
     float a[TN];
 #pragma unroll
     for (int t = 0; t < TN; t++) a[t] = an[t];
     if (i + 1 < NS) load(i + 1, an);
     __builtin_amdgcn_sched_barrier(0);
-    const float* sd = star + d * 81;
     float b[9];
 #pragma unroll
     for (int p = 0; p < 9; p++) b[p] = prev[p][u][v];
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-      const float z = star_row<SP>(sd, q, b);
+      const float z = sc.row(q, b);
 #pragma unroll
       for (int t = 0; t < TN; t++) cur[q][t] = mfma(a[t], z, cur[q][t]);
     }
@@ -380,7 +399,8 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
     load(0, 0, an, bn);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      const float* sd = star + d * 81;
+      StarCoef<SP> sc;
+      sc.load(star + d * 81);
 #pragma unroll
       for (int ks = 0; ks < KV; ks++) {
         float a[NT], b[9];
@@ -395,7 +415,7 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-          const float z = star_row<SP>(sd, q, b);
+          const float z = sc.row(q, b);
 #pragma unroll
           for (int t = 0; t < NT; t++) acc[q][t] = mfma(a[t], z, acc[q][t]);
         }
