@@ -14,6 +14,7 @@ resident in HBM before the timed region.  Prints one JSON line.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -37,6 +38,50 @@ def alg_bytes(order, runs, s, s_c):
     dense star + two flux-solver sets, connectivity."""
     n = order * (order + 1) * (order + 2) // 6
     return s * 9 * n * runs * 4 + s_c * (3 * 81 + 2 * 4 * 81) + 80
+
+
+def local_bytes(order, runs, prec=4):
+    """Compulsory bytes per element of one local launch: DOFs read and
+    written, tInt written, dense star matrices and local flux solvers read."""
+    n = order * (order + 1) * (order + 2) // 6
+    return 3 * 9 * n * runs * prec + (3 * 81 + 4 * 81) * prec
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_k_local_o5_fp32_r16_196608tets.md")
+
+
+def read_traffic(path, n_el, order, runs):
+    """HBM bytes per launch of the local kernel from a committed rocprofv3
+    summary: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by
+    bytes / counter for a copy kernel of known size.  None when the summary
+    is absent or describes another workload."""
+    try:
+        kv = {}
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"^\s*([a-z_0-9]+)\s*=\s*([-+0-9.eE]+)\s*$", line)
+                if m:
+                    kv[m.group(1)] = float(m.group(2))
+        if (int(kv["elements"]), int(kv["order"]), int(kv["runs"])) != (n_el, order, runs):
+            return None
+        return (kv["local_fetch_kib"] * kv["fetch_scale"] + kv["local_write_kib"] * kv["write_scale"]) * 1024.0
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size (an elementwise negation: reads and
+    writes `mib` MiB once each), for scaling the FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5,
+                                           "ok": bool((dst == -1).all())}}))
 
 
 def build_problem(n_hex, order, seed=3343):
@@ -114,11 +159,16 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "the element update's multi-GPU sharding is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     import torch
     from edge_amd.solver import ElasticSolver
 
@@ -157,8 +207,11 @@ def main():
     roof = None
     if fl:
         achieved = fl["local"] * runs * m.n_el / res["fp32"]["local"]
+        traffic = read_traffic(TRAFFIC_PROFILE, m.n_el, order, runs)
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32_mfma"] / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK["fp32_mfma"], "traffic": None, "kernel": "local (CK + volume + local surface)",
+                "frac": achieved / PEAK["fp32_mfma"], "traffic": traffic,
+                "traffic_alg": local_bytes(order, runs) * m.n_el,
+                "kernel": "local (CK + volume + local surface)",
                 "neigh_tflops": fl["neigh"] * runs * m.n_el / res["fp32"]["neigh"] / 1e12}
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
