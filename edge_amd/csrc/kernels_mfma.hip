@@ -64,6 +64,13 @@ __device__ __forceinline__ int64_t xcd_element(int64_t n) {
 
 __device__ __forceinline__ v4f zero4() { return v4f{0.f, 0.f, 0.f, 0.f}; }
 
+// The value x, opaque to the optimiser: index arithmetic derived from it is
+// computed where it is used instead of being hoisted (and spilled) early.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // Guarded load without a branch: the index is clamped into the valid range
 // (so the load is always in bounds) and the value masked afterwards.
 __device__ __forceinline__ float ldg(const float* __restrict__ p, int idx, int clamp_idx, bool ok) {
@@ -109,6 +116,15 @@ __device__ __forceinline__ void store_d(float* __restrict__ x, int rows, int t, 
   }
 }
 
+// LDS index of row k, column c of quantity q in an element tile, or of a
+// dummy slot past the tile (kLdsPad floats) when k is not below `rows`:
+// branch-free guarded LDS accesses.
+constexpr int kLdsPad = 16;
+template <int O>
+__device__ __forceinline__ int lds_idx(int q, int k, int rows, int c) {
+  return k < rows ? (q * Dims<O>::N + k) * R + c : Dims<O>::SZ + c;
+}
+
 // Coefficients of one star matrix kept in (scalar) registers across a k-loop:
 // the 24 elastic non-zeros in row order, or all 81 entries.
 template <bool SP>
@@ -145,24 +161,31 @@ struct StarCoef {
 };
 
 // ---------------------------------------------------------------- local
-// CK stage 1: der_1[q] = sum_d (-K_d^T M^-1)^T (sum_p S_d[q][p] Q_p), B operands
-// read from the DOFs in global memory in row order.
+// CK stage 1 together with tInt's first term.  Each row of Q is read once
+// (row order, k = 4 s + g): dt Q goes to the tInt tile in LDS, and
+// der_1[q] = sum_d (-K_d^T M^-1)^T (sum_p S_d[q][p] Q_p) accumulates for all
+// three directions from the same B values.
 template <int O, bool SP>
 __device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, const float* __restrict__ star,
-                                         const float* __restrict__ q0, v4f (&der)[9][tiles(n_modes(O - 1))], int g,
-                                         int c) {
+                                         const float* __restrict__ q0, float dt, float* __restrict__ l_ti,
+                                         v4f (&der)[9][tiles(n_modes(O - 1))], int g, int c) {
   constexpr int N = Dims<O>::N;
   constexpr int NK = N, NN = n_modes(O - 1), TN = tiles(NN), KS = ksteps(NK);
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
     for (int t = 0; t < TN; t++) der[q][t] = zero4();
-  // operands of step (d, s) are loaded one step ahead of their use
-  auto load = [&](int d, int s, float (&a)[TN], float (&b)[9]) {
-    const float* m = stiffTn + d * N * N;
+  StarCoef<SP> sc[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) sc[d].load(star + d * 81);
+  // operands of k-step s are loaded one step ahead of their use
+  auto load = [&](int s, float (&a)[3][TN], float (&b)[9]) {
     const int k = 4 * s + g;
 #pragma unroll
-    for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int t = 0; t < TN; t++)
+        a[d][t] = ldg(stiffTn + d * N * N, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
 #pragma unroll
     for (int p = 0; p < 9; p++) {
       if (4 * s + 3 < NK)  // constant after unrolling
@@ -171,31 +194,30 @@ __device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, cons
         b[p] = ldg(q0, (p * N + k) * R + c, c, k < NK);
     }
   };
-  float an[TN], bn[9];
-  load(0, 0, an, bn);
+  float an[3][TN], bn[9];
+  load(0, an, bn);
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    StarCoef<SP> sc;
-    sc.load(star + d * 81);
+  for (int s = 0; s < KS; s++) {
+    float a[3][TN], b[9];
 #pragma unroll
-    for (int s = 0; s < KS; s++) {
-      float a[TN], b[9];
+    for (int d = 0; d < 3; d++)
 #pragma unroll
-      for (int t = 0; t < TN; t++) a[t] = an[t];
+      for (int t = 0; t < TN; t++) a[d][t] = an[d][t];
 #pragma unroll
-      for (int p = 0; p < 9; p++) b[p] = bn[p];
-      if (s + 1 < KS)
-        load(d, s + 1, an, bn);
-      else if (d + 1 < 3)
-        load(d + 1, 0, an, bn);
-      __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+    for (int p = 0; p < 9; p++) b[p] = bn[p];
+    if (s + 1 < KS) load(s + 1, an, bn);
+    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+    const int k = 4 * s + g;
+#pragma unroll
+    for (int p = 0; p < 9; p++) l_ti[lds_idx<O>(p, k, NK, c)] = dt * b[p];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        const float z = sc.row(q, b);
+        const float z = sc[d].row(q, b);
 #pragma unroll
-        for (int t = 0; t < TN; t++) der[q][t] = mfma(a[t], z, der[q][t]);
+        for (int t = 0; t < TN; t++) der[q][t] = mfma(a[d][t], z, der[q][t]);
       }
-    }
   }
 }
 
@@ -253,33 +275,34 @@ __device__ __forceinline__ void ck_next(const float* __restrict__ stiffTn, const
   }
 }
 
-// tInt (LDS) += sca * der, each lane updating the entries it holds
-template <int O, int TN>
-__device__ __forceinline__ void add_tint(float* __restrict__ l_ti, int rows, float sca, const v4f (&der)[9][TN],
-                                         int g, int c) {
-  constexpr int N = Dims<O>::N;
+// tInt (LDS) += x for tile t of a D-layout array (rows below `rows`)
+template <int O>
+__device__ __forceinline__ void add_tint_tile(float* __restrict__ l_ti, int rows, int t, const v4f (&x)[9], int g,
+                                              int c) {
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
-    for (int t = 0; t < TN; t++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int k = 16 * t + 4 * g + v;
-        if (k < rows) l_ti[(q * N + k) * R + c] += sca * der[q][t][v];
-      }
+    for (int v = 0; v < 4; v++) {
+      const int i = lds_idx<O>(q, 16 * t + 4 * g + v, rows, c);
+      l_ti[i] += x[q][v];
+    }
 }
 
+// CK stages DE >= 2.  Their derivatives have at most 16 rows, so their tInt
+// terms sum into t0 (rows 0..15, registers) that is added to LDS at the end.
 template <int O, int DE, bool SP>
 __device__ __forceinline__ void ck_rest(const float* stiffTn, const float* star,
-                                        const v4f (&prev)[9][tiles(n_modes(O - DE + 1))], float* l_ti, float dt,
+                                        const v4f (&prev)[9][tiles(n_modes(O - DE + 1))], v4f (&t0)[9], float dt,
                                         float sca, int g, int c) {
   if constexpr (DE < O) {
     constexpr int NN = n_modes(O - DE);
-    v4f cur[9][tiles(NN)];
+    static_assert(tiles(NN) == 1, "derivatives of stage 2 and up fit one tile");
+    v4f cur[9][1];
     ck_next<O, DE, SP>(stiffTn, star, prev, cur, g, c);
     sca *= dt / float(DE + 1);
-    add_tint<O, tiles(NN)>(l_ti, NN, sca, cur, g, c);
-    ck_rest<O, DE + 1, SP>(stiffTn, star, cur, l_ti, dt, sca, g, c);
+#pragma unroll
+    for (int q = 0; q < 9; q++) t0[q] += sca * cur[q][0];
+    ck_rest<O, DE + 1, SP>(stiffTn, star, cur, t0, dt, sca, g, c);
   }
 }
 
@@ -353,7 +376,7 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   using D = Dims<O>;
   constexpr int N = D::N, NF = D::NF, NT = D::NT;
   constexpr int NV = n_modes(O - 1);  // rows of tInt the volume integral reads
-  __shared__ float l_ti[D::SZ];       // time-integrated DOFs
+  __shared__ __align__(16) float l_ti[D::SZ + kLdsPad];  // time-integrated DOFs, then the output
   const int64_t off = xcd_element(n);
   if (off >= n) return;
   const int64_t el = first + off;
@@ -361,32 +384,51 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   const float* q0 = dofs + el * D::SZ;
   const float* star = s.star + el * 243;
 
-  // ---- CK: tInt = dt Q + sum_de dt^{de+1}/(de+1)! der_de, accumulated in LDS
-  for (int i = lane; i < D::SZ; i += 64) l_ti[i] = dt * q0[i];
+  // ---- CK: tInt = dt Q + sum_de dt^{de+1}/(de+1)! der_de, tile in LDS
   {
     v4f der[9][tiles(NV)];
-    ck_first<O, SP>(s.dg.stiffTn, star, q0, der, g, c);
+    ck_first<O, SP>(s.dg.stiffTn, star, q0, dt, l_ti, der, g, c);
     const float sca = dt * dt / 2.f;
-    add_tint<O, tiles(NV)>(l_ti, NV, sca, der, g, c);
-    ck_rest<O, 2, SP>(s.dg.stiffTn, star, der, l_ti, dt, sca, g, c);
+    v4f t0[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) t0[q] = sca * der[q][0];
+#pragma unroll
+    for (int t = 1; t < tiles(NV); t++) {
+      v4f x[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) x[q] = sca * der[q][t];
+      add_tint_tile<O>(l_ti, NV, t, x, g, c);
+    }
+    ck_rest<O, 2, SP>(s.dg.stiffTn, star, der, t0, dt, sca, g, c);
+    add_tint_tile<O>(l_ti, NV, 0, t0, g, c);
   }
   __syncthreads();
-  float* gti = t_int + el * D::SZ;
-  for (int i = lane; i < D::SZ; i += 64) gti[i] = l_ti[i];
 
-  // ---- volume: acc_q += sum_d stiffV_d^T (sum_p S_d[q][p] tInt_p)
+  // ---- volume accumulators start from Q; tInt goes to global memory meanwhile
   v4f acc[9][NT];
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[q][t] = load_d(q0 + q * N * R, N, t, g, c);
+  {
+    float4* gti = reinterpret_cast<float4*>(t_int + el * D::SZ);
+    const float4* lti = reinterpret_cast<const float4*>(l_ti);
+    for (int i = lane; i < D::NVEC; i += 64) gti[i] = lti[i];
+  }
+
+  // ---- volume: acc_q += sum_d stiffV_d^T (sum_p S_d[q][p] tInt_p), all d per B load
   constexpr int KV = ksteps(NV);
   {
-    auto load = [&](int d, int ks, float (&a)[NT], float (&b)[9]) {
-      const float* m = s.dg.stiffV + d * N * N;
+    StarCoef<SP> sc[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) sc[d].load(star + d * 81);
+    auto load = [&](int ks, float (&a)[3][NT], float (&b)[9]) {
       const int k = 4 * ks + g;
 #pragma unroll
-      for (int t = 0; t < NT; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NV && 16 * t + c < N);
+      for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          a[d][t] = ldg(s.dg.stiffV + d * N * N, k * N + 16 * t + c, 0, k < NV && 16 * t + c < N);
 #pragma unroll
       for (int p = 0; p < 9; p++) {
         if (4 * ks + 3 < NV)  // constant after unrolling
@@ -395,31 +437,27 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
           b[p] = ldg(l_ti, (p * N + k) * R + c, c, k < NV);
       }
     };
-    float an[NT], bn[9];
-    load(0, 0, an, bn);
+    float an[3][NT], bn[9];
+    load(0, an, bn);
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      StarCoef<SP> sc;
-      sc.load(star + d * 81);
+    for (int ks = 0; ks < KV; ks++) {
+      float a[3][NT], b[9];
 #pragma unroll
-      for (int ks = 0; ks < KV; ks++) {
-        float a[NT], b[9];
+      for (int d = 0; d < 3; d++)
 #pragma unroll
-        for (int t = 0; t < NT; t++) a[t] = an[t];
+        for (int t = 0; t < NT; t++) a[d][t] = an[d][t];
 #pragma unroll
-        for (int p = 0; p < 9; p++) b[p] = bn[p];
-        if (ks + 1 < KV)
-          load(d, ks + 1, an, bn);
-        else if (d + 1 < 3)
-          load(d + 1, 0, an, bn);
-        __builtin_amdgcn_sched_barrier(0);
+      for (int p = 0; p < 9; p++) b[p] = bn[p];
+      if (ks + 1 < KV) load(ks + 1, an, bn);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int d = 0; d < 3; d++)
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-          const float z = sc.row(q, b);
+          const float z = sc[d].row(q, b);
 #pragma unroll
-          for (int t = 0; t < NT; t++) acc[q][t] = mfma(a[t], z, acc[q][t]);
+          for (int t = 0; t < NT; t++) acc[q][t] = mfma(a[d][t], z, acc[q][t]);
         }
-      }
     }
   }
 
@@ -429,11 +467,23 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   for (int fa = 0; fa < 4; fa++)
     face_update<O>(s.dg.fluxLN + fa * N * NF, s.dg.fluxT + fa * NF * N, fs0 + fa * 81, l_ti, acc, g, c);
 
-  float* out = dofs + el * D::SZ;
+  // ---- DOFs out through LDS (16-byte global stores)
+  __syncthreads();  // the surface integral's reads of tInt are done
+  {
+    const int ln = opaque(lane), go = ln >> 4, co = ln & 15;
 #pragma unroll
-  for (int q = 0; q < 9; q++)
+    for (int q = 0; q < 9; q++)
 #pragma unroll
-    for (int t = 0; t < NT; t++) store_d(out + q * N * R, N, t, g, c, acc[q][t]);
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) l_ti[lds_idx<O>(q, 16 * t + 4 * go + v, N, co)] = acc[q][t][v];
+  }
+  __syncthreads();
+  {
+    float4* gq = reinterpret_cast<float4*>(dofs + el * D::SZ);
+    const float4* lq = reinterpret_cast<const float4*>(l_ti);
+    for (int i = lane; i < D::NVEC; i += 64) gq[i] = lq[i];
+  }
 }
 
 // ---------------------------------------------------------------- neigh
