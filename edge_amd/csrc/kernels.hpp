@@ -31,6 +31,16 @@ struct SolverDev {
   const T* star;  // [nEl][3][81]
   const T* fs;    // [2][nEl][4][81]
   bool sparse_star;  // every star matrix has the elastic 24-entry pattern
+  // rank-3 factors of the flux solvers [2][nEl][4][54] (U [9][3], then V^T
+  // [3][9]); null unless the MFMA-tiled kernels run
+  const T* fsf = nullptr;
+  bool fs_rank3 = false;  // every flux solver is rank 3 and fsf holds its factors
+  // face projections written by the local step for the neighbour step
+  // ([nEl][4][3][16][16]); null: the neighbour step reads tInt tiles
+  T* proj = nullptr;
+  const int32_t* snd_v = nullptr;  // [nEl][4]: receiving element-face (el * 4 + fa) of this face, -1: none
+  const uint8_t* snd_fm = nullptr; // [nEl][4]: flux matrix (index into fluxLN) of that projection
+  const int32_t* rcv = nullptr;    // [nEl][4]: projection slot (el * 4 + f) feeding this face, -1: none
 };
 
 // Non-zero pattern of the elastic star matrices: the union of the patterns

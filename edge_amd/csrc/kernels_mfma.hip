@@ -369,7 +369,102 @@ __device__ __forceinline__ void face_update(const float* __restrict__ fl, const 
   }
 }
 
-template <int O, bool SP>
+// Rank-3 face routines.  A flux solver factored as U V^T (U [9][3], V^T
+// [3][9], wave-uniform) turns the 9-quantity projection / solver / lift chain
+// into three combined columns: y_j = sum_p V^T[j][p] X_p on the B operands,
+// w_j = F^T y_j (3 products instead of 9), lifted tiles L_j = F^T_fa^T w_j
+// (3 instead of 9), and acc_q += sum_j U[q][j] L_j lane-locally.
+
+// w_j (D layout: face-mode rows 4 g + v, run c) = F^T (V^T X)_j for the tile
+// X in LDS ([q][N][R]) and F (fluxL or fluxN, [N][NF]).
+template <int O>
+__device__ __forceinline__ void project3(const float* __restrict__ fl, const float* __restrict__ vt,
+                                         const float* __restrict__ x, v4f (&w)[3], int g, int c) {
+  constexpr int N = Dims<O>::N, NF = Dims<O>::NF;
+  constexpr int KS = ksteps(N);
+  float cv[27];
+#pragma unroll
+  for (int i = 0; i < 27; i++) cv[i] = vt[i];
+  auto load = [&](int ks, float& al, float (&b)[9]) {
+    const int k = 4 * ks + g;
+    al = ldg(fl, k * NF + c, 0, k < N && c < NF);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      if (4 * ks + 3 < N)  // constant after unrolling
+        b[q] = x[(q * N + k) * R + c];
+      else
+        b[q] = ldg(x, (q * N + k) * R + c, c, k < N);
+    }
+  };
+#pragma unroll
+  for (int j = 0; j < 3; j++) w[j] = zero4();
+  float aln, bn[9];
+  load(0, aln, bn);
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    const float al = aln;
+    float b[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) b[q] = bn[q];
+    if (ks + 1 < KS) load(ks + 1, aln, bn);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      float y = 0.f;
+#pragma unroll
+      for (int p = 0; p < 9; p++) y += cv[j * 9 + p] * b[p];
+      w[j] = mfma(al, y, w[j]);
+    }
+  }
+}
+
+// acc_q += sum_j U[q][j] (F^T_fa^T w_j) with the lift F^T_fa ([NF][N]) on the
+// three projected columns (B operands straight from the w tiles, k = 4 g + v).
+template <int O>
+__device__ __forceinline__ void lift3(const float* __restrict__ ft, const float* __restrict__ u, const v4f (&w)[3],
+                                      v4f (&acc)[9][Dims<O>::NT], int g, int c) {
+  constexpr int N = Dims<O>::N, NF = Dims<O>::NF, NT = Dims<O>::NT;
+  v4f l[3][NT];
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) l[j][t] = zero4();
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int fr = 4 * g + v;
+    float at[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) at[t] = ldg(ft, fr * N + 16 * t + c, 0, fr < NF && 16 * t + c < N);
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) l[j][t] = mfma(at[t], w[j][v], l[j][t]);
+  }
+  float cu[27];
+#pragma unroll
+  for (int i = 0; i < 27; i++) cu[i] = u[i];
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[q][t] += cu[q * 3] * l[0][t] + cu[q * 3 + 1] * l[1][t] + cu[q * 3 + 2] * l[2][t];
+}
+
+// Face projections exchanged between the local and the neighbour step: per
+// element face 3 tiles of 16 x 16 (w_j, face modes x runs), lane-major
+// (lane l holds the 4 values of its D-layout registers at [j][l][0..3]).
+constexpr int kProjFace = 3 * 256;
+
+__device__ __forceinline__ void store_proj(float* __restrict__ p, const v4f (&w)[3], int lane) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) reinterpret_cast<v4f*>(p + j * 256)[lane] = w[j];
+}
+
+__device__ __forceinline__ void load_proj(const float* __restrict__ p, v4f (&w)[3], int lane) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) w[j] = reinterpret_cast<const v4f*>(p + j * 256)[lane];
+}
+
+template <int O, bool SP, bool R3>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __restrict__ dofs,
              float* __restrict__ t_int) {
@@ -462,10 +557,33 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   }
 
   // ---- local surface integral
-  const float* fs0 = s.fs + el * 324;
+  if constexpr (R3) {
 #pragma unroll 1
-  for (int fa = 0; fa < 4; fa++)
-    face_update<O>(s.dg.fluxLN + fa * N * NF, s.dg.fluxT + fa * NF * N, fs0 + fa * 81, l_ti, acc, g, c);
+    for (int fa = 0; fa < 4; fa++) {
+      const float* f = s.fsf + (el * 4 + fa) * 54;
+      v4f w[3];
+      project3<O>(s.dg.fluxLN + fa * N * NF, f + 27, l_ti, w, g, c);
+      lift3<O>(s.dg.fluxT + fa * NF * N, f, w, acc, g, c);
+    }
+    // ---- this element's tInt as each face-adjacent element sees it (the
+    // neighbour's flux matrix and solver side), for the neighbour step
+    if (s.proj) {
+#pragma unroll 1
+      for (int f = 0; f < 4; f++) {
+        const int32_t rv = s.snd_v[el * 4 + f];
+        if (rv < 0) continue;  // nobody receives through this face (uniform across the wave)
+        const int fm = s.snd_fm[el * 4 + f];
+        v4f w[3];
+        project3<O>(s.dg.fluxLN + fm * N * NF, s.fsf + (s.n_el * 4 + rv) * 54 + 27, l_ti, w, g, c);
+        store_proj(s.proj + (el * 4 + f) * kProjFace, w, lane);
+      }
+    }
+  } else {
+    const float* fs0 = s.fs + el * 324;
+#pragma unroll 1
+    for (int fa = 0; fa < 4; fa++)
+      face_update<O>(s.dg.fluxLN + fa * N * NF, s.dg.fluxT + fa * NF * N, fs0 + fa * 81, l_ti, acc, g, c);
+  }
 
   // ---- DOFs out through LDS (16-byte global stores)
   __syncthreads();  // the surface integral's reads of tInt are done
@@ -487,7 +605,7 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
 }
 
 // ---------------------------------------------------------------- neigh
-template <int O>
+template <int O, bool R3>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_neigh(SolverDev<float> s, int64_t first, int64_t n, const float* __restrict__ t_int,
              float* __restrict__ dofs) {
@@ -513,13 +631,62 @@ void k_neigh(SolverDev<float> s, int64_t first, int64_t n, const float* __restri
     wait_vm_all();
     __syncthreads();
     const int fm = s.nb.fmat[el * 4 + fa];
-    face_update<O>(s.dg.fluxLN + fm * N * NF, s.dg.fluxT + fa * NF * N, s.fs + ((s.n_el + el) * 4 + fa) * 81,
-                   l_nb, acc, g, c);
+    if constexpr (R3) {
+      const float* f = s.fsf + ((s.n_el + el) * 4 + fa) * 54;
+      v4f w[3];
+      project3<O>(s.dg.fluxLN + fm * N * NF, f + 27, l_nb, w, g, c);
+      lift3<O>(s.dg.fluxT + fa * NF * N, f, w, acc, g, c);
+    } else {
+      face_update<O>(s.dg.fluxLN + fm * N * NF, s.dg.fluxT + fa * NF * N, s.fs + ((s.n_el + el) * 4 + fa) * 81,
+                     l_nb, acc, g, c);
+    }
   }
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
     for (int t = 0; t < NT; t++) store_d(q0 + q * N * R, N, t, g, c, acc[q][t]);
+}
+
+// Neighbour step from the face projections the local step wrote: per face
+// only the lift of three tiles (the neighbour's tInt is not read).
+template <int O>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_neigh_proj(SolverDev<float> s, int64_t first, int64_t n, float* __restrict__ dofs) {
+  using D = Dims<O>;
+  constexpr int N = D::N, NT = D::NT, NF = D::NF;
+  __shared__ __align__(16) float l_out[D::SZ + kLdsPad];
+  const int64_t off = xcd_element(n);
+  if (off >= n) return;
+  const int64_t el = first + off;
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  float* q0 = dofs + el * D::SZ;
+  v4f acc[9][NT];
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[q][t] = load_d(q0 + q * N * R, N, t, g, c);
+  (void)NF;
+#pragma unroll 1
+  for (int fa = 0; fa < 4; fa++) {
+    const int32_t r = s.rcv[el * 4 + fa];
+    if (r < 0) continue;  // outflow (uniform across the wave)
+    v4f w[3];
+    load_proj(s.proj + (int64_t)r * kProjFace, w, lane);
+    lift3<O>(s.dg.fluxT + fa * D::NF * N, s.fsf + ((s.n_el + el) * 4 + fa) * 54, w, acc, g, c);
+  }
+  {
+    const int ln = opaque(lane), go = ln >> 4, co = ln & 15;
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) l_out[lds_idx<O>(q, 16 * t + 4 * go + v, N, co)] = acc[q][t][v];
+  }
+  __syncthreads();
+  float4* gq = reinterpret_cast<float4*>(q0);
+  const float4* lq = reinterpret_cast<const float4*>(l_out);
+  for (int i = lane; i < D::NVEC; i += 64) gq[i] = lq[i];
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)(8 * ((n + 7) / 8)); }
@@ -528,10 +695,26 @@ template <int O>
 void launch_local_o(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs, float* t_int,
                     hipStream_t st) {
   const dim3 grid(grid_of(n)), block(64);
-  if (s.sparse_star)
-    hipLaunchKernelGGL((k_local<O, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+  if (s.sparse_star && s.fs_rank3)
+    hipLaunchKernelGGL((k_local<O, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+  else if (s.sparse_star)
+    hipLaunchKernelGGL((k_local<O, true, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+  else if (s.fs_rank3)
+    hipLaunchKernelGGL((k_local<O, false, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
   else
-    hipLaunchKernelGGL((k_local<O, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local<O, false, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+}
+
+template <int O>
+void launch_neigh_o(const SolverDev<float>& s, int64_t first, int64_t n, const float* t_int, float* dofs,
+                    hipStream_t st) {
+  const dim3 grid(grid_of(n)), block(64);
+  if (s.proj)
+    hipLaunchKernelGGL(k_neigh_proj<O>, grid, block, 0, st, s, first, n, dofs);
+  else if (s.fs_rank3)
+    hipLaunchKernelGGL((k_neigh<O, true>), grid, block, 0, st, s, first, n, t_int, dofs);
+  else
+    hipLaunchKernelGGL((k_neigh<O, false>), grid, block, 0, st, s, first, n, t_int, dofs);
 }
 
 }  // namespace
@@ -559,12 +742,11 @@ template <>
 int launch_neigh_mfma<float>(const SolverDev<float>& s, int64_t first, int64_t n, const float* t_int, float* dofs,
                              hipStream_t st) {
   if (n <= 0) return EDGE_OK;
-  const dim3 grid(grid_of(n)), block(64);
   switch (s.order) {
-    case 2: hipLaunchKernelGGL(k_neigh<2>, grid, block, 0, st, s, first, n, t_int, dofs); break;
-    case 3: hipLaunchKernelGGL(k_neigh<3>, grid, block, 0, st, s, first, n, t_int, dofs); break;
-    case 4: hipLaunchKernelGGL(k_neigh<4>, grid, block, 0, st, s, first, n, t_int, dofs); break;
-    case 5: hipLaunchKernelGGL(k_neigh<5>, grid, block, 0, st, s, first, n, t_int, dofs); break;
+    case 2: launch_neigh_o<2>(s, first, n, t_int, dofs, st); break;
+    case 3: launch_neigh_o<3>(s, first, n, t_int, dofs, st); break;
+    case 4: launch_neigh_o<4>(s, first, n, t_int, dofs, st); break;
+    case 5: launch_neigh_o<5>(s, first, n, t_int, dofs, st); break;
     default: return fail(EDGE_ERR_UNSUPPORTED, "order");
   }
   EDGE_HIP_TRY(hipGetLastError());

@@ -152,11 +152,75 @@ __device__ void face_solvers(double rL, double rR, double lL, double lR, double 
   mm9(tmp, Ti, fsR);
 }
 
+// Rank-3 factors of a flux solver F (9 x 9): F = U V^T with U = [u_0 u_1 u_2]
+// orthonormal (Gram-Schmidt with column pivoting; each new column is
+// re-orthogonalised against the earlier ones) and V^T = U^T F.  Writes U
+// row-major [9][3] then V^T row-major [3][9].  Returns false when the
+// remainder F - U U^T F exceeds 1e-10 of F (Frobenius norms).  The elastic
+// Godunov solvers have rank 3: the flux A_L q* depends on either side's state
+// only through its three waves travelling towards the face.
+template <typename T>
+__device__ bool rank3_factors(const double* F, T* out) {
+  double W[81], U[27];
+  double fn2 = 0;
+  for (int i = 0; i < 81; i++) {
+    W[i] = F[i];
+    fn2 += F[i] * F[i];
+  }
+  for (int i = 0; i < 27; i++) U[i] = 0;
+  for (int j = 0; j < 3; j++) {
+    int best = -1;
+    double bn = 0;
+    for (int c = 0; c < 9; c++) {
+      double nr = 0;
+      for (int r = 0; r < 9; r++) nr += W[r * 9 + c] * W[r * 9 + c];
+      if (nr > bn) {
+        bn = nr;
+        best = c;
+      }
+    }
+    if (best < 0 || !(bn > 1e-28 * fn2)) break;  // rank below 3: the remaining columns stay zero
+    double u[9];
+    for (int r = 0; r < 9; r++) u[r] = W[r * 9 + best];
+    for (int i = 0; i < j; i++) {
+      double d = 0;
+      for (int r = 0; r < 9; r++) d += U[r * 3 + i] * u[r];
+      for (int r = 0; r < 9; r++) u[r] -= d * U[r * 3 + i];
+    }
+    double nu = 0;
+    for (int r = 0; r < 9; r++) nu += u[r] * u[r];
+    const double inv = 1.0 / sqrt(nu);
+    for (int r = 0; r < 9; r++) U[r * 3 + j] = u[r] * inv;
+    for (int c = 0; c < 9; c++) {
+      double d = 0;
+      for (int r = 0; r < 9; r++) d += U[r * 3 + j] * W[r * 9 + c];
+      for (int r = 0; r < 9; r++) W[r * 9 + c] -= d * U[r * 3 + j];
+    }
+  }
+  double rem = 0;
+  for (int i = 0; i < 81; i++) rem += W[i] * W[i];
+  for (int i = 0; i < 27; i++) out[i] = (T)U[i];
+  for (int j = 0; j < 3; j++)
+    for (int p = 0; p < 9; p++) {
+      double v = 0;
+      for (int q = 0; q < 9; q++) v += U[q * 3 + j] * F[q * 9 + p];
+      out[27 + j * 9 + p] = (T)v;
+    }
+  return rem <= 1e-20 * fn2;
+}
+
+template <typename T>
+__global__ void k_fs_factor(int64_t n_mats, const double* fs, T* fsf, int* bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mats) return;
+  if (!rank3_factors(fs + i * 81, fsf + i * 54)) atomicOr(bad, 8);
+}
+
 template <typename T>
 __global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el, const int64_t* el_fa,
                                const int32_t* fa_type, const double* normals, const double* tangents,
                                const double* areas, const double* jdet, const double* rho, const double* lam,
-                               const double* mu, T* fs, int* bad) {
+                               const double* mu, T* fs, T* fsf, int* bad) {
   const int64_t fa = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (fa >= n_fa) return;
   const int64_t eL = fa_el[fa * 2], eR = fa_el[fa * 2 + 1];
@@ -182,30 +246,31 @@ __global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el,
   const double* t = tangents + fa * 6 + 3;
   double a[81], b[81];
   face_solvers(rL, rR, lL, lR, mL, mR, n, s, t, fsurf, a, b);
-  double sc = -2.0 * areas[fa] / jdet[eL];
-  T* o0 = fs + (eL * 4 + fL) * 81;
-  T* o1 = fs + ((n_el + eL) * 4 + fL) * 81;
-  for (int i = 0; i < 81; i++) {
-    o0[i] = (T)(a[i] * sc);
-    o1[i] = (T)(b[i] * sc);
-  }
+  auto emit = [&](int64_t e, int f, double sc) {
+    for (int i = 0; i < 81; i++) {
+      a[i] *= sc;
+      b[i] *= sc;
+      fs[(e * 4 + f) * 81 + i] = (T)a[i];
+      fs[((n_el + e) * 4 + f) * 81 + i] = (T)b[i];
+    }
+    if (fsf) {
+      bool ok = rank3_factors(a, fsf + (e * 4 + f) * 54);
+      ok = rank3_factors(b, fsf + ((n_el + e) * 4 + f) * 54) && ok;
+      if (!ok) atomicOr(bad, 8);
+    }
+  };
+  emit(eL, fL, -2.0 * areas[fa] / jdet[eL]);
   if (eR >= 0) {
     const double nm[3] = {-n[0], -n[1], -n[2]};
     face_solvers(rR, rL, lR, lL, mR, mL, nm, s, t, false, a, b);
-    sc = -2.0 * areas[fa] / jdet[eR];
-    o0 = fs + (eR * 4 + fR) * 81;
-    o1 = fs + ((n_el + eR) * 4 + fR) * 81;
-    for (int i = 0; i < 81; i++) {
-      o0[i] = (T)(a[i] * sc);
-      o1[i] = (T)(b[i] * sc);
-    }
+    emit(eR, fR, -2.0 * areas[fa] / jdet[eR]);
   }
 }
 
 }  // namespace
 
 template <typename T>
-int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, hipStream_t st) {
+int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_fsf, bool* rank3, hipStream_t st) {
   double *d_crds = nullptr, *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr, *d_nrm = nullptr,
          *d_tan = nullptr, *d_area = nullptr;
   int64_t *d_elve = nullptr, *d_fael = nullptr, *d_elfa = nullptr;
@@ -257,7 +322,7 @@ int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, hipStr
                      d_lam, d_mu, d_star, d_jdet, d_bad);
   EDGE_SETUP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_flux_solvers<T>, dim3((unsigned)((nf + bs - 1) / bs)), dim3(bs), 0, st, nf, ne, d_fael,
-                     d_elfa, d_type, d_nrm, d_tan, d_area, d_jdet, d_rho, d_lam, d_mu, d_fs, d_bad);
+                     d_elfa, d_type, d_nrm, d_tan, d_area, d_jdet, d_rho, d_lam, d_mu, d_fs, d_fsf, d_bad);
   EDGE_SETUP_TRY(hipGetLastError());
   int bad = 0;
   EDGE_SETUP_TRY(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -265,12 +330,39 @@ int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, hipStr
   if (bad & 1) rc = fail(EDGE_ERR_MESH, "element with non-positive Jacobian determinant");
   else if (bad & 2) rc = fail(EDGE_ERR_MESH, "inconsistent face-element relations");
   else if (bad & 4) rc = fail(EDGE_ERR_MESH, "face with non-positive area");
+  if (rank3) *rank3 = d_fsf && !(bad & 8);
   cleanup();
 #undef EDGE_SETUP_TRY
   return rc;
 }
 
-template int build_setup<double>(const SetupInput&, double*, double*, double*, hipStream_t);
-template int build_setup<float>(const SetupInput&, float*, float*, double*, hipStream_t);
+template int build_setup<double>(const SetupInput&, double*, double*, double*, double*, bool*, hipStream_t);
+template int build_setup<float>(const SetupInput&, float*, float*, double*, float*, bool*, hipStream_t);
+
+template <typename T>
+int factor_flux_solvers(const double* h_fs, int64_t n_mats, T* d_fsf, bool* rank3) {
+  double* d_fs = nullptr;
+  int* d_bad = nullptr;
+  int bad = 0;
+  hipError_t e = hipMalloc(&d_fs, sizeof(double) * 81 * n_mats);
+  if (e == hipSuccess) e = hipMalloc(&d_bad, sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(d_fs, h_fs, sizeof(double) * 81 * n_mats, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(d_bad, 0, sizeof(int));
+  if (e == hipSuccess) {
+    const int bs = 128;
+    hipLaunchKernelGGL(k_fs_factor<T>, dim3((unsigned)((n_mats + bs - 1) / bs)), dim3(bs), 0, 0, n_mats, d_fs,
+                       d_fsf, d_bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost);
+  hipFree(d_fs);
+  hipFree(d_bad);
+  if (e != hipSuccess) return fail(EDGE_ERR_HIP, std::string("flux-solver factors: ") + hipGetErrorString(e));
+  *rank3 = !(bad & 8);
+  return EDGE_OK;
+}
+
+template int factor_flux_solvers<double>(const double*, int64_t, double*, bool*);
+template int factor_flux_solvers<float>(const double*, int64_t, float*, bool*);
 
 }  // namespace edge

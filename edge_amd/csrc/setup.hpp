@@ -23,8 +23,16 @@ struct SetupInput {  // host arrays
 };
 
 // Writes star [nEl][3][81], flux solvers [2][nEl][4][81] (device, type T)
-// and the Jacobian determinants d_jdet [nEl] (device, FP64).
+// and the Jacobian determinants d_jdet [nEl] (device, FP64).  With d_fsf
+// (may be null) also the rank-3 factors [2][nEl][4][54] of every flux solver
+// (U [9][3], then V^T [3][9], from FP64); *rank3 tells whether every solver
+// is rank 3 (then fs = U V^T to rounding).
 template <typename T>
-int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, hipStream_t st);
+int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_fsf, bool* rank3, hipStream_t st);
+
+// Rank-3 factors of n_mats host FP64 flux solvers [n_mats][81] into d_fsf
+// [n_mats][54] (device); *rank3 is false when any solver is not rank 3.
+template <typename T>
+int factor_flux_solvers(const double* h_fs, int64_t n_mats, T* d_fsf, bool* rank3);
 
 }  // namespace edge

@@ -27,6 +27,15 @@ struct edge_el_solver {
   double* jdet = nullptr;  // [nEl] Jacobian determinants (geometry-built solvers only)
   bool sparse_star = false;  // all star matrices have the elastic 24-entry pattern
   double* mass = nullptr;  // [N] diagonal reference mass matrix
+  // MFMA-tiled path: rank-3 flux-solver factors and face projections
+  void* fsf = nullptr;       // [2][nEl][4][54]
+  bool fs_rank3 = false;
+  bool want_proj = false;    // kernel selector 2: exchange face projections when possible
+  void* proj = nullptr;      // [nEl][4][768]
+  int32_t* snd_v = nullptr;  // [nEl][4]
+  uint8_t* snd_fm = nullptr; // [nEl][4]
+  int32_t* rcv = nullptr;    // [nEl][4]
+  bool proj_ok = false;      // every projection slot feeds at most one element face
   // graph path
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -55,6 +64,12 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.star = static_cast<const T*>(s->star);
   d.fs = static_cast<const T*>(s->fs);
   d.sparse_star = s->sparse_star;
+  d.fsf = static_cast<const T*>(s->fsf);
+  d.fs_rank3 = s->fs_rank3 && s->fsf;
+  d.proj = d.fs_rank3 ? static_cast<T*>(s->proj) : nullptr;
+  d.snd_v = s->snd_v;
+  d.snd_fm = s->snd_fm;
+  d.rcv = s->rcv;
   return d;
 }
 
@@ -80,7 +95,7 @@ int check_cfg(const edge_el_config* c) {
     return edge::fail(EDGE_ERR_UNSUPPORTED, "order must be in 2..5");
   if (c->n_runs != 1 && c->n_runs != 16) return edge::fail(EDGE_ERR_UNSUPPORTED, "n_runs must be 1 or 16");
   if (c->precision != 64 && c->precision != 32) return edge::fail(EDGE_ERR_UNSUPPORTED, "precision must be 32 or 64");
-  if (c->kernel < 0 || c->kernel > 2) return edge::fail(EDGE_ERR_ARG, "kernel selector");
+  if (c->kernel < 0 || c->kernel > 3) return edge::fail(EDGE_ERR_ARG, "kernel selector");
   return EDGE_OK;
 }
 
@@ -122,20 +137,57 @@ int build_neigh(edge_el_solver* s, const edge_mesh_in* m) {
   EDGE_HIP_TRY(hipMalloc(&s->fmat, 4 * ne));
   EDGE_HIP_TRY(hipMemcpy(s->ne, nb.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice));
   EDGE_HIP_TRY(hipMemcpy(s->fmat, fm.data(), 4 * ne, hipMemcpyHostToDevice));
+  if (s->kernel != 2) return EDGE_OK;
+  // Face projections: element el's face fa receives the projection that its
+  // neighbour n computes through the shared face (n's face fId, or fa itself
+  // on a free surface) with el's flux matrix and neighbour-side solver.
+  std::vector<int32_t> snd_v(4 * ne, -1), rcv(4 * ne, -1);
+  std::vector<uint8_t> snd_fm(4 * ne, 0);
+  s->proj_ok = true;
+  for (int64_t el = 0; el < ne && s->proj_ok; el++)
+    for (int fa = 0; fa < 4; fa++) {
+      const int32_t n = nb[el * 4 + fa];
+      if (n < 0) continue;
+      const int f = fm[el * 4 + fa] < 4 ? fa : (int)m->f_id[el * 4 + fa];
+      const int64_t slot = (int64_t)n * 4 + f;
+      if (snd_v[slot] >= 0) {
+        s->proj_ok = false;
+        break;
+      }
+      snd_v[slot] = (int32_t)(el * 4 + fa);
+      snd_fm[slot] = fm[el * 4 + fa];
+      rcv[el * 4 + fa] = (int32_t)slot;
+    }
+  if (!s->proj_ok) return EDGE_OK;
+  EDGE_HIP_TRY(hipMalloc(&s->snd_v, sizeof(int32_t) * 4 * ne));
+  EDGE_HIP_TRY(hipMalloc(&s->snd_fm, 4 * ne));
+  EDGE_HIP_TRY(hipMalloc(&s->rcv, sizeof(int32_t) * 4 * ne));
+  EDGE_HIP_TRY(hipMemcpy(s->snd_v, snd_v.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice));
+  EDGE_HIP_TRY(hipMemcpy(s->snd_fm, snd_fm.data(), 4 * ne, hipMemcpyHostToDevice));
+  EDGE_HIP_TRY(hipMemcpy(s->rcv, rcv.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice));
+  return EDGE_OK;
+}
+
+// after the flux solvers (and their factors) exist: projection buffer
+int finish_mfma(edge_el_solver* s) {
+  if (s->kernel == 2 && s->want_proj && s->fs_rank3 && s->proj_ok)
+    EDGE_HIP_TRY(hipMalloc(&s->proj, sizeof(float) * 4 * 768 * s->n_el));
   return EDGE_OK;
 }
 
 int pick_kernel(edge_el_solver* s) {
   const bool mfma_ok = edge::mfma_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision);
   const bool gen_ok = edge::generic_supported(s->cfg.order, s->cfg.n_runs, (int)s->real_bytes);
-  if (s->cfg.kernel == 2) {
+  if (s->cfg.kernel == 2 || s->cfg.kernel == 3) {
     if (!mfma_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no MFMA-tiled kernel for this configuration");
     s->kernel = 2;
+    s->want_proj = s->cfg.kernel == 2;
   } else if (s->cfg.kernel == 1) {
     if (!gen_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "generic kernel does not fit this configuration");
     s->kernel = 1;
   } else {
     s->kernel = mfma_ok ? 2 : 1;
+    s->want_proj = mfma_ok;
     if (!mfma_ok && !gen_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no kernel for this configuration");
   }
   return EDGE_OK;
@@ -163,6 +215,7 @@ int create_common(const edge_el_config* cfg, const edge_mesh_in* mesh, edge_el_s
   if ((rc = build_neigh(s, mesh))) return rc;
   EDGE_HIP_TRY(hipMalloc(&s->star, s->real_bytes * 243 * s->n_el));
   EDGE_HIP_TRY(hipMalloc(&s->fs, s->real_bytes * 2 * 324 * s->n_el));
+  if (s->kernel == 2) EDGE_HIP_TRY(hipMalloc(&s->fsf, s->real_bytes * 2 * 4 * 54 * s->n_el));
   EDGE_HIP_TRY(hipMalloc(&s->mass, sizeof(double) * t.n));
   EDGE_HIP_TRY(hipMemcpy(s->mass, t.mass, sizeof(double) * t.n, hipMemcpyHostToDevice));
   return EDGE_OK;
@@ -226,8 +279,12 @@ extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mes
     if (hipMalloc(&s->jdet, sizeof(double) * s->n_el) != hipSuccess)
       rc = edge::fail(EDGE_ERR_HIP, "allocation of Jacobian determinants");
     else
-      rc = cfg->precision == 64 ? edge::build_setup<double>(in, (double*)s->star, (double*)s->fs, s->jdet, nullptr)
-                                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, nullptr);
+      rc = cfg->precision == 64
+               ? edge::build_setup<double>(in, (double*)s->star, (double*)s->fs, s->jdet, (double*)s->fsf,
+                                           &s->fs_rank3, nullptr)
+               : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
+                                          &s->fs_rank3, nullptr);
+    if (rc == EDGE_OK) rc = finish_mfma(s);
   }
   if (rc != EDGE_OK) {
     if (s) edge_el_destroy(s);
@@ -262,6 +319,10 @@ extern "C" int edge_el_create_raw(const edge_el_config* cfg, const edge_mesh_in*
           hipMemcpy(s->fs, b.data(), 4 * nf, hipMemcpyHostToDevice) != hipSuccess)
         rc = edge::fail(EDGE_ERR_HIP, "upload of star matrices / flux solvers");
     }
+    if (rc == EDGE_OK && s->fsf)
+      rc = cfg->precision == 64 ? edge::factor_flux_solvers<double>(fs, 8 * s->n_el, (double*)s->fsf, &s->fs_rank3)
+                                : edge::factor_flux_solvers<float>(fs, 8 * s->n_el, (float*)s->fsf, &s->fs_rank3);
+    if (rc == EDGE_OK) rc = finish_mfma(s);
   }
   if (rc != EDGE_OK) {
     if (s) edge_el_destroy(s);
@@ -281,6 +342,11 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   hipFree(s->fmat);
   hipFree(s->jdet);
   hipFree(s->mass);
+  hipFree(s->fsf);
+  hipFree(s->proj);
+  hipFree(s->snd_v);
+  hipFree(s->snd_fm);
+  hipFree(s->rcv);
   delete s;
   return EDGE_OK;
 }
@@ -303,6 +369,16 @@ extern "C" int edge_el_get_setup(const edge_el_solver* s, double* star, double* 
       EDGE_HIP_TRY(hipMemcpy(a.data(), s->fs, 4 * nf, hipMemcpyDeviceToHost));
       for (size_t i = 0; i < nf; i++) fs[i] = a[i];
     }
+  }
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_kernel_info(const edge_el_solver* s, int32_t* kernel, int32_t* flags) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (kernel) *kernel = s->kernel;
+  if (flags) {
+    const bool r3 = s->kernel == 2 && s->fs_rank3 && s->fsf;
+    *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (r3 && s->proj ? 4 : 0);
   }
   return EDGE_OK;
 }

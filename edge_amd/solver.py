@@ -12,7 +12,8 @@ import numpy as np
 from . import _lib
 from .setups import n_modes
 
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA = 0, 1, 2
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES = 0, 1, 2, 3
+PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS = 1, 2, 4
 
 
 def _stream_ptr(stream):
@@ -51,6 +52,12 @@ class ElasticSolver:
         self._h = h
 
     # ------------------------------------------------------------------
+    def kernel_info(self):
+        """(kernel, flags): 1 generic / 2 MFMA-tiled; flags PATH_* bits."""
+        k, f = C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().edge_el_kernel_info(self._h, C.byref(k), C.byref(f)))
+        return k.value, f.value
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib.lib().edge_el_destroy(self._h)

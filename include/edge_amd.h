@@ -115,7 +115,8 @@ typedef struct {
   int32_t order;     /* O: 2..5 */
   int32_t n_runs;    /* fused runs R (innermost DOF axis): 1 or 16 */
   int32_t precision; /* 64 or 32 (arithmetic and storage type of DOFs) */
-  int32_t kernel;    /* 0 = pick automatically, 1 = generic, 2 = MFMA-tiled */
+  int32_t kernel;    /* 0 = pick automatically, 1 = generic, 2 = MFMA-tiled,
+                        3 = MFMA-tiled with the neighbour step reading tInt tiles */
 } edge_el_config;
 
 typedef struct { /* host arrays in the reference's entity layout */
@@ -162,7 +163,12 @@ int edge_el_local(edge_el_solver *s, int64_t first, int64_t n_els, double dt, vo
 
 /* Neighbour step for elements [first, first+n): surface integrals of the
  * face-adjacent elements' tInt (free surface: own tInt and local flux
- * matrix; outflow: skipped).  Mirrors AderDg::neigh (AderDg.hpp:553-722). */
+ * matrix; outflow: skipped).  Mirrors AderDg::neigh (AderDg.hpp:553-722).
+ * With the MFMA-tiled kernels (selector 0 or 2, FP32, 16 runs) the local step
+ * also writes, per element face, the projection of the element's tInt that
+ * its neighbour needs (the reference's neighFluxInt, AderDg.hpp:419-473), and
+ * this step reads those instead of t_int: t_int must then be the array the
+ * local steps of the neighbouring elements wrote, unchanged since. */
 int edge_el_neigh(edge_el_solver *s, int64_t first, int64_t n_els, const void *t_int, void *dofs,
                   void *stream);
 
@@ -175,6 +181,12 @@ int edge_el_step(edge_el_solver *s, int64_t n_steps, double dt, void *dofs, void
 /* Number of elements / modes / runs / bytes of one DOF array of a solver. */
 int edge_el_info(const edge_el_solver *s, int64_t *n_el, int32_t *n_modes, int32_t *n_runs,
                  int64_t *dof_bytes);
+
+/* Which kernels a solver runs: *kernel = 1 (generic) or 2 (MFMA-tiled);
+ * *flags bit 0: sparse (elastic-pattern) star matrices, bit 1: rank-3
+ * factored flux solvers, bit 2: the neighbour step reads face projections.
+ * Diagnostic with no counterpart in the reference (for tests and tools). */
+int edge_el_kernel_info(const edge_el_solver *s, int32_t *kernel, int32_t *flags);
 
 /* FP64 L2 norms per quantity and run, exact through the diagonal mass
  * matrix: for device DOF arrays a (element precision prec_a: 32 or 64) and

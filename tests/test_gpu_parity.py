@@ -10,7 +10,8 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import KERNEL_GENERIC, KERNEL_MFMA, ElasticSolver
+from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES, PATH_PROJECTIONS, PATH_RANK3_FS,
+                             PATH_SPARSE_STAR, ElasticSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -233,3 +234,58 @@ def test_mfma_random_matrices_match_oracle(gpu, order):
     got = gd.cpu().numpy()
     for r in range(16):
         assert rel_l2(got[..., r], d[..., r]) <= FP32_TOL
+
+
+@pytest.mark.parametrize("kernel", [KERNEL_MFMA, KERNEL_MFMA_TILES])
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+def test_mfma_physical_solvers_match_oracle(gpu, order, kernel):
+    """The MFMA-tiled FP32 fused-16 kernels with physical set-up: sparse star
+    matrices and rank-3 factored flux solvers; the neighbour step from the
+    face projections the local step wrote (selector 2) or from tInt tiles
+    (selector 3).  Free-surface, outflow and periodic faces; local and
+    neighbour passes separately, then steps, against the FP64 oracle."""
+    m, rho, lam, mu = _problem(n=3, bc=(FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC), seed=10 + order)
+    s = ElasticSolver(m, order, 16, 32, rho, lam, mu, kernel=kernel)
+    want = PATH_SPARSE_STAR | PATH_RANK3_FS | (PATH_PROJECTIONS if kernel == KERNEL_MFMA else 0)
+    assert s.kernel_info() == (2, want)
+    mo = orc.MeshOracle(m, order, 16, rho=rho, lam=lam, mu=mu)
+    dofs0 = _ic(m, order, 16, seed=order)
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    d, t = dofs0.copy(), np.zeros_like(dofs0)
+    mo.local(d, t, dt)
+    gd, gt = _to_dev(dofs0, 32), _to_dev(np.zeros_like(dofs0), 32)
+    s.local(gd, gt, dt)
+    assert rel_l2(gt.cpu().numpy(), t) <= FP32_TOL
+    assert rel_l2(gd.cpu().numpy(), d) <= FP32_TOL
+    mo.neigh(d, t)
+    s.neigh(gd, gt)
+    assert rel_l2(gd.cpu().numpy(), d) <= FP32_TOL
+    mo.step(d, t, dt, 5)
+    s.step(gd, gt, dt, 5)
+    got = gd.cpu().numpy()
+    for r in range(16):
+        assert rel_l2(got[..., r], d[..., r]) <= FP32_TOL
+
+
+def test_mfma_raw_physical_solvers_are_factored(gpu):
+    """Caller-given FP64 star matrices and flux solvers with physical
+    structure take the sparse-star / rank-3 / projection path (factored on
+    the device from FP64) and match the oracle; dense random ones do not."""
+    m, rho, lam, mu = _problem(n=3, bc=(FA_PERIODIC, FA_FREE_SURFACE, FA_OUTFLOW), seed=8)
+    star, fs, _ = orc.setup_mesh(m, rho, lam, mu)
+    order = 5
+    s = ElasticSolver(m, order, 16, 32, star=star, fs=fs)
+    assert s.kernel_info() == (2, PATH_SPARSE_STAR | PATH_RANK3_FS | PATH_PROJECTIONS)
+    mo = orc.MeshOracle(m, order, 16, star=star, fs=fs)
+    dofs0 = _ic(m, order, 16, seed=1)
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    d, t = dofs0.copy(), np.zeros_like(dofs0)
+    mo.step(d, t, dt, 5)
+    gd, gt = _to_dev(dofs0, 32), _to_dev(np.zeros_like(dofs0), 32)
+    s.step(gd, gt, dt, 5)
+    got = gd.cpu().numpy()
+    for r in range(16):
+        assert rel_l2(got[..., r], d[..., r]) <= FP32_TOL
+    rng = np.random.default_rng(2)
+    s2 = ElasticSolver(m, order, 16, 32, star=rng.normal(size=star.shape), fs=rng.normal(size=fs.shape))
+    assert s2.kernel_info() == (2, 0)
