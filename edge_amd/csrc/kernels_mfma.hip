@@ -275,6 +275,172 @@ __device__ __forceinline__ void ck_next(const float* __restrict__ stiffTn, const
   }
 }
 
+// Elastic star pattern: stress rows (0..5) have non-zeros only in the
+// velocity columns 6..8 and velocity rows (6..8) only in stress columns.  A
+// stress output sum_p S_d[q][p] X_p is then a combination of the three raw
+// velocity tiles, so its stiffness product is formed from the three products
+// K_d X_{6+i} (per direction) and combined afterwards, lane-locally: 6
+// instead of 9 products per direction and k-step.
+constexpr int kVelCols[3][3] = {{0, 3, 5}, {1, 3, 4}, {2, 4, 5}};  // rows 6, 7, 8
+constexpr int kStrNnz[6] = {3, 3, 3, 2, 2, 2};
+constexpr int kStrVel[6][3] = {{0, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 1, 0}, {1, 2, 0}, {0, 2, 0}};  // column 6 + i
+
+struct VelRows {  // S_d[6 + r][kVelCols[r][j]]
+  float v[9];
+  __device__ __forceinline__ void load(const float* __restrict__ sd) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) v[r * 3 + j] = sd[(6 + r) * 9 + kVelCols[r][j]];
+  }
+  __device__ __forceinline__ float row(int r, const float (&b)[9]) const {
+    return v[r * 3] * b[kVelCols[r][0]] + v[r * 3 + 1] * b[kVelCols[r][1]] + v[r * 3 + 2] * b[kVelCols[r][2]];
+  }
+};
+
+// out_q (+)= sum_i S_d[q][6 + i] P_i for the stress rows q = 0..5 (D tiles)
+template <int TN>
+__device__ __forceinline__ void add_stress(const float* __restrict__ sd, const v4f (&p)[3][TN], v4f (&out)[9][TN],
+                                           bool first) {
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+#pragma unroll
+    for (int t = 0; t < TN; t++) {
+      v4f z = first ? zero4() : out[q][t];
+#pragma unroll
+      for (int j = 0; j < kStrNnz[q]; j++) z += sd[q * 9 + 6 + kStrVel[q][j]] * p[kStrVel[q][j]][t];
+      out[q][t] = z;
+    }
+}
+
+// CK stage 1 for the elastic pattern (see ck_first): tInt's first term, the
+// velocity outputs from star-combined B values, the stress outputs from the
+// raw velocity products P[d][i] combined after the k-loop.
+template <int O>
+__device__ __forceinline__ void ck_first_el(const float* __restrict__ stiffTn, const float* __restrict__ star,
+                                            const float* __restrict__ q0, float dt, float* __restrict__ l_ti,
+                                            v4f (&der)[9][tiles(n_modes(O - 1))], int g, int c) {
+  constexpr int N = Dims<O>::N;
+  constexpr int NK = N, NN = n_modes(O - 1), TN = tiles(NN), KS = ksteps(NK);
+  v4f pv[3][3][TN];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int t = 0; t < TN; t++) pv[d][i][t] = zero4();
+#pragma unroll
+  for (int r = 6; r < 9; r++)
+#pragma unroll
+    for (int t = 0; t < TN; t++) der[r][t] = zero4();
+  VelRows sv[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) sv[d].load(star + d * 81);
+  auto load = [&](int s, float (&a)[3][TN], float (&b)[9]) {
+    const int k = 4 * s + g;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int t = 0; t < TN; t++)
+        a[d][t] = ldg(stiffTn + d * N * N, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+      if (4 * s + 3 < NK)  // constant after unrolling
+        b[p] = q0[(p * N + k) * R + c];
+      else
+        b[p] = ldg(q0, (p * N + k) * R + c, c, k < NK);
+    }
+  };
+  float an[3][TN], bn[9];
+  load(0, an, bn);
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    float a[3][TN], b[9];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int t = 0; t < TN; t++) a[d][t] = an[d][t];
+#pragma unroll
+    for (int p = 0; p < 9; p++) b[p] = bn[p];
+    if (s + 1 < KS) load(s + 1, an, bn);
+    __builtin_amdgcn_sched_barrier(0);
+    const int k = 4 * s + g;
+#pragma unroll
+    for (int p = 0; p < 9; p++) l_ti[lds_idx<O>(p, k, NK, c)] = dt * b[p];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int t = 0; t < TN; t++) pv[d][i][t] = mfma(a[d][t], b[6 + i], pv[d][i][t]);
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        const float z = sv[d].row(r, b);
+#pragma unroll
+        for (int t = 0; t < TN; t++) der[6 + r][t] = mfma(a[d][t], z, der[6 + r][t]);
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) add_stress<TN>(star + d * 81, pv[d], der, d == 0);
+}
+
+// CK stage DE >= 2 for the elastic pattern (see ck_next), direction-major k-steps.
+template <int O, int DE>
+__device__ __forceinline__ void ck_next_el(const float* __restrict__ stiffTn, const float* __restrict__ star,
+                                           const v4f (&prev)[9][tiles(n_modes(O - DE + 1))],
+                                           v4f (&cur)[9][tiles(n_modes(O - DE))], int g, int c) {
+  constexpr int N = Dims<O>::N;
+  constexpr int NK = n_modes(O - DE + 1), NN = n_modes(O - DE), TP = tiles(NK), TN = tiles(NN);
+  constexpr int PER_D = (TP - 1) * 4 + (NK - 16 * (TP - 1) < 4 ? NK - 16 * (TP - 1) : 4);
+#pragma unroll
+  for (int r = 6; r < 9; r++)
+#pragma unroll
+    for (int t = 0; t < TN; t++) cur[r][t] = zero4();
+  auto load = [&](int i, float (&a)[TN]) {
+    const int d = i / PER_D, j = i % PER_D, u = j / 4, v = j % 4;
+    const float* m = stiffTn + d * N * N;
+    const int k = 16 * u + 4 * g + v;
+#pragma unroll
+    for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+  };
+  float an[TN];
+  load(0, an);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    VelRows sv;
+    sv.load(star + d * 81);
+    v4f pv[3][TN];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int t = 0; t < TN; t++) pv[i][t] = zero4();
+#pragma unroll
+    for (int j = 0; j < PER_D; j++) {
+      const int i = d * PER_D + j, u = j / 4, v = j % 4;
+      float a[TN];
+#pragma unroll
+      for (int t = 0; t < TN; t++) a[t] = an[t];
+      if (i + 1 < 3 * PER_D) load(i + 1, an);
+      __builtin_amdgcn_sched_barrier(0);
+      float b[9];
+#pragma unroll
+      for (int p = 0; p < 9; p++) b[p] = prev[p][u][v];
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int t = 0; t < TN; t++) pv[q][t] = mfma(a[t], b[6 + q], pv[q][t]);
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        const float z = sv.row(r, b);
+#pragma unroll
+        for (int t = 0; t < TN; t++) cur[6 + r][t] = mfma(a[t], z, cur[6 + r][t]);
+      }
+    }
+    add_stress<TN>(star + d * 81, pv, cur, d == 0);
+  }
+}
+
 // tInt (LDS) += x for tile t of a D-layout array (rows below `rows`)
 template <int O>
 __device__ __forceinline__ void add_tint_tile(float* __restrict__ l_ti, int rows, int t, const v4f (&x)[9], int g,
@@ -298,7 +464,10 @@ __device__ __forceinline__ void ck_rest(const float* stiffTn, const float* star,
     constexpr int NN = n_modes(O - DE);
     static_assert(tiles(NN) == 1, "derivatives of stage 2 and up fit one tile");
     v4f cur[9][1];
-    ck_next<O, DE, SP>(stiffTn, star, prev, cur, g, c);
+    if constexpr (SP)
+      ck_next_el<O, DE>(stiffTn, star, prev, cur, g, c);
+    else
+      ck_next<O, DE, SP>(stiffTn, star, prev, cur, g, c);
     sca *= dt / float(DE + 1);
 #pragma unroll
     for (int q = 0; q < 9; q++) t0[q] += sca * cur[q][0];
@@ -482,7 +651,10 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   // ---- CK: tInt = dt Q + sum_de dt^{de+1}/(de+1)! der_de, tile in LDS
   {
     v4f der[9][tiles(NV)];
-    ck_first<O, SP>(s.dg.stiffTn, star, q0, dt, l_ti, der, g, c);
+    if constexpr (SP)
+      ck_first_el<O>(s.dg.stiffTn, star, q0, dt, l_ti, der, g, c);
+    else
+      ck_first<O, SP>(s.dg.stiffTn, star, q0, dt, l_ti, der, g, c);
     const float sca = dt * dt / 2.f;
     v4f t0[9];
 #pragma unroll
@@ -511,9 +683,60 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
     for (int i = lane; i < D::NVEC; i += 64) gti[i] = lti[i];
   }
 
-  // ---- volume: acc_q += sum_d stiffV_d^T (sum_p S_d[q][p] tInt_p), all d per B load
+  // ---- volume: acc_q += sum_d stiffV_d^T (sum_p S_d[q][p] tInt_p)
   constexpr int KV = ksteps(NV);
-  {
+  if constexpr (SP) {
+    // elastic pattern: per direction the velocity outputs from star-combined
+    // B values, the stress outputs from the three raw velocity products
+    auto load = [&](int d, int ks, float (&a)[NT], float (&b)[9]) {
+      const int k = 4 * ks + g;
+#pragma unroll
+      for (int t = 0; t < NT; t++) a[t] = ldg(s.dg.stiffV + d * N * N, k * N + 16 * t + c, 0, k < NV && 16 * t + c < N);
+#pragma unroll
+      for (int p = 0; p < 9; p++) {
+        if (4 * ks + 3 < NV)  // constant after unrolling
+          b[p] = l_ti[(p * N + k) * R + c];
+        else
+          b[p] = ldg(l_ti, (p * N + k) * R + c, c, k < NV);
+      }
+    };
+    float an[NT], bn[9];
+    load(0, 0, an, bn);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      VelRows sv;
+      sv.load(star + d * 81);
+      v4f pv[3][NT];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) pv[i][t] = zero4();
+#pragma unroll
+      for (int ks = 0; ks < KV; ks++) {
+        float a[NT], b[9];
+#pragma unroll
+        for (int t = 0; t < NT; t++) a[t] = an[t];
+#pragma unroll
+        for (int p = 0; p < 9; p++) b[p] = bn[p];
+        if (ks + 1 < KV)
+          load(d, ks + 1, an, bn);
+        else if (d + 1 < 3)
+          load(d + 1, 0, an, bn);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int t = 0; t < NT; t++) pv[i][t] = mfma(a[t], b[6 + i], pv[i][t]);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          const float z = sv.row(r, b);
+#pragma unroll
+          for (int t = 0; t < NT; t++) acc[6 + r][t] = mfma(a[t], z, acc[6 + r][t]);
+        }
+      }
+      add_stress<NT>(star + d * 81, pv, acc, false);
+    }
+  } else {
     StarCoef<SP> sc[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) sc[d].load(star + d * 81);
