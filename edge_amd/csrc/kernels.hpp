@@ -14,6 +14,7 @@ struct DgDev {
   const T* stiffV;   // [3][N][N]  K_d M^-1
   const T* fluxLN;   // [16][N][NF] local (0..3) then neighbouring (4 + v*4 + f)
   const T* fluxT;    // [4][NF][N]
+  const T* pk = nullptr;  // MFMA A operands packed in lane order (mfma_pack.hpp), or null
 };
 
 // Per element-face neighbour data, precomputed from the mesh.

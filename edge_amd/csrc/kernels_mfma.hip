@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels_mfma.hpp"
+#include "mfma_pack.hpp"
 
 namespace edge {
 namespace {
@@ -161,63 +162,63 @@ struct StarCoef {
 };
 
 // ---------------------------------------------------------------- local
-// CK stage 1 together with tInt's first term.  Each row of Q is read once
-// (row order, k = 4 s + g): dt Q goes to the tInt tile in LDS, and
-// der_1[q] = sum_d (-K_d^T M^-1)^T (sum_p S_d[q][p] Q_p) accumulates for all
-// three directions from the same B values.
+// CK stage 1 together with tInt's first term, any star matrices.  The LDS
+// tile holds Q on entry; B values are read in row order (k = 4 s + g) once
+// per direction, and the last direction replaces each row by dt Q;
+// der_1[q] = sum_d (-K_d^T M^-1)^T (sum_p S_d[q][p] Q_p).
 template <int O, bool SP>
 __device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, const float* __restrict__ star,
-                                         const float* __restrict__ q0, float dt, float* __restrict__ l_ti,
-                                         v4f (&der)[9][tiles(n_modes(O - 1))], int g, int c) {
+                                         float dt, float* l_ti, v4f (&der)[9][tiles(n_modes(O - 1))], int g,
+                                         int c) {
   constexpr int N = Dims<O>::N;
   constexpr int NK = N, NN = n_modes(O - 1), TN = tiles(NN), KS = ksteps(NK);
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
     for (int t = 0; t < TN; t++) der[q][t] = zero4();
-  StarCoef<SP> sc[3];
-#pragma unroll
-  for (int d = 0; d < 3; d++) sc[d].load(star + d * 81);
-  // operands of k-step s are loaded one step ahead of their use
-  auto load = [&](int s, float (&a)[3][TN], float (&b)[9]) {
+  // operands of step (d, s) are loaded one step ahead of their use
+  auto load = [&](int d, int s, float (&a)[TN], float (&b)[9]) {
     const int k = 4 * s + g;
 #pragma unroll
-    for (int d = 0; d < 3; d++)
-#pragma unroll
-      for (int t = 0; t < TN; t++)
-        a[d][t] = ldg(stiffTn + d * N * N, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+    for (int t = 0; t < TN; t++) a[t] = ldg(stiffTn + d * N * N, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
 #pragma unroll
     for (int p = 0; p < 9; p++) {
       if (4 * s + 3 < NK)  // constant after unrolling
-        b[p] = q0[(p * N + k) * R + c];
+        b[p] = l_ti[(p * N + k) * R + c];
       else
-        b[p] = ldg(q0, (p * N + k) * R + c, c, k < NK);
+        b[p] = ldg(l_ti, (p * N + k) * R + c, c, k < NK);
     }
   };
-  float an[3][TN], bn[9];
-  load(0, an, bn);
+  float an[TN], bn[9];
+  load(0, 0, an, bn);
 #pragma unroll
-  for (int s = 0; s < KS; s++) {
-    float a[3][TN], b[9];
+  for (int d = 0; d < 3; d++) {
+    StarCoef<SP> sc;
+    sc.load(star + d * 81);
 #pragma unroll
-    for (int d = 0; d < 3; d++)
+    for (int s = 0; s < KS; s++) {
+      float a[TN], b[9];
 #pragma unroll
-      for (int t = 0; t < TN; t++) a[d][t] = an[d][t];
+      for (int t = 0; t < TN; t++) a[t] = an[t];
 #pragma unroll
-    for (int p = 0; p < 9; p++) b[p] = bn[p];
-    if (s + 1 < KS) load(s + 1, an, bn);
-    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
-    const int k = 4 * s + g;
+      for (int p = 0; p < 9; p++) b[p] = bn[p];
+      if (s + 1 < KS)
+        load(d, s + 1, an, bn);
+      else if (d + 1 < 3)
+        load(d + 1, 0, an, bn);
+      __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+      if (d == 2) {
+        const int k = 4 * s + g;
 #pragma unroll
-    for (int p = 0; p < 9; p++) l_ti[lds_idx<O>(p, k, NK, c)] = dt * b[p];
-#pragma unroll
-    for (int d = 0; d < 3; d++)
+        for (int p = 0; p < 9; p++) l_ti[lds_idx<O>(p, k, NK, c)] = dt * b[p];
+      }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        const float z = sc[d].row(q, b);
+        const float z = sc.row(q, b);
 #pragma unroll
-        for (int t = 0; t < TN; t++) der[q][t] = mfma(a[d][t], z, der[q][t]);
+        for (int t = 0; t < TN; t++) der[q][t] = mfma(a[t], z, der[q][t]);
       }
+    }
   }
 }
 
@@ -315,13 +316,16 @@ __device__ __forceinline__ void add_stress(const float* __restrict__ sd, const v
 
 // CK stage 1 for the elastic pattern (see ck_first): tInt's first term, the
 // velocity outputs from star-combined B values, the stress outputs from the
-// raw velocity products P[d][i] combined after the k-loop.
+// raw velocity products P[d][i] combined after the k-loop.  A operands from
+// the packed table (four k-steps per load, one group ahead).
 template <int O>
-__device__ __forceinline__ void ck_first_el(const float* __restrict__ stiffTn, const float* __restrict__ star,
-                                            const float* __restrict__ q0, float dt, float* __restrict__ l_ti,
-                                            v4f (&der)[9][tiles(n_modes(O - 1))], int g, int c) {
+__device__ __forceinline__ void ck_first_el(const float* __restrict__ pk, const float* __restrict__ star, float dt,
+                                            float* l_ti, v4f (&der)[9][tiles(n_modes(O - 1))], int g, int c,
+                                            int lane) {
   constexpr int N = Dims<O>::N;
-  constexpr int NK = N, NN = n_modes(O - 1), TN = tiles(NN), KS = ksteps(NK);
+  constexpr int NK = N, NN = n_modes(O - 1), TN = tiles(NN), KS = ksteps(NK), NG = pk::groups(KS);
+  constexpr pk::Layout L{O};
+  const v4f* pa = reinterpret_cast<const v4f*>(pk + L.ck1());  // [d][t][group][lane]
   v4f pv[3][3][TN];
 #pragma unroll
   for (int d = 0; d < 3; d++)
@@ -336,34 +340,34 @@ __device__ __forceinline__ void ck_first_el(const float* __restrict__ stiffTn, c
   VelRows sv[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) sv[d].load(star + d * 81);
-  auto load = [&](int s, float (&a)[3][TN], float (&b)[9]) {
-    const int k = 4 * s + g;
+  auto load_a = [&](int gr, v4f (&a)[3][TN]) {
 #pragma unroll
     for (int d = 0; d < 3; d++)
 #pragma unroll
-      for (int t = 0; t < TN; t++)
-        a[d][t] = ldg(stiffTn + d * N * N, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
+      for (int t = 0; t < TN; t++) a[d][t] = pa[((d * TN + t) * NG + gr) * 64 + lane];
+  };
+  auto load_b = [&](int s, float (&b)[9]) {
+    const int k = 4 * s + g;
 #pragma unroll
     for (int p = 0; p < 9; p++) {
       if (4 * s + 3 < NK)  // constant after unrolling
-        b[p] = q0[(p * N + k) * R + c];
+        b[p] = l_ti[(p * N + k) * R + c];
       else
-        b[p] = ldg(q0, (p * N + k) * R + c, c, k < NK);
+        b[p] = ldg(l_ti, (p * N + k) * R + c, c, k < NK);
     }
   };
-  float an[3][TN], bn[9];
-  load(0, an, bn);
+  v4f a4[3][TN], a4n[3][TN];
+  float bn[9];
+  load_a(0, a4);
+  load_b(0, bn);
 #pragma unroll
   for (int s = 0; s < KS; s++) {
-    float a[3][TN], b[9];
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-#pragma unroll
-      for (int t = 0; t < TN; t++) a[d][t] = an[d][t];
+    if (s % 4 == 0 && s / 4 + 1 < NG) load_a(s / 4 + 1, a4n);
+    float b[9];
 #pragma unroll
     for (int p = 0; p < 9; p++) b[p] = bn[p];
-    if (s + 1 < KS) load(s + 1, an, bn);
-    __builtin_amdgcn_sched_barrier(0);
+    if (s + 1 < KS) load_b(s + 1, bn);
+    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
     const int k = 4 * s + g;
 #pragma unroll
     for (int p = 0; p < 9; p++) l_ti[lds_idx<O>(p, k, NK, c)] = dt * b[p];
@@ -372,72 +376,67 @@ __device__ __forceinline__ void ck_first_el(const float* __restrict__ stiffTn, c
 #pragma unroll
       for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int t = 0; t < TN; t++) pv[d][i][t] = mfma(a[d][t], b[6 + i], pv[d][i][t]);
+        for (int t = 0; t < TN; t++) pv[d][i][t] = mfma(a4[d][t][s % 4], b[6 + i], pv[d][i][t]);
 #pragma unroll
       for (int r = 0; r < 3; r++) {
         const float z = sv[d].row(r, b);
 #pragma unroll
-        for (int t = 0; t < TN; t++) der[6 + r][t] = mfma(a[d][t], z, der[6 + r][t]);
+        for (int t = 0; t < TN; t++) der[6 + r][t] = mfma(a4[d][t][s % 4], z, der[6 + r][t]);
       }
+    }
+    if (s % 4 == 3) {
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int t = 0; t < TN; t++) a4[d][t] = a4n[d][t];
     }
   }
 #pragma unroll
   for (int d = 0; d < 3; d++) add_stress<TN>(star + d * 81, pv[d], der, d == 0);
 }
 
-// CK stage DE >= 2 for the elastic pattern (see ck_next), direction-major k-steps.
+// CK stage DE >= 2 for the elastic pattern (see ck_next), direction-major
+// k-steps (u, v); A operands from the packed table, one 4-step group ahead.
 template <int O, int DE>
-__device__ __forceinline__ void ck_next_el(const float* __restrict__ stiffTn, const float* __restrict__ star,
+__device__ __forceinline__ void ck_next_el(const float* __restrict__ pk, const float* __restrict__ star,
                                            const v4f (&prev)[9][tiles(n_modes(O - DE + 1))],
-                                           v4f (&cur)[9][tiles(n_modes(O - DE))], int g, int c) {
-  constexpr int N = Dims<O>::N;
-  constexpr int NK = n_modes(O - DE + 1), NN = n_modes(O - DE), TP = tiles(NK), TN = tiles(NN);
-  constexpr int PER_D = (TP - 1) * 4 + (NK - 16 * (TP - 1) < 4 ? NK - 16 * (TP - 1) : 4);
+                                           v4f (&cur)[9][tiles(n_modes(O - DE))], int lane) {
+  constexpr int NN = n_modes(O - DE), TN = tiles(NN);
+  static_assert(TN == 1, "derivatives of stage 2 and up fit one tile");
+  constexpr int PER_D = pk::next_steps(O, DE), NG = pk::groups(PER_D);
+  constexpr pk::Layout L{O};
+  const v4f* pa = reinterpret_cast<const v4f*>(pk + L.ck(DE));  // [d][group][lane]
 #pragma unroll
-  for (int r = 6; r < 9; r++)
-#pragma unroll
-    for (int t = 0; t < TN; t++) cur[r][t] = zero4();
-  auto load = [&](int i, float (&a)[TN]) {
-    const int d = i / PER_D, j = i % PER_D, u = j / 4, v = j % 4;
-    const float* m = stiffTn + d * N * N;
-    const int k = 16 * u + 4 * g + v;
-#pragma unroll
-    for (int t = 0; t < TN; t++) a[t] = ldg(m, k * N + 16 * t + c, 0, k < NK && 16 * t + c < NN);
-  };
-  float an[TN];
-  load(0, an);
+  for (int r = 6; r < 9; r++) cur[r][0] = zero4();
+  v4f a4 = pa[lane], a4n = a4;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     VelRows sv;
     sv.load(star + d * 81);
-    v4f pv[3][TN];
+    v4f pv[3][1];
 #pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int t = 0; t < TN; t++) pv[i][t] = zero4();
+    for (int i = 0; i < 3; i++) pv[i][0] = zero4();
 #pragma unroll
     for (int j = 0; j < PER_D; j++) {
-      const int i = d * PER_D + j, u = j / 4, v = j % 4;
-      float a[TN];
-#pragma unroll
-      for (int t = 0; t < TN; t++) a[t] = an[t];
-      if (i + 1 < 3 * PER_D) load(i + 1, an);
+      const int u = j / 4, v = j % 4;
+      if (v == 0) {  // next group: (d, u + 1) or (d + 1, 0)
+        if (u + 1 < NG)
+          a4n = pa[(d * NG + u + 1) * 64 + lane];
+        else if (d + 1 < 3)
+          a4n = pa[((d + 1) * NG) * 64 + lane];
+      }
       __builtin_amdgcn_sched_barrier(0);
+      const float a = a4[v];
       float b[9];
 #pragma unroll
       for (int p = 0; p < 9; p++) b[p] = prev[p][u][v];
 #pragma unroll
-      for (int q = 0; q < 3; q++)
+      for (int q = 0; q < 3; q++) pv[q][0] = mfma(a, b[6 + q], pv[q][0]);
 #pragma unroll
-        for (int t = 0; t < TN; t++) pv[q][t] = mfma(a[t], b[6 + q], pv[q][t]);
-#pragma unroll
-      for (int r = 0; r < 3; r++) {
-        const float z = sv.row(r, b);
-#pragma unroll
-        for (int t = 0; t < TN; t++) cur[6 + r][t] = mfma(a[t], z, cur[6 + r][t]);
-      }
+      for (int r = 0; r < 3; r++) cur[6 + r][0] = mfma(a, sv.row(r, b), cur[6 + r][0]);
+      if (v == 3 || j == PER_D - 1) a4 = a4n;
     }
-    add_stress<TN>(star + d * 81, pv, cur, d == 0);
+    add_stress<1>(star + d * 81, pv, cur, d == 0);
   }
 }
 
@@ -457,21 +456,21 @@ __device__ __forceinline__ void add_tint_tile(float* __restrict__ l_ti, int rows
 // CK stages DE >= 2.  Their derivatives have at most 16 rows, so their tInt
 // terms sum into t0 (rows 0..15, registers) that is added to LDS at the end.
 template <int O, int DE, bool SP>
-__device__ __forceinline__ void ck_rest(const float* stiffTn, const float* star,
+__device__ __forceinline__ void ck_rest(const DgDev<float>& dg, const float* star,
                                         const v4f (&prev)[9][tiles(n_modes(O - DE + 1))], v4f (&t0)[9], float dt,
-                                        float sca, int g, int c) {
+                                        float sca, int g, int c, int lane) {
   if constexpr (DE < O) {
     constexpr int NN = n_modes(O - DE);
     static_assert(tiles(NN) == 1, "derivatives of stage 2 and up fit one tile");
     v4f cur[9][1];
     if constexpr (SP)
-      ck_next_el<O, DE>(stiffTn, star, prev, cur, g, c);
+      ck_next_el<O, DE>(dg.pk, star, prev, cur, lane);
     else
-      ck_next<O, DE, SP>(stiffTn, star, prev, cur, g, c);
+      ck_next<O, DE, SP>(dg.stiffTn, star, prev, cur, g, c);
     sca *= dt / float(DE + 1);
 #pragma unroll
     for (int q = 0; q < 9; q++) t0[q] += sca * cur[q][0];
-    ck_rest<O, DE + 1, SP>(stiffTn, star, cur, t0, dt, sca, g, c);
+    ck_rest<O, DE + 1, SP>(dg, star, cur, t0, dt, sca, g, c, lane);
   }
 }
 
@@ -545,18 +544,18 @@ __device__ __forceinline__ void face_update(const float* __restrict__ fl, const 
 // (3 instead of 9), and acc_q += sum_j U[q][j] L_j lane-locally.
 
 // w_j (D layout: face-mode rows 4 g + v, run c) = F^T (V^T X)_j for the tile
-// X in LDS ([q][N][R]) and F (fluxL or fluxN, [N][NF]).
+// X in LDS ([q][N][R]) and F (fluxL or fluxN, [N][NF]) as packed A operands
+// pm ([group][lane], four k-steps per vector).
 template <int O>
-__device__ __forceinline__ void project3(const float* __restrict__ fl, const float* __restrict__ vt,
-                                         const float* __restrict__ x, v4f (&w)[3], int g, int c) {
-  constexpr int N = Dims<O>::N, NF = Dims<O>::NF;
-  constexpr int KS = ksteps(N);
+__device__ __forceinline__ void project3(const v4f* __restrict__ pm, const float* __restrict__ vt,
+                                         const float* __restrict__ x, v4f (&w)[3], int g, int c, int lane) {
+  constexpr int N = Dims<O>::N;
+  constexpr int KS = ksteps(N), NG = pk::groups(KS);
   float cv[27];
 #pragma unroll
   for (int i = 0; i < 27; i++) cv[i] = vt[i];
-  auto load = [&](int ks, float& al, float (&b)[9]) {
+  auto load_b = [&](int ks, float (&b)[9]) {
     const int k = 4 * ks + g;
-    al = ldg(fl, k * NF + c, 0, k < N && c < NF);
 #pragma unroll
     for (int q = 0; q < 9; q++) {
       if (4 * ks + 3 < N)  // constant after unrolling
@@ -567,16 +566,18 @@ __device__ __forceinline__ void project3(const float* __restrict__ fl, const flo
   };
 #pragma unroll
   for (int j = 0; j < 3; j++) w[j] = zero4();
-  float aln, bn[9];
-  load(0, aln, bn);
+  v4f a4 = pm[lane], a4n = a4;
+  float bn[9];
+  load_b(0, bn);
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
-    const float al = aln;
+    if (ks % 4 == 0 && ks / 4 + 1 < NG) a4n = pm[(ks / 4 + 1) * 64 + lane];
     float b[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) b[q] = bn[q];
-    if (ks + 1 < KS) load(ks + 1, aln, bn);
+    if (ks + 1 < KS) load_b(ks + 1, bn);
     __builtin_amdgcn_sched_barrier(0);
+    const float al = a4[ks % 4];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       float y = 0.f;
@@ -584,31 +585,31 @@ __device__ __forceinline__ void project3(const float* __restrict__ fl, const flo
       for (int p = 0; p < 9; p++) y += cv[j * 9 + p] * b[p];
       w[j] = mfma(al, y, w[j]);
     }
+    if (ks % 4 == 3) a4 = a4n;
   }
 }
 
-// acc_q += sum_j U[q][j] (F^T_fa^T w_j) with the lift F^T_fa ([NF][N]) on the
-// three projected columns (B operands straight from the w tiles, k = 4 g + v).
+// acc_q += sum_j U[q][j] (F^T_fa^T w_j) with the lift F^T_fa ([NF][N]) as
+// packed A operands pl ([t][lane], the four register columns v per vector) on
+// the three projected columns (B operands straight from the w tiles, k = 4 g + v).
 template <int O>
-__device__ __forceinline__ void lift3(const float* __restrict__ ft, const float* __restrict__ u, const v4f (&w)[3],
-                                      v4f (&acc)[9][Dims<O>::NT], int g, int c) {
-  constexpr int N = Dims<O>::N, NF = Dims<O>::NF, NT = Dims<O>::NT;
+__device__ __forceinline__ void lift3(const v4f* __restrict__ pl, const float* __restrict__ u, const v4f (&w)[3],
+                                      v4f (&acc)[9][Dims<O>::NT], int lane) {
+  constexpr int NT = Dims<O>::NT;
+  v4f at[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) at[t] = pl[t * 64 + lane];
   v4f l[3][NT];
 #pragma unroll
   for (int j = 0; j < 3; j++)
 #pragma unroll
     for (int t = 0; t < NT; t++) l[j][t] = zero4();
 #pragma unroll
-  for (int v = 0; v < 4; v++) {
-    const int fr = 4 * g + v;
-    float at[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) at[t] = ldg(ft, fr * N + 16 * t + c, 0, fr < NF && 16 * t + c < N);
+  for (int v = 0; v < 4; v++)
 #pragma unroll
     for (int j = 0; j < 3; j++)
 #pragma unroll
-      for (int t = 0; t < NT; t++) l[j][t] = mfma(at[t], w[j][v], l[j][t]);
-  }
+      for (int t = 0; t < NT; t++) l[j][t] = mfma(at[t][v], w[j][v], l[j][t]);
   float cu[27];
 #pragma unroll
   for (int i = 0; i < 27; i++) cu[i] = u[i];
@@ -640,21 +641,24 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   using D = Dims<O>;
   constexpr int N = D::N, NF = D::NF, NT = D::NT;
   constexpr int NV = n_modes(O - 1);  // rows of tInt the volume integral reads
-  __shared__ __align__(16) float l_ti[D::SZ + kLdsPad];  // time-integrated DOFs, then the output
+  // Q, then the time-integrated DOFs, then the output
+  __shared__ __align__(16) float l_ti[D::SZ + kLdsPad > D::CHUNKS * 256 ? D::SZ + kLdsPad : D::CHUNKS * 256];
   const int64_t off = xcd_element(n);
   if (off >= n) return;
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const float* q0 = dofs + el * D::SZ;
   const float* star = s.star + el * 243;
+  tile_to_lds<D::CHUNKS>(q0, l_ti, D::NVEC, lane);
 
   // ---- CK: tInt = dt Q + sum_de dt^{de+1}/(de+1)! der_de, tile in LDS
   {
     v4f der[9][tiles(NV)];
+    wait_vm_all();
     if constexpr (SP)
-      ck_first_el<O>(s.dg.stiffTn, star, q0, dt, l_ti, der, g, c);
+      ck_first_el<O>(s.dg.pk, star, dt, l_ti, der, g, c, lane);
     else
-      ck_first<O, SP>(s.dg.stiffTn, star, q0, dt, l_ti, der, g, c);
+      ck_first<O, SP>(s.dg.stiffTn, star, dt, l_ti, der, g, c);
     const float sca = dt * dt / 2.f;
     v4f t0[9];
 #pragma unroll
@@ -666,7 +670,7 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
       for (int q = 0; q < 9; q++) x[q] = sca * der[q][t];
       add_tint_tile<O>(l_ti, NV, t, x, g, c);
     }
-    ck_rest<O, 2, SP>(s.dg.stiffTn, star, der, t0, dt, sca, g, c);
+    ck_rest<O, 2, SP>(s.dg, star, der, t0, dt, sca, g, c, lane);
     add_tint_tile<O>(l_ti, NV, 0, t0, g, c);
   }
   __syncthreads();
@@ -687,7 +691,72 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   constexpr int KV = ksteps(NV);
   if constexpr (SP) {
     // elastic pattern: per direction the velocity outputs from star-combined
-    // B values, the stress outputs from the three raw velocity products
+    // B values, the stress outputs from the three raw velocity products;
+    // A operands from the packed table, one 4-step group ahead
+    constexpr int NG = pk::groups(KV);
+    constexpr pk::Layout L{O};
+    const v4f* pa = reinterpret_cast<const v4f*>(s.dg.pk + L.vol());  // [d][t][group][lane]
+    auto load_a = [&](int d, int gr, v4f (&a)[NT]) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) a[t] = pa[((d * NT + t) * NG + gr) * 64 + lane];
+    };
+    auto load_b = [&](int ks, float (&b)[9]) {
+      const int k = 4 * ks + g;
+#pragma unroll
+      for (int p = 0; p < 9; p++) {
+        if (4 * ks + 3 < NV)  // constant after unrolling
+          b[p] = l_ti[(p * N + k) * R + c];
+        else
+          b[p] = ldg(l_ti, (p * N + k) * R + c, c, k < NV);
+      }
+    };
+    v4f a4[NT], a4n[NT];
+    float bn[9];
+    load_a(0, 0, a4);
+    load_b(0, bn);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      VelRows sv;
+      sv.load(star + d * 81);
+      v4f pv[3][NT];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) pv[i][t] = zero4();
+#pragma unroll
+      for (int ks = 0; ks < KV; ks++) {
+        if (ks % 4 == 0) {  // next group: (d, ks / 4 + 1) or (d + 1, 0)
+          if (ks / 4 + 1 < NG)
+            load_a(d, ks / 4 + 1, a4n);
+          else if (d + 1 < 3)
+            load_a(d + 1, 0, a4n);
+        }
+        float b[9];
+#pragma unroll
+        for (int p = 0; p < 9; p++) b[p] = bn[p];
+        if (ks + 1 < KV)
+          load_b(ks + 1, bn);
+        else if (d + 1 < 3)
+          load_b(0, bn);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int t = 0; t < NT; t++) pv[i][t] = mfma(a4[t][ks % 4], b[6 + i], pv[i][t]);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          const float z = sv.row(r, b);
+#pragma unroll
+          for (int t = 0; t < NT; t++) acc[6 + r][t] = mfma(a4[t][ks % 4], z, acc[6 + r][t]);
+        }
+        if (ks % 4 == 3 || ks == KV - 1) {
+#pragma unroll
+          for (int t = 0; t < NT; t++) a4[t] = a4n[t];
+        }
+      }
+      add_stress<NT>(star + d * 81, pv, acc, false);
+    }
+  } else {
     auto load = [&](int d, int ks, float (&a)[NT], float (&b)[9]) {
       const int k = 4 * ks + g;
 #pragma unroll
@@ -704,13 +773,8 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
     load(0, 0, an, bn);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      VelRows sv;
-      sv.load(star + d * 81);
-      v4f pv[3][NT];
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int t = 0; t < NT; t++) pv[i][t] = zero4();
+      StarCoef<SP> sc;
+      sc.load(star + d * 81);
 #pragma unroll
       for (int ks = 0; ks < KV; ks++) {
         float a[NT], b[9];
@@ -724,69 +788,27 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
           load(d + 1, 0, an, bn);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-          for (int t = 0; t < NT; t++) pv[i][t] = mfma(a[t], b[6 + i], pv[i][t]);
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-          const float z = sv.row(r, b);
-#pragma unroll
-          for (int t = 0; t < NT; t++) acc[6 + r][t] = mfma(a[t], z, acc[6 + r][t]);
-        }
-      }
-      add_stress<NT>(star + d * 81, pv, acc, false);
-    }
-  } else {
-    StarCoef<SP> sc[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) sc[d].load(star + d * 81);
-    auto load = [&](int ks, float (&a)[3][NT], float (&b)[9]) {
-      const int k = 4 * ks + g;
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-          a[d][t] = ldg(s.dg.stiffV + d * N * N, k * N + 16 * t + c, 0, k < NV && 16 * t + c < N);
-#pragma unroll
-      for (int p = 0; p < 9; p++) {
-        if (4 * ks + 3 < NV)  // constant after unrolling
-          b[p] = l_ti[(p * N + k) * R + c];
-        else
-          b[p] = ldg(l_ti, (p * N + k) * R + c, c, k < NV);
-      }
-    };
-    float an[3][NT], bn[9];
-    load(0, an, bn);
-#pragma unroll
-    for (int ks = 0; ks < KV; ks++) {
-      float a[3][NT], b[9];
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-#pragma unroll
-        for (int t = 0; t < NT; t++) a[d][t] = an[d][t];
-#pragma unroll
-      for (int p = 0; p < 9; p++) b[p] = bn[p];
-      if (ks + 1 < KV) load(ks + 1, an, bn);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-#pragma unroll
         for (int q = 0; q < 9; q++) {
-          const float z = sc[d].row(q, b);
+          const float z = sc.row(q, b);
 #pragma unroll
-          for (int t = 0; t < NT; t++) acc[q][t] = mfma(a[d][t], z, acc[q][t]);
+          for (int t = 0; t < NT; t++) acc[q][t] = mfma(a[t], z, acc[q][t]);
         }
+      }
     }
   }
 
   // ---- local surface integral
   if constexpr (R3) {
+    constexpr pk::Layout L{O};
+    constexpr int NGP = pk::groups(ksteps(N));
+    const v4f* pprj = reinterpret_cast<const v4f*>(s.dg.pk + L.prj());
+    const v4f* plft = reinterpret_cast<const v4f*>(s.dg.pk + L.lft());
 #pragma unroll 1
     for (int fa = 0; fa < 4; fa++) {
       const float* f = s.fsf + (el * 4 + fa) * 54;
       v4f w[3];
-      project3<O>(s.dg.fluxLN + fa * N * NF, f + 27, l_ti, w, g, c);
-      lift3<O>(s.dg.fluxT + fa * NF * N, f, w, acc, g, c);
+      project3<O>(pprj + fa * NGP * 64, f + 27, l_ti, w, g, c, lane);
+      lift3<O>(plft + fa * NT * 64, f, w, acc, lane);
     }
     // ---- this element's tInt as each face-adjacent element sees it (the
     // neighbour's flux matrix and solver side), for the neighbour step
@@ -797,7 +819,7 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
         if (rv < 0) continue;  // nobody receives through this face (uniform across the wave)
         const int fm = s.snd_fm[el * 4 + f];
         v4f w[3];
-        project3<O>(s.dg.fluxLN + fm * N * NF, s.fsf + (s.n_el * 4 + rv) * 54 + 27, l_ti, w, g, c);
+        project3<O>(pprj + fm * NGP * 64, s.fsf + (s.n_el * 4 + rv) * 54 + 27, l_ti, w, g, c, lane);
         store_proj(s.proj + (el * 4 + f) * kProjFace, w, lane);
       }
     }
@@ -855,10 +877,12 @@ void k_neigh(SolverDev<float> s, int64_t first, int64_t n, const float* __restri
     __syncthreads();
     const int fm = s.nb.fmat[el * 4 + fa];
     if constexpr (R3) {
+      constexpr pk::Layout L{O};
+      constexpr int NGP = pk::groups(ksteps(N));
       const float* f = s.fsf + ((s.n_el + el) * 4 + fa) * 54;
       v4f w[3];
-      project3<O>(s.dg.fluxLN + fm * N * NF, f + 27, l_nb, w, g, c);
-      lift3<O>(s.dg.fluxT + fa * NF * N, f, w, acc, g, c);
+      project3<O>(reinterpret_cast<const v4f*>(s.dg.pk + L.prj()) + fm * NGP * 64, f + 27, l_nb, w, g, c, lane);
+      lift3<O>(reinterpret_cast<const v4f*>(s.dg.pk + L.lft()) + fa * NT * 64, f, w, acc, lane);
     } else {
       face_update<O>(s.dg.fluxLN + fm * N * NF, s.dg.fluxT + fa * NF * N, s.fs + ((s.n_el + el) * 4 + fa) * 81,
                      l_nb, acc, g, c);
@@ -895,7 +919,8 @@ void k_neigh_proj(SolverDev<float> s, int64_t first, int64_t n, float* __restric
     if (r < 0) continue;  // outflow (uniform across the wave)
     v4f w[3];
     load_proj(s.proj + (int64_t)r * kProjFace, w, lane);
-    lift3<O>(s.dg.fluxT + fa * D::NF * N, s.fsf + ((s.n_el + el) * 4 + fa) * 54, w, acc, g, c);
+    lift3<O>(reinterpret_cast<const v4f*>(s.dg.pk + pk::Layout{O}.lft()) + fa * NT * 64,
+             s.fsf + ((s.n_el + el) * 4 + fa) * 54, w, acc, lane);
   }
   {
     const int ln = opaque(lane), go = ln >> 4, co = ln & 15;
@@ -941,6 +966,63 @@ void launch_neigh_o(const SolverDev<float>& s, int64_t first, int64_t n, const f
 }
 
 }  // namespace
+
+std::vector<float> mfma_pack_operands(const DgTables& tb) {
+  const int O = tb.order, N = tb.n, NF = tb.nf;
+  const pk::Layout L{O};
+  std::vector<float> out((size_t)L.size(), 0.f);
+  // value of block `blk` (ng groups) at k-step s for lane l
+  auto put = [&](int64_t off, int blk, int ng, int s, int l, double v) {
+    out[(size_t)(off + (((int64_t)blk * ng + s / 4) * 64 + l) * 4 + s % 4)] = (float)v;
+  };
+  const int NT = pk::tiles(N);
+  for (int l = 0; l < 64; l++) {
+    const int g = l / 16, c = l % 16;
+    {  // CK stage 1: -K^T M^-1 (stiffT negated), rows k = 4 s + g < N, columns < n_modes(O - 1)
+      const int nn = n_modes(O - 1), tn = pk::tiles(nn), ks = pk::ksteps(N), ng = pk::groups(ks);
+      for (int d = 0; d < 3; d++)
+        for (int t = 0; t < tn; t++)
+          for (int s = 0; s < ks; s++) {
+            const int k = 4 * s + g, col = 16 * t + c;
+            if (k < N && col < nn) put(L.ck1(), d * tn + t, ng, s, l, -tb.stiffT[((size_t)d * N + k) * N + col]);
+          }
+    }
+    for (int de = 2; de < O; de++) {  // CK stages >= 2: rows 16 u + 4 g + v < n_modes(O - de + 1)
+      const int nk = n_modes(O - de + 1), nn = n_modes(O - de), st = pk::next_steps(O, de), ng = pk::groups(st);
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < st; s++) {
+          const int k = 16 * (s / 4) + 4 * g + s % 4;
+          if (k < nk && c < nn) put(L.ck(de), d, ng, s, l, -tb.stiffT[((size_t)d * N + k) * N + c]);
+        }
+    }
+    {  // volume: K M^-1, rows k = 4 s + g < n_modes(O - 1), all N columns
+      const int nv = n_modes(O - 1), ks = pk::ksteps(nv), ng = pk::groups(ks);
+      for (int d = 0; d < 3; d++)
+        for (int t = 0; t < NT; t++)
+          for (int s = 0; s < ks; s++) {
+            const int k = 4 * s + g, col = 16 * t + c;
+            if (k < nv && col < N) put(L.vol(), d * NT + t, ng, s, l, tb.stiffV[((size_t)d * N + k) * N + col]);
+          }
+    }
+    {  // projections: fluxL (m < 4) or fluxN (m - 4), [N][NF], rows k = 4 s + g
+      const int ks = pk::ksteps(N), ng = pk::groups(ks);
+      for (int m = 0; m < 16; m++) {
+        const double* f = m < 4 ? tb.fluxL + (size_t)m * N * NF : tb.fluxN + (size_t)(m - 4) * N * NF;
+        for (int s = 0; s < ks; s++) {
+          const int k = 4 * s + g;
+          if (k < N && c < NF) put(L.prj(), m, ng, s, l, f[(size_t)k * NF + c]);
+        }
+      }
+    }
+    for (int fa = 0; fa < 4; fa++)  // lifts: fluxT [NF][N], rows 4 g + v
+      for (int t = 0; t < NT; t++)
+        for (int v = 0; v < 4; v++) {
+          const int k = 4 * g + v, col = 16 * t + c;
+          if (k < NF && col < N) put(L.lft(), fa * NT + t, 1, v, l, tb.fluxT[((size_t)fa * NF + k) * N + col]);
+        }
+  }
+  return out;
+}
 
 bool mfma_supported(int order, int n_runs, int precision) {
   return precision == 32 && n_runs == 16 && order >= 2 && order <= 5;

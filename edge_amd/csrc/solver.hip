@@ -10,6 +10,7 @@
 
 #include "kernels.hpp"
 #include "kernels_mfma.hpp"
+#include "mfma_pack.hpp"
 #include "setup.hpp"
 
 struct edge_el_solver {
@@ -20,6 +21,7 @@ struct edge_el_solver {
   size_t real_bytes = 8;
   // device allocations
   void* mats = nullptr;  // stiffTn | stiffV | fluxLN | fluxT  (precision T)
+  float* pk = nullptr;   // MFMA-tiled path: A operands packed in lane order
   void* star = nullptr;  // [nEl][3][81]
   void* fs = nullptr;    // [2][nEl][4][81]
   int32_t* ne = nullptr;
@@ -59,6 +61,7 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.dg.stiffV = m + 3 * N * N;
   d.dg.fluxLN = m + 6 * N * N;
   d.dg.fluxT = m + 6 * N * N + 16 * N * NF;
+  if constexpr (sizeof(T) == 4) d.dg.pk = s->pk;
   d.nb.ne = s->ne;
   d.nb.fmat = s->fmat;
   d.star = static_cast<const T*>(s->star);
@@ -86,6 +89,11 @@ int upload_mats(edge_el_solver* s, const edge::DgTables& t) {
   for (size_t i = 0; i < 4 * NF * N; i++) h[o++] = (T)t.fluxT[i];
   EDGE_HIP_TRY(hipMalloc(&s->mats, sizeof(T) * total));
   EDGE_HIP_TRY(hipMemcpy(s->mats, h.data(), sizeof(T) * total, hipMemcpyHostToDevice));
+  if (s->kernel == 2) {
+    const std::vector<float> p = edge::mfma_pack_operands(t);
+    EDGE_HIP_TRY(hipMalloc(&s->pk, sizeof(float) * p.size()));
+    EDGE_HIP_TRY(hipMemcpy(s->pk, p.data(), sizeof(float) * p.size(), hipMemcpyHostToDevice));
+  }
   return EDGE_OK;
 }
 
@@ -336,6 +344,7 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   if (!s) return EDGE_OK;
   drop_graph(s);
   hipFree(s->mats);
+  hipFree(s->pk);
   hipFree(s->star);
   hipFree(s->fs);
   hipFree(s->ne);
