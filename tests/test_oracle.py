@@ -180,6 +180,28 @@ def test_flux_solver_closed_form_properties():
         np.testing.assert_allclose(T @ Ti, np.eye(9), atol=1e-14)
 
 
+def test_flux_solvers_have_rank_three():
+    """Different materials on both sides, free surfaces included: each flux
+    solver has rank exactly 3 (the flux depends on a side's state only
+    through its three waves travelling towards the face), the structure the
+    MFMA kernels factor as U V^T.  A product of a rank-3 factor pair
+    reproduces each solver to rounding."""
+    rng = np.random.default_rng(17)
+    for _ in range(50):
+        n, s, t = _frame(rng)
+        rl, rr = rng.uniform(1, 3, 2)
+        vp = rng.uniform(3, 6, 2)
+        vs = vp / rng.uniform(1.5, 2.5, 2)
+        ml, mr = rl * vs[0] ** 2, rr * vs[1] ** 2
+        ll, lr = rl * vp[0] ** 2 - 2 * ml, rr * vp[1] ** 2 - 2 * mr
+        for free in (False, True):
+            for f in orc.fs_face(rl, rr, ll, lr, ml, mr, n, s, t, free):
+                sv = np.linalg.svd(f, compute_uv=False)
+                assert sv[2] > 1e-3 * sv[0] and sv[3] < 1e-13 * sv[0], sv
+                u, w, vt = np.linalg.svd(f)
+                np.testing.assert_allclose((u[:, :3] * w[:3]) @ vt[:3], f, atol=1e-13 * sv[0])
+
+
 # --------------------------------------------------------------- the scheme
 @pytest.mark.parametrize("order", [2, 3, 4])
 def test_oracle_plane_wave_convergence(order):
