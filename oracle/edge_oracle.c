@@ -20,6 +20,10 @@
  */
 #include <math.h>
 #include <stdint.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -498,6 +502,223 @@ void orc_init_fs(int64_t n_fas, const int64_t *fa_el /*[fa][2]*/, const int64_t 
 
 /* Element time step by CFL, src/impl/seismic/common.hpp:84-98, with the
  * reference's scale 1/2 (const.inc:135-137). */
+/* -------------------------------------------------------------------------
+ * Viscoelastic set-up, src/impl/seismic/setups/ViscoElasticity.{h,cpp}.
+ * ---------------------------------------------------------------------- */
+/* 2n-1 relaxation frequencies log-spaced around 2 pi f_cen over the ratio
+ * f_rat, ViscoElasticity.h:169-182. */
+void orc_visco_frequencies(int n_mechs, double f_cen, double f_rat, double *freqs) {
+  freqs[0] = 2 * M_PI * f_cen / sqrt(f_rat);
+  for (int fr = 1; fr < 2 * n_mechs - 1; fr++)
+    freqs[fr] = exp(log(freqs[0]) + (fr * log(f_rat)) / (2 * (n_mechs - 1)));
+}
+
+/* Least-squares fit of the anelastic coefficients for a constant quality
+ * factor q over the 2n-1 frequencies, ViscoElasticity.cpp:69-128 (the
+ * reference solves with an SVD; the full-rank least-squares solution is
+ * unique, here by Householder QR). */
+void orc_anelastic_coeff_ps(int n_mechs, double q, const double *freqs, double *coeffs) {
+  const int m = 2 * n_mechs - 1, n = n_mechs;
+  const double qi = 1.0 / q;
+  double A[64], b[16];
+  for (int fr = 0; fr < m; fr++) {
+    for (int me = 0; me < n; me++) {
+      const double mf = freqs[me * 2];
+      A[fr * n + me] = (mf * freqs[fr] + mf * mf * qi) / (freqs[fr] * freqs[fr] + mf * mf);
+    }
+    b[fr] = qi;
+  }
+  for (int k = 0; k < n; k++) { /* Householder reflection of column k */
+    double nr = 0;
+    for (int i = k; i < m; i++) nr += A[i * n + k] * A[i * n + k];
+    nr = sqrt(nr);
+    const double alpha = A[k * n + k] > 0 ? -nr : nr;
+    double v[16];
+    for (int i = 0; i < m; i++) v[i] = i < k ? 0 : A[i * n + k];
+    v[k] -= alpha;
+    double vn = 0;
+    for (int i = k; i < m; i++) vn += v[i] * v[i];
+    if (vn == 0) continue;
+    for (int j = k; j < n; j++) {
+      double d = 0;
+      for (int i = k; i < m; i++) d += v[i] * A[i * n + j];
+      for (int i = k; i < m; i++) A[i * n + j] -= 2 * d / vn * v[i];
+    }
+    double d = 0;
+    for (int i = k; i < m; i++) d += v[i] * b[i];
+    for (int i = k; i < m; i++) b[i] -= 2 * d / vn * v[i];
+  }
+  for (int k = n - 1; k >= 0; k--) { /* R y = Q^T b */
+    double r = b[k];
+    for (int j = k + 1; j < n; j++) r -= A[k * n + j] * coeffs[j];
+    coeffs[k] = r / A[k * n + k];
+  }
+}
+
+/* Unrelaxed (elastic) Lame parameters from the phase-velocity ones at the
+ * reference frequency, ViscoElasticity.cpp:30-67. */
+void orc_elastic_lame(int n_mechs, double f_ref, const double *freqs, const double *c_kappa,
+                      const double *c_mu, double lam_phase, double mu_phase, double *lam_e, double *mu_e) {
+  double th1[2] = {1, 1}, th2[2] = {0, 0};
+  for (int me = 0; me < n_mechs; me++) {
+    const double rat = f_ref / freqs[me * 2], sca = 1.0 / (1 + rat * rat);
+    th1[0] -= c_kappa[me] * sca;
+    th1[1] -= c_mu[me] * sca;
+    th2[0] += c_kappa[me] * rat * sca;
+    th2[1] += c_mu[me] * rat * sca;
+  }
+  double r[2];
+  for (int i = 0; i < 2; i++) r[i] = sqrt(th1[i] * th1[i] + th2[i] * th2[i]);
+  *mu_e = mu_phase * (r[1] + th1[1]) / (2 * r[1] * r[1]);
+  const double kappa = (lam_phase + 2 * mu_phase) * (r[0] + th1[0]) / (2 * r[0] * r[0]);
+  *lam_e = kappa - 2 * *mu_e;
+}
+
+/* Anelastic coefficients in Lame form (lambda part, then mu part) and the
+ * elastic Lame parameters, ViscoElasticity.cpp:130-176. */
+void orc_anelastic_coeff_lame(int n_mechs, double f_cen, double f_rat, double qp, double qs, double lam_phase,
+                              double mu_phase, double *lam_e, double *mu_e, double *coeffs /*[2n]*/) {
+  double freqs[16];
+  orc_visco_frequencies(n_mechs, f_cen, f_rat, freqs);
+  orc_anelastic_coeff_ps(n_mechs, qp, freqs, coeffs);
+  orc_anelastic_coeff_ps(n_mechs, qs, freqs, coeffs + n_mechs);
+  orc_elastic_lame(n_mechs, 2 * M_PI * f_cen, freqs, coeffs, coeffs + n_mechs, lam_phase, mu_phase, lam_e, mu_e);
+  for (int me = 0; me < n_mechs; me++) {
+    const double sca = 2.0 * *mu_e / *lam_e;
+    coeffs[me] = (1.0 + sca) * coeffs[me] - sca * coeffs[n_mechs + me];
+  }
+}
+
+/* Anelastic source matrices acting on the stresses, [n][6][6] per element,
+ * and the elastic Lame parameters, ViscoElasticity.h:380-458. */
+void orc_src_a(int n_mechs, double f_cen, double f_rat, double qp, double qs, double lam_phase, double mu_phase,
+               double *lam_e, double *mu_e, double *src /*[n][36]*/) {
+  double co[32];
+  orc_anelastic_coeff_lame(n_mechs, f_cen, f_rat, qp, qs, lam_phase, mu_phase, lam_e, mu_e, co);
+  for (int me = 0; me < n_mechs; me++) {
+    double *o = src + me * 36;
+    for (int i = 0; i < 36; i++) o[i] = 0;
+    const double d = -*lam_e * co[me] - 2 * *mu_e * co[n_mechs + me];
+    const double sh = -2 * *mu_e * co[n_mechs + me];
+    const double off = -*lam_e * co[me];
+    for (int i = 0; i < 3; i++) {
+      o[i * 6 + i] = d;
+      o[(3 + i) * 6 + 3 + i] = sh;
+      for (int j = 0; j < 3; j++)
+        if (j != i) o[i * 6 + j] = off;
+    }
+  }
+}
+
+/* Anelastic star matrices [3][6][3] (rows: memory variables, columns:
+ * velocities), ViscoElasticity.cpp:233-316. */
+void orc_star_a(const double jinv[9], double star_a[54]) {
+  for (int i = 0; i < 54; i++) star_a[i] = 0;
+  for (int d = 0; d < 3; d++) {
+    double *o = star_a + d * 18;
+    const double j0 = jinv[0 * 3 + d], j1 = jinv[1 * 3 + d], j2 = jinv[2 * 3 + d];
+    o[0 * 3 + 0] += -1.0 * j0;
+    o[3 * 3 + 1] += -0.5 * j0;
+    o[5 * 3 + 2] += -0.5 * j0;
+    o[1 * 3 + 1] += -1.0 * j1;
+    o[3 * 3 + 0] += -0.5 * j1;
+    o[4 * 3 + 2] += -0.5 * j1;
+    o[2 * 3 + 2] += -1.0 * j2;
+    o[4 * 3 + 1] += -0.5 * j2;
+    o[5 * 3 + 0] += -0.5 * j2;
+  }
+}
+
+/* Middle-state part of the anelastic flux solvers in face coordinates,
+ * [6][9] each, ViscoElasticity.cpp:376-441. */
+void orc_fs_mid_a(double rL, double rR, double lL, double lR, double mL, double mR, double midL[54],
+                  double midR[54]) {
+  const double cpL = sqrt((lL + 2 * mL) / rL), cpR = sqrt((lR + 2 * mR) / rR);
+  const double csL = sqrt(mL / rL), csR = sqrt(mR / rR);
+  for (int i = 0; i < 54; i++) midL[i] = midR[i] = 0;
+  const double dp = lL * cpR + lR * cpL + 2 * mL * cpR + 2 * mR * cpL;
+  const double ds = 2 * mL * csR + 2 * mR * csL;
+  midL[0 * 9 + 0] = cpL * cpR / dp;
+  midL[0 * 9 + 6] = -cpR * (lL + 2 * mL) / dp;
+  midL[3 * 9 + 3] = midL[5 * 9 + 5] = csL * csR / ds;
+  midL[3 * 9 + 7] = midL[5 * 9 + 8] = -mL * csR / ds;
+  midR[0 * 9 + 0] = -cpL * cpR / dp;
+  midR[0 * 9 + 6] = -cpL * (lR + 2 * mR) / dp;
+  midR[3 * 9 + 3] = midR[5 * 9 + 5] = -csL * csR / ds;
+  midR[3 * 9 + 7] = midR[5 * 9 + 8] = -mR * csL / ds;
+}
+
+/* Anelastic flux solvers [6][9] of both sides in global coordinates:
+ * T_A mid T^-1 with T_A the stress block of T, the right state mirrored
+ * on a free surface, ViscoElasticity.cpp:505-570 and
+ * solvers/AderDgInit.hpp:288-370. */
+void orc_fs_a(double rL, double rR, double lL, double lR, double mL, double mR, const double n[3],
+              const double s[3], const double t[3], int free_surface, double fsAl[54], double fsAr[54]) {
+  double T[81], Ti[81], mid[2][54], tmp[2][54];
+  orc_trafo(n, s, t, T, Ti);
+  orc_fs_mid_a(rL, rR, lL, lR, mL, mR, mid[0], mid[1]);
+  for (int sd = 0; sd < 2; sd++)
+    for (int r = 0; r < 6; r++)
+      for (int c = 0; c < 9; c++) {
+        double v = 0;
+        for (int k = 0; k < 6; k++) v += T[r * 9 + k] * mid[sd][k * 9 + c];
+        tmp[sd][r * 9 + c] = v;
+      }
+  if (free_surface)
+    for (int r = 0; r < 6; r++) {
+      tmp[1][r * 9 + 0] *= -1;
+      tmp[1][r * 9 + 3] *= -1;
+      tmp[1][r * 9 + 5] *= -1;
+    }
+  for (int sd = 0; sd < 2; sd++) {
+    double *o = sd == 0 ? fsAl : fsAr;
+    for (int r = 0; r < 6; r++)
+      for (int c = 0; c < 9; c++) {
+        double v = 0;
+        for (int k = 0; k < 9; k++) v += tmp[sd][r * 9 + k] * Ti[k * 9 + c];
+        o[r * 9 + c] = v;
+      }
+  }
+}
+
+/* Anelastic flux solvers of all element faces, [el][4][54] per side, with
+ * the elastic solvers' scaling -2 area / |J| (AderDgInit.hpp:617-663). */
+void orc_init_fs_a(int64_t n_fas, const int64_t *fa_el, const int64_t *el_fa, const int32_t *fa_type,
+                   const double *normals, const double *tangents, const double *areas, const double *jdet,
+                   const double *rho, const double *lam, const double *mu, double *fsA0, double *fsA1) {
+  for (int64_t fa = 0; fa < n_fas; fa++) {
+    const int64_t eL = fa_el[fa * 2], eR = fa_el[fa * 2 + 1];
+    const int exR = eR >= 0;
+    int fL = -1, fR = -1;
+    for (int f = 0; f < 4; f++) {
+      if (el_fa[eL * 4 + f] == fa) fL = f;
+      if (exR && el_fa[eR * 4 + f] == fa) fR = f;
+    }
+    const int fs = (fa_type[fa] & FA_FREE_SURFACE) == FA_FREE_SURFACE;
+    const double rL = rho[eL], lL = lam[eL], mL = mu[eL];
+    const double rR = exR ? rho[eR] : rL, lR = exR ? lam[eR] : lL, mR = exR ? mu[eR] : mL;
+    const double *n = normals + fa * 3, *s = tangents + fa * 6, *t = tangents + fa * 6 + 3;
+    double a[54], b[54];
+    if (fL >= 0) {
+      orc_fs_a(rL, rR, lL, lR, mL, mR, n, s, t, fs, a, b);
+      const double sc = -2.0 * areas[fa] / jdet[eL];
+      for (int i = 0; i < 54; i++) {
+        fsA0[(eL * 4 + fL) * 54 + i] = a[i] * sc;
+        fsA1[(eL * 4 + fL) * 54 + i] = b[i] * sc;
+      }
+    }
+    if (exR && fR >= 0) {
+      const double nm[3] = {-n[0], -n[1], -n[2]};
+      orc_fs_a(rR, rL, lR, lL, mR, mL, nm, s, t, 0, a, b);
+      const double sc = -2.0 * areas[fa] / jdet[eR];
+      for (int i = 0; i < 54; i++) {
+        fsA0[(eR * 4 + fR) * 54 + i] = a[i] * sc;
+        fsA1[(eR * 4 + fR) * 54 + i] = b[i] * sc;
+      }
+    }
+  }
+}
+
 double orc_dt_cfl(double rho, double lam, double mu, double in_dia, int order) {
   return in_dia * 0.5 / sqrt((lam + 2 * mu) / rho) / (2.0 * order - 1);
 }

@@ -38,7 +38,9 @@ def lib():
         _lib.orc_dt_cfl.argtypes = [C.c_double] * 5 + [C.c_int]
         for f in ["orc_ck", "orc_vol", "orc_surf_local", "orc_surf_neigh", "orc_neigh_flux_int", "orc_local",
                   "orc_neigh", "orc_star", "orc_fs", "orc_jac", "orc_trafo", "orc_fs_mid", "orc_init_fs",
-                  "orc_integrate", "orc_eval_time_pred", "orc_scatter_update_a"]:
+                  "orc_integrate", "orc_eval_time_pred", "orc_scatter_update_a", "orc_visco_frequencies",
+                  "orc_anelastic_coeff_ps", "orc_elastic_lame", "orc_anelastic_coeff_lame", "orc_src_a",
+                  "orc_star_a", "orc_fs_mid_a", "orc_fs_a", "orc_init_fs_a"]:
             getattr(_lib, f).restype = None
     return _lib
 
@@ -54,7 +56,13 @@ def ref_lib():
 
 
 def _p(a):
-    return None if a is None else C.c_void_p(a.ctypes.data)
+    """Pointer to an array's data that keeps the array alive as long as the
+    pointer object (arguments built from temporaries stay valid for the call)."""
+    if a is None:
+        return None
+    v = C.c_void_p(a.ctypes.data)
+    v._keep = a
+    return v
 
 
 def _c(a):
@@ -168,6 +176,97 @@ def setup_mesh(mesh, rho, lam, mu):
     return st, fs, jdet
 
 
+# ---------------------------------------------------------------- viscoelastic set-up
+_d = C.c_double
+
+
+def visco_frequencies(n_mechs, f_cen, f_rat):
+    out = np.zeros(2 * n_mechs - 1)
+    lib().orc_visco_frequencies(C.c_int(n_mechs), _d(f_cen), _d(f_rat), _p(out))
+    return out
+
+
+def anelastic_coeff_ps(n_mechs, q, freqs):
+    out = np.zeros(n_mechs)
+    fr = _c(freqs)  # held for the call
+    lib().orc_anelastic_coeff_ps(C.c_int(n_mechs), _d(q), _p(fr), _p(out))
+    return out
+
+
+def elastic_lame(n_mechs, f_ref, freqs, c_kappa, c_mu, lam, mu):
+    le, me = _d(), _d()
+    fr, ck, cm = _c(freqs), _c(c_kappa), _c(c_mu)  # held for the call
+    lib().orc_elastic_lame(C.c_int(n_mechs), _d(f_ref), _p(fr), _p(ck), _p(cm), _d(lam), _d(mu), C.byref(le),
+                           C.byref(me))
+    return le.value, me.value
+
+
+def anelastic_coeff_lame(n_mechs, f_cen, f_rat, qp, qs, lam, mu):
+    le, me = _d(), _d()
+    co = np.zeros(2 * n_mechs)
+    lib().orc_anelastic_coeff_lame(C.c_int(n_mechs), _d(f_cen), _d(f_rat), _d(qp), _d(qs), _d(lam), _d(mu),
+                                   C.byref(le), C.byref(me), _p(co))
+    return le.value, me.value, co
+
+
+def src_a(n_mechs, f_cen, f_rat, qp, qs, lam, mu):
+    """(lam_elastic, mu_elastic, src [n][6][6])."""
+    le, me = _d(), _d()
+    out = np.zeros((n_mechs, 36))
+    lib().orc_src_a(C.c_int(n_mechs), _d(f_cen), _d(f_rat), _d(qp), _d(qs), _d(lam), _d(mu), C.byref(le),
+                    C.byref(me), _p(out))
+    return le.value, me.value, out.reshape(n_mechs, 6, 6)
+
+
+def star_a(jinv):
+    out = np.zeros(54)
+    ji = _c(np.asarray(jinv, dtype=np.float64).ravel())  # held for the call
+    lib().orc_star_a(_p(ji), _p(out))
+    return out.reshape(3, 6, 3)
+
+
+def fs_mid_a(rL, rR, lL, lR, mL, mR):
+    a, b = np.zeros(54), np.zeros(54)
+    lib().orc_fs_mid_a(*[_d(x) for x in (rL, rR, lL, lR, mL, mR)], _p(a), _p(b))
+    return a.reshape(6, 9), b.reshape(6, 9)
+
+
+def fs_a(rL, rR, lL, lR, mL, mR, n, s, t, free_surface=False):
+    a, b = np.zeros(54), np.zeros(54)
+    n, s, t = _c(n), _c(s), _c(t)  # held for the call
+    lib().orc_fs_a(*[_d(x) for x in (rL, rR, lL, lR, mL, mR)], _p(n), _p(s), _p(t), C.c_int(int(free_surface)),
+                   _p(a), _p(b))
+    return a.reshape(6, 9), b.reshape(6, 9)
+
+
+def setup_mesh_visco(mesh, rho, lam, mu, qp, qs, n_rms, f_cen, f_rat):
+    """Viscoelastic set-up of a mesh (AderDgInit.hpp:91-117, :129-174,
+    :394-665 with AderDg.hpp:248-258): per element the anelastic source
+    matrices and the unrelaxed Lame parameters, which then build the elastic
+    star matrices and flux solvers; anelastic star matrices and flux solvers;
+    relaxation frequencies (every second of the 2n-1 frequencies)."""
+    ne = mesh.n_el
+    lam_e, mu_e = np.zeros(ne), np.zeros(ne)
+    src = np.zeros((ne, n_rms, 36))
+    for e in range(ne):
+        lam_e[e], mu_e[e], sm = src_a(n_rms, f_cen, f_rat, qp[e], qs[e], lam[e], mu[e])
+        src[e] = sm.reshape(n_rms, 36)
+    starE, fsE, jdet = setup_mesh(mesh, rho, lam_e, mu_e)
+    ve = mesh.el_crds()
+    starA = np.zeros((ne, 3, 18))
+    for e in range(ne):
+        _, _, ji = jac(ve[e])
+        starA[e] = star_a(ji).reshape(3, 18)
+    fsA = np.full((2, ne, 4, 54), np.nan)
+    lib().orc_init_fs_a(C.c_int64(mesh.n_fa), _p(np.ascontiguousarray(mesh.fa_el)),
+                        _p(np.ascontiguousarray(mesh.el_fa)), _p(np.ascontiguousarray(mesh.fa_type)),
+                        _p(_c(mesh.normals)), _p(_c(mesh.tangents)), _p(_c(mesh.areas)), _p(_c(jdet)),
+                        _p(_c(rho)), _p(_c(lam_e)), _p(_c(mu_e)), _p(fsA[0]), _p(fsA[1]))
+    rfs = visco_frequencies(n_rms, f_cen, f_rat)[::2].copy()
+    return {"starE": starE, "fsE": fsE, "starA": starA, "srcA": src, "fsA": fsA, "rfs": rfs,
+            "lam_e": lam_e, "mu_e": mu_e, "jdet": jdet}
+
+
 def dt_cfl(rho, lam, mu, in_dia, order):
     return lib().orc_dt_cfl(rho, lam, mu, in_dia, order)
 
@@ -176,9 +275,17 @@ def dt_cfl(rho, lam, mu, in_dia, order):
 class MeshOracle:
     """FP64 CPU time stepping of a whole mesh (AderDg::local + neigh, GTS)."""
 
-    def __init__(self, mesh, order, n_crs=1, star=None, fs=None, rho=None, lam=None, mu=None):
+    def __init__(self, mesh, order, n_crs=1, star=None, fs=None, rho=None, lam=None, mu=None, visco=None):
+        """visco: the dict of setup_mesh_visco (anelastic terms, N_RMS > 0)."""
         self.mesh = mesh
-        self.dg = Dg(order, n_crs)
+        self.visco = visco
+        if visco is not None:
+            star, fs = visco["starE"], visco["fsE"]
+            self.star_a, self.src_a, self.fs_a = _c(visco["starA"]), _c(visco["srcA"]), _c(visco["fsA"])
+        else:
+            self.star_a = self.src_a = None
+            self.fs_a = [None, None]
+        self.dg = Dg(order, n_crs, rfs=None if visco is None else visco["rfs"])
         if star is None:
             star, fs, _ = setup_mesh(mesh, rho, lam, mu)
         self.star = _c(star)
@@ -189,21 +296,24 @@ class MeshOracle:
         self.v_id = np.ascontiguousarray(mesh.v_id, dtype=np.uint16)
         self.fa_type = np.ascontiguousarray(mesh.fa_type, dtype=np.int32)
 
-    def local(self, dofs, tint, dt, first=0, n=None):
+    def local(self, dofs, tint, dt, first=0, n=None, dofs_a=None):
+        """dofs [el][9][N][R], tint [el][9][N][R]; dofs_a [el][N_RMS][6][N][R]
+        with anelastic terms."""
         n = self.mesh.n_el - first if n is None else n
-        lib().orc_local(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), C.c_double(dt), _p(self.star), None,
-                        None, _p(self.fs[0]), None, _p(dofs), None, _p(tint))
+        lib().orc_local(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), C.c_double(dt), _p(self.star),
+                        _p(self.star_a), _p(self.src_a), _p(self.fs[0]), _p(self.fs_a[0]), _p(dofs), _p(dofs_a),
+                        _p(tint))
 
-    def neigh(self, dofs, tint, first=0, n=None):
+    def neigh(self, dofs, tint, first=0, n=None, dofs_a=None):
         n = self.mesh.n_el - first if n is None else n
         lib().orc_neigh(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), _p(self.fa_type), _p(self.el_fa),
-                        _p(self.el_fa_el), _p(self.f_id), _p(self.v_id), _p(self.fs[1]), None, _p(tint), _p(dofs),
-                        None)
+                        _p(self.el_fa_el), _p(self.f_id), _p(self.v_id), _p(self.fs[1]), _p(self.fs_a[1]),
+                        _p(tint), _p(dofs), _p(dofs_a))
 
-    def step(self, dofs, tint, dt, n_steps=1):
+    def step(self, dofs, tint, dt, n_steps=1, dofs_a=None):
         for _ in range(n_steps):
-            self.local(dofs, tint, dt)
-            self.neigh(dofs, tint)
+            self.local(dofs, tint, dt, dofs_a=dofs_a)
+            self.neigh(dofs, tint, dofs_a=dofs_a)
 
 
 # ---------------------------------------------------------------- CPU baseline sample
@@ -238,6 +348,8 @@ class SampleOracle(MeshOracle):
         self.dg = Dg(order, n_crs)
         self.star = _c(star)
         self.fs = _c(fs)
+        self.visco = self.star_a = self.src_a = None
+        self.fs_a = [None, None]
         n_s = star.shape[0]
 
         class _M:

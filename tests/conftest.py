@@ -36,3 +36,10 @@ def gpu():
     from edge_amd import _lib
     _lib.lib()  # raises when libedge_amd.so is missing
     return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="session")
+def visco_fx():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "ref_visco_fixtures.json")) as f:
+        return json.load(f)

@@ -18,6 +18,10 @@ Sources (all read as text):
   (test_fluxTet).
 * src/dg/generated/basis_tet.inc, basis_tria.inc -- the reference's generated
   basis polynomials, evaluated here at seeded points (values only are stored).
+* src/impl/seismic/setups/ViscoElasticity.test.cpp -- expected values of the
+  viscoelastic set-up tests (relaxation frequencies, anelastic coefficients,
+  elastic Lame parameters, source and star matrices, 2D middle-state flux
+  solvers), stored as (array, index, value).
 """
 
 from fractions import Fraction
@@ -109,6 +113,60 @@ def c_expr_basis(path, var_name, nvar, max_fn, pts):
     return out
 
 
+VISCO_CASES = {  # reference TEST_CASE title -> fixture key
+    "Derivation of relaxation frequencies.": "frequencies",
+    "Derivation of anelastic coefficients, p/s-version": "coeff_ps",
+    "Derivation of elastic Lame parameters.": "elastic_lame",
+    "Derivation of anelastic coefficients, lame-version": "coeff_lame",
+    "Derivation of the source matrix in 3D": "src3d",
+    "Derivation of the viscoelastic part of the star matrix in 3D": "star3d",
+    "Derivation of the viscoelastic part of the flux solvers in 2D": "fs_mid2d",
+}
+
+
+def _num_expr(tok):
+    """Value of a literal or a product of literals such as '-0.5 * 4'."""
+    v = 1.0
+    for f in tok.split("*"):
+        v *= float(f)
+    return v
+
+
+def visco_golden(text):
+    """(array name, indices, expected value) of every REQUIRE of the
+    viscoelastic set-up test cases, in file order."""
+    out = {}
+    parts = re.split(r'TEST_CASE\(\s*"', text)[1:]
+    for part in parts:
+        title = part.split('"')[0]
+        key = VISCO_CASES.get(title)
+        if key is None:
+            continue
+        rows = []
+        for name, idx, val in re.findall(
+                r"REQUIRE\(\s*l_(\w+)((?:\[[^\]]+\])*)\s*==\s*(?:Approx\(\s*)?([-+0-9.eE* ]+?)\s*\)?\s*\);", part):
+            try:
+                ids = [eval_index(i) for i in re.findall(r"\[([^\]]+)\]", idx)]
+            except ValueError:  # indexed by a loop variable (checks of zero rows)
+                continue
+            rows.append([name, ids, _num_expr(val)])
+        out[key] = rows
+    return out
+
+
+def eval_index(tok):
+    """Value of an index expression made of integer literals, + - and *."""
+    tok = tok.replace(" ", "")
+    total = 0
+    for term in re.findall(r"[+-]?[^+-]+", tok):
+        sign = -1 if term.startswith("-") else 1
+        v = 1
+        for f in term.lstrip("+-").split("*"):
+            v *= int(f)
+        total += sign * v
+    return total
+
+
 def main():
     fx = {}
 
@@ -181,6 +239,11 @@ def main():
     mats["basis_tria_val"] = c_expr_basis(os.path.join(gen, "basis_tria.inc"), "val", 2, nb2, pts2).tolist()
     with open(os.path.join(HERE, "ref_matrix_fixtures.json"), "w") as f:
         json.dump(mats, f)
+
+    # ---------------- viscoelastic set-up ----------------
+    t = _read(os.path.join(REF, "src/impl/seismic/setups/ViscoElasticity.test.cpp"))
+    with open(os.path.join(HERE, "ref_visco_fixtures.json"), "w") as f:
+        json.dump(visco_golden(t), f, indent=0)
     print("wrote fixtures to", HERE)
 
 

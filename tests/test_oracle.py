@@ -223,3 +223,89 @@ def test_oracle_plane_wave_convergence(order):
         errs.append(np.sqrt(l2.sum()))
     rate = np.log2(errs[0] / errs[1])
     assert rate > order - 0.3, (errs, rate)
+
+
+# --------------------------------------------------------------- viscoelastic set-up
+# Inputs of the reference's viscoelastic set-up tests
+# (src/impl/seismic/setups/ViscoElasticity.test.cpp:147-330): 3 mechanisms,
+# central frequency 2, frequency ratio 10, Q_p = 200, Q_s = 300 (the Lame-form
+# tests; the p/s test fits Q = 200 then Q = 300), lambda = 20.8e9, mu = 10.4e9.
+def _golden(fx, key, name):
+    return [(tuple(i), v) for n, i, v in fx[key] if n == name]
+
+
+def test_visco_frequencies_and_fits_match_reference_tests(visco_fx):
+    f = orc.visco_frequencies(3, 2, 10)
+    for (i,), v in _golden(visco_fx, "frequencies", "freqs1"):
+        assert f[i] == pytest.approx(v, rel=1e-12)
+    fits = _golden(visco_fx, "coeff_ps", "fit1")
+    for q, part in ((200.0, fits[:3]), (300.0, fits[3:])):
+        c = orc.anelastic_coeff_ps(3, q, f)
+        for (i,), v in part:
+            assert c[i] == pytest.approx(v, rel=1e-11)
+
+
+def test_visco_lame_and_source_match_reference_tests(visco_fx):
+    f = [v for _, v in _golden(visco_fx, "frequencies", "freqs1")]
+    fits = [v for _, v in _golden(visco_fx, "coeff_ps", "fit1")]
+    le, me = orc.elastic_lame(3, 2 * np.pi * 2, f, fits[:3], fits[3:], 20.8e9, 10.4e9)
+    gold = dict((n, v) for n, _, v in visco_fx["elastic_lame"])
+    assert le == pytest.approx(gold["lamElastic"], rel=1e-12)
+    assert me == pytest.approx(gold["muElastic"], rel=1e-12)
+    le, me, co = orc.anelastic_coeff_lame(3, 2, 10, 200.0, 300.0, 20.8e9, 10.4e9)
+    for (i,), v in _golden(visco_fx, "coeff_lame", "lameE"):
+        assert (le, me)[i] == pytest.approx(v, rel=1e-12)
+    for (i,), v in _golden(visco_fx, "coeff_lame", "coeffs1"):
+        assert co[i] == pytest.approx(v, rel=1e-11)
+    le, me, src = orc.src_a(3, 2, 10, 200.0, 300.0, 20.8e9, 10.4e9)
+    checked = np.zeros((3, 36), bool)
+    for (m, i), v in _golden(visco_fx, "src3d", "srcElastic"):
+        assert src[m].ravel()[i] == pytest.approx(v, rel=1e-11, abs=1e-3)
+        checked[m, i] = True
+    assert checked.sum() >= 36  # every non-zero of the three mechanisms and the zeros the test lists
+
+
+def test_visco_star_matches_reference_test(visco_fx):
+    ji = np.array([[4, 8, 0], [2, 8, 4], [8, 0, 4]], dtype=float)
+    st = orc.star_a(ji)  # [3][6][3]
+    gold = _golden(visco_fx, "star3d", "starA")
+    assert len(gold) == 9
+    for (d, i), v in gold:
+        assert st[d].ravel()[i] == pytest.approx(v, rel=1e-15)
+    # the other directions follow the same pattern with column d of J^-1
+    for d in range(3):
+        np.testing.assert_allclose(st[d], orc.star_a(ji[:, [d, d, d]] * 1.0)[0], rtol=0, atol=0)
+
+
+def test_visco_flux_mid_reduces_to_reference_2d(visco_fx):
+    """The 3D anelastic middle states reduce to the 2D ones the reference
+    tests (P: sigma_nn / u_n, S: sigma_ns / u_s and sigma_nt / u_t)."""
+    midL, midR = orc.fs_mid_a(1.2e3, 1.1e3, 20.8e9, 19.4e9, 10.4e9, 13.1e9)
+    g = {(n, i): v for n, i, v in ((n, tuple(i), v) for n, i, v in visco_fx["fs_mid2d"])}
+    pos = {(0, 0): (0, 0), (0, 3): (0, 6), (2, 2): (3, 3), (2, 4): (3, 7)}
+    for side, mid in (("fsMidL1", midL), ("fsMidR1", midR)):
+        expect = np.zeros((6, 9))
+        for (r2, c2), (r3, c3) in pos.items():
+            expect[r3, c3] = g[(side, (r2, c2))]
+        expect[5, 5], expect[5, 8] = expect[3, 3], expect[3, 7]  # second shear direction
+        np.testing.assert_allclose(mid, expect, rtol=1e-12, atol=0)
+
+
+def test_visco_flux_solvers_rotate_the_middle_state():
+    """In the global frame (n, s, t) = (x, y, z) the rotation is the identity:
+    the solvers are the middle states; a free surface negates the right
+    solver's traction columns (0, 3, 5).  Rotated frames: T_A mid T^-1."""
+    args = (1.2, 1.5, 2.0, 1.7, 1.1, 0.9)
+    e = np.eye(3)
+    l, r = orc.fs_a(*args, e[0], e[1], e[2])
+    ml, mr = orc.fs_mid_a(*args)
+    np.testing.assert_allclose(l, ml, atol=1e-15)
+    np.testing.assert_allclose(r, mr, atol=1e-15)
+    _, rf = orc.fs_a(*args, e[0], e[1], e[2], True)
+    np.testing.assert_allclose(rf, mr * np.array([-1, 1, 1, -1, 1, -1, 1, 1, 1.0]), atol=1e-15)
+    rng = np.random.default_rng(4)
+    n, s, t = _frame(rng)
+    T, Ti = orc.trafo(n, s, t)
+    l, r = orc.fs_a(*args, n, s, t)
+    np.testing.assert_allclose(l, T[:6, :6] @ ml @ Ti, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(r, T[:6, :6] @ mr @ Ti, rtol=1e-13, atol=1e-14)
