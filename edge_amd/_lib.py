@@ -42,6 +42,10 @@ class ElConfig(C.Structure):
     _fields_ = [("order", C.c_int32), ("n_runs", C.c_int32), ("precision", C.c_int32), ("kernel", C.c_int32)]
 
 
+class ViscoConfig(C.Structure):
+    _fields_ = [("n_rms", C.c_int32), ("freq_cen", C.c_double), ("freq_rat", C.c_double)]
+
+
 # symbol -> (restype, argtypes)
 _SIGS = {
     "edge_last_error": (C.c_char_p, []),
@@ -66,6 +70,13 @@ _SIGS = {
                                C.c_void_p]),
     "edge_el_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                C.POINTER(C.c_int64)]),
+    "edge_el_create_visco": (C.c_int, [C.POINTER(ElConfig), C.POINTER(ViscoConfig), C.POINTER(MeshIn)] +
+                             [C.c_void_p] * 5 + [C.POINTER(C.c_void_p)]),
+    "edge_el_local_a": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]),
+    "edge_el_neigh_a": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "edge_el_get_setup_a": (C.c_int, [C.c_void_p] + [C.c_void_p] * 6),
     "edge_el_kernel_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edge_el_l2_norms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),

@@ -42,6 +42,12 @@ struct SolverDev {
   const int32_t* snd_v = nullptr;  // [nEl][4]: receiving element-face (el * 4 + fa) of this face, -1: none
   const uint8_t* snd_fm = nullptr; // [nEl][4]: flux matrix (index into fluxLN) of that projection
   const int32_t* rcv = nullptr;    // [nEl][4]: projection slot (el * 4 + f) feeding this face, -1: none
+  // viscoelastic solvers (n_rms > 0)
+  int n_rms = 0;
+  const T* rfs = nullptr;     // [n_rms] relaxation frequencies
+  const T* star_a = nullptr;  // [nEl][3][6][3] anelastic star matrices
+  const T* src_a = nullptr;   // [nEl][n_rms][6][6] anelastic source matrices
+  const T* fs_a = nullptr;    // [2][nEl][4][6][9] anelastic flux solvers
 };
 
 // Non-zero pattern of the elastic star matrices: the union of the patterns
@@ -57,5 +63,14 @@ int launch_local_generic(const SolverDev<T>& s, int64_t first, int64_t n, T dt, 
 template <typename T>
 int launch_neigh_generic(const SolverDev<T>& s, int64_t first, int64_t n, const T* t_int, T* dofs, hipStream_t st);
 bool generic_supported(int order, int n_runs, int prec_bytes);
+
+// Viscoelastic kernels (kernels_visco.hip): one workgroup per element, 1 or 8 runs.
+template <typename T>
+int launch_local_visco(const SolverDev<T>& s, int64_t first, int64_t n, T dt, T* dofs, T* dofs_a, T* t_int,
+                       hipStream_t st);
+template <typename T>
+int launch_neigh_visco(const SolverDev<T>& s, int64_t first, int64_t n, const T* t_int, T* dofs, T* dofs_a,
+                       hipStream_t st);
+bool visco_supported(int order, int n_runs, int prec_bytes, int n_rms);
 
 }  // namespace edge

@@ -14,12 +14,12 @@
 //   Q += sum_fa ((fs1_fa (tInt_ne F^N_{v*4+f})) F^T_fa)
 #include <hip/hip_runtime.h>
 
+#include "gemm_lds.hpp"
 #include "kernels.hpp"
 
 namespace edge {
 namespace {
 
-constexpr int kThreads = 256;
 
 template <typename T, int O, int R>
 struct Sizes {
@@ -30,30 +30,6 @@ struct Sizes {
   static constexpr size_t kLocalLds = sizeof(T) * (5 * SZ + 2 * SZF + 4 * 81 + 3 * 81);
   static constexpr size_t kNeighLds = sizeof(T) * (2 * SZ + 2 * SZF + 81);
 };
-
-// C[m][n][r] (+)= sum_k A[m][k][r] B[k][n]  (A, C in LDS; B row-major, ld = ldb)
-template <typename T, int R>
-__device__ inline void gemm_ac(int m, int n, int k, int lda, int ldb, int ldc, bool beta, const T* A, const T* B,
-                               T* C) {
-  for (int i = threadIdx.x; i < m * n * R; i += kThreads) {
-    const int r = i % R, in = (i / R) % n, im = i / (R * n);
-    T acc = beta ? C[(im * ldc + in) * R + r] : T(0);
-    for (int ik = 0; ik < k; ik++) acc += A[(im * lda + ik) * R + r] * B[ik * ldb + in];
-    C[(im * ldc + in) * R + r] = acc;
-  }
-}
-
-// C[m][n][r] (+)= sum_k A[m][k] B[k][n][r]
-template <typename T, int R>
-__device__ inline void gemm_bc(int m, int n, int k, int lda, int ldb, int ldc, bool beta, const T* A, const T* B,
-                               T* C) {
-  for (int i = threadIdx.x; i < m * n * R; i += kThreads) {
-    const int r = i % R, in = (i / R) % n, im = i / (R * n);
-    T acc = beta ? C[(im * ldc + in) * R + r] : T(0);
-    for (int ik = 0; ik < k; ik++) acc += A[im * lda + ik] * B[(ik * ldb + in) * R + r];
-    C[(im * ldc + in) * R + r] = acc;
-  }
-}
 
 template <typename T, int O, int R>
 __global__ __launch_bounds__(kThreads) void k_local(SolverDev<T> s, int64_t first, T dt, T* __restrict__ dofs,
@@ -198,7 +174,7 @@ int dispatch_neigh(int order, const SolverDev<T>& s, int64_t first, int64_t n, c
 }  // namespace
 
 bool generic_supported(int order, int n_runs, int prec_bytes) {
-  if (order < 2 || order > 5 || (n_runs != 1 && n_runs != 16)) return false;
+  if (order < 2 || order > 5 || (n_runs != 1 && n_runs != 8 && n_runs != 16)) return false;
   const size_t sz = 9 * n_modes(order) * n_runs, szf = 9 * n_face_modes(order) * n_runs;
   return prec_bytes * (5 * sz + 2 * szf + 7 * 81) <= 160 * 1024;
 }
@@ -207,6 +183,7 @@ template <typename T>
 int launch_local_generic(const SolverDev<T>& s, int64_t first, int64_t n, T dt, T* dofs, T* t_int, hipStream_t st) {
   if (n <= 0) return EDGE_OK;
   if (s.n_runs == 1) return dispatch_local<T, 1>(s.order, s, first, n, dt, dofs, t_int, st);
+  if (s.n_runs == 8) return dispatch_local<T, 8>(s.order, s, first, n, dt, dofs, t_int, st);
   if (s.n_runs == 16) return dispatch_local<T, 16>(s.order, s, first, n, dt, dofs, t_int, st);
   return fail(EDGE_ERR_UNSUPPORTED, "runs");
 }
@@ -215,6 +192,7 @@ template <typename T>
 int launch_neigh_generic(const SolverDev<T>& s, int64_t first, int64_t n, const T* t_int, T* dofs, hipStream_t st) {
   if (n <= 0) return EDGE_OK;
   if (s.n_runs == 1) return dispatch_neigh<T, 1>(s.order, s, first, n, t_int, dofs, st);
+  if (s.n_runs == 8) return dispatch_neigh<T, 8>(s.order, s, first, n, t_int, dofs, st);
   if (s.n_runs == 16) return dispatch_neigh<T, 16>(s.order, s, first, n, t_int, dofs, st);
   return fail(EDGE_ERR_UNSUPPORTED, "runs");
 }

@@ -45,7 +45,7 @@ __device__ void load_ve(const double* crds, const int64_t* el_ve, int64_t el, do
 
 template <typename T>
 __global__ void k_star(int64_t n_el, const double* crds, const int64_t* el_ve, const double* rho,
-                       const double* lam, const double* mu, T* star, double* jdet, int* bad) {
+                       const double* lam, const double* mu, T* star, double* jdet, T* star_a, int* bad) {
   const int64_t el = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (el >= n_el) return;
   double ve[12], det, ji[9];
@@ -65,6 +65,26 @@ __global__ void k_star(int64_t n_el, const double* crds, const int64_t* el_ve, c
     for (int k = 0; k < 3; k++)
       for (int e = 0; e < 8; e++) s[d * 81 + rc[k][e][0] * 9 + rc[k][e][1]] += va[k][e] * ji[k * 3 + d];
   for (int i = 0; i < 243; i++) star[el * 243 + i] = (T)s[i];
+  if (star_a) {
+    // anelastic star matrices [3][6][3]: memory variables from the velocity
+    // gradients, the Jacobians of src/impl/seismic/setups/ViscoElasticity.cpp:233-316
+    double a[54];
+    for (int i = 0; i < 54; i++) a[i] = 0;
+    for (int d = 0; d < 3; d++) {
+      double* o = a + d * 18;
+      const double j0 = ji[0 * 3 + d], j1 = ji[1 * 3 + d], j2 = ji[2 * 3 + d];
+      o[0 * 3 + 0] -= j0;
+      o[3 * 3 + 1] -= 0.5 * j0;
+      o[5 * 3 + 2] -= 0.5 * j0;
+      o[1 * 3 + 1] -= j1;
+      o[3 * 3 + 0] -= 0.5 * j1;
+      o[4 * 3 + 2] -= 0.5 * j1;
+      o[2 * 3 + 2] -= j2;
+      o[4 * 3 + 1] -= 0.5 * j2;
+      o[5 * 3 + 0] -= 0.5 * j2;
+    }
+    for (int i = 0; i < 54; i++) star_a[el * 54 + i] = (T)a[i];
+  }
 }
 
 // quantities face-aligned (n, s, t) -> xyz and back, Voigt order xx yy zz xy yz xz
@@ -152,6 +172,53 @@ __device__ void face_solvers(double rL, double rR, double lL, double lR, double 
   mm9(tmp, Ti, fsR);
 }
 
+// Anelastic flux solvers [6][9] of both sides: the middle state's anelastic
+// part in face coordinates (memory variables of sigma_nn / sigma_ns /
+// sigma_nt driven by the P and S waves, src/impl/seismic/setups/
+// ViscoElasticity.cpp:376-441), rotated with the stress block T_A of T and
+// T^-1, the right state mirrored on a free surface (ViscoElasticity.cpp:
+// 505-570, solvers/AderDgInit.hpp:288-370).
+__device__ void anelastic_solvers(double rL, double rR, double lL, double lR, double mL, double mR, const double n[3],
+                                  const double s[3], const double t[3], bool free_surface, double fsAl[54],
+                                  double fsAr[54]) {
+  double T[81], Ti[81], mid[2][54], tmp[54];
+  trafo(n, s, t, T, Ti);
+  const double cpL = sqrt((lL + 2 * mL) / rL), cpR = sqrt((lR + 2 * mR) / rR);
+  const double csL = sqrt(mL / rL), csR = sqrt(mR / rR);
+  for (int i = 0; i < 54; i++) mid[0][i] = mid[1][i] = 0;
+  const double dp = lL * cpR + lR * cpL + 2 * mL * cpR + 2 * mR * cpL;
+  const double ds = 2 * mL * csR + 2 * mR * csL;
+  mid[0][0 * 9 + 0] = cpL * cpR / dp;
+  mid[0][0 * 9 + 6] = -cpR * (lL + 2 * mL) / dp;
+  mid[0][3 * 9 + 3] = mid[0][5 * 9 + 5] = csL * csR / ds;
+  mid[0][3 * 9 + 7] = mid[0][5 * 9 + 8] = -mL * csR / ds;
+  mid[1][0 * 9 + 0] = -cpL * cpR / dp;
+  mid[1][0 * 9 + 6] = -cpL * (lR + 2 * mR) / dp;
+  mid[1][3 * 9 + 3] = mid[1][5 * 9 + 5] = -csL * csR / ds;
+  mid[1][3 * 9 + 7] = mid[1][5 * 9 + 8] = -mR * csL / ds;
+  for (int sd = 0; sd < 2; sd++) {
+    for (int r = 0; r < 6; r++)
+      for (int c = 0; c < 9; c++) {
+        double v = 0;
+        for (int k = 0; k < 6; k++) v += T[r * 9 + k] * mid[sd][k * 9 + c];
+        tmp[r * 9 + c] = v;
+      }
+    if (sd == 1 && free_surface)
+      for (int r = 0; r < 6; r++) {
+        tmp[r * 9 + 0] = -tmp[r * 9 + 0];
+        tmp[r * 9 + 3] = -tmp[r * 9 + 3];
+        tmp[r * 9 + 5] = -tmp[r * 9 + 5];
+      }
+    double* o = sd == 0 ? fsAl : fsAr;
+    for (int r = 0; r < 6; r++)
+      for (int c = 0; c < 9; c++) {
+        double v = 0;
+        for (int k = 0; k < 9; k++) v += tmp[r * 9 + k] * Ti[k * 9 + c];
+        o[r * 9 + c] = v;
+      }
+  }
+}
+
 // Rank-3 factors of a flux solver F (9 x 9): F = U V^T with U = [u_0 u_1 u_2]
 // orthonormal (Gram-Schmidt with column pivoting; each new column is
 // re-orthogonalised against the earlier ones) and V^T = U^T F.  Writes U
@@ -220,7 +287,7 @@ template <typename T>
 __global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el, const int64_t* el_fa,
                                const int32_t* fa_type, const double* normals, const double* tangents,
                                const double* areas, const double* jdet, const double* rho, const double* lam,
-                               const double* mu, T* fs, T* fsf, int* bad) {
+                               const double* mu, T* fs, T* fsf, T* fs_a, int* bad) {
   const int64_t fa = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (fa >= n_fa) return;
   const int64_t eL = fa_el[fa * 2], eR = fa_el[fa * 2 + 1];
@@ -260,17 +327,36 @@ __global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el,
     }
   };
   emit(eL, fL, -2.0 * areas[fa] / jdet[eL]);
+  if (fs_a) {
+    double al[54], ar[54];
+    anelastic_solvers(rL, rR, lL, lR, mL, mR, n, s, t, fsurf, al, ar);
+    const double sc = -2.0 * areas[fa] / jdet[eL];
+    for (int i = 0; i < 54; i++) {
+      fs_a[(eL * 4 + fL) * 54 + i] = (T)(al[i] * sc);
+      fs_a[((n_el + eL) * 4 + fL) * 54 + i] = (T)(ar[i] * sc);
+    }
+  }
   if (eR >= 0) {
     const double nm[3] = {-n[0], -n[1], -n[2]};
     face_solvers(rR, rL, lR, lL, mR, mL, nm, s, t, false, a, b);
     emit(eR, fR, -2.0 * areas[fa] / jdet[eR]);
+    if (fs_a) {
+      double al[54], ar[54];
+      anelastic_solvers(rR, rL, lR, lL, mR, mL, nm, s, t, false, al, ar);
+      const double sc = -2.0 * areas[fa] / jdet[eR];
+      for (int i = 0; i < 54; i++) {
+        fs_a[(eR * 4 + fR) * 54 + i] = (T)(al[i] * sc);
+        fs_a[((n_el + eR) * 4 + fR) * 54 + i] = (T)(ar[i] * sc);
+      }
+    }
   }
 }
 
 }  // namespace
 
 template <typename T>
-int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_fsf, bool* rank3, hipStream_t st) {
+int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_fsf, bool* rank3, T* d_star_a,
+                T* d_fs_a, hipStream_t st) {
   double *d_crds = nullptr, *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr, *d_nrm = nullptr,
          *d_tan = nullptr, *d_area = nullptr;
   int64_t *d_elve = nullptr, *d_fael = nullptr, *d_elfa = nullptr;
@@ -317,12 +403,13 @@ int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_f
   EDGE_SETUP_TRY(hipMemsetAsync(d_bad, 0, sizeof(int), st));
   // sentinel values: every (element, face) slot must be written by its face
   EDGE_SETUP_TRY(hipMemsetAsync(d_fs, 0xFF, sizeof(T) * 2 * 4 * 81 * ne, st));
+  if (d_fs_a) EDGE_SETUP_TRY(hipMemsetAsync(d_fs_a, 0xFF, sizeof(T) * 2 * 4 * 54 * ne, st));
   const int bs = 128;
   hipLaunchKernelGGL(k_star<T>, dim3((unsigned)((ne + bs - 1) / bs)), dim3(bs), 0, st, ne, d_crds, d_elve, d_rho,
-                     d_lam, d_mu, d_star, d_jdet, d_bad);
+                     d_lam, d_mu, d_star, d_jdet, d_star_a, d_bad);
   EDGE_SETUP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_flux_solvers<T>, dim3((unsigned)((nf + bs - 1) / bs)), dim3(bs), 0, st, nf, ne, d_fael,
-                     d_elfa, d_type, d_nrm, d_tan, d_area, d_jdet, d_rho, d_lam, d_mu, d_fs, d_fsf, d_bad);
+                     d_elfa, d_type, d_nrm, d_tan, d_area, d_jdet, d_rho, d_lam, d_mu, d_fs, d_fsf, d_fs_a, d_bad);
   EDGE_SETUP_TRY(hipGetLastError());
   int bad = 0;
   EDGE_SETUP_TRY(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -336,8 +423,10 @@ int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_f
   return rc;
 }
 
-template int build_setup<double>(const SetupInput&, double*, double*, double*, double*, bool*, hipStream_t);
-template int build_setup<float>(const SetupInput&, float*, float*, double*, float*, bool*, hipStream_t);
+template int build_setup<double>(const SetupInput&, double*, double*, double*, double*, bool*, double*, double*,
+                                  hipStream_t);
+template int build_setup<float>(const SetupInput&, float*, float*, double*, float*, bool*, float*, float*,
+                                 hipStream_t);
 
 template <typename T>
 int factor_flux_solvers(const double* h_fs, int64_t n_mats, T* d_fsf, bool* rank3) {

@@ -12,6 +12,7 @@
 #include "kernels_mfma.hpp"
 #include "mfma_pack.hpp"
 #include "setup.hpp"
+#include "visco.hpp"
 
 struct edge_el_solver {
   edge_el_config cfg;
@@ -38,6 +39,13 @@ struct edge_el_solver {
   uint8_t* snd_fm = nullptr; // [nEl][4]
   int32_t* rcv = nullptr;    // [nEl][4]
   bool proj_ok = false;      // every projection slot feeds at most one element face
+  // viscoelastic solvers
+  int n_rms = 0;
+  void* rfs = nullptr;       // [n_rms]
+  void* star_a = nullptr;    // [nEl][54]
+  void* src_a = nullptr;     // [nEl][n_rms][36]
+  void* fs_a = nullptr;      // [2][nEl][4][54]
+  std::vector<double> lam_e, mu_e, h_rfs;
   // graph path
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -73,6 +81,11 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.snd_v = s->snd_v;
   d.snd_fm = s->snd_fm;
   d.rcv = s->rcv;
+  d.n_rms = s->n_rms;
+  d.rfs = static_cast<const T*>(s->rfs);
+  d.star_a = static_cast<const T*>(s->star_a);
+  d.src_a = static_cast<const T*>(s->src_a);
+  d.fs_a = static_cast<const T*>(s->fs_a);
   return d;
 }
 
@@ -101,7 +114,8 @@ int check_cfg(const edge_el_config* c) {
   if (!c) return edge::fail(EDGE_ERR_ARG, "config missing");
   if (c->order < edge::kMinOrder || c->order > edge::kMaxOrder)
     return edge::fail(EDGE_ERR_UNSUPPORTED, "order must be in 2..5");
-  if (c->n_runs != 1 && c->n_runs != 16) return edge::fail(EDGE_ERR_UNSUPPORTED, "n_runs must be 1 or 16");
+  if (c->n_runs != 1 && c->n_runs != 8 && c->n_runs != 16)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "n_runs must be 1, 8 or 16");
   if (c->precision != 64 && c->precision != 32) return edge::fail(EDGE_ERR_UNSUPPORTED, "precision must be 32 or 64");
   if (c->kernel < 0 || c->kernel > 3) return edge::fail(EDGE_ERR_ARG, "kernel selector");
   return EDGE_OK;
@@ -231,6 +245,7 @@ int create_common(const edge_el_config* cfg, const edge_mesh_in* mesh, edge_el_s
 
 template <typename T>
 int local_t(edge_el_solver* s, int64_t first, int64_t n, double dt, void* dofs, void* t_int, hipStream_t st) {
+  if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_local_a)");
   const auto d = dev_view<T>(s);
   if (s->kernel == 2) return edge::launch_local_mfma<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
   return edge::launch_local_generic<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
@@ -238,6 +253,7 @@ int local_t(edge_el_solver* s, int64_t first, int64_t n, double dt, void* dofs, 
 
 template <typename T>
 int neigh_t(edge_el_solver* s, int64_t first, int64_t n, const void* t_int, void* dofs, hipStream_t st) {
+  if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_neigh_a)");
   const auto d = dev_view<T>(s);
   if (s->kernel == 2) return edge::launch_neigh_mfma<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
   return edge::launch_neigh_generic<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
@@ -289,9 +305,9 @@ extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mes
     else
       rc = cfg->precision == 64
                ? edge::build_setup<double>(in, (double*)s->star, (double*)s->fs, s->jdet, (double*)s->fsf,
-                                           &s->fs_rank3, nullptr)
+                                           &s->fs_rank3, nullptr, nullptr, nullptr)
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
-                                          &s->fs_rank3, nullptr);
+                                          &s->fs_rank3, nullptr, nullptr, nullptr);
     if (rc == EDGE_OK) rc = finish_mfma(s);
   }
   if (rc != EDGE_OK) {
@@ -356,6 +372,10 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   hipFree(s->snd_v);
   hipFree(s->snd_fm);
   hipFree(s->rcv);
+  hipFree(s->rfs);
+  hipFree(s->star_a);
+  hipFree(s->src_a);
+  hipFree(s->fs_a);
   delete s;
   return EDGE_OK;
 }
@@ -382,12 +402,148 @@ extern "C" int edge_el_get_setup(const edge_el_solver* s, double* star, double* 
   return EDGE_OK;
 }
 
+namespace {
+
+template <typename T>
+int upload_as(void** dst, const std::vector<double>& h) {
+  std::vector<T> v(h.begin(), h.end());
+  EDGE_HIP_TRY(hipMalloc(dst, sizeof(T) * v.size()));
+  EDGE_HIP_TRY(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return EDGE_OK;
+}
+
+template <typename T>
+int download_as(double* dst, const void* src, size_t n) {
+  std::vector<T> v(n);
+  EDGE_HIP_TRY(hipMemcpy(v.data(), src, sizeof(T) * n, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; i++) dst[i] = v[i];
+  return EDGE_OK;
+}
+
+}  // namespace
+
+extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_config* vc, const edge_mesh_in* mesh,
+                                    const double* rho, const double* lam, const double* mu, const double* qp,
+                                    const double* qs, edge_el_solver** out) {
+  if (out) *out = nullptr;
+  if (!cfg || !vc) return edge::fail(EDGE_ERR_ARG, "configuration missing");
+  if (vc->n_rms < 1 || vc->n_rms > edge::kMaxMechs)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "relaxation mechanisms must be 1..5");
+  if (!(vc->freq_cen > 0) || !(vc->freq_rat > 1))
+    return edge::fail(EDGE_ERR_ARG, "central frequency must be positive and the frequency ratio above 1");
+  if (!mesh || !mesh->ve_crds || !mesh->el_ve || !mesh->normals || !mesh->tangents || !mesh->areas || !rho ||
+      !lam || !mu || !qp || !qs)
+    return edge::fail(EDGE_ERR_ARG, "mesh geometry, material or quality factors missing");
+  if (cfg->kernel > 1) return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic solvers run the generic kernels");
+  if (!edge::visco_supported(cfg->order, cfg->n_runs, cfg->precision / 8, vc->n_rms))
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic kernels: 1 or 8 runs whose tiles fit LDS");
+  const int64_t ne = mesh->n_el;
+  for (int64_t el = 0; el < ne; el++)
+    if (!(rho[el] > 0) || !(mu[el] > 0) || !(lam[el] + 2 * mu[el] > 0) || !(qp[el] > 0) || !(qs[el] > 0))
+      return edge::fail(EDGE_ERR_ARG, "non-physical material parameters or quality factors");
+  const int nr = vc->n_rms;
+  // constant-Q fit per element (FP64, host)
+  double freqs[2 * edge::kMaxMechs];
+  edge::visco_frequencies(nr, vc->freq_cen, vc->freq_rat, freqs);
+  std::vector<double> lam_e(ne), mu_e(ne), src((size_t)ne * nr * 36), rfs(nr);
+  for (int rm = 0; rm < nr; rm++) rfs[rm] = freqs[2 * rm];
+  for (int64_t el = 0; el < ne; el++) {
+    edge::visco_element(nr, freqs, vc->freq_cen, qp[el], qs[el], lam[el], mu[el], &lam_e[el], &mu_e[el],
+                        src.data() + (size_t)el * nr * 36);
+    if (!(mu_e[el] > 0) || !(lam_e[el] + 2 * mu_e[el] > 0))
+      return edge::fail(EDGE_ERR_ARG, "constant-Q fit gives non-physical unrelaxed Lame parameters");
+  }
+  edge_el_config c = *cfg;
+  c.kernel = 1;
+  edge_el_solver* s = nullptr;
+  int rc = create_common(&c, mesh, out, &s);
+  if (rc == EDGE_OK) {
+    s->n_rms = nr;
+    s->lam_e = lam_e;
+    s->mu_e = mu_e;
+    s->h_rfs = rfs;
+    int64_t nve = 0;
+    for (int64_t i = 0; i < 4 * ne; i++) nve = std::max(nve, mesh->el_ve[i] + 1);
+    edge::SetupInput in{ne, mesh->n_fa, nve, mesh->ve_crds, mesh->el_ve, mesh->el_fa, mesh->fa_el, mesh->fa_type,
+                        mesh->normals, mesh->tangents, mesh->areas, rho, s->lam_e.data(), s->mu_e.data()};
+    s->sparse_star = true;
+    const size_t rb = s->real_bytes;
+    if (hipMalloc(&s->jdet, sizeof(double) * ne) != hipSuccess || hipMalloc(&s->star_a, rb * 54 * ne) != hipSuccess ||
+        hipMalloc(&s->fs_a, rb * 2 * 4 * 54 * ne) != hipSuccess)
+      rc = edge::fail(EDGE_ERR_HIP, "allocation of the viscoelastic set-up");
+    if (rc == EDGE_OK)
+      rc = cfg->precision == 64 ? upload_as<double>(&s->src_a, src) : upload_as<float>(&s->src_a, src);
+    if (rc == EDGE_OK)
+      rc = cfg->precision == 64 ? upload_as<double>(&s->rfs, rfs) : upload_as<float>(&s->rfs, rfs);
+    if (rc == EDGE_OK)
+      rc = cfg->precision == 64
+               ? edge::build_setup<double>(in, (double*)s->star, (double*)s->fs, s->jdet, nullptr, nullptr,
+                                           (double*)s->star_a, (double*)s->fs_a, nullptr)
+               : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, nullptr, nullptr,
+                                          (float*)s->star_a, (float*)s->fs_a, nullptr);
+  }
+  if (rc != EDGE_OK) {
+    if (s) edge_el_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_local_a(edge_el_solver* s, int64_t first, int64_t n_els, double dt, void* dofs, void* dofs_a,
+                               void* t_int, void* stream) {
+  int rc = check_range(s, first, n_els);
+  if (rc) return rc;
+  if (s->n_rms == 0) return edge_el_local(s, first, n_els, dt, dofs, t_int, stream);
+  if (!dofs || !dofs_a || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  hipStream_t st = (hipStream_t)stream;
+  if (s->cfg.precision == 64)
+    return edge::launch_local_visco<double>(dev_view<double>(s), first, n_els, dt, (double*)dofs, (double*)dofs_a,
+                                            (double*)t_int, st);
+  return edge::launch_local_visco<float>(dev_view<float>(s), first, n_els, (float)dt, (float*)dofs, (float*)dofs_a,
+                                         (float*)t_int, st);
+}
+
+extern "C" int edge_el_neigh_a(edge_el_solver* s, int64_t first, int64_t n_els, const void* t_int, void* dofs,
+                               void* dofs_a, void* stream) {
+  int rc = check_range(s, first, n_els);
+  if (rc) return rc;
+  if (s->n_rms == 0) return edge_el_neigh(s, first, n_els, t_int, dofs, stream);
+  if (!dofs || !dofs_a || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  hipStream_t st = (hipStream_t)stream;
+  if (s->cfg.precision == 64)
+    return edge::launch_neigh_visco<double>(dev_view<double>(s), first, n_els, (const double*)t_int, (double*)dofs,
+                                            (double*)dofs_a, st);
+  return edge::launch_neigh_visco<float>(dev_view<float>(s), first, n_els, (const float*)t_int, (float*)dofs,
+                                         (float*)dofs_a, st);
+}
+
+extern "C" int edge_el_get_setup_a(const edge_el_solver* s, double* star_a, double* src_a, double* fs_a, double* rfs,
+                                   double* lam_e, double* mu_e) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (s->n_rms == 0) return edge::fail(EDGE_ERR_ARG, "not a viscoelastic solver");
+  const size_t ne = s->n_el, nr = s->n_rms;
+  const bool d = s->cfg.precision == 64;
+  int rc = EDGE_OK;
+  if (star_a && rc == EDGE_OK) rc = d ? download_as<double>(star_a, s->star_a, 54 * ne) : download_as<float>(star_a, s->star_a, 54 * ne);
+  if (src_a && rc == EDGE_OK) rc = d ? download_as<double>(src_a, s->src_a, 36 * nr * ne) : download_as<float>(src_a, s->src_a, 36 * nr * ne);
+  if (fs_a && rc == EDGE_OK) rc = d ? download_as<double>(fs_a, s->fs_a, 432 * ne) : download_as<float>(fs_a, s->fs_a, 432 * ne);
+  if (rc) return rc;
+  if (rfs)
+    for (size_t i = 0; i < nr; i++) rfs[i] = s->h_rfs[i];
+  if (lam_e)
+    for (size_t i = 0; i < ne; i++) lam_e[i] = s->lam_e[i];
+  if (mu_e)
+    for (size_t i = 0; i < ne; i++) mu_e[i] = s->mu_e[i];
+  return EDGE_OK;
+}
+
 extern "C" int edge_el_kernel_info(const edge_el_solver* s, int32_t* kernel, int32_t* flags) {
   if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
   if (kernel) *kernel = s->kernel;
   if (flags) {
     const bool r3 = s->kernel == 2 && s->fs_rank3 && s->fsf;
-    *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (r3 && s->proj ? 4 : 0);
+    *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (r3 && s->proj ? 4 : 0) | (s->n_rms > 0 ? 8 : 0);
   }
   return EDGE_OK;
 }

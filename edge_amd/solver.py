@@ -13,7 +13,7 @@ from . import _lib
 from .setups import n_modes
 
 KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES = 0, 1, 2, 3
-PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS = 1, 2, 4
+PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS, PATH_VISCO = 1, 2, 4, 8
 
 
 def _stream_ptr(stream):
@@ -128,3 +128,69 @@ class ElasticSolver:
         _lib.check(_lib.lib().edge_el_l2_norms(self._h, a.data_ptr(), pa, None if b is None else b.data_ptr(),
                                                pb, d.ctypes.data, n.ctypes.data, _stream_ptr(stream)))
         return d, n
+
+
+class ViscoSolver(ElasticSolver):
+    """Viscoelastic ADER-DG update (N_RMS relaxation mechanisms) on one GPU:
+    AderDg with TL_N_RMS > 0.  Built from phase-velocity Lame parameters and
+    P / S quality factors per element; the constant-Q fit over the band
+    (freq_cen, freq_rat) gives the unrelaxed Lame parameters and the
+    anelastic source matrices.  Anelastic DOFs: [nEl][n_rms][6][N][R]."""
+
+    def __init__(self, mesh, order, n_runs, precision, rho, lam, mu, qp, qs, n_rms=3, freq_cen=1.0,
+                 freq_rat=10.0, kernel=KERNEL_AUTO):
+        self.mesh = mesh
+        self.order = order
+        self.n_runs = n_runs
+        self.precision = precision
+        self.n_modes = n_modes(order)
+        self.n_rms = n_rms
+        cfg = _lib.ElConfig(order, n_runs, precision, kernel)
+        vc = _lib.ViscoConfig(n_rms, freq_cen, freq_rat)
+        mi = mesh.mesh_in()
+        h = C.c_void_p()
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, lam, mu, qp, qs)]
+        _lib.check(_lib.lib().edge_el_create_visco(C.byref(cfg), C.byref(vc), C.byref(mi),
+                                                   *[a.ctypes.data for a in arrs], C.byref(h)))
+        self._h = h
+
+    def dof_a_shape(self):
+        return (self.mesh.n_el, self.n_rms, 6, self.n_modes, self.n_runs)
+
+    def alloc_a(self, device="cuda"):
+        import torch
+        return torch.zeros(self.dof_a_shape(), dtype=self.dtype, device=device)
+
+    def setup_a(self):
+        """dict of the anelastic set-up as FP64 host arrays."""
+        ne, nr = self.mesh.n_el, self.n_rms
+        out = {"starA": np.zeros((ne, 3, 18)), "srcA": np.zeros((ne, nr, 36)), "fsA": np.zeros((2, ne, 4, 54)),
+               "rfs": np.zeros(nr), "lam_e": np.zeros(ne), "mu_e": np.zeros(ne)}
+        _lib.check(_lib.lib().edge_el_get_setup_a(self._h, *[out[k].ctypes.data for k in
+                                                             ("starA", "srcA", "fsA", "rfs", "lam_e", "mu_e")]))
+        return out
+
+    def _check_a(self, a):
+        if tuple(a.shape) != self.dof_a_shape() or a.dtype != self.dtype or not a.is_cuda or not a.is_contiguous():
+            raise ValueError(f"expected a contiguous {self.dtype} device tensor of shape {self.dof_a_shape()}")
+
+    def local(self, dofs, t_int, dt, dofs_a=None, first=0, n=None, stream=None):
+        self._check_arr(dofs)
+        self._check_arr(t_int)
+        self._check_a(dofs_a)
+        n = self.mesh.n_el - first if n is None else n
+        _lib.check(_lib.lib().edge_el_local_a(self._h, first, n, float(dt), dofs.data_ptr(), dofs_a.data_ptr(),
+                                              t_int.data_ptr(), _stream_ptr(stream)))
+
+    def neigh(self, dofs, t_int, dofs_a=None, first=0, n=None, stream=None):
+        self._check_arr(dofs)
+        self._check_arr(t_int)
+        self._check_a(dofs_a)
+        n = self.mesh.n_el - first if n is None else n
+        _lib.check(_lib.lib().edge_el_neigh_a(self._h, first, n, t_int.data_ptr(), dofs.data_ptr(),
+                                              dofs_a.data_ptr(), _stream_ptr(stream)))
+
+    def step(self, dofs, t_int, dt, n_steps=1, dofs_a=None, stream=None):
+        for _ in range(n_steps):
+            self.local(dofs, t_int, dt, dofs_a, stream=stream)
+            self.neigh(dofs, t_int, dofs_a, stream=stream)

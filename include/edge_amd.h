@@ -113,7 +113,7 @@ typedef struct edge_el_solver edge_el_solver;
 
 typedef struct {
   int32_t order;     /* O: 2..5 */
-  int32_t n_runs;    /* fused runs R (innermost DOF axis): 1 or 16 */
+  int32_t n_runs;    /* fused runs R (innermost DOF axis): 1, 8 or 16 */
   int32_t precision; /* 64 or 32 (arithmetic and storage type of DOFs) */
   int32_t kernel;    /* 0 = pick automatically, 1 = generic, 2 = MFMA-tiled,
                         3 = MFMA-tiled with the neighbour step reading tInt tiles */
@@ -182,9 +182,47 @@ int edge_el_step(edge_el_solver *s, int64_t n_steps, double dt, void *dofs, void
 int edge_el_info(const edge_el_solver *s, int64_t *n_el, int32_t *n_modes, int32_t *n_runs,
                  int64_t *dof_bytes);
 
+/* ====================================================================== */
+/* Viscoelastic (attenuating) solvers: AderDg with TL_N_RMS > 0            */
+/* (src/impl/seismic/solvers/AderDg.hpp:225-312, :338-534, :553-722) with  */
+/* the set-up of AderDgInit::initSrcA / initStar / initFs                  */
+/* (AderDgInit.hpp:91-117, :129-174, :394-665) and the constant-Q fit of   */
+/* src/impl/seismic/setups/ViscoElasticity.{h,cpp}.                        */
+/* ====================================================================== */
+typedef struct {
+  int32_t n_rms;   /* relaxation mechanisms, 1..5 */
+  double freq_cen; /* central frequency of the constant-Q band */
+  double freq_rat; /* ratio of the band's highest to its lowest frequency */
+} edge_visco_config;
+
+/* Builds a viscoelastic solver from phase-velocity Lame parameters and
+ * quality factors per element (host arrays [nEl]).  The fit replaces lam /
+ * mu by the unrelaxed ones for the elastic star matrices and flux solvers.
+ * Runs 1 or 8 fused runs (the generic kernels). */
+int edge_el_create_visco(const edge_el_config *cfg, const edge_visco_config *vc, const edge_mesh_in *mesh,
+                         const double *rho, const double *lam, const double *mu, const double *qp, const double *qs,
+                         edge_el_solver **out);
+
+/* Local / neighbour steps with the anelastic DOFs dofs_a [nEl][n_rms][6][mode][run]
+ * (device, the solver's precision) beside the elastic ones; as
+ * edge_el_local / edge_el_neigh otherwise.  dofs_a may be NULL for an
+ * elastic solver. */
+int edge_el_local_a(edge_el_solver *s, int64_t first, int64_t n_els, double dt, void *dofs, void *dofs_a,
+                    void *t_int, void *stream);
+int edge_el_neigh_a(edge_el_solver *s, int64_t first, int64_t n_els, const void *t_int, void *dofs, void *dofs_a,
+                    void *stream);
+
+/* Copies a viscoelastic solver's set-up to host FP64 arrays (any may be
+ * NULL): anelastic star matrices [nEl][3][6][3], source matrices
+ * [nEl][n_rms][6][6], anelastic flux solvers [2][nEl][4][6][9], relaxation
+ * frequencies [n_rms], unrelaxed Lame parameters [nEl]. */
+int edge_el_get_setup_a(const edge_el_solver *s, double *star_a, double *src_a, double *fs_a, double *rfs,
+                        double *lam_e, double *mu_e);
+
 /* Which kernels a solver runs: *kernel = 1 (generic) or 2 (MFMA-tiled);
  * *flags bit 0: sparse (elastic-pattern) star matrices, bit 1: rank-3
- * factored flux solvers, bit 2: the neighbour step reads face projections.
+ * factored flux solvers, bit 2: the neighbour step reads face projections,
+ * bit 3: viscoelastic (anelastic DOFs).
  * Diagnostic with no counterpart in the reference (for tests and tools). */
 int edge_el_kernel_info(const edge_el_solver *s, int32_t *kernel, int32_t *flags);
 

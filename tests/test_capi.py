@@ -51,3 +51,32 @@ def test_solver_arguments_checked_before_device_use():
                           rho.ctypes.data, C.byref(h))
     assert rc == 1 and b"material" in L.edge_last_error()
     assert L.edge_el_local(None, 0, 1, 0.1, None, None, None) == 1
+
+
+def test_visco_create_rejects_bad_arguments():
+    """Argument checks of edge_el_create_visco that return before any device work."""
+    import ctypes as C
+    import numpy as np
+    from edge_amd import _lib
+    from edge_amd import mesh as M
+    m = M.box(3)
+    ne = m.n_el
+    one = np.ones(ne)
+    good = [np.full(ne, 2.0), np.full(ne, 3.0), np.full(ne, 1.5), np.full(ne, 50.0), np.full(ne, 25.0)]
+    L = _lib.lib()
+
+    def create(order=4, runs=1, prec=64, n_rms=3, fc=1.0, fr=10.0, arrs=good):
+        cfg = _lib.ElConfig(order, runs, prec, 0)
+        vc = _lib.ViscoConfig(n_rms, fc, fr)
+        h = C.c_void_p()
+        rc = L.edge_el_create_visco(C.byref(cfg), C.byref(vc), C.byref(m.mesh_in()), *[a.ctypes.data for a in arrs],
+                                    C.byref(h))
+        return rc, L.edge_last_error().decode()
+
+    assert create(n_rms=0)[0] != _lib.EDGE_OK
+    assert create(n_rms=6)[0] != _lib.EDGE_OK
+    assert create(fr=1.0)[0] != _lib.EDGE_OK
+    assert create(fc=0.0)[0] != _lib.EDGE_OK
+    assert create(runs=16, prec=32)[0] != _lib.EDGE_OK  # the anelastic tiles of 16 runs do not fit LDS
+    rc, msg = create(arrs=good[:3] + [-one, one])
+    assert rc != _lib.EDGE_OK and "quality" in msg
