@@ -46,6 +46,10 @@ class ViscoConfig(C.Structure):
     _fields_ = [("n_rms", C.c_int32), ("freq_cen", C.c_double), ("freq_rat", C.c_double)]
 
 
+class RuptureConfig(C.Structure):
+    _fields_ = [("n_sub_faces", C.c_int32), ("n_sub_cells", C.c_int32), ("n_runs", C.c_int32)]
+
+
 # symbol -> (restype, argtypes)
 _SIGS = {
     "edge_last_error": (C.c_char_p, []),
@@ -77,6 +81,11 @@ _SIGS = {
     "edge_el_neigh_a": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "edge_el_get_setup_a": (C.c_int, [C.c_void_p] + [C.c_void_p] * 6),
+    "edge_rupture_create": (C.c_int, [C.POINTER(RuptureConfig), C.c_int64] + [C.c_void_p] * 10 +
+                            [C.POINTER(C.c_void_p)]),
+    "edge_rupture_destroy": (C.c_int, [C.c_void_p]),
+    "edge_rupture_net_updates": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double] + [C.c_void_p] * 5),
+    "edge_rupture_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "edge_el_kernel_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edge_el_l2_norms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),

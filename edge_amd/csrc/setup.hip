@@ -12,6 +12,7 @@
 // FlMid / FrMid are derived here from the welded-interface Riemann problem.
 #include <hip/hip_runtime.h>
 
+#include "face_frame.hpp"
 #include "setup.hpp"
 
 namespace edge {
@@ -87,30 +88,6 @@ __global__ void k_star(int64_t n_el, const double* crds, const int64_t* el_ve, c
   }
 }
 
-// quantities face-aligned (n, s, t) -> xyz and back, Voigt order xx yy zz xy yz xz
-__device__ void trafo(const double n[3], const double s[3], const double t[3], double T[81], double Ti[81]) {
-  const double* f[3] = {n, s, t};
-  const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
-  for (int i = 0; i < 81; i++) T[i] = Ti[i] = 0;
-  for (int r = 0; r < 6; r++)
-    for (int c = 0; c < 6; c++) {
-      const int i = vi[r][0], j = vi[r][1], a = vi[c][0], b = vi[c][1];
-      double v = f[a][i] * f[b][j];
-      if (a != b) v += f[b][i] * f[a][j];
-      T[r * 9 + c] = v;
-      // inverse: face-aligned (a', b') = vi[r] from xyz (i', j') = vi[c]
-      const int ap = vi[r][0], bp = vi[r][1], ip = vi[c][0], jp = vi[c][1];
-      double w = f[ap][ip] * f[bp][jp];
-      if (ip != jp) w += f[ap][jp] * f[bp][ip];
-      Ti[r * 9 + c] = w;
-    }
-  for (int i = 0; i < 3; i++)
-    for (int a = 0; a < 3; a++) {
-      T[(6 + i) * 9 + 6 + a] = f[a][i];
-      Ti[(6 + a) * 9 + 6 + i] = f[a][i];
-    }
-}
-
 // derivative of the left element's flux A_L q* w.r.t. the left (fl) and the
 // right (fr) face-aligned states; q* is the welded-interface middle state
 __device__ void fs_mid(double rhoL, double rhoR, double lamL, double lamR, double muL, double muR,
@@ -158,7 +135,7 @@ __device__ void face_solvers(double rL, double rR, double lL, double lR, double 
                              const double s[3], const double t[3], bool free_surface, double fsL[81],
                              double fsR[81]) {
   double T[81], Ti[81], fl[81], fr[81], tmp[81];
-  trafo(n, s, t, T, Ti);
+  face_trafo(n, s, t, T, Ti);
   fs_mid(rL, rR, lL, lR, mL, mR, fl, fr);
   mm9(T, fl, tmp);
   mm9(tmp, Ti, fsL);
@@ -182,7 +159,7 @@ __device__ void anelastic_solvers(double rL, double rR, double lL, double lR, do
                                   const double s[3], const double t[3], bool free_surface, double fsAl[54],
                                   double fsAr[54]) {
   double T[81], Ti[81], mid[2][54], tmp[54];
-  trafo(n, s, t, T, Ti);
+  face_trafo(n, s, t, T, Ti);
   const double cpL = sqrt((lL + 2 * mL) / rL), cpR = sqrt((lR + 2 * mR) / rR);
   const double csL = sqrt(mL / rL), csR = sqrt(mR / rR);
   for (int i = 0; i < 54; i++) mid[0][i] = mid[1][i] = 0;

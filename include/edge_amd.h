@@ -219,6 +219,49 @@ int edge_el_neigh_a(edge_el_solver *s, int64_t first, int64_t n_els, const void 
 int edge_el_get_setup_a(const edge_el_solver *s, double *star_a, double *src_a, double *fs_a, double *rfs,
                         double *lam_e, double *mu_e);
 
+/* ====================================================================== */
+/* Dynamic rupture: the internal-boundary net updates of fault faces with  */
+/* linear slip-weakening friction.  Replaces                               */
+/* InternalBoundarySolvers<TET4,O,3>::init (src/impl/seismic/solvers/      */
+/* InternalBoundary.hpp:743-923), the face data of RuptureInit             */
+/* (src/impl/seismic/setups/RuptureInit.hpp:170-206) and, per time step,   */
+/* InternalBoundary::netUpdates (InternalBoundary.hpp:184-280) with        */
+/* FrictionLaws<3,R>::linSlipWeak (FrictionLaws.hpp:309-456).  FP64.       */
+/* ====================================================================== */
+typedef struct edge_rupture edge_rupture;
+
+typedef struct {
+  int32_t n_sub_faces; /* sub-faces per fault face (n_sf) */
+  int32_t n_sub_cells; /* sub-cells per element (scales the flux solvers) */
+  int32_t n_runs;      /* fused runs R */
+} edge_rupture_config;
+
+/* Per fault face (host arrays): outward normal [nF][3] and tangents
+ * [nF][2][3] w.r.t. the left element, area [nF], the fault frame [3][3]
+ * (normal, strike, dip; unit vectors) that orients every face, density /
+ * lambda / mu of the left and right elements [nF][3], their volumes [nF];
+ * friction parameters [3][R] (static and dynamic coefficient, 1/Dc) and the
+ * initial sub-face state [nF][n_sf][R][14]: sn0, co0, ss0[2], ss0A, muf, st,
+ * tr[2], sr[2], dd[2] (t_LinSlipWeakSubFace, FrictionLaws.type:58-148). */
+int edge_rupture_create(const edge_rupture_config *cfg, int64_t n_faces, const double *normals,
+                        const double *tangents, const double *areas, const double *fault_frame, const double *mat_l,
+                        const double *mat_r, const double *vol_l, const double *vol_r, const double *glob,
+                        const double *state0, edge_rupture **out);
+int edge_rupture_destroy(edge_rupture *r);
+
+/* Net updates of faces [first, first+n) for the sub-cell DOFs of both sides
+ * dofs_l / dofs_r [nF][9][n_sf][R] (device): net_l / net_r [nF][9][n_sf][R]
+ * (device; the left-going updates negative, as the reference adds both).
+ * Advances the friction state (slip, slip rate, traction, friction
+ * coefficient) with the time step dt. */
+int edge_rupture_net_updates(edge_rupture *r, int64_t first, int64_t n_faces, double dt, const double *dofs_l,
+                             const double *dofs_r, double *net_l, double *net_r, void *stream);
+
+/* Copies the face solvers [nF][4][9][9] (T^-1, jump, left and right flux),
+ * the sub-face state [nF][n_sf][R][14] and the failure flags [nF][n_sf][R]
+ * to the host (any may be NULL). */
+int edge_rupture_get(const edge_rupture *r, double *solvers, double *state, uint8_t *per);
+
 /* Which kernels a solver runs: *kernel = 1 (generic) or 2 (MFMA-tiled);
  * *flags bit 0: sparse (elastic-pattern) star matrices, bit 1: rank-3
  * factored flux solvers, bit 2: the neighbour step reads face projections,

@@ -719,6 +719,238 @@ void orc_init_fs_a(int64_t n_fas, const int64_t *fa_el, const int64_t *el_fa, co
   }
 }
 
+/* -------------------------------------------------------------------------
+ * Dynamic rupture: linear slip-weakening friction
+ * (src/impl/seismic/solvers/FrictionLaws.hpp:309-456) and the sub-face net
+ * updates of the internal boundary (InternalBoundary.hpp:184-280) with the
+ * face solvers of InternalBoundarySolvers<TET4,O,3>::init (:743-923).
+ * ---------------------------------------------------------------------- */
+/* Middle state of the left side from the jump qR - qL in face coordinates:
+ * ms = qL + J (qR - qL), the waves travelling into the left medium
+ * (P: sigma_nn, u_n; S: sigma_ns / sigma_nt, u_s / u_t; the tangential
+ * normal stresses follow the P wave), as generated/MiddleStateJumpL3D.inc. */
+void orc_ms_jump_l(double lamL, double lamR, double muL, double muR, double rhoL, double rhoR, double J[81]) {
+  const double zpL = sqrt((lamL + 2 * muL) * rhoL), zpR = sqrt((lamR + 2 * muR) * rhoR);
+  const double zsL = sqrt(muL * rhoL), zsR = sqrt(muR * rhoR);
+  const double dp = zpL + zpR, ds = zsL + zsR, lr = lamL / (lamL + 2 * muL);
+  for (int i = 0; i < 81; i++) J[i] = 0;
+  J[0 * 9 + 0] = zpL / dp;
+  J[0 * 9 + 6] = zpL * zpR / dp;
+  for (int r = 1; r < 3; r++) {
+    J[r * 9 + 0] = lr * J[0];
+    J[r * 9 + 6] = lr * J[6];
+  }
+  J[3 * 9 + 3] = J[5 * 9 + 5] = zsL / ds;
+  J[3 * 9 + 7] = J[5 * 9 + 8] = zsL * zsR / ds;
+  J[6 * 9 + 0] = 1 / dp;
+  J[6 * 9 + 6] = zpR / dp;
+  J[7 * 9 + 3] = J[8 * 9 + 5] = 1 / ds;
+  J[7 * 9 + 7] = J[8 * 9 + 8] = zsR / ds;
+}
+
+/* Middle state of the right side: ms = qR + J (qR - qL), the waves
+ * travelling into the right medium, as generated/MiddleStateJumpR3D.inc. */
+void orc_ms_jump_r(double lamL, double lamR, double muL, double muR, double rhoL, double rhoR, double J[81]) {
+  const double zpL = sqrt((lamL + 2 * muL) * rhoL), zpR = sqrt((lamR + 2 * muR) * rhoR);
+  const double zsL = sqrt(muL * rhoL), zsR = sqrt(muR * rhoR);
+  const double dp = zpL + zpR, ds = zsL + zsR, lr = lamR / (lamR + 2 * muR);
+  for (int i = 0; i < 81; i++) J[i] = 0;
+  J[0 * 9 + 0] = -zpR / dp;
+  J[0 * 9 + 6] = zpL * zpR / dp;
+  for (int r = 1; r < 3; r++) {
+    J[r * 9 + 0] = lr * J[0];
+    J[r * 9 + 6] = lr * J[6];
+  }
+  J[3 * 9 + 3] = J[5 * 9 + 5] = -zsR / ds;
+  J[3 * 9 + 7] = J[5 * 9 + 8] = zsL * zsR / ds;
+  J[6 * 9 + 0] = 1 / dp;
+  J[6 * 9 + 6] = -zpL / dp;
+  J[7 * 9 + 3] = J[8 * 9 + 5] = 1 / ds;
+  J[7 * 9 + 7] = J[8 * 9 + 8] = -zsL / ds;
+}
+
+/* Flux of a face-coordinate middle state in physical coordinates: T A_n,
+ * A_n the normal Jacobian in face coordinates, as
+ * generated/MiddleStateFlux3D.inc. */
+void orc_ms_flux(const double n[3], const double s[3], const double t[3], double lam, double mu, double rho,
+                 double F[81]) {
+  double T[81], Ti[81], A[81];
+  orc_trafo(n, s, t, T, Ti);
+  for (int i = 0; i < 81; i++) A[i] = 0;
+  A[0 * 9 + 6] = -(lam + 2 * mu);
+  A[1 * 9 + 6] = A[2 * 9 + 6] = -lam;
+  A[3 * 9 + 7] = A[5 * 9 + 8] = -mu;
+  A[6 * 9 + 0] = A[7 * 9 + 3] = A[8 * 9 + 5] = -1 / rho;
+  mm9(T, A, F);
+}
+
+/* Rotation taking the direction of v0 to that of v1 (Rodrigues),
+ * src/linalg/Geom.hpp:939-989. */
+void orc_rot_mat(const double v0[3], const double v1[3], double R[9]) {
+  double a[3], b[3], na = 0, nb = 0;
+  for (int i = 0; i < 3; i++) {
+    na += v0[i] * v0[i];
+    nb += v1[i] * v1[i];
+  }
+  na = sqrt(na);
+  nb = sqrt(nb);
+  for (int i = 0; i < 3; i++) {
+    a[i] = v0[i] / na;
+    b[i] = v1[i] / nb;
+  }
+  const double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double sm[9] = {0, -c[2], c[1], c[2], 0, -c[0], -c[1], c[0], 0};
+  const double sca = 1.0 / (1.0 + a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = 0;
+      for (int k = 0; k < 3; k++) v += sm[i * 3 + k] * sm[k * 3 + j];
+      R[i * 3 + j] = v * sca + sm[i * 3 + j] + (i == j ? 1.0 : 0.0);
+    }
+}
+
+/* Solvers of one rupture face [4][81]: T^-1 of the face frame, the middle
+ * state jump solver, the flux solvers of the left and right elements scaled
+ * by (area / n_sf) / (volume / n_sc); with a boundary frame bnd [3][3] the
+ * face frame follows the boundary's normal orientation and tangents
+ * (InternalBoundary.hpp:743-923).  Returns 1 if the face normal points in
+ * the boundary's preferred direction. */
+int orc_ib_solvers(const double normal[3], const double tangent0[3], const double tangent1[3],
+                   const double *bnd /*[3][3] or NULL*/, double rhoL, double lamL, double muL, double rhoR,
+                   double lamR, double muR, double area, double volL, double volR, int n_sf, int n_sc,
+                   double out[4 * 81]) {
+  double f[3][3];
+  for (int d = 0; d < 3; d++) {
+    f[0][d] = normal[d];
+    f[1][d] = tangent0[d];
+    f[2][d] = tangent1[d];
+  }
+  int pref = 1;
+  if (bnd) {
+    pref = (bnd[0] * f[0][0] + bnd[1] * f[0][1] + bnd[2] * f[0][2]) > 0;
+    if (!pref)
+      for (int d = 0; d < 3; d++) f[0][d] = -f[0][d];
+    double R[9];
+    orc_rot_mat(bnd, f[0], R);
+    for (int k = 1; k < 3; k++)
+      for (int i = 0; i < 3; i++) {
+        double v = 0;
+        for (int j = 0; j < 3; j++) v += R[i * 3 + j] * bnd[k * 3 + j];
+        f[k][i] = v;
+      }
+  }
+  double T[81];
+  orc_trafo(f[0], f[1], f[2], T, out);
+  if (pref) {
+    orc_ms_jump_l(lamL, lamR, muL, muR, rhoL, rhoR, out + 81);
+  } else {
+    orc_ms_jump_r(lamR, lamL, muR, muL, rhoR, rhoL, out + 81);
+    for (int i = 0; i < 81; i++) out[81 + i] = -out[81 + i];
+  }
+  orc_ms_flux(f[0], f[1], f[2], lamL, muL, rhoL, out + 162);
+  orc_ms_flux(f[0], f[1], f[2], lamR, muR, rhoR, out + 243);
+  for (int sd = 0; sd < 2; sd++) {
+    double sca = (area / n_sf) / ((sd == 0 ? volL : volR) / n_sc);
+    if (!pref) sca = -sca;
+    for (int i = 0; i < 81; i++) out[(2 + sd) * 81 + i] *= sca;
+  }
+  return pref;
+}
+
+/* Linear slip weakening for one sub-face and run: strength, failure,
+ * perturbed minus / plus middle states, slip rate, slip and friction
+ * coefficient (FrictionLaws.hpp:309-392).  state: sn0, co0, ss0[2], ss0A,
+ * muf, st, tr[2], sr[2], dd[2] (14 values). */
+void orc_lsw(double dt, double csDmuM, double csDmuP, double mus, double mud, double dcInv, double *st_,
+             const double ms[9], double msM[9], double msP[9], int *per) {
+  double *sn0 = st_ + 0, *co0 = st_ + 1, *ss0 = st_ + 2, *muf = st_ + 5, *st = st_ + 6, *tr = st_ + 7,
+         *sr = st_ + 9, *dd = st_ + 11;
+  for (int q = 0; q < 9; q++) msM[q] = msP[q] = ms[q];
+  double strength = *muf * (*sn0 + ms[0]) + *co0;
+  strength = strength < 0 ? -strength : 0; /* only compression holds the fault */
+  *st = strength;
+  const double s1 = ss0[0] + ms[3], s2 = ss0[1] + ms[5];
+  const double shear = sqrt(s1 * s1 + s2 * s2);
+  *per = shear > strength;
+  const double scale = strength / shear;
+  tr[0] = *per ? s1 * scale - ss0[0] : ms[3];
+  tr[1] = *per ? s2 * scale - ss0[1] : ms[5];
+  msM[3] = msP[3] = tr[0];
+  msM[5] = msP[5] = tr[1];
+  const double d1 = tr[0] - ms[3], d2 = tr[1] - ms[5];
+  msM[7] -= csDmuM * d1;
+  msM[8] -= csDmuM * d2;
+  msP[7] += csDmuP * d1;
+  msP[8] += csDmuP * d2;
+  sr[0] = msP[7] - msM[7];
+  sr[1] = msP[8] - msM[8];
+  dd[0] += fabs(sr[0]) * dt;
+  dd[1] += fabs(sr[1]) * dt;
+  const double dda = sqrt(dd[0] * dd[0] + dd[1] * dd[1]);
+  const double arg = mus - (mus - mud) * dcInv * dda;
+  *muf = arg > mud ? arg : mud;
+  tr[0] += ss0[0];
+  tr[1] += ss0[1];
+}
+
+/* Net updates of one rupture face for n_sf sub-faces and R runs
+ * (InternalBoundary.hpp:184-280 with FrictionLaws<3,R>::perturb):
+ * solv [4][81] from orc_ib_solvers, dofs [9][n_sf][R] of the two sides'
+ * face-adjacent sub-cells, l_eq_m: the left element is the minus side;
+ * glob [3][R]: mus, mud, 1/Dc; state [n_sf][R][14] (see orc_lsw). */
+void orc_net_updates(int n_sf, int R, const double *solv, const double *dofsL, const double *dofsR, double dt,
+                     int l_eq_m, double csDmuM, double csDmuP, const double *glob, double *state, double *netL,
+                     double *netR, int *per) {
+  const double *tm1 = solv, *J = solv + 81, *FL = solv + 162, *FR = solv + 243;
+  const size_t sz = (size_t)9 * n_sf * R;
+  double *rot = (double *)malloc(sizeof(double) * 2 * sz), *ms2 = (double *)malloc(sizeof(double) * 2 * sz);
+  for (int sd = 0; sd < 2; sd++) {
+    const double *d = sd == 0 ? dofsL : dofsR;
+    for (int q = 0; q < 9; q++)
+      for (int i = 0; i < n_sf * R; i++) {
+        double v = 0;
+        for (int p = 0; p < 9; p++) v += tm1[q * 9 + p] * d[p * n_sf * R + i];
+        rot[sd * sz + q * n_sf * R + i] = v;
+      }
+  }
+  for (int sf = 0; sf < n_sf; sf++)
+    for (int r = 0; r < R; r++) {
+      const int i = sf * R + r;
+      double qL[9], jmp[9], ms[9], msA[9], msB[9];
+      for (int q = 0; q < 9; q++) {
+        qL[q] = rot[q * n_sf * R + i];
+        jmp[q] = rot[sz + q * n_sf * R + i] - qL[q];
+      }
+      for (int q = 0; q < 9; q++) {
+        double v = qL[q];
+        for (int p = 0; p < 9; p++) v += J[q * 9 + p] * jmp[p];
+        ms[q] = v;
+      }
+      int pf;
+      orc_lsw(dt, csDmuM, csDmuP, glob[r], glob[R + r], glob[2 * R + r], state + (size_t)i * 14, ms, msA, msB,
+              &pf);
+      per[i] = pf;
+      /* minus / plus to left / right */
+      const double *mL = l_eq_m ? msA : msB, *mR = l_eq_m ? msB : msA;
+      for (int q = 0; q < 9; q++) {
+        ms2[q * n_sf * R + i] = -mL[q] * dt;
+        ms2[sz + q * n_sf * R + i] = mR[q] * dt;
+      }
+    }
+  for (int sd = 0; sd < 2; sd++) {
+    const double *F = sd == 0 ? FL : FR;
+    double *o = sd == 0 ? netL : netR;
+    for (int q = 0; q < 9; q++)
+      for (int i = 0; i < n_sf * R; i++) {
+        double v = 0;
+        for (int p = 0; p < 9; p++) v += F[q * 9 + p] * ms2[sd * sz + p * n_sf * R + i];
+        o[q * n_sf * R + i] = v;
+      }
+  }
+  free(rot);
+  free(ms2);
+}
+
 double orc_dt_cfl(double rho, double lam, double mu, double in_dia, int order) {
   return in_dia * 0.5 / sqrt((lam + 2 * mu) / rho) / (2.0 * order - 1);
 }

@@ -40,7 +40,8 @@ def lib():
                   "orc_neigh", "orc_star", "orc_fs", "orc_jac", "orc_trafo", "orc_fs_mid", "orc_init_fs",
                   "orc_integrate", "orc_eval_time_pred", "orc_scatter_update_a", "orc_visco_frequencies",
                   "orc_anelastic_coeff_ps", "orc_elastic_lame", "orc_anelastic_coeff_lame", "orc_src_a",
-                  "orc_star_a", "orc_fs_mid_a", "orc_fs_a", "orc_init_fs_a"]:
+                  "orc_star_a", "orc_fs_mid_a", "orc_fs_a", "orc_init_fs_a", "orc_ms_jump_l", "orc_ms_jump_r",
+                  "orc_ms_flux", "orc_rot_mat", "orc_lsw", "orc_net_updates"]:
             getattr(_lib, f).restype = None
     return _lib
 
@@ -50,7 +51,7 @@ def ref_lib():
     global _ref
     if _ref is None and os.path.exists(REF_LIB):
         _ref = C.CDLL(REF_LIB)
-        for f in ["ref_star", "ref_trafo", "ref_fs"]:
+        for f in ["ref_star", "ref_trafo", "ref_fs", "ref_ms_jump_l", "ref_ms_jump_r", "ref_ms_flux"]:
             getattr(_ref, f).restype = None
     return _ref
 
@@ -265,6 +266,63 @@ def setup_mesh_visco(mesh, rho, lam, mu, qp, qs, n_rms, f_cen, f_rat):
     rfs = visco_frequencies(n_rms, f_cen, f_rat)[::2].copy()
     return {"starE": starE, "fsE": fsE, "starA": starA, "srcA": src, "fsA": fsA, "rfs": rfs,
             "lam_e": lam_e, "mu_e": mu_e, "jdet": jdet}
+
+
+# ---------------------------------------------------------------- rupture
+def ms_jump(lamL, lamR, muL, muR, rhoL, rhoR, side="l"):
+    out = np.zeros(81)
+    f = lib().orc_ms_jump_l if side == "l" else lib().orc_ms_jump_r
+    f(*[_d(x) for x in (lamL, lamR, muL, muR, rhoL, rhoR)], _p(out))
+    return out.reshape(9, 9)
+
+
+def ms_flux(n, s, t, lam, mu, rho):
+    out = np.zeros(81)
+    lib().orc_ms_flux(_p(_c(n)), _p(_c(s)), _p(_c(t)), _d(lam), _d(mu), _d(rho), _p(out))
+    return out.reshape(9, 9)
+
+
+def rot_mat(v0, v1):
+    out = np.zeros(9)
+    lib().orc_rot_mat(_p(_c(v0)), _p(_c(v1)), _p(out))
+    return out.reshape(3, 3)
+
+
+def ib_solvers(normal, t0, t1, bnd, matL, matR, area, volL, volR, n_sf, n_sc):
+    """(solvers [4][9][9]: T^-1, jump, flux left, flux right; preferred direction)."""
+    lib().orc_ib_solvers.restype = C.c_int
+    out = np.zeros(4 * 81)
+    pref = lib().orc_ib_solvers(_p(_c(normal)), _p(_c(t0)), _p(_c(t1)),
+                                None if bnd is None else _p(_c(np.asarray(bnd).ravel())), *[_d(x) for x in matL],
+                                *[_d(x) for x in matR], _d(area), _d(volL), _d(volR), C.c_int(n_sf), C.c_int(n_sc),
+                                _p(out))
+    return out.reshape(4, 9, 9), bool(pref)
+
+
+LSW_STATE = ["sn0", "co0", "ss0_0", "ss0_1", "ss0A", "muf", "st", "tr_0", "tr_1", "sr_0", "sr_1", "dd_0", "dd_1"]
+
+
+def lsw_state(n, **vals):
+    """Friction state [n][14] (columns: sn0, co0, ss0[2], ss0A, muf, st, tr[2], sr[2], dd[2])."""
+    st = np.zeros((n, 14))
+    cols = {"sn0": 0, "co0": 1, "ss0_0": 2, "ss0_1": 3, "ss0A": 4, "muf": 5, "st": 6, "tr_0": 7, "tr_1": 8,
+            "sr_0": 9, "sr_1": 10, "dd_0": 11, "dd_1": 12}
+    for k, v in vals.items():
+        st[:, cols[k]] = v
+    return st
+
+
+def net_updates(solv, dofsL, dofsR, dt, l_eq_m, cs_dmu_m, cs_dmu_p, glob, state):
+    """One rupture face: dofs [9][n_sf][R]; glob [3][R] (mus, mud, 1/Dc);
+    state [n_sf*R][14] (updated in place).  Returns (netL, netR, per)."""
+    n_sf, R = dofsL.shape[1], dofsL.shape[2]
+    netL, netR = np.zeros((9, n_sf, R)), np.zeros((9, n_sf, R))
+    per = np.zeros((n_sf, R), dtype=np.int32)
+    solv, dofsL, dofsR, glob = _c(solv), _c(dofsL), _c(dofsR), _c(glob)
+    assert state.dtype == np.float64 and state.flags.c_contiguous
+    lib().orc_net_updates(C.c_int(n_sf), C.c_int(R), _p(solv), _p(dofsL), _p(dofsR), _d(dt), C.c_int(int(l_eq_m)),
+                          _d(cs_dmu_m), _d(cs_dmu_p), _p(glob), _p(state), _p(netL), _p(netR), _p(per))
+    return netL, netR, per.astype(bool)
 
 
 def dt_cfl(rho, lam, mu, in_dia, order):

@@ -43,3 +43,10 @@ def visco_fx():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "ref_visco_fixtures.json")) as f:
         return json.load(f)
+
+
+@pytest.fixture(scope="session")
+def rupture_fx():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "ref_rupture_fixtures.json")) as f:
+        return json.load(f)

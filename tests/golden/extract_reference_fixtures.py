@@ -18,6 +18,9 @@ Sources (all read as text):
   (test_fluxTet).
 * src/dg/generated/basis_tet.inc, basis_tria.inc -- the reference's generated
   basis polynomials, evaluated here at seeded points (values only are stored).
+* src/impl/seismic/solvers/InternalBoundary.test.cpp -- the 3D internal-boundary
+  test: left / right quantities and the expected perturbed middle states after
+  linear slip weakening (lines 339-628).
 * src/impl/seismic/setups/ViscoElasticity.test.cpp -- expected values of the
   viscoelastic set-up tests (relaxation frequencies, anelastic coefficients,
   elastic Lame parameters, source and star matrices, 2D middle-state flux
@@ -239,6 +242,16 @@ def main():
     mats["basis_tria_val"] = c_expr_basis(os.path.join(gen, "basis_tria.inc"), "val", 2, nb2, pts2).tolist()
     with open(os.path.join(HERE, "ref_matrix_fixtures.json"), "w") as f:
         json.dump(mats, f)
+
+    # ---------------- internal boundary (rupture), 3D test ----------------
+    t = _read(os.path.join(REF, "src/impl/seismic/solvers/InternalBoundary.test.cpp"))
+    t3 = t[t.index('"Internal Boundary: 3D solvers with middle state perturbation."'):]
+    ib = {}
+    for name in ["qL", "qR", "msLp", "msRp"]:
+        m = re.search(r"double\s+l_%s\[9\]\s*=\s*\{(.*?)\};" % name, t3, re.S)
+        ib[name] = [float(x) for x in re.findall(r"[-+]?[0-9]*\.?[0-9]+(?:[eE][-+]?[0-9]+)?", m.group(1))]
+    with open(os.path.join(HERE, "ref_rupture_fixtures.json"), "w") as f:
+        json.dump(ib, f, indent=0)
 
     # ---------------- viscoelastic set-up ----------------
     t = _read(os.path.join(REF, "src/impl/seismic/setups/ViscoElasticity.test.cpp"))

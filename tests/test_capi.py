@@ -80,3 +80,37 @@ def test_visco_create_rejects_bad_arguments():
     assert create(runs=16, prec=32)[0] != _lib.EDGE_OK  # the anelastic tiles of 16 runs do not fit LDS
     rc, msg = create(arrs=good[:3] + [-one, one])
     assert rc != _lib.EDGE_OK and "quality" in msg
+
+
+def test_rupture_create_rejects_bad_arguments():
+    """Argument checks of edge_rupture_create that return before any device work."""
+    import ctypes as C
+    import numpy as np
+    from edge_amd import _lib
+    L = _lib.lib()
+    nf, R = 2, 1
+    e = np.eye(3)
+    good = dict(normals=np.tile(e[0], (nf, 1)), tangents=np.tile(e[1:], (nf, 1, 1)), areas=np.ones(nf),
+                fault_frame=e.copy(), mat_l=np.tile([2700.0, 3e10, 3e10], (nf, 1)),
+                mat_r=np.tile([2700.0, 3e10, 3e10], (nf, 1)), vol_l=np.ones(nf), vol_r=np.ones(nf),
+                glob=np.array([[0.6], [0.5], [2.0]]), state0=np.zeros((nf, 1, R, 14)))
+
+    def create(n_sf=1, n_sc=1, runs=R, **over):
+        a = dict(good, **over)
+        h = C.c_void_p()
+        rc = L.edge_rupture_create(C.byref(_lib.RuptureConfig(n_sf, n_sc, runs)), nf,
+                                   *[np.ascontiguousarray(a[k], dtype=np.float64).ctypes.data for k in good],
+                                   C.byref(h))
+        assert not h.value
+        return rc, L.edge_last_error().decode()
+
+    assert create(n_sf=0)[0] == 1
+    assert create(n_sc=-1)[0] == 1
+    rc, msg = create(fault_frame=2 * e)
+    assert rc == 1 and "unit" in msg
+    rc, msg = create(glob=np.array([[0.6], [0.5], [0.0]]))
+    assert rc == 1 and "1/Dc" in msg
+    rc, msg = create(areas=np.array([1.0, -1.0]))
+    assert rc == 1 and "area" in msg
+    assert L.edge_rupture_net_updates(None, 0, 1, 0.1, None, None, None, None, None) == 1
+    assert L.edge_rupture_destroy(None) == 0
