@@ -65,6 +65,17 @@ __device__ __forceinline__ int64_t xcd_element(int64_t n) {
 
 __device__ __forceinline__ v4f zero4() { return v4f{0.f, 0.f, 0.f, 0.f}; }
 
+// Per-element tables that no kernel writes (star matrices, factored flux
+// solvers, face ids) are read through the constant address space, so that
+// wave-uniform values come in by scalar loads into SGPRs instead of vector
+// loads into VGPRs.
+typedef const __attribute__((address_space(4))) float cfloat;
+typedef const __attribute__((address_space(4))) int32_t cint32;
+__device__ __forceinline__ const cfloat* as_const(const float* p) { return (const cfloat*)p; }
+typedef const __attribute__((address_space(4))) uint8_t cuint8;
+__device__ __forceinline__ const cint32* as_const(const int32_t* p) { return (const cint32*)p; }
+__device__ __forceinline__ const cuint8* as_const(const uint8_t* p) { return (const cuint8*)p; }
+
 // The value x, opaque to the optimiser: index arithmetic derived from it is
 // computed where it is used instead of being hoisted (and spilled) early.
 __device__ __forceinline__ int opaque(int x) {
@@ -132,7 +143,7 @@ template <bool SP>
 struct StarCoef {
   static constexpr int NC = SP ? 24 : 81;
   float v[NC];
-  __device__ __forceinline__ void load(const float* __restrict__ sd) {
+  __device__ __forceinline__ void load(const cfloat* __restrict__ sd) {
     if constexpr (SP) {
       constexpr int cols[24] = {6, 7, 8, 6, 7, 8, 6, 7, 8, 6, 7, 7, 8, 6, 8, 0, 3, 5, 1, 3, 4, 2, 4, 5};
       constexpr int rows[24] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 6, 7, 7, 7, 8, 8, 8};
@@ -167,7 +178,7 @@ struct StarCoef {
 // per direction, and the last direction replaces each row by dt Q;
 // der_1[q] = sum_d (-K_d^T M^-1)^T (sum_p S_d[q][p] Q_p).
 template <int O, bool SP>
-__device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, const float* __restrict__ star,
+__device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, const cfloat* __restrict__ star,
                                          float dt, float* l_ti, v4f (&der)[9][tiles(n_modes(O - 1))], int g,
                                          int c) {
   constexpr int N = Dims<O>::N;
@@ -225,7 +236,7 @@ __device__ __forceinline__ void ck_first(const float* __restrict__ stiffTn, cons
 // CK stage DE >= 2 from the previous derivative held in registers
 // (permuted k: row 16 u + 4 g + v is register v of tile u).
 template <int O, int DE, bool SP>
-__device__ __forceinline__ void ck_next(const float* __restrict__ stiffTn, const float* __restrict__ star,
+__device__ __forceinline__ void ck_next(const float* __restrict__ stiffTn, const cfloat* __restrict__ star,
                                         const v4f (&prev)[9][tiles(n_modes(O - DE + 1))],
                                         v4f (&cur)[9][tiles(n_modes(O - DE))], int g, int c) {
   constexpr int N = Dims<O>::N;
@@ -288,7 +299,7 @@ constexpr int kStrVel[6][3] = {{0, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 1, 0}, {1, 2
 
 struct VelRows {  // S_d[6 + r][kVelCols[r][j]]
   float v[9];
-  __device__ __forceinline__ void load(const float* __restrict__ sd) {
+  __device__ __forceinline__ void load(const cfloat* __restrict__ sd) {
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
@@ -301,7 +312,7 @@ struct VelRows {  // S_d[6 + r][kVelCols[r][j]]
 
 // out_q (+)= sum_i S_d[q][6 + i] P_i for the stress rows q = 0..5 (D tiles)
 template <int TN>
-__device__ __forceinline__ void add_stress(const float* __restrict__ sd, const v4f (&p)[3][TN], v4f (&out)[9][TN],
+__device__ __forceinline__ void add_stress(const cfloat* __restrict__ sd, const v4f (&p)[3][TN], v4f (&out)[9][TN],
                                            bool first) {
 #pragma unroll
   for (int q = 0; q < 6; q++)
@@ -319,7 +330,7 @@ __device__ __forceinline__ void add_stress(const float* __restrict__ sd, const v
 // raw velocity products P[d][i] combined after the k-loop.  A operands from
 // the packed table (four k-steps per load, one group ahead).
 template <int O>
-__device__ __forceinline__ void ck_first_el(const float* __restrict__ pk, const float* __restrict__ star, float dt,
+__device__ __forceinline__ void ck_first_el(const float* __restrict__ pk, const cfloat* __restrict__ star, float dt,
                                             float* l_ti, v4f (&der)[9][tiles(n_modes(O - 1))], int g, int c,
                                             int lane) {
   constexpr int N = Dims<O>::N;
@@ -398,7 +409,7 @@ __device__ __forceinline__ void ck_first_el(const float* __restrict__ pk, const 
 // CK stage DE >= 2 for the elastic pattern (see ck_next), direction-major
 // k-steps (u, v); A operands from the packed table, one 4-step group ahead.
 template <int O, int DE>
-__device__ __forceinline__ void ck_next_el(const float* __restrict__ pk, const float* __restrict__ star,
+__device__ __forceinline__ void ck_next_el(const float* __restrict__ pk, const cfloat* __restrict__ star,
                                            const v4f (&prev)[9][tiles(n_modes(O - DE + 1))],
                                            v4f (&cur)[9][tiles(n_modes(O - DE))], int lane) {
   constexpr int NN = n_modes(O - DE), TN = tiles(NN);
@@ -440,23 +451,28 @@ __device__ __forceinline__ void ck_next_el(const float* __restrict__ pk, const f
   }
 }
 
-// tInt (LDS) += x for tile t of a D-layout array (rows below `rows`)
+// tInt (LDS) += x for tile t of a D-layout array (rows below `rows`).  All
+// reads are issued before any write: rows past `rows` share the dummy slot,
+// so the compiler cannot prove the 36 read-modify-writes independent and
+// would otherwise wait out the LDS latency of each one in turn.
 template <int O>
 __device__ __forceinline__ void add_tint_tile(float* __restrict__ l_ti, int rows, int t, const v4f (&x)[9], int g,
                                               int c) {
+  float o[9][4];
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int i = lds_idx<O>(q, 16 * t + 4 * g + v, rows, c);
-      l_ti[i] += x[q][v];
-    }
+    for (int v = 0; v < 4; v++) o[q][v] = l_ti[lds_idx<O>(q, 16 * t + 4 * g + v, rows, c)];
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) l_ti[lds_idx<O>(q, 16 * t + 4 * g + v, rows, c)] = o[q][v] + x[q][v];
 }
 
 // CK stages DE >= 2.  Their derivatives have at most 16 rows, so their tInt
 // terms sum into t0 (rows 0..15, registers) that is added to LDS at the end.
 template <int O, int DE, bool SP>
-__device__ __forceinline__ void ck_rest(const DgDev<float>& dg, const float* star,
+__device__ __forceinline__ void ck_rest(const DgDev<float>& dg, const cfloat* star,
                                         const v4f (&prev)[9][tiles(n_modes(O - DE + 1))], v4f (&t0)[9], float dt,
                                         float sca, int g, int c, int lane) {
   if constexpr (DE < O) {
@@ -546,8 +562,9 @@ __device__ __forceinline__ void face_update(const float* __restrict__ fl, const 
 // w_j (D layout: face-mode rows 4 g + v, run c) = F^T (V^T X)_j for the tile
 // X in LDS ([q][N][R]) and F (fluxL or fluxN, [N][NF]) as packed A operands
 // pm ([group][lane], four k-steps per vector).
+// a_first: pm[lane], the first group, loaded by the caller ahead of time.
 template <int O>
-__device__ __forceinline__ void project3(const v4f* __restrict__ pm, const float* __restrict__ vt,
+__device__ __forceinline__ void project3(const v4f* __restrict__ pm, v4f a_first, const cfloat* __restrict__ vt,
                                          const float* __restrict__ x, v4f (&w)[3], int g, int c, int lane) {
   constexpr int N = Dims<O>::N;
   constexpr int KS = ksteps(N), NG = pk::groups(KS);
@@ -566,7 +583,7 @@ __device__ __forceinline__ void project3(const v4f* __restrict__ pm, const float
   };
 #pragma unroll
   for (int j = 0; j < 3; j++) w[j] = zero4();
-  v4f a4 = pm[lane], a4n = a4;
+  v4f a4 = a_first, a4n = a4;
   float bn[9];
   load_b(0, bn);
 #pragma unroll
@@ -593,12 +610,16 @@ __device__ __forceinline__ void project3(const v4f* __restrict__ pm, const float
 // packed A operands pl ([t][lane], the four register columns v per vector) on
 // the three projected columns (B operands straight from the w tiles, k = 4 g + v).
 template <int O>
-__device__ __forceinline__ void lift3(const v4f* __restrict__ pl, const float* __restrict__ u, const v4f (&w)[3],
-                                      v4f (&acc)[9][Dims<O>::NT], int lane) {
-  constexpr int NT = Dims<O>::NT;
-  v4f at[NT];
+__device__ __forceinline__ void load_lift(const v4f* __restrict__ pl, v4f (&at)[Dims<O>::NT], int lane) {
 #pragma unroll
-  for (int t = 0; t < NT; t++) at[t] = pl[t * 64 + lane];
+  for (int t = 0; t < Dims<O>::NT; t++) at[t] = pl[t * 64 + lane];
+}
+
+// at: the face's lift operands (load_lift), loaded by the caller ahead of time.
+template <int O>
+__device__ __forceinline__ void lift3(const v4f (&at)[Dims<O>::NT], const cfloat* __restrict__ u, const v4f (&w)[3],
+                                      v4f (&acc)[9][Dims<O>::NT]) {
+  constexpr int NT = Dims<O>::NT;
   v4f l[3][NT];
 #pragma unroll
   for (int j = 0; j < 3; j++)
@@ -648,7 +669,7 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const float* q0 = dofs + el * D::SZ;
-  const float* star = s.star + el * 243;
+  const cfloat* star = as_const(s.star + el * 243);
   tile_to_lds<D::CHUNKS>(q0, l_ti, D::NVEC, lane);
 
   // ---- CK: tInt = dt Q + sum_de dt^{de+1}/(de+1)! der_de, tile in LDS
@@ -675,12 +696,13 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   }
   __syncthreads();
 
-  // ---- volume accumulators start from Q; tInt goes to global memory meanwhile
+  // ---- volume accumulators start from Q (from zero with rank-3 faces, where
+  // Q comes back through LDS at the end); tInt goes to global memory meanwhile
   v4f acc[9][NT];
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[q][t] = load_d(q0 + q * N * R, N, t, g, c);
+    for (int t = 0; t < NT; t++) acc[q][t] = R3 ? zero4() : load_d(q0 + q * N * R, N, t, g, c);
   {
     float4* gti = reinterpret_cast<float4*>(t_int + el * D::SZ);
     const float4* lti = reinterpret_cast<const float4*>(l_ti);
@@ -803,25 +825,46 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
     constexpr int NGP = pk::groups(ksteps(N));
     const v4f* pprj = reinterpret_cast<const v4f*>(s.dg.pk + L.prj());
     const v4f* plft = reinterpret_cast<const v4f*>(s.dg.pk + L.lft());
-#pragma unroll 1
-    for (int fa = 0; fa < 4; fa++) {
-      const float* f = s.fsf + (el * 4 + fa) * 54;
-      v4f w[3];
-      project3<O>(pprj + fa * NGP * 64, f + 27, l_ti, w, g, c, lane);
-      lift3<O>(plft + fa * NT * 64, f, w, acc, lane);
-    }
+    // Every projection first (the last reads of tInt), then Q streams into
+    // LDS while the lifts run.  The first A group of each projection is
+    // loaded one projection ahead.
     // ---- this element's tInt as each face-adjacent element sees it (the
     // neighbour's flux matrix and solver side), for the neighbour step
+    v4f a0 = pprj[(s.proj ? as_const(s.snd_fm)[el * 4] : 0) * NGP * 64 + lane];
     if (s.proj) {
 #pragma unroll 1
       for (int f = 0; f < 4; f++) {
-        const int32_t rv = s.snd_v[el * 4 + f];
-        if (rv < 0) continue;  // nobody receives through this face (uniform across the wave)
-        const int fm = s.snd_fm[el * 4 + f];
-        v4f w[3];
-        project3<O>(pprj + fm * NGP * 64, s.fsf + (s.n_el * 4 + rv) * 54 + 27, l_ti, w, g, c, lane);
-        store_proj(s.proj + (el * 4 + f) * kProjFace, w, lane);
+        const int32_t rv = as_const(s.snd_v)[el * 4 + f];
+        const int fm = as_const(s.snd_fm)[el * 4 + f];
+        const int fmn = f < 3 ? as_const(s.snd_fm)[el * 4 + f + 1] : 0;  // then local face 0
+        const v4f a0n = pprj[fmn * NGP * 64 + lane];
+        if (rv >= 0) {  // somebody receives through this face (uniform across the wave)
+          v4f w[3];
+          project3<O>(pprj + fm * NGP * 64, a0, as_const(s.fsf + (s.n_el * 4 + rv) * 54 + 27), l_ti, w, g, c, lane);
+          store_proj(s.proj + (el * 4 + f) * kProjFace, w, lane);
+        }
+        a0 = a0n;
       }
+    } else {
+      a0 = pprj[lane];
+    }
+    // ---- the local faces' projections, kept for the lifts
+    v4f wl[4][3];
+#pragma unroll
+    for (int fa = 0; fa < 4; fa++) {
+      __builtin_amdgcn_sched_barrier(0);  // one face at a time (register pressure)
+      const v4f a0n = pprj[(fa < 3 ? fa + 1 : fa) * NGP * 64 + lane];
+      project3<O>(pprj + fa * NGP * 64, a0, as_const(s.fsf + (el * 4 + fa) * 54 + 27), l_ti, wl[fa], g, c, lane);
+      a0 = a0n;
+    }
+    __syncthreads();  // tInt's last reads are done: Q may overwrite it
+    tile_to_lds<D::CHUNKS>(q0, l_ti, D::NVEC, lane);
+#pragma unroll
+    for (int fa = 0; fa < 4; fa++) {
+      __builtin_amdgcn_sched_barrier(0);
+      v4f at[NT];
+      load_lift<O>(plft + fa * NT * 64, at, lane);
+      lift3<O>(at, as_const(s.fsf + (el * 4 + fa) * 54), wl[fa], acc);
     }
   } else {
     const float* fs0 = s.fs + el * 324;
@@ -831,8 +874,30 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   }
 
   // ---- DOFs out through LDS (16-byte global stores)
-  __syncthreads();  // the surface integral's reads of tInt are done
-  {
+  if constexpr (R3) {
+    wait_vm_all();  // Q is in LDS
+    // DOFs = Q + acc, three quantities at a time with every read before the
+    // writes (rows past N share the dummy slot)
+    const int ln = opaque(lane), go = ln >> 4, co = ln & 15;
+#pragma unroll
+    for (int qb = 0; qb < 9; qb += 3) {
+      float o[3][NT][4];
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) o[q][t][v] = l_ti[lds_idx<O>(qb + q, 16 * t + 4 * go + v, N, co)];
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int v = 0; v < 4; v++)
+            l_ti[lds_idx<O>(qb + q, 16 * t + 4 * go + v, N, co)] = o[q][t][v] + acc[qb + q][t][v];
+    }
+  } else {
+    __syncthreads();  // the surface integral's reads of tInt are done
     const int ln = opaque(lane), go = ln >> 4, co = ln & 15;
 #pragma unroll
     for (int q = 0; q < 9; q++)
@@ -879,10 +944,12 @@ void k_neigh(SolverDev<float> s, int64_t first, int64_t n, const float* __restri
     if constexpr (R3) {
       constexpr pk::Layout L{O};
       constexpr int NGP = pk::groups(ksteps(N));
-      const float* f = s.fsf + ((s.n_el + el) * 4 + fa) * 54;
-      v4f w[3];
-      project3<O>(reinterpret_cast<const v4f*>(s.dg.pk + L.prj()) + fm * NGP * 64, f + 27, l_nb, w, g, c, lane);
-      lift3<O>(reinterpret_cast<const v4f*>(s.dg.pk + L.lft()) + fa * NT * 64, f, w, acc, lane);
+      const cfloat* f = as_const(s.fsf + ((s.n_el + el) * 4 + fa) * 54);
+      const v4f* pm = reinterpret_cast<const v4f*>(s.dg.pk + L.prj()) + fm * NGP * 64;
+      v4f w[3], at[NT];
+      load_lift<O>(reinterpret_cast<const v4f*>(s.dg.pk + L.lft()) + fa * NT * 64, at, lane);
+      project3<O>(pm, pm[lane], f + 27, l_nb, w, g, c, lane);
+      lift3<O>(at, f, w, acc);
     } else {
       face_update<O>(s.dg.fluxLN + fm * N * NF, s.dg.fluxT + fa * NF * N, s.fs + ((s.n_el + el) * 4 + fa) * 81,
                      l_nb, acc, g, c);
@@ -907,20 +974,39 @@ void k_neigh_proj(SolverDev<float> s, int64_t first, int64_t n, float* __restric
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   float* q0 = dofs + el * D::SZ;
+  (void)NF;
+  // At order 5 (two waves per SIMD either way) every face's projection is
+  // requested before the DOFs and each face's lift operands one face ahead;
+  // below, the registers this takes would halve the occupancy.
+  constexpr bool AHEAD = O >= 5;
+  const v4f* plft = reinterpret_cast<const v4f*>(s.dg.pk + pk::Layout{O}.lft());
+  int32_t rcv[4];
+  v4f w[AHEAD ? 4 : 1][3];
+#pragma unroll
+  for (int fa = 0; fa < 4; fa++) {
+    rcv[fa] = as_const(s.rcv)[el * 4 + fa];
+    if (AHEAD && rcv[fa] >= 0) load_proj(s.proj + (int64_t)rcv[fa] * kProjFace, w[AHEAD ? fa : 0], lane);
+  }
+  v4f at[NT];
+  load_lift<O>(plft, at, lane);
   v4f acc[9][NT];
 #pragma unroll
   for (int q = 0; q < 9; q++)
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[q][t] = load_d(q0 + q * N * R, N, t, g, c);
-  (void)NF;
-#pragma unroll 1
+#pragma unroll
   for (int fa = 0; fa < 4; fa++) {
-    const int32_t r = s.rcv[el * 4 + fa];
-    if (r < 0) continue;  // outflow (uniform across the wave)
-    v4f w[3];
-    load_proj(s.proj + (int64_t)r * kProjFace, w, lane);
-    lift3<O>(reinterpret_cast<const v4f*>(s.dg.pk + pk::Layout{O}.lft()) + fa * NT * 64,
-             s.fsf + ((s.n_el + el) * 4 + fa) * 54, w, acc, lane);
+    v4f atn[NT];
+    if (AHEAD && fa < 3) load_lift<O>(plft + (fa + 1) * NT * 64, atn, lane);
+    if (!AHEAD && fa > 0) load_lift<O>(plft + fa * NT * 64, at, lane);
+    if (rcv[fa] >= 0) {  // else outflow: no projection (uniform across the wave)
+      if (!AHEAD) load_proj(s.proj + (int64_t)rcv[fa] * kProjFace, w[0], lane);
+      lift3<O>(at, as_const(s.fsf + ((s.n_el + el) * 4 + fa) * 54), w[AHEAD ? fa : 0], acc);
+    }
+    if (AHEAD && fa < 3) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) at[t] = atn[t];
+    }
   }
   {
     const int ln = opaque(lane), go = ln >> 4, co = ln & 15;
