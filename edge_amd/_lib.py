@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libedge_amd.so")
 
 EDGE_OK = 0
+EDGE_ERR_ARG = 1
 EDGE_NO_ID16 = 0xFFFF
 FA_INTERIOR = 0
 FA_FREE_SURFACE = 101
@@ -81,6 +82,13 @@ _SIGS = {
     "edge_el_neigh_a": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "edge_el_get_setup_a": (C.c_int, [C.c_void_p] + [C.c_void_p] * 6),
+    "edge_el_point_basis": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "edge_recv_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "edge_recv_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                 C.c_void_p]),
+    "edge_src_create": (C.c_int, [C.c_void_p, C.c_int64] + [C.c_void_p] * 8 + [C.POINTER(C.c_void_p)]),
+    "edge_src_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
+    "edge_points_destroy": (C.c_int, [C.c_void_p]),
     "edge_rupture_create": (C.c_int, [C.POINTER(RuptureConfig), C.c_int64] + [C.c_void_p] * 10 +
                             [C.POINTER(C.c_void_p)]),
     "edge_rupture_destroy": (C.c_int, [C.c_void_p]),

@@ -73,4 +73,21 @@ int launch_neigh_visco(const SolverDev<T>& s, int64_t first, int64_t n, const T*
                        hipStream_t st);
 bool visco_supported(int order, int n_runs, int prec_bytes, int n_rms);
 
+// What kernels outside solver.hip need of a solver handle.
+struct SolverRef {
+  int precision = 64, order = 0, n_runs = 0, n_modes = 0, n_rms = 0;
+  int64_t n_el = 0;
+  const double* jdet = nullptr;  // [nEl] Jacobian determinants (device), null for caller-given solvers
+  const double* mass = nullptr;  // [N] diagonal reference mass matrix (device)
+  SolverDev<float> f{};
+  SolverDev<double> d{};
+};
+int solver_ref(const edge_el_solver* s, SolverRef* out);
+
+// Receivers and point sources (points.hip).
+template <typename T>
+int launch_recv_eval(const SolverDev<T>& s, int64_t n, const int64_t* el, const double* basis, int n_tau,
+                     const double* tau, const T* dofs, T* out, hipStream_t st);
+bool recv_supported(int order, int n_runs, int prec_bytes);
+
 }  // namespace edge

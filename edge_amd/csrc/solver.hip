@@ -538,6 +538,25 @@ extern "C" int edge_el_get_setup_a(const edge_el_solver* s, double* star_a, doub
   return EDGE_OK;
 }
 
+namespace edge {
+int solver_ref(const edge_el_solver* s, SolverRef* out) {
+  if (!s) return fail(EDGE_ERR_ARG, "solver handle missing");
+  out->precision = s->cfg.precision;
+  out->order = s->cfg.order;
+  out->n_runs = s->cfg.n_runs;
+  out->n_modes = s->n_modes;
+  out->n_rms = s->n_rms;
+  out->n_el = s->n_el;
+  out->jdet = s->jdet;
+  out->mass = s->mass;
+  if (s->cfg.precision == 64)
+    out->d = dev_view<double>(s);
+  else
+    out->f = dev_view<float>(s);
+  return EDGE_OK;
+}
+}  // namespace edge
+
 extern "C" int edge_el_kernel_info(const edge_el_solver* s, int32_t* kernel, int32_t* flags) {
   if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
   if (kernel) *kernel = s->kernel;

@@ -220,6 +220,49 @@ int edge_el_get_setup_a(const edge_el_solver *s, double *star_a, double *src_a, 
                         double *lam_e, double *mu_e);
 
 /* ====================================================================== */
+/* Receivers and point sources.                                           */
+/* ====================================================================== */
+typedef struct edge_points edge_points;
+
+/* The N basis functions at reference points xi[n][3] of elements el[n]
+ * (src/io/Receivers.cpp:137-145); with src_scaled != 0 divided by the
+ * diagonal mass and the element's Jacobian determinant, as the point
+ * sources' bEvals (src/impl/seismic/setups/PointSources.hpp:389-470).
+ * out[n][N]. */
+int edge_el_point_basis(const edge_el_solver *s, int64_t n, const int64_t *el, const double *xi, int32_t src_scaled,
+                        double *out);
+
+/* Receivers in elements el[n] with basis values basis[n][N] (edge_el_point_basis,
+ * src_scaled 0); replaces io::Receivers' per-element evaluation
+ * (src/io/Receivers.cpp:250-290) fed by AderDg::local's evalTimePrediction
+ * (src/impl/seismic/solvers/AderDg.hpp:475-496). */
+int edge_recv_create(const edge_el_solver *s, int64_t n, const int64_t *el, const double *basis, edge_points **out);
+
+/* Evaluates every receiver at n_tau (<= 8) relative times tau[n][n_tau] of
+ * the coming step from the DOFs at its start (device, the solver's layout and
+ * precision): out[n][n_tau][9][R] (device, the solver's precision), the time
+ * prediction of the element at the receiver's point.  Call before the step's
+ * local update; the values for tau < 0 are not meaningful. */
+int edge_recv_eval(edge_points *p, const edge_el_solver *s, const void *dofs, int32_t n_tau, const double *tau,
+                   void *out, void *stream);
+
+/* Point sources (PointSources::t_ps, PointSources.hpp:82-107, one run per
+ * source): element el[n], fused run run[n], series start times t0[n] and
+ * sampling steps dts[n], basis evaluations bev[n][N] (edge_el_point_basis,
+ * src_scaled 1), quantity scalings sca[n][9], series samples ts[] with
+ * offsets ts_ptr[n+1] (ts_ptr[0] = 0). */
+int edge_src_create(const edge_el_solver *s, int64_t n, const int64_t *el, const int32_t *run, const double *t0,
+                    const double *dts, const double *bev, const double *sca, const int64_t *ts_ptr,
+                    const double *ts, edge_points **out);
+
+/* Adds the sources' contributions of [t1, t2] to the DOFs (device):
+ * PointSources::apply (PointSources.hpp:887-930) with Series<1>::integrate
+ * (src/linalg/Series.hpp:59-133), FP64. */
+int edge_src_apply(edge_points *p, const edge_el_solver *s, double t1, double t2, void *dofs, void *stream);
+
+int edge_points_destroy(edge_points *p);
+
+/* ====================================================================== */
 /* Dynamic rupture: the internal-boundary net updates of fault faces with  */
 /* linear slip-weakening friction.  Replaces                               */
 /* InternalBoundarySolvers<TET4,O,3>::init (src/impl/seismic/solvers/      */

@@ -951,6 +951,50 @@ void orc_net_updates(int n_sf, int R, const double *solv, const double *dofsL, c
   free(ms2);
 }
 
+/* Integral over [x1, x2] of a series sampled at x0 + i dx (i < n), linear
+ * between samples; where [x1, x2] leaves the samples the series takes the
+ * value na.  Series<1>::integrate, src/linalg/Series.hpp:59-133 (FP64
+ * tolerance 1e-9, src/constants.hpp:289-300). */
+double orc_series_integrate(double dx, double x0, long n, const double *v, double x1, double x2, double na) {
+  const double tol = 1e-9;
+  double a = (x1 - x0 + dx) / dx, b = (x2 - x0) / dx;
+  size_t first = (size_t)(a > 0 ? a : 0), last = (size_t)(b > 0 ? b : 0);
+  if (last > (size_t)(n - 1)) last = (size_t)(n - 1);
+  if (first > last) first = last;
+  double s = 0;
+  for (size_t i = first + 1; i <= last; i++) s += 0.5 * dx * (v[i] - v[i - 1]) + v[i - 1] * dx;
+  const double t_end = x0 + (n - 1) * dx;
+  const int in1 = x1 + tol > x0 && x1 - tol < t_end, in2 = x2 + tol > x0 && x2 - tol < t_end;
+  double d = x0 + first * dx - x1; /* the part before the first covered sample */
+  if (first > 0 && in1)
+    s += 0.5 * d * (d / dx) * (v[first - 1] - v[first]) + v[first] * d;
+  else
+    s += na * d;
+  d = x2 - (x0 + last * dx); /* the part after the last covered sample */
+  if (last < (size_t)(n - 1) && in2)
+    s += 0.5 * d * (d / dx) * (v[last + 1] - v[last]) + v[last] * d;
+  else
+    s += na * d;
+  return s;
+}
+
+/* Point sources of one step (PointSources::apply, src/impl/seismic/setups/
+ * PointSources.hpp:887-930): for every source (element el, run, basis
+ * evaluations scaled by the inverse mass and 1/|J| bev[N], quantity
+ * scalings sca[9], series ts[ptr[i]..ptr[i+1]) sampled from t0 with dts):
+ * dofs[el][q][md][run] += int_{t1}^{t2} s(t) dt * sca[q] * bev[md]. */
+void orc_src_apply(int n_modes, int n_runs, long n_src, const long *el, const int *run, const double *t0,
+                   const double *dts, const double *bev, const double *sca, const long *ptr, const double *ts,
+                   double t1, double t2, double *dofs) {
+  for (long i = 0; i < n_src; i++) {
+    const double in = orc_series_integrate(dts[i], t0[i], ptr[i + 1] - ptr[i], ts + ptr[i], t1, t2, 0.0);
+    for (int q = 0; q < NQ; q++)
+      for (int md = 0; md < n_modes; md++)
+        dofs[((size_t)el[i] * NQ + q) * n_modes * n_runs + (size_t)md * n_runs + run[i]] +=
+            in * sca[i * NQ + q] * bev[i * n_modes + md];
+  }
+}
+
 double orc_dt_cfl(double rho, double lam, double mu, double in_dia, int order) {
   return in_dia * 0.5 / sqrt((lam + 2 * mu) / rho) / (2.0 * order - 1);
 }

@@ -41,7 +41,7 @@ def lib():
                   "orc_integrate", "orc_eval_time_pred", "orc_scatter_update_a", "orc_visco_frequencies",
                   "orc_anelastic_coeff_ps", "orc_elastic_lame", "orc_anelastic_coeff_lame", "orc_src_a",
                   "orc_star_a", "orc_fs_mid_a", "orc_fs_a", "orc_init_fs_a", "orc_ms_jump_l", "orc_ms_jump_r",
-                  "orc_ms_flux", "orc_rot_mat", "orc_lsw", "orc_net_updates"]:
+                  "orc_ms_flux", "orc_rot_mat", "orc_lsw", "orc_net_updates", "orc_src_apply"]:
             getattr(_lib, f).restype = None
     return _lib
 
@@ -323,6 +323,42 @@ def net_updates(solv, dofsL, dofsR, dt, l_eq_m, cs_dmu_m, cs_dmu_p, glob, state)
     lib().orc_net_updates(C.c_int(n_sf), C.c_int(R), _p(solv), _p(dofsL), _p(dofsR), _d(dt), C.c_int(int(l_eq_m)),
                           _d(cs_dmu_m), _d(cs_dmu_p), _p(glob), _p(state), _p(netL), _p(netR), _p(per))
     return netL, netR, per.astype(bool)
+
+
+# ---------------------------------------------------------------- sources
+def series_integrate(dx, x0, vals, x1, x2, na=0.0):
+    v = _c(vals)
+    f = lib().orc_series_integrate
+    f.restype = C.c_double
+    return f(C.c_double(dx), C.c_double(x0), C.c_long(len(v)), _p(v), C.c_double(x1), C.c_double(x2),
+             C.c_double(na))
+
+
+def src_apply(dofs, el, run, t0, dts, bev, sca, series, t1, t2):
+    """dofs [nEl][9][N][R] updated in place; series: list of sampled arrays."""
+    assert dofs.dtype == np.float64 and dofs.flags.c_contiguous
+    n = len(el)
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum([len(x) for x in series])
+    ts = np.concatenate([np.asarray(x, dtype=np.float64) for x in series])
+    lib().orc_src_apply(C.c_int(dofs.shape[2]), C.c_int(dofs.shape[3]), C.c_long(n),
+                        _p(np.asarray(el, dtype=np.int64)), _p(np.asarray(run, dtype=np.int32)), _p(_c(t0)),
+                        _p(_c(dts)), _p(_c(bev)), _p(_c(sca)), _p(ptr), _p(ts), C.c_double(t1), C.c_double(t2),
+                        _p(dofs))
+
+
+def integrate(dg, dt, der):
+    """Taylor integral over [0, dt] of der [O][9][N][R] (TimePred::integrate)."""
+    out = np.zeros(der.shape[1:])
+    lib().orc_integrate(C.byref(dg.s), C.c_double(dt), _p(_c(der)), _p(out))
+    return out
+
+
+def eval_time_pred(dg, t, der):
+    """Taylor evaluation at relative time t of der [O][9][N][R] (evalTimePrediction)."""
+    out = np.zeros(der.shape[1:])
+    lib().orc_eval_time_pred(C.byref(dg.s), C.c_double(t), _p(_c(der)), _p(out))
+    return out
 
 
 def dt_cfl(rho, lam, mu, in_dia, order):

@@ -21,6 +21,10 @@ Sources (all read as text):
 * src/impl/seismic/solvers/InternalBoundary.test.cpp -- the 3D internal-boundary
   test: left / right quantities and the expected perturbed middle states after
   linear slip weakening (lines 339-628).
+* src/linalg/Series.test.cpp -- sampled values, arguments and expected
+  integrals of every ``Series::integrate`` call.
+* src/impl/advection/kernels/TimePred.test.inc -- time derivatives, time step
+  and the [0, dt] and [0, dt/2] Taylor integrals of the advection CK test.
 * src/impl/seismic/setups/ViscoElasticity.test.cpp -- expected values of the
   viscoelastic set-up tests (relaxation frequencies, anelastic coefficients,
   elastic Lame parameters, source and star matrices, 2D middle-state flux
@@ -252,6 +256,28 @@ def main():
         ib[name] = [float(x) for x in re.findall(r"[-+]?[0-9]*\.?[0-9]+(?:[eE][-+]?[0-9]+)?", m.group(1))]
     with open(os.path.join(HERE, "ref_rupture_fixtures.json"), "w") as f:
         json.dump(ib, f, indent=0)
+
+    # ---------------- time series integration and Taylor integration ----------------
+    # src/linalg/Series.test.cpp: the 5 sample values (first series) and every
+    # integrate(dx, x0, n, vals, x1, x2, out[, na]) call with its expected value
+    t = _read(os.path.join(REF, "src/linalg/Series.test.cpp"))
+    m = re.search(r"double\s+l_vals\[5\]\[8\]\s*=\s*\{(.*?)\};", t, re.S)
+    vals = [float(x) for x in re.findall(r"[-+]?[0-9]*\.?[0-9]+", m.group(1))]
+    num = r"([-+]?[0-9]*\.?[0-9]+)"
+    calls = re.findall(r"integrate\(\s*" + num + r",\s*" + num + r",\s*(\d+),\s*l_vals,\s*" + num + r",\s*" + num +
+                       r",\s*l_re\s*(?:,\s*" + num + r")?\s*\);.*?Approx\((.*?)\)\s*\);", t, re.S)
+    series = {"vals": vals[::8], "cases": []}
+    for dx, x0, npt, x1, x2, na, exp in calls:
+        series["cases"].append({"dx": float(dx), "x0": float(x0), "n": int(npt), "x1": float(x1), "x2": float(x2),
+                                "na": float(na) if na else 0.0, "expected": float(eval(exp, {"__builtins__": {}}))})
+    # src/impl/advection/kernels/TimePred.test.inc: derivatives, dt and the
+    # [0, dt] and [0, dt/2] Taylor integrals of the advection CK test
+    t = _read(os.path.join(REF, "src/impl/advection/kernels/TimePred.test.inc"))
+    adv = {"dt": float(re.search(r"float\s+l_utDt\s*=\s*([-+0-9.eE]+);", t).group(1)),
+           "ders": c_array(t, "l_utDers").tolist(), "tint": c_array(t, "l_utTint").tolist(),
+           "tint12": c_array(t, "l_utTint12").tolist()}
+    with open(os.path.join(HERE, "ref_series_fixtures.json"), "w") as f:
+        json.dump({"series": series, "advection_time_pred": adv}, f, indent=0)
 
     # ---------------- viscoelastic set-up ----------------
     t = _read(os.path.join(REF, "src/impl/seismic/setups/ViscoElasticity.test.cpp"))
