@@ -90,4 +90,16 @@ int launch_recv_eval(const SolverDev<T>& s, int64_t n, const int64_t* el, const 
                      const double* tau, const T* dofs, T* out, hipStream_t st);
 bool recv_supported(int order, int n_runs, int prec_bytes);
 
+// Rate-2 local time stepping (lts.hip): tDofs[2] of the listed elements,
+// tDofs[1] sums, and the interface faces of the listed elements.
+template <typename T>
+int launch_lts_half(const SolverDev<T>& s, const int64_t* list, int64_t n, T h, const T* dofs, T* t2,
+                    hipStream_t st);
+template <typename T>
+int launch_lts_sum(const int64_t* list, int64_t n, int sz, int first_ts, const T* t0, T* t1, hipStream_t st);
+template <typename T>
+int launch_lts_faces(const SolverDev<T>& s, const uint8_t* fa_lts, const int64_t* list, int64_t n, int first_ts,
+                     const T* t0, const T* t1, const T* t2, T* dofs, hipStream_t st);
+bool lts_supported(int order, int n_runs, int prec_bytes);
+
 }  // namespace edge

@@ -3,9 +3,11 @@
 // the local / neighbour steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -46,6 +48,14 @@ struct edge_el_solver {
   void* src_a = nullptr;     // [nEl][n_rms][36]
   void* fs_a = nullptr;      // [2][nEl][4][54]
   std::vector<double> lam_e, mu_e, h_rfs;
+  // rate-2 local time stepping
+  uint8_t* fa_lts = nullptr;   // [nEl][4]: 0 equal steps, 1 this element's step greater, 2 smaller
+  int32_t* ne_lts = nullptr;   // ne with the interface faces skipped
+  int32_t* rcv_lts = nullptr;  // rcv with the interface faces skipped
+  int64_t* lts_lt = nullptr;   // elements with a larger-step neighbour (tDofs[1]), ascending
+  int64_t* lts_gt = nullptr;   // elements with a smaller-step neighbour (tDofs[2]), ascending
+  int64_t* lts_if = nullptr;   // elements with interface faces, ascending
+  std::vector<int64_t> h_lts_lt, h_lts_gt, h_lts_if;
   // graph path
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -259,10 +269,56 @@ int neigh_t(edge_el_solver* s, int64_t first, int64_t n, const void* t_int, void
   return edge::launch_neigh_generic<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
 }
 
+void drop_lts(edge_el_solver* s) {
+  for (void* p : {(void*)s->fa_lts, (void*)s->ne_lts, (void*)s->rcv_lts, (void*)s->lts_lt, (void*)s->lts_gt,
+                  (void*)s->lts_if})
+    if (p) hipFree(p);
+  s->fa_lts = nullptr;
+  s->ne_lts = s->rcv_lts = nullptr;
+  s->lts_lt = s->lts_gt = s->lts_if = nullptr;
+  s->h_lts_lt.clear();
+  s->h_lts_gt.clear();
+  s->h_lts_if.clear();
+}
+
+// [lo, hi) of the entries of an ascending element list inside [first, first + n)
+std::pair<int64_t, int64_t> list_range(const std::vector<int64_t>& l, int64_t first, int64_t n) {
+  const auto a = std::lower_bound(l.begin(), l.end(), first), b = std::lower_bound(l.begin(), l.end(), first + n);
+  return {a - l.begin(), b - l.begin()};
+}
+
 int check_range(const edge_el_solver* s, int64_t first, int64_t n) {
   if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
   if (first < 0 || n < 0 || first + n > s->n_el) return edge::fail(EDGE_ERR_ARG, "element range out of bounds");
   return EDGE_OK;
+}
+
+template <typename T>
+int local_lts_t(edge_el_solver* s, int64_t first, int64_t n, double dt, int first_ts, void* dofs, void* t0, void* t1,
+                void* t2, hipStream_t st) {
+  const auto g = list_range(s->h_lts_gt, first, n);
+  int rc = edge::launch_lts_half<T>(dev_view<T>(s), s->lts_gt + g.first, g.second - g.first, (T)(0.5 * dt),
+                                    (const T*)dofs, (T*)t2, st);
+  if (rc) return rc;
+  rc = local_t<T>(s, first, n, dt, dofs, t0, st);
+  if (rc) return rc;
+  const auto l = list_range(s->h_lts_lt, first, n);
+  return edge::launch_lts_sum<T>(s->lts_lt + l.first, l.second - l.first, 9 * s->n_modes * s->cfg.n_runs,
+                                 first_ts, (const T*)t0, (T*)t1, st);
+}
+
+template <typename T>
+int neigh_lts_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, const void* t0, const void* t1,
+                const void* t2, void* dofs, hipStream_t st) {
+  auto d = dev_view<T>(s);
+  d.nb.ne = s->ne_lts;
+  if (d.rcv) d.rcv = s->rcv_lts;
+  int rc = s->kernel == 2 ? edge::launch_neigh_mfma<T>(d, first, n, (const T*)t0, (T*)dofs, st)
+                          : edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
+  if (rc) return rc;
+  const auto f = list_range(s->h_lts_if, first, n);
+  return edge::launch_lts_faces<T>(dev_view<T>(s), s->fa_lts, s->lts_if + f.first, f.second - f.first, first_ts,
+                                   (const T*)t0, (const T*)t1, (const T*)t2, (T*)dofs, st);
 }
 
 int one_step(edge_el_solver* s, double dt, void* dofs, void* t_int, hipStream_t st) {
@@ -376,6 +432,7 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   hipFree(s->star_a);
   hipFree(s->src_a);
   hipFree(s->fs_a);
+  drop_lts(s);
   delete s;
   return EDGE_OK;
 }
@@ -585,6 +642,88 @@ extern "C" int edge_el_neigh(edge_el_solver* s, int64_t first, int64_t n_els, co
   hipStream_t st = (hipStream_t)stream;
   return s->cfg.precision == 64 ? neigh_t<double>(s, first, n_els, t_int, dofs, st)
                                 : neigh_t<float>(s, first, n_els, t_int, dofs, st);
+}
+
+extern "C" int edge_el_set_lts(edge_el_solver* s, const uint8_t* el_lts, const uint8_t* fa_lts) {
+  using edge::fail;
+  if (!s) return fail(EDGE_ERR_ARG, "solver handle missing");
+  drop_lts(s);
+  if (!el_lts && !fa_lts) return EDGE_OK;
+  if (!el_lts || !fa_lts) return fail(EDGE_ERR_ARG, "local time stepping: element and face characteristics");
+  if (s->n_rms > 0) return fail(EDGE_ERR_UNSUPPORTED, "local time stepping: viscoelastic solvers");
+  if (!edge::lts_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8))
+    return fail(EDGE_ERR_UNSUPPORTED, "local time stepping: order, runs and precision do not fit one workgroup");
+  const int64_t ne4 = s->n_el * 4;
+  std::vector<int32_t> ne(ne4), rcv;
+  EDGE_HIP_TRY(hipMemcpy(ne.data(), s->ne, sizeof(int32_t) * ne4, hipMemcpyDeviceToHost));
+  if (s->rcv) {
+    rcv.resize(ne4);
+    EDGE_HIP_TRY(hipMemcpy(rcv.data(), s->rcv, sizeof(int32_t) * ne4, hipMemcpyDeviceToHost));
+  }
+  std::vector<int64_t> lt, gt, itf;
+  for (int64_t el = 0; el < s->n_el; el++) {
+    bool any1 = false, any2 = false;
+    for (int fa = 0; fa < 4; fa++) {
+      const int m = fa_lts[el * 4 + fa];
+      if (m > 2) return fail(EDGE_ERR_ARG, "local time stepping: face characteristic not 0, 1 or 2");
+      if (m != 0 && (ne[el * 4 + fa] < 0 || ne[el * 4 + fa] == el))
+        return fail(EDGE_ERR_ARG, "local time stepping: an interface face without a neighbouring element");
+      any1 |= m == 1;
+      any2 |= m == 2;
+      if (m != 0) {
+        ne[el * 4 + fa] = -1;
+        if (!rcv.empty()) rcv[el * 4 + fa] = -1;
+      }
+    }
+    if (el_lts[el] > 3 || ((el_lts[el] & 1) != 0) != any2 || ((el_lts[el] & 2) != 0) != any1)
+      return fail(EDGE_ERR_ARG, "local time stepping: element characteristics do not match its faces");
+    if (any2) lt.push_back(el);
+    if (any1) gt.push_back(el);
+    if (any1 || any2) itf.push_back(el);
+  }
+  hipError_t e = hipMalloc((void**)&s->fa_lts, ne4);
+  if (e == hipSuccess) e = hipMemcpy(s->fa_lts, fa_lts, ne4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->ne_lts, sizeof(int32_t) * ne4);
+  if (e == hipSuccess) e = hipMemcpy(s->ne_lts, ne.data(), sizeof(int32_t) * ne4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !rcv.empty()) e = hipMalloc((void**)&s->rcv_lts, sizeof(int32_t) * ne4);
+  if (e == hipSuccess && !rcv.empty())
+    e = hipMemcpy(s->rcv_lts, rcv.data(), sizeof(int32_t) * ne4, hipMemcpyHostToDevice);
+  for (auto pr : {std::make_pair(&s->lts_lt, &lt), std::make_pair(&s->lts_gt, &gt), std::make_pair(&s->lts_if, &itf)}) {
+    if (e == hipSuccess) e = hipMalloc((void**)pr.first, sizeof(int64_t) * (pr.second->size() + 1));
+    if (e == hipSuccess && !pr.second->empty())
+      e = hipMemcpy(*pr.first, pr.second->data(), sizeof(int64_t) * pr.second->size(), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    drop_lts(s);
+    return fail(EDGE_ERR_HIP, std::string("local time stepping: ") + hipGetErrorString(e));
+  }
+  s->h_lts_lt = lt;
+  s->h_lts_gt = gt;
+  s->h_lts_if = itf;
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_local_lts(edge_el_solver* s, int64_t first, int64_t n_els, double dt, int32_t first_ts,
+                                 void* dofs, void* t_int0, void* t_int1, void* t_int2, void* stream) {
+  int rc = check_range(s, first, n_els);
+  if (rc) return rc;
+  if (!s->fa_lts) return edge::fail(EDGE_ERR_ARG, "local time stepping is not set up (edge_el_set_lts)");
+  if (!dofs || !t_int0 || !t_int1 || !t_int2) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  hipStream_t st = (hipStream_t)stream;
+  return s->cfg.precision == 64 ? local_lts_t<double>(s, first, n_els, dt, first_ts, dofs, t_int0, t_int1, t_int2, st)
+                                : local_lts_t<float>(s, first, n_els, dt, first_ts, dofs, t_int0, t_int1, t_int2, st);
+}
+
+extern "C" int edge_el_neigh_lts(edge_el_solver* s, int64_t first, int64_t n_els, int32_t first_ts,
+                                 const void* t_int0, const void* t_int1, const void* t_int2, void* dofs,
+                                 void* stream) {
+  int rc = check_range(s, first, n_els);
+  if (rc) return rc;
+  if (!s->fa_lts) return edge::fail(EDGE_ERR_ARG, "local time stepping is not set up (edge_el_set_lts)");
+  if (!dofs || !t_int0 || !t_int1 || !t_int2) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  hipStream_t st = (hipStream_t)stream;
+  return s->cfg.precision == 64 ? neigh_lts_t<double>(s, first, n_els, first_ts, t_int0, t_int1, t_int2, dofs, st)
+                                : neigh_lts_t<float>(s, first, n_els, first_ts, t_int0, t_int1, t_int2, dofs, st);
 }
 
 extern "C" int edge_el_step(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* t_int, int use_graph,

@@ -117,6 +117,31 @@ class ElasticSolver:
         _lib.check(_lib.lib().edge_el_step(self._h, int(n_steps), float(dt), dofs.data_ptr(), t_int.data_ptr(),
                                            1 if graph else 0, _stream_ptr(stream)))
 
+    # ---- rate-2 local time stepping
+    def set_lts(self, el_lts, fa_lts):
+        """el_lts [nEl] bits (1: a larger-step neighbour, 2: a smaller-step
+        one), fa_lts [nEl][4] (0 equal, 1 greater, 2 smaller); None, None: off."""
+        if el_lts is None:
+            _lib.check(_lib.lib().edge_el_set_lts(self._h, None, None))
+            return
+        el_lts = np.ascontiguousarray(el_lts, dtype=np.uint8)
+        fa_lts = np.ascontiguousarray(fa_lts, dtype=np.uint8)
+        _lib.check(_lib.lib().edge_el_set_lts(self._h, el_lts.ctypes.data, fa_lts.ctypes.data))
+
+    def local_lts(self, dofs, t0, t1, t2, dt, first_ts, first=0, n=None, stream=None):
+        for a in (dofs, t0, t1, t2):
+            self._check_arr(a)
+        n = self.mesh.n_el - first if n is None else n
+        _lib.check(_lib.lib().edge_el_local_lts(self._h, first, n, float(dt), 1 if first_ts else 0, dofs.data_ptr(),
+                                                t0.data_ptr(), t1.data_ptr(), t2.data_ptr(), _stream_ptr(stream)))
+
+    def neigh_lts(self, dofs, t0, t1, t2, first_ts, first=0, n=None, stream=None):
+        for a in (dofs, t0, t1, t2):
+            self._check_arr(a)
+        n = self.mesh.n_el - first if n is None else n
+        _lib.check(_lib.lib().edge_el_neigh_lts(self._h, first, n, 1 if first_ts else 0, t0.data_ptr(), t1.data_ptr(),
+                                                t2.data_ptr(), dofs.data_ptr(), _stream_ptr(stream)))
+
     def l2_norms(self, a, b=None, stream=None):
         """FP64 (||a-b||^2, ||b||^2) per [9][R] through the mass matrix."""
         import torch

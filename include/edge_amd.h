@@ -305,6 +305,31 @@ int edge_rupture_net_updates(edge_rupture *r, int64_t first, int64_t n_faces, do
  * to the host (any may be NULL). */
 int edge_rupture_get(const edge_rupture *r, double *solvers, double *state, uint8_t *per);
 
+/* ---------------------------------------------------------------------- */
+/* Rate-2 local time stepping (AderDg.hpp:384-417 and :620-674): elements   */
+/* are sorted by time group and advanced group-range by group-range.        */
+/* ---------------------------------------------------------------------- */
+/* The C_LTS_EL / C_LTS_AD bits of the elements' spType: el_lts[nEl] bit 0
+ * "an adjacent element has a larger time step" (the element keeps tDofs[1]),
+ * bit 1 "an adjacent element has a smaller time step" (tDofs[2]);
+ * fa_lts[nEl][4]: 0 equal steps, 1 the element's step is the greater, 2 the
+ * smaller (interior faces only).  NULL, NULL switches local time stepping
+ * off.  Elastic solvers. */
+int edge_el_set_lts(edge_el_solver *s, const uint8_t *el_lts, const uint8_t *fa_lts);
+
+/* AderDg::local with local time stepping over elements [first, first+n):
+ * t_int0 = tDofs[0] (as edge_el_local), t_int1 = tDofs[1] (reset on the
+ * first step of a pair, first_ts != 0, then summed), t_int2 = tDofs[2]
+ * (integral over [0, dt/2]); device arrays [nEl][9][N][R]. */
+int edge_el_local_lts(edge_el_solver *s, int64_t first, int64_t n_els, double dt, int32_t first_ts, void *dofs,
+                      void *t_int0, void *t_int1, void *t_int2, void *stream);
+
+/* AderDg::neigh with local time stepping: faces between equal steps take the
+ * neighbour's tDofs[0], faces to a smaller step its tDofs[1], faces to a
+ * larger step its tDofs[2] (first_ts != 0) or tDofs[0] - tDofs[2]. */
+int edge_el_neigh_lts(edge_el_solver *s, int64_t first, int64_t n_els, int32_t first_ts, const void *t_int0,
+                      const void *t_int1, const void *t_int2, void *dofs, void *stream);
+
 /* Which kernels a solver runs: *kernel = 1 (generic) or 2 (MFMA-tiled);
  * *flags bit 0: sparse (elastic-pattern) star matrices, bit 1: rank-3
  * factored flux solvers, bit 2: the neighbour step reads face projections,

@@ -318,6 +318,78 @@ void orc_neigh(const orc_dg *dg, int64_t first, int64_t n_els, const int32_t *fa
 }
 
 /* -------------------------------------------------------------------------
+ * Local time stepping (rate 2), elastic: the tDofs[1] / tDofs[2] buffers of
+ * AderDg::local (AderDg.hpp:384-417) and the neighbour data AderDg::neigh
+ * takes per face (:620-674).
+ *   el_lts[el] bit 0: an adjacent element has a larger time step
+ *                     (tDofs[1] sums tDofs[0] over the element's steps since
+ *                     the first of every pair);
+ *              bit 1: an adjacent element has a smaller time step
+ *                     (tDofs[2] = the Taylor integral over [0, dt/2]).
+ *   fa_lts[el][fa]: 0 equal steps: the neighbour's tDofs[0];
+ *                   1 the element's step is the greater: tDofs[1];
+ *                   2 the element's step is the smaller: tDofs[2] on the
+ *                     first step of a pair, tDofs[0] - tDofs[2] on the second.
+ * ---------------------------------------------------------------------- */
+void orc_local_lts(const orc_dg *dg, int64_t first, int64_t n_els, double dt, int first_ts, const double *starE,
+                   const double *fsE0, const uint8_t *el_lts, double *dofsE, double *t0, double *t1, double *t2) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs;
+  const size_t sz = (size_t)NQ * N * R;
+#pragma omp parallel
+  {
+    double *der = (double *)malloc(sizeof(double) * O * sz);
+#pragma omp for schedule(static)
+    for (int64_t el = first; el < first + n_els; el++) {
+      orc_ck(dg, dt, starE + el * 243, NULL, NULL, dofsE + el * sz, NULL, der, NULL, t0 + el * sz, NULL);
+      if (el_lts[el] & 1) {
+        for (size_t i = 0; i < sz; i++) t1[el * sz + i] = (first_ts ? 0.0 : t1[el * sz + i]) + t0[el * sz + i];
+      }
+      if (el_lts[el] & 2) orc_integrate(dg, 0.5 * dt, der, t2 + el * sz);
+      orc_vol(dg, starE + el * 243, NULL, NULL, t0 + el * sz, NULL, dofsE + el * sz, NULL);
+      orc_surf_local(dg, fsE0 + el * 324, NULL, t0 + el * sz, dofsE + el * sz, NULL);
+    }
+    free(der);
+  }
+}
+
+void orc_neigh_lts(const orc_dg *dg, int64_t first, int64_t n_els, int first_ts, const int32_t *fa_type,
+                   const int64_t *el_fa, const int64_t *el_fa_el, const uint16_t *f_id, const uint16_t *v_id,
+                   const double *fsE1, const uint8_t *fa_lts, const double *t0, const double *t1,
+                   const double *t2, double *dofsE) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs;
+  const size_t sz = (size_t)NQ * N * R;
+#pragma omp parallel
+  {
+    double *tn = (double *)malloc(sizeof(double) * sz);
+#pragma omp for schedule(static)
+    for (int64_t el = first; el < first + n_els; el++) {
+      for (int fa = 0; fa < 4; fa++) {
+        const int64_t fid = el_fa[el * 4 + fa];
+        const int32_t ty = fa_type[fid];
+        if ((ty & FA_OUTFLOW) == FA_OUTFLOW) continue;
+        const int fs = (ty & FA_FREE_SURFACE) == FA_FREE_SURFACE;
+        const int64_t ne = fs ? el : el_fa_el[el * 4 + fa];
+        unsigned vId = NO_ID16, fId = NO_ID16;
+        if (!fs) {
+          vId = v_id[el * 4 + fa];
+          fId = f_id[el * 4 + fa];
+        }
+        const int mode = fa_lts[el * 4 + fa];
+        const double *src = t0 + ne * sz;
+        if (mode == 1) src = t1 + ne * sz;
+        if (mode == 2 && first_ts) src = t2 + ne * sz;
+        if (mode == 2 && !first_ts) {
+          for (size_t i = 0; i < sz; i++) tn[i] = t0[ne * sz + i] - t2[ne * sz + i];
+          src = tn;
+        }
+        orc_surf_neigh(dg, fa, vId, fId, fsE1 + (el * 4 + fa) * 81, NULL, src, dofsE + el * sz, NULL);
+      }
+    }
+    free(tn);
+  }
+}
+
+/* -------------------------------------------------------------------------
  * Geometry: Jacobian, src/linalg/Mappings.hpp:189-230 with the tet4 shape
  * derivatives (jac[i][j] = dx_j / dxi_i); determinant and inverse.
  * ve: [4][3] vertex coordinates.

@@ -41,7 +41,8 @@ def lib():
                   "orc_integrate", "orc_eval_time_pred", "orc_scatter_update_a", "orc_visco_frequencies",
                   "orc_anelastic_coeff_ps", "orc_elastic_lame", "orc_anelastic_coeff_lame", "orc_src_a",
                   "orc_star_a", "orc_fs_mid_a", "orc_fs_a", "orc_init_fs_a", "orc_ms_jump_l", "orc_ms_jump_r",
-                  "orc_ms_flux", "orc_rot_mat", "orc_lsw", "orc_net_updates", "orc_src_apply"]:
+                  "orc_ms_flux", "orc_rot_mat", "orc_lsw", "orc_net_updates", "orc_src_apply", "orc_local_lts",
+                  "orc_neigh_lts"]:
             getattr(_lib, f).restype = None
     return _lib
 
@@ -408,6 +409,40 @@ class MeshOracle:
         for _ in range(n_steps):
             self.local(dofs, tint, dt, dofs_a=dofs_a)
             self.neigh(dofs, tint, dofs_a=dofs_a)
+
+    # ---- local time stepping (rate 2, elastic)
+    def set_lts(self, el_lts, fa_lts):
+        self.el_lts = np.ascontiguousarray(el_lts, dtype=np.uint8)
+        self.fa_lts = np.ascontiguousarray(fa_lts, dtype=np.uint8)
+
+    def local_lts(self, dofs, t0, t1, t2, dt, first_ts, first=0, n=None):
+        n = self.mesh.n_el - first if n is None else n
+        lib().orc_local_lts(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), C.c_double(dt),
+                            C.c_int(int(first_ts)), _p(self.star), _p(self.fs[0]), _p(self.el_lts), _p(dofs),
+                            _p(t0), _p(t1), _p(t2))
+
+    def neigh_lts(self, dofs, t0, t1, t2, first_ts, first=0, n=None):
+        n = self.mesh.n_el - first if n is None else n
+        lib().orc_neigh_lts(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), C.c_int(int(first_ts)),
+                            _p(self.fa_type), _p(self.el_fa), _p(self.el_fa_el), _p(self.f_id), _p(self.v_id),
+                            _p(self.fs[1]), _p(self.fa_lts), _p(t0), _p(t1), _p(t2), _p(dofs))
+
+
+def lts_groups(mesh, group):
+    """Rate-2 LTS characteristics from a time group per element (group g
+    steps with 2^g times the fundamental step): el_lts [nEl] (bit 0: an
+    adjacent element has a larger step, bit 1: a smaller one) and fa_lts
+    [nEl][4] (0 equal, 1 the element's step is the greater, 2 the smaller).
+    Outflow faces and free surfaces count as equal."""
+    group = np.asarray(group)
+    ne = np.asarray(mesh.el_fa_el)
+    ty = np.asarray(mesh.fa_type)[np.asarray(mesh.el_fa)]
+    own = (ty == 101) | (ty == 105) | (ne < 0)
+    gn = np.where(own, group[:, None], group[np.where(ne < 0, 0, ne)])
+    g = group[:, None]
+    fa = np.where(gn == g, 0, np.where(g > gn, 1, 2)).astype(np.uint8)
+    el = ((fa == 2).any(axis=1) * 1 + (fa == 1).any(axis=1) * 2).astype(np.uint8)
+    return el, fa
 
 
 # ---------------------------------------------------------------- CPU baseline sample
