@@ -51,6 +51,11 @@ class RuptureConfig(C.Structure):
     _fields_ = [("n_sub_faces", C.c_int32), ("n_sub_cells", C.c_int32), ("n_runs", C.c_int32)]
 
 
+class TetsIn(C.Structure):
+    _fields_ = [("n_ve", C.c_int64), ("n_el", C.c_int64), ("ve_crds", C.c_void_p), ("el_ve", C.c_void_p),
+                ("n_bf", C.c_int64), ("bf_ve", C.c_void_p), ("bf_type", C.c_void_p), ("default_bc", C.c_int32)]
+
+
 # symbol -> (restype, argtypes)
 _SIGS = {
     "edge_last_error": (C.c_char_p, []),
@@ -82,6 +87,8 @@ _SIGS = {
     "edge_el_neigh_a": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "edge_el_get_setup_a": (C.c_int, [C.c_void_p] + [C.c_void_p] * 6),
+    "edge_mesh_tets_sizes": (C.c_int, [C.POINTER(TetsIn), C.POINTER(C.c_int64)]),
+    "edge_mesh_tets": (C.c_int, [C.POINTER(TetsIn), C.c_void_p]),
     "edge_el_set_lts": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "edge_el_local_lts": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_int32] + [C.c_void_p] * 5),
     "edge_el_neigh_lts": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32] + [C.c_void_p] * 5),

@@ -13,6 +13,7 @@
 //  * in-diameter 12 V / sum |2 A_face| (tools/edge_v/src/geom/Tet4.cpp:49-78).
 // Periodic faces are stored once and shared by both elements, so a periodic
 // face looks like an interior face to the solver; its type is PERIODIC.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -44,6 +45,137 @@ void cross(const double a[3], const double b[3], double c[3]) {
 }
 double dot(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 double norm(const double a[3]) { return std::sqrt(dot(a, a)); }
+
+struct TriKey {
+  int64_t v[3];
+  bool operator==(const TriKey& o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2]; }
+};
+struct TriHash {
+  size_t operator()(const TriKey& k) const {
+    uint64_t h = 1469598103934665603ull;
+    for (int i = 0; i < 3; i++) h = (h ^ (uint64_t)k.v[i]) * 1099511628211ull;
+    return (size_t)(h ^ (h >> 29));
+  }
+};
+TriKey tri_key(int64_t a, int64_t b, int64_t c) {
+  if (a > b) std::swap(a, b);
+  if (b > c) std::swap(b, c);
+  if (a > b) std::swap(a, b);
+  return TriKey{{a, b, c}};
+}
+
+// Faces of positively oriented tets (el_ve, volume set), matched through
+// canonical vertex triples (canon: periodic identification), with their
+// types (face_type(eL, local face, eR or -1, &type)), geometry and the
+// element-face-element relations.  Writes everything of m but ve_crds,
+// el_ve and volume (at most max_fa faces); *n_fa receives the face count.
+template <typename Canon, typename FaceType>
+int connect(edge_mesh_out* m, int64_t nel, Canon canon, FaceType face_type, int64_t max_fa, int64_t* n_fa) {
+  std::unordered_map<TriKey, int64_t, TriHash> faces;
+  faces.reserve((size_t)nel * 3);
+  int64_t fa_count = 0;
+  for (int64_t e = 0; e < nel; e++) {
+    for (int lf = 0; lf < 4; lf++) {
+      const TriKey key = tri_key(canon(m->el_ve[e * 4 + kFaVe[lf][0]]), canon(m->el_ve[e * 4 + kFaVe[lf][1]]),
+                                 canon(m->el_ve[e * 4 + kFaVe[lf][2]]));
+      auto it = faces.find(key);
+      if (it == faces.end()) {
+        if (fa_count >= max_fa) return edge::fail(EDGE_ERR_MESH, "more faces than the output arrays hold");
+        faces.emplace(key, fa_count);
+        m->fa_el[fa_count * 2] = e;
+        m->fa_el[fa_count * 2 + 1] = -1;
+        m->el_fa[e * 4 + lf] = fa_count;
+        fa_count++;
+      } else {
+        const int64_t f = it->second;
+        if (m->fa_el[f * 2 + 1] != -1) return edge::fail(EDGE_ERR_MESH, "face shared by more than two tets");
+        m->fa_el[f * 2 + 1] = e;
+        m->el_fa[e * 4 + lf] = f;
+      }
+    }
+  }
+  *n_fa = fa_count;
+  // face types, neighbours, geometry
+  for (int64_t f = 0; f < fa_count; f++) {
+    const int64_t eL = m->fa_el[f * 2], eR = m->fa_el[f * 2 + 1];
+    int lf = -1;
+    for (int s = 0; s < 4; s++)
+      if (m->el_fa[eL * 4 + s] == f) lf = s;
+    const int st = face_type(eL, lf, eR, &m->fa_type[f]);
+    if (st != EDGE_OK) return st;
+
+    // geometry of the left element's face
+    double x[3][3], np[3];
+    for (int s = 0; s < 3; s++)
+      for (int d = 0; d < 3; d++) x[s][d] = m->ve_crds[3 * m->el_ve[eL * 4 + kFaVe[lf][s]] + d];
+    {
+      int opp = 0;
+      for (int v = 0; v < 4; v++) {
+        bool in = false;
+        for (int s = 0; s < 3; s++) in = in || kFaVe[lf][s] == v;
+        if (!in) opp = v;
+      }
+      for (int d = 0; d < 3; d++) np[d] = m->ve_crds[3 * m->el_ve[eL * 4 + opp] + d];
+    }
+    double d0[3], d1[3], c[3];
+    for (int d = 0; d < 3; d++) {
+      d0[d] = x[1][d] - x[0][d];
+      d1[d] = x[2][d] - x[0][d];
+    }
+    cross(d0, d1, c);
+    m->areas[f] = 0.5 * norm(c);
+    // tangents: d0 and its in-plane normal n0; swapped if d0 x n0 points inwards
+    const double alpha = -dot(d0, d1) / dot(d0, d0);
+    double n0[3];
+    for (int d = 0; d < 3; d++) n0[d] = d1[d] + alpha * d0[d];
+    const double l0 = norm(d0), l1 = norm(n0);
+    for (int d = 0; d < 3; d++) {
+      d0[d] /= l0;
+      n0[d] /= l1;
+    }
+    double n1[3], dn[3];
+    cross(d0, n0, n1);
+    for (int d = 0; d < 3; d++) dn[d] = np[d] - x[0][d];
+    if (dot(n1, dn) > 0)
+      for (int d = 0; d < 3; d++) std::swap(d0[d], n0[d]);
+    double nrm[3];
+    cross(d0, n0, nrm);
+    for (int d = 0; d < 3; d++) {
+      m->tangents[f * 6 + d] = d0[d];
+      m->tangents[f * 6 + 3 + d] = n0[d];
+      m->normals[f * 3 + d] = nrm[d];
+    }
+  }
+
+  // element-face-element relations
+  for (int64_t e = 0; e < nel; e++) {
+    double asum = 0;
+    for (int lf = 0; lf < 4; lf++) {
+      const int64_t f = m->el_fa[e * 4 + lf];
+      asum += 2.0 * m->areas[f];
+      const int64_t ne = (m->fa_el[f * 2] == e) ? m->fa_el[f * 2 + 1] : m->fa_el[f * 2];
+      m->el_fa_el[e * 4 + lf] = ne;
+      if (ne < 0) {
+        m->f_id[e * 4 + lf] = EDGE_NO_ID16;
+        m->v_id[e * 4 + lf] = EDGE_NO_ID16;
+        continue;
+      }
+      int nf = -1;
+      for (int s = 0; s < 4; s++)
+        if (m->el_fa[ne * 4 + s] == f) nf = s;
+      if (nf < 0) return edge::fail(EDGE_ERR_MESH, "neighbour misses the shared face");
+      const int64_t c0 = canon(m->el_ve[e * 4 + (lf <= 2 ? 0 : 1)]);
+      int pos = -1;
+      for (int s = 0; s < 3; s++)
+        if (canon(m->el_ve[ne * 4 + kFaVe[nf][s]]) == c0) pos = s;
+      if (pos < 0) return edge::fail(EDGE_ERR_MESH, "shared face vertex not found in the neighbour");
+      m->f_id[e * 4 + lf] = (uint16_t)nf;
+      m->v_id[e * 4 + lf] = (uint16_t)pos;
+    }
+    m->in_dia[e] = 12.0 * m->volume[e] / asum;
+  }
+  return EDGE_OK;
+}
 
 }  // namespace
 
@@ -150,132 +282,105 @@ extern "C" int edge_mesh_box(const edge_box_params* p, edge_mesh_out* m) {
     }
   }
 
-  // faces, matched through canonical vertex triples
-  std::unordered_map<uint64_t, int64_t> faces;
-  faces.reserve((size_t)nfa * 2);
-  int64_t fa_count = 0;
-  for (int64_t e = 0; e < nel; e++) {
-    for (int lf = 0; lf < 4; lf++) {
-      int64_t cv[3];
-      for (int s = 0; s < 3; s++) cv[s] = canon(m->el_ve[e * 4 + kFaVe[lf][s]]);
-      if (cv[0] > cv[1]) std::swap(cv[0], cv[1]);
-      if (cv[1] > cv[2]) std::swap(cv[1], cv[2]);
-      if (cv[0] > cv[1]) std::swap(cv[0], cv[1]);
-      const uint64_t key = ((uint64_t)cv[0] << 42) | ((uint64_t)cv[1] << 21) | (uint64_t)cv[2];
-      auto it = faces.find(key);
-      if (it == faces.end()) {
-        if (fa_count >= nfa) return edge::fail(EDGE_ERR_MESH, "face count mismatch");
-        faces.emplace(key, fa_count);
-        m->fa_el[fa_count * 2] = e;
-        m->fa_el[fa_count * 2 + 1] = -1;
-        m->el_fa[e * 4 + lf] = fa_count;
-        fa_count++;
-      } else {
-        const int64_t f = it->second;
-        if (m->fa_el[f * 2 + 1] != -1) return edge::fail(EDGE_ERR_MESH, "face shared by more than two tets");
-        m->fa_el[f * 2 + 1] = e;
-        m->el_fa[e * 4 + lf] = f;
-      }
-    }
-  }
-  if (fa_count != nfa) return edge::fail(EDGE_ERR_MESH, "face count mismatch");
-  if ((uint64_t)nve >= (1ull << 21)) return edge::fail(EDGE_ERR_MESH, "box too large for face keys");
-
-  // face types, neighbours, geometry
-  for (int64_t f = 0; f < nfa; f++) {
-    const int64_t eL = m->fa_el[f * 2], eR = m->fa_el[f * 2 + 1];
-    int lf = -1;
-    for (int s = 0; s < 4; s++)
-      if (m->el_fa[eL * 4 + s] == f) lf = s;
-    // which box side (if any) the face lies on, from the left element's vertices
-    int side = -1;
-    for (int d = 0; d < 3 && side < 0; d++) {
-      for (int hi = 0; hi < 2 && side < 0; hi++) {
+  // which box side (if any) a face of element e (local face lf) lies on
+  auto side_of = [&](int64_t e, int lf) {
+    for (int d = 0; d < 3; d++)
+      for (int hi = 0; hi < 2; hi++) {
         bool on = true;
         for (int s = 0; s < 3; s++) {
-          const int64_t v = m->el_ve[eL * 4 + kFaVe[lf][s]];
+          const int64_t v = m->el_ve[e * 4 + kFaVe[lf][s]];
           const int64_t ijk[3] = {v % nv[0], (v / nv[0]) % nv[1], v / (nv[0] * nv[1])};
           on = on && ijk[d] == (hi ? n[d] : 0);
         }
-        if (on) side = 2 * d + hi;
+        if (on) return 2 * d + hi;
       }
-    }
+    return -1;
+  };
+  auto face_type = [&](int64_t eL, int lf, int64_t eR, int32_t* ty) {
+    const int side = side_of(eL, lf);
     if (eR < 0) {
       if (side < 0) return edge::fail(EDGE_ERR_MESH, "unmatched interior face");
-      m->fa_type[f] = p->bc[side / 2];
+      *ty = p->bc[side / 2];
     } else {
-      m->fa_type[f] = (side >= 0) ? EDGE_FA_PERIODIC : 0;
+      *ty = (side >= 0) ? EDGE_FA_PERIODIC : 0;
     }
-
-    // geometry of the left element's face
-    double x[3][3], np[3];
-    for (int s = 0; s < 3; s++)
-      for (int d = 0; d < 3; d++) x[s][d] = m->ve_crds[3 * m->el_ve[eL * 4 + kFaVe[lf][s]] + d];
-    {
-      int opp = 0;
-      for (int v = 0; v < 4; v++) {
-        bool in = false;
-        for (int s = 0; s < 3; s++) in = in || kFaVe[lf][s] == v;
-        if (!in) opp = v;
-      }
-      for (int d = 0; d < 3; d++) np[d] = m->ve_crds[3 * m->el_ve[eL * 4 + opp] + d];
-    }
-    double d0[3], d1[3], c[3];
-    for (int d = 0; d < 3; d++) {
-      d0[d] = x[1][d] - x[0][d];
-      d1[d] = x[2][d] - x[0][d];
-    }
-    cross(d0, d1, c);
-    m->areas[f] = 0.5 * norm(c);
-    // tangents: d0 and its in-plane normal n0; swapped if d0 x n0 points inwards
-    const double alpha = -dot(d0, d1) / dot(d0, d0);
-    double n0[3];
-    for (int d = 0; d < 3; d++) n0[d] = d1[d] + alpha * d0[d];
-    const double l0 = norm(d0), l1 = norm(n0);
-    for (int d = 0; d < 3; d++) {
-      d0[d] /= l0;
-      n0[d] /= l1;
-    }
-    double n1[3], dn[3];
-    cross(d0, n0, n1);
-    for (int d = 0; d < 3; d++) dn[d] = np[d] - x[0][d];
-    if (dot(n1, dn) > 0)
-      for (int d = 0; d < 3; d++) std::swap(d0[d], n0[d]);
-    double nrm[3];
-    cross(d0, n0, nrm);
-    for (int d = 0; d < 3; d++) {
-      m->tangents[f * 6 + d] = d0[d];
-      m->tangents[f * 6 + 3 + d] = n0[d];
-      m->normals[f * 3 + d] = nrm[d];
-    }
-  }
-
-  // element-face-element relations
-  for (int64_t e = 0; e < nel; e++) {
-    double asum = 0;
-    for (int lf = 0; lf < 4; lf++) {
-      const int64_t f = m->el_fa[e * 4 + lf];
-      asum += 2.0 * m->areas[f];
-      const int64_t ne = (m->fa_el[f * 2] == e) ? m->fa_el[f * 2 + 1] : m->fa_el[f * 2];
-      m->el_fa_el[e * 4 + lf] = ne;
-      if (ne < 0) {
-        m->f_id[e * 4 + lf] = EDGE_NO_ID16;
-        m->v_id[e * 4 + lf] = EDGE_NO_ID16;
-        continue;
-      }
-      int nf = -1;
-      for (int s = 0; s < 4; s++)
-        if (m->el_fa[ne * 4 + s] == f) nf = s;
-      if (nf < 0) return edge::fail(EDGE_ERR_MESH, "neighbour misses the shared face");
-      const int64_t c0 = canon(m->el_ve[e * 4 + (lf <= 2 ? 0 : 1)]);
-      int pos = -1;
-      for (int s = 0; s < 3; s++)
-        if (canon(m->el_ve[ne * 4 + kFaVe[nf][s]]) == c0) pos = s;
-      if (pos < 0) return edge::fail(EDGE_ERR_MESH, "shared face vertex not found in the neighbour");
-      m->f_id[e * 4 + lf] = (uint16_t)nf;
-      m->v_id[e * 4 + lf] = (uint16_t)pos;
-    }
-    m->in_dia[e] = 12.0 * m->volume[e] / asum;
-  }
+    return EDGE_OK;
+  };
+  int64_t nfa_built = 0;
+  st = connect(m, nel, canon, face_type, nfa, &nfa_built);
+  if (st != EDGE_OK) return st;
+  if (nfa_built != nfa) return edge::fail(EDGE_ERR_MESH, "face count mismatch");
   return EDGE_OK;
+}
+
+// ---------------------------------------------------------------- any tet mesh
+namespace {
+
+int tets_check(const edge_tets_in* in) {
+  if (!in || !in->ve_crds || !in->el_ve || (in->n_bf > 0 && (!in->bf_ve || !in->bf_type)))
+    return edge::fail(EDGE_ERR_ARG, "tet mesh: argument missing");
+  if (in->n_ve < 4 || in->n_el < 1 || in->n_bf < 0) return edge::fail(EDGE_ERR_ARG, "tet mesh: sizes");
+  for (int64_t i = 0; i < 4 * in->n_el; i++)
+    if (in->el_ve[i] < 0 || in->el_ve[i] >= in->n_ve) return edge::fail(EDGE_ERR_MESH, "tet mesh: vertex id out of range");
+  for (int64_t i = 0; i < 3 * in->n_bf; i++)
+    if (in->bf_ve[i] < 0 || in->bf_ve[i] >= in->n_ve) return edge::fail(EDGE_ERR_MESH, "tet mesh: vertex id out of range");
+  return EDGE_OK;
+}
+
+}  // namespace
+
+extern "C" int edge_mesh_tets_sizes(const edge_tets_in* in, int64_t* n_fa) {
+  int st = tets_check(in);
+  if (st != EDGE_OK) return st;
+  if (!n_fa) return edge::fail(EDGE_ERR_ARG, "null argument");
+  std::unordered_map<TriKey, int, TriHash> faces;
+  faces.reserve((size_t)in->n_el * 3);
+  for (int64_t e = 0; e < in->n_el; e++)
+    for (int lf = 0; lf < 4; lf++) {
+      const int64_t* v = in->el_ve + e * 4;
+      const int c = ++faces[tri_key(v[kFaVe[lf][0]], v[kFaVe[lf][1]], v[kFaVe[lf][2]])];
+      if (c > 2) return edge::fail(EDGE_ERR_MESH, "face shared by more than two tets");
+    }
+  *n_fa = (int64_t)faces.size();
+  return EDGE_OK;
+}
+
+extern "C" int edge_mesh_tets(const edge_tets_in* in, edge_mesh_out* m) {
+  int64_t nfa = 0;
+  int st = edge_mesh_tets_sizes(in, &nfa);
+  if (st != EDGE_OK) return st;
+  if (!m || !m->ve_crds || !m->el_ve || !m->el_fa || !m->fa_el || !m->el_fa_el || !m->f_id || !m->v_id ||
+      !m->fa_type || !m->normals || !m->tangents || !m->areas || !m->in_dia || !m->volume)
+    return edge::fail(EDGE_ERR_ARG, "mesh output arrays missing");
+  std::memcpy(m->ve_crds, in->ve_crds, sizeof(double) * 3 * in->n_ve);
+  // positively oriented vertex order (vertices 2 and 3 swapped on a negative determinant)
+  for (int64_t e = 0; e < in->n_el; e++) {
+    int64_t ve[4];
+    for (int v = 0; v < 4; v++) ve[v] = in->el_ve[e * 4 + v];
+    double a[3][3], cr[3];
+    for (int r = 0; r < 3; r++)
+      for (int d = 0; d < 3; d++) a[r][d] = in->ve_crds[3 * ve[r + 1] + d] - in->ve_crds[3 * ve[0] + d];
+    cross(a[1], a[2], cr);
+    const double det = dot(a[0], cr);
+    const double scale = norm(a[0]) * norm(a[1]) * norm(a[2]);
+    if (!(std::fabs(det) > 1e-12 * scale)) return edge::fail(EDGE_ERR_MESH, "degenerate tet");
+    if (det < 0) std::swap(ve[2], ve[3]);
+    for (int v = 0; v < 4; v++) m->el_ve[e * 4 + v] = ve[v];
+    m->volume[e] = std::fabs(det) / 6.0;
+  }
+  std::unordered_map<TriKey, int32_t, TriHash> tags;
+  for (int64_t i = 0; i < in->n_bf; i++)
+    tags[tri_key(in->bf_ve[3 * i], in->bf_ve[3 * i + 1], in->bf_ve[3 * i + 2])] = in->bf_type[i];
+  auto face_type = [&](int64_t eL, int lf, int64_t eR, int32_t* ty) {
+    const int64_t* v = m->el_ve + eL * 4;
+    const auto it = tags.find(tri_key(v[kFaVe[lf][0]], v[kFaVe[lf][1]], v[kFaVe[lf][2]]));
+    *ty = it != tags.end() ? it->second : (eR < 0 ? in->default_bc : 0);
+    if (eR < 0 && (*ty == 0 || *ty == EDGE_FA_PERIODIC))
+      return edge::fail(EDGE_ERR_MESH, "a boundary face needs a boundary type (free surface or outflow)");
+    return EDGE_OK;
+  };
+  int64_t built = 0;
+  st = connect(m, in->n_el, [](int64_t v) { return v; }, face_type, nfa, &built);
+  if (st != EDGE_OK) return st;
+  return built == nfa ? EDGE_OK : edge::fail(EDGE_ERR_MESH, "face count mismatch");
 }

@@ -63,3 +63,34 @@ def box(n, extent=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), bc=(FA_PERIODIC,) * 3
         setattr(m, k, v.ctypes.data)
     _lib.check(L.edge_mesh_box(C.byref(p), C.byref(m)))
     return Mesh(**a)
+
+
+def _out_arrays(nve, nel, nfa):
+    return dict(
+        ve_crds=np.zeros((nve, 3)), el_ve=np.zeros((nel, 4), np.int64), el_fa=np.zeros((nel, 4), np.int64),
+        fa_el=np.zeros((nfa, 2), np.int64), el_fa_el=np.zeros((nel, 4), np.int64),
+        f_id=np.zeros((nel, 4), np.uint16), v_id=np.zeros((nel, 4), np.uint16), fa_type=np.zeros(nfa, np.int32),
+        normals=np.zeros((nfa, 3)), tangents=np.zeros((nfa, 2, 3)), areas=np.zeros(nfa), in_dia=np.zeros(nel),
+        volume=np.zeros(nel))
+
+
+def from_tets(ve_crds, el_ve, bf_ve=None, bf_type=None, default_bc=FA_OUTFLOW):
+    """Mesh of any tet4 elements: vertex coordinates [nVe][3], element
+    vertices [nEl][4] (any orientation), tagged triangles bf_ve [nBf][3] with
+    their face types bf_type [nBf] (boundary conditions, internal
+    boundaries); untagged boundary faces get default_bc."""
+    ve_crds = np.ascontiguousarray(ve_crds, dtype=np.float64)
+    el_ve = np.ascontiguousarray(el_ve, dtype=np.int64)
+    bf_ve = np.ascontiguousarray(np.zeros((0, 3)) if bf_ve is None else bf_ve, dtype=np.int64)
+    bf_type = np.ascontiguousarray(np.zeros(0) if bf_type is None else bf_type, dtype=np.int32)
+    t = _lib.TetsIn(len(ve_crds), len(el_ve), ve_crds.ctypes.data, el_ve.ctypes.data, len(bf_ve),
+                    bf_ve.ctypes.data, bf_type.ctypes.data, int(default_bc))
+    L = _lib.lib()
+    nfa = C.c_int64()
+    _lib.check(L.edge_mesh_tets_sizes(C.byref(t), C.byref(nfa)))
+    a = _out_arrays(len(ve_crds), len(el_ve), nfa.value)
+    m = _lib.MeshOut()
+    for k, v in a.items():
+        setattr(m, k, v.ctypes.data)
+    _lib.check(L.edge_mesh_tets(C.byref(t), C.byref(m)))
+    return Mesh(**a)

@@ -102,6 +102,25 @@ typedef struct { /* caller-allocated outputs, sizes from edge_mesh_box_sizes */
 int edge_mesh_box_sizes(const edge_box_params *p, int64_t *n_ve, int64_t *n_el, int64_t *n_fa);
 int edge_mesh_box(const edge_box_params *p, edge_mesh_out *m);
 
+/* Any tet4 mesh (the entity layout EDGE-V writes for the solver,
+ * src/mesh/EdgeV.cpp:182-466, built here from vertices and elements): the
+ * elements' vertex order is made positive, faces are matched through their
+ * vertices, and each face gets the type of a tagged triangle bf_ve with the
+ * same vertices (boundary conditions and internal boundaries, EDGE-V's
+ * physical groups), else 0 inside and default_bc on the boundary. */
+typedef struct {
+  int64_t n_ve, n_el;
+  const double *ve_crds;  /* [nVe][3] */
+  const int64_t *el_ve;   /* [nEl][4] */
+  int64_t n_bf;           /* tagged triangles */
+  const int64_t *bf_ve;   /* [nBf][3] */
+  const int32_t *bf_type; /* [nBf] */
+  int32_t default_bc;     /* EDGE_FA_FREE_SURFACE or EDGE_FA_OUTFLOW */
+} edge_tets_in;
+
+int edge_mesh_tets_sizes(const edge_tets_in *in, int64_t *n_fa);
+int edge_mesh_tets(const edge_tets_in *in, edge_mesh_out *m);
+
 /* ====================================================================== */
 /* Element-update solver: replaces                                          */
 /* edge::seismic::solvers::AderDg<real, 0, TET4, O, O, R>                   */
