@@ -8,8 +8,11 @@ single-run update on the same mesh is timed (the drop-in), and the FP64
 oracle (the vanilla-semantics CPU restatement) on a bounded sample of the
 same workload on the host cores.
 
-A step is one local + one neighbour pass over all elements.  Inputs are
-resident in HBM before the timed region.  Prints one JSON line.
+A step of the fused-16 run is the reference's (src/impl/seismic/steps.inc):
+the receivers' time prediction at the step start (64 receivers), one local
+pass over all elements, the point sources over the step (one Ricker
+moment-tensor source per run), one neighbour pass.  Inputs are resident in
+HBM before the timed region.  Prints one JSON line.
 """
 import argparse
 import json
@@ -100,27 +103,69 @@ def build_problem(n_hex, order, seed=3343):
     return m, rho, lam, mu, dt, q0
 
 
-def time_solver(solver, dofs, tint, dt, steps, warmup):
+def point_setup(solver, m, runs, seed=3343, n_recv=64):
+    """One Ricker point source per run (peak frequency U[1, 5], a random
+    symmetric moment tensor with N(0, 1) entries, at a seeded point) and
+    n_recv seeded receivers recording every run at every step start."""
+    from edge_amd.points import PointSources, Receivers, moment_tensor_scalings, ricker
+    rng = np.random.default_rng(seed + 1)
+    lo, hi = m.ve_crds.min(axis=0), m.ve_crds.max(axis=0)
+    pts = lo + (hi - lo) * rng.uniform(0.05, 0.95, size=(runs, 3))
+    f0 = rng.uniform(1.0, 5.0, runs)
+    dts = 1e-3 / f0  # 1,000 samples per period
+    t0 = -3.0 / f0
+    series = [ricker(f0[r], 0.0 - t0[r], dts[r], int(6.0 / f0[r] / dts[r]) + 1) for r in range(runs)]
+    scas = []
+    for r in range(runs):
+        a = rng.normal(size=(3, 3))
+        scas.append(moment_tensor_scalings(np.triu(a) + np.triu(a, 1).T))
+    src = PointSources(solver, pts, np.arange(runs), np.array(scas), t0, dts, series)
+    recv = Receivers(solver, lo + (hi - lo) * rng.uniform(0.05, 0.95, size=(n_recv, 3)), freq=1.0)
+    return src, recv
+
+
+def time_solver(solver, dofs, tint, dt, steps, warmup, points=None):
     """Times exactly `steps` steps between synchronisations; per-kernel
     durations from HIP events recorded around each launch on the stream
-    the kernels run on."""
+    the kernels run on.  With points = (sources, receivers), every step is
+    the reference's: receivers at the step start, local, sources over the
+    step, neigh."""
     import torch
     stream = torch.cuda.current_stream()
-    for _ in range(warmup):
-        solver.step(dofs, tint, dt, 1, stream=stream)
+    src, recv = points if points is not None else (None, None)
+    out = None
+    if recv is not None:
+        out = torch.empty((recv.n, 1, 9, solver.n_runs), dtype=dofs.dtype, device=dofs.device)
+        tau = np.zeros((recv.n, 1))
+
+    def one(i, ev=None):
+        t = i * dt
+        if recv is not None:
+            recv.eval_into(dofs, tau, out, stream)
+        if ev is not None:
+            ev[0].record(stream)
+        solver.local(dofs, tint, dt, stream=stream)
+        if ev is not None:
+            ev[1].record(stream)
+        if src is not None:
+            src.apply(t, t + dt, dofs, stream)
+        if ev is not None:
+            ev[2].record(stream)
+        solver.neigh(dofs, tint, stream=stream)
+        if ev is not None:
+            ev[3].record(stream)
+
+    for i in range(warmup):
+        one(i)
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
     t0 = time.perf_counter()
     for i in range(steps):
-        ev[i][0].record(stream)
-        solver.local(dofs, tint, dt, stream=stream)
-        ev[i][1].record(stream)
-        solver.neigh(dofs, tint, stream=stream)
-        ev[i][2].record(stream)
+        one(warmup + i, ev[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     loc = np.mean([e[0].elapsed_time(e[1]) for e in ev]) * 1e-3
-    nei = np.mean([e[1].elapsed_time(e[2]) for e in ev]) * 1e-3
+    nei = np.mean([e[2].elapsed_time(e[3]) for e in ev]) * 1e-3
     return wall, loc, nei
 
 
@@ -164,6 +209,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 MFMA-tiled")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-points", action="store_true",
+                    help="time the element update alone (no point sources and receivers in the step)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration)")
@@ -190,8 +237,12 @@ def main():
     proj = bool(s32.kernel_info()[1] & PATH_PROJECTIONS)
     d32 = (torch.from_numpy(q0)[..., None] * scale).to(torch.float32).cuda().contiguous()
     t32 = torch.zeros_like(d32)
+    points = None if args.no_points else point_setup(s32, m, runs)
     print("[bench] fp32 fused-16 timing", file=sys.stderr, flush=True)
-    wall, loc, nei = time_solver(s32, d32, t32, dt, args.steps, args.warmup)
+    wall, loc, nei = time_solver(s32, d32, t32, dt, args.steps, args.warmup, points)
+    if points is not None:
+        for p in points:
+            p.close()
     ok = bool(torch.isfinite(d32).all())
     res["fp32"] = dict(wall=wall, local=loc, neigh=nei, finite=ok)
     del d32, t32
@@ -230,7 +281,9 @@ def main():
         "warmup": args.warmup, "ms_per_step": 1e3 * res["fp32"]["wall"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"elastic tet4 O{order} FP32 fused-{runs}, {m.n_el} tets (32^3 hexes x 6, "
-                               f"+-10% jitter, random rho/vp), plane-wave initial DOFs scaled per run",
+                               f"+-10% jitter, random rho/vp), plane-wave initial DOFs scaled per run"
+                               + ("" if args.no_points else f", one Ricker moment-tensor point source per run, "
+                                  f"64 receivers sampled every step"),
                    "elements": m.n_el, "order": order, "runs": runs, "dt": dt, "finite": res["fp32"]["finite"]},
         "roofline": roof,
     }

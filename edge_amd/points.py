@@ -128,6 +128,14 @@ class Receivers:
                     self.values[i].append(vals[i, j])
         return k
 
+    def eval_into(self, dofs, tau, out, stream=None):
+        """Evaluates every receiver at relative times tau [n][k] of the coming
+        step into the device tensor out [n][k][9][R], without host copies of
+        the results (no output-time bookkeeping)."""
+        tau = np.ascontiguousarray(tau, dtype=np.float64)
+        _lib.check(_lib.lib().edge_recv_eval(self._h, self.solver._h, dofs.data_ptr(), tau.shape[1],
+                                             tau.ctypes.data, out.data_ptr(), _stream_ptr(stream)))
+
     def seismogram(self, i):
         """(times [k], values [k][9][R]) of receiver i."""
         return np.array(self.times[i]), np.array(self.values[i]).reshape(-1, 9, self.solver.n_runs)
