@@ -73,6 +73,13 @@ int launch_neigh_visco(const SolverDev<T>& s, int64_t first, int64_t n, const T*
                        hipStream_t st);
 bool visco_supported(int order, int n_runs, int prec_bytes, int n_rms);
 
+// FP64 single-run kernels on FP64 MFMA (kernels_plain.hip).
+bool plain64_supported(int order, int n_runs, int prec_bytes);
+int launch_local_plain64(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs,
+                         double* t_int, hipStream_t st);
+int launch_neigh_plain64(const SolverDev<double>& s, int64_t first, int64_t n, const double* t_int, double* dofs,
+                         hipStream_t st);
+
 // What kernels outside solver.hip need of a solver handle.
 struct SolverRef {
   int precision = 64, order = 0, n_runs = 0, n_modes = 0, n_rms = 0;

@@ -127,7 +127,7 @@ int check_cfg(const edge_el_config* c) {
   if (c->n_runs != 1 && c->n_runs != 8 && c->n_runs != 16)
     return edge::fail(EDGE_ERR_UNSUPPORTED, "n_runs must be 1, 8 or 16");
   if (c->precision != 64 && c->precision != 32) return edge::fail(EDGE_ERR_UNSUPPORTED, "precision must be 32 or 64");
-  if (c->kernel < 0 || c->kernel > 3) return edge::fail(EDGE_ERR_ARG, "kernel selector");
+  if (c->kernel < 0 || c->kernel > 4) return edge::fail(EDGE_ERR_ARG, "kernel selector");
   return EDGE_OK;
 }
 
@@ -210,6 +210,7 @@ int finish_mfma(edge_el_solver* s) {
 int pick_kernel(edge_el_solver* s) {
   const bool mfma_ok = edge::mfma_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision);
   const bool gen_ok = edge::generic_supported(s->cfg.order, s->cfg.n_runs, (int)s->real_bytes);
+  const bool p64_ok = edge::plain64_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8);
   if (s->cfg.kernel == 2 || s->cfg.kernel == 3) {
     if (!mfma_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no MFMA-tiled kernel for this configuration");
     s->kernel = 2;
@@ -217,10 +218,13 @@ int pick_kernel(edge_el_solver* s) {
   } else if (s->cfg.kernel == 1) {
     if (!gen_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "generic kernel does not fit this configuration");
     s->kernel = 1;
+  } else if (s->cfg.kernel == 4) {
+    if (!p64_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "the FP64 MFMA kernel needs FP64 and one run");
+    s->kernel = 4;
   } else {
-    s->kernel = mfma_ok ? 2 : 1;
+    s->kernel = mfma_ok ? 2 : (p64_ok ? 4 : 1);
     s->want_proj = mfma_ok;
-    if (!mfma_ok && !gen_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no kernel for this configuration");
+    if (!mfma_ok && !p64_ok && !gen_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no kernel for this configuration");
   }
   return EDGE_OK;
 }
@@ -258,6 +262,8 @@ int local_t(edge_el_solver* s, int64_t first, int64_t n, double dt, void* dofs, 
   if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_local_a)");
   const auto d = dev_view<T>(s);
   if (s->kernel == 2) return edge::launch_local_mfma<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
+  if constexpr (sizeof(T) == 8)
+    if (s->kernel == 4) return edge::launch_local_plain64(d, first, n, dt, (double*)dofs, (double*)t_int, st);
   return edge::launch_local_generic<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
 }
 
@@ -266,6 +272,8 @@ int neigh_t(edge_el_solver* s, int64_t first, int64_t n, const void* t_int, void
   if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_neigh_a)");
   const auto d = dev_view<T>(s);
   if (s->kernel == 2) return edge::launch_neigh_mfma<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
+  if constexpr (sizeof(T) == 8)
+    if (s->kernel == 4) return edge::launch_neigh_plain64(d, first, n, (const double*)t_int, (double*)dofs, st);
   return edge::launch_neigh_generic<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
 }
 
@@ -313,8 +321,14 @@ int neigh_lts_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, const
   auto d = dev_view<T>(s);
   d.nb.ne = s->ne_lts;
   if (d.rcv) d.rcv = s->rcv_lts;
-  int rc = s->kernel == 2 ? edge::launch_neigh_mfma<T>(d, first, n, (const T*)t0, (T*)dofs, st)
-                          : edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
+  int rc;
+  if (s->kernel == 2)
+    rc = edge::launch_neigh_mfma<T>(d, first, n, (const T*)t0, (T*)dofs, st);
+  else if constexpr (sizeof(T) == 8)
+    rc = s->kernel == 4 ? edge::launch_neigh_plain64(d, first, n, (const double*)t0, (double*)dofs, st)
+                        : edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
+  else
+    rc = edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
   if (rc) return rc;
   const auto f = list_range(s->h_lts_if, first, n);
   return edge::launch_lts_faces<T>(dev_view<T>(s), s->fa_lts, s->lts_if + f.first, f.second - f.first, first_ts,

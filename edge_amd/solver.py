@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib
 from .setups import n_modes
 
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES, KERNEL_MFMA64 = 0, 1, 2, 3, 4
 PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS, PATH_VISCO = 1, 2, 4, 8
 
 
@@ -53,7 +53,7 @@ class ElasticSolver:
 
     # ------------------------------------------------------------------
     def kernel_info(self):
-        """(kernel, flags): 1 generic / 2 MFMA-tiled; flags PATH_* bits."""
+        """(kernel, flags): 1 generic / 2 MFMA-tiled / 4 FP64 MFMA; flags PATH_* bits."""
         k, f = C.c_int32(), C.c_int32()
         _lib.check(_lib.lib().edge_el_kernel_info(self._h, C.byref(k), C.byref(f)))
         return k.value, f.value

@@ -134,8 +134,9 @@ typedef struct {
   int32_t order;     /* O: 2..5 */
   int32_t n_runs;    /* fused runs R (innermost DOF axis): 1, 8 or 16 */
   int32_t precision; /* 64 or 32 (arithmetic and storage type of DOFs) */
-  int32_t kernel;    /* 0 = pick automatically, 1 = generic, 2 = MFMA-tiled,
-                        3 = MFMA-tiled with the neighbour step reading tInt tiles */
+  int32_t kernel;    /* 0 = pick automatically, 1 = generic, 2 = MFMA-tiled (FP32, 16 runs),
+                        3 = MFMA-tiled with the neighbour step reading tInt tiles,
+                        4 = FP64 single run on FP64 MFMA */
 } edge_el_config;
 
 typedef struct { /* host arrays in the reference's entity layout */
@@ -349,7 +350,8 @@ int edge_el_local_lts(edge_el_solver *s, int64_t first, int64_t n_els, double dt
 int edge_el_neigh_lts(edge_el_solver *s, int64_t first, int64_t n_els, int32_t first_ts, const void *t_int0,
                       const void *t_int1, const void *t_int2, void *dofs, void *stream);
 
-/* Which kernels a solver runs: *kernel = 1 (generic) or 2 (MFMA-tiled);
+/* Which kernels a solver runs: *kernel = 1 (generic), 2 (MFMA-tiled) or 4
+ * (FP64 single run on FP64 MFMA);
  * *flags bit 0: sparse (elastic-pattern) star matrices, bit 1: rank-3
  * factored flux solvers, bit 2: the neighbour step reads face projections,
  * bit 3: viscoelastic (anelastic DOFs).

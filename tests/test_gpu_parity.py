@@ -10,8 +10,8 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES, PATH_PROJECTIONS, PATH_RANK3_FS,
-                             PATH_SPARSE_STAR, ElasticSolver)
+from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, PATH_PROJECTIONS,
+                             PATH_RANK3_FS, PATH_SPARSE_STAR, ElasticSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -58,10 +58,12 @@ def test_setup_matches_oracle(gpu, bc):
     assert rel_l2(fs, ofs) <= 1e-13
 
 
+@pytest.mark.parametrize("kernel", [KERNEL_GENERIC, KERNEL_MFMA64])
 @pytest.mark.parametrize("order", [2, 3, 4, 5])
-def test_fp64_local_neigh_match_oracle(gpu, order):
+def test_fp64_local_neigh_match_oracle(gpu, order, kernel):
     m, rho, lam, mu = _problem()
-    s = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=KERNEL_GENERIC)
+    s = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=kernel)
+    assert s.kernel_info()[0] == kernel
     mo = orc.MeshOracle(m, order, rho=rho, lam=lam, mu=mu)
     dofs0 = _ic(m, order, 1)
     dt = S.dt_cfl(m, rho, lam, mu, order)
@@ -81,6 +83,7 @@ def test_fp64_local_neigh_match_oracle(gpu, order):
 def test_fp64_steps_match_oracle(gpu, order, bc):
     m, rho, lam, mu = _problem(bc=bc)
     s = ElasticSolver(m, order, 1, 64, rho, lam, mu)
+    assert s.kernel_info()[0] == KERNEL_MFMA64  # the FP64 single-run default
     mo = orc.MeshOracle(m, order, rho=rho, lam=lam, mu=mu)
     dofs0 = _ic(m, order, 1)
     dt = S.dt_cfl(m, rho, lam, mu, order)
@@ -100,7 +103,7 @@ def test_random_star_and_flux_solvers_match_oracle(gpu):
     order = 4
     star = rng.normal(size=(m.n_el, 3, 81)) * 0.2
     fs = rng.normal(size=(2, m.n_el, 4, 81)) * 0.2
-    for prec, kern in [(64, KERNEL_GENERIC), (32, KERNEL_GENERIC)]:
+    for prec, kern in [(64, KERNEL_GENERIC), (64, KERNEL_MFMA64), (32, KERNEL_GENERIC)]:
         s = ElasticSolver(m, order, 1, prec, star=star, fs=fs, kernel=kern)
         mo = orc.MeshOracle(m, order, star=star, fs=fs)
         dofs0 = rng.normal(size=(m.n_el, 9, S.n_modes(order), 1))
