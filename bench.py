@@ -199,6 +199,116 @@ def cpu_baseline(m, order, runs, rho, lam, mu, dt, q0, budget_s=15.0):
                       f"{wall:.1f} s, OpenMP over elements; vanilla-semantics restatement, LIBXSMM unavailable"}
 
 
+def visco_flops(order, n_rms):
+    """Dense vanilla FLOPs per element-step and run of the viscoelastic
+    update (TimePredVanilla / VolIntVanilla / SurfIntVanilla with N_RMS > 0,
+    no CK mode truncation): (local, neigh)."""
+    n = order * (order + 1) * (order + 2) // 6
+    nf = order * (order + 1) // 2
+    nv = (order - 1) * order * (order + 1) // 6
+    ck_stage = 3 * (2 * 9 * n * n + 2 * 81 * n + 2 * 18 * n) + n_rms * (2 * 36 * n + 2 * 6 * n) + 2 * (9 + 6 * n_rms) * n
+    ck = (order - 1) * ck_stage
+    vol = 3 * (2 * 9 * nv * n + 2 * 81 * n + 2 * 18 * n) + n_rms * (2 * 36 * n + 3 * 6 * n)
+    face = 2 * 9 * n * nf + 2 * 81 * nf + 2 * 9 * nf * n + 2 * 54 * nf + 2 * 6 * nf * n
+    return ck + vol + 4 * face + n_rms * 2 * 6 * n, 4 * face + n_rms * 2 * 6 * n
+
+
+def rows(args):
+    """Per-row measurements beside the headline (one JSON line per row):
+    viscoelastic updates, local time stepping, receivers and sources, and the
+    rupture net updates, each with HIP events around its launches."""
+    import torch
+    from edge_amd import mesh as M
+    from edge_amd import setups as S
+    from edge_amd.lts import Rate2Lts
+    from edge_amd.rupture import Rupture
+    from edge_amd.solver import ElasticSolver, ViscoSolver
+
+    def timed(fn, reps):
+        st = torch.cuda.current_stream()
+        fn(st)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn(st)
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / reps
+
+    m, rho, lam, mu, dt, q0 = build_problem(args.hex, args.order)
+    n = q0.shape[2]
+    # ---- viscoelastic (3 mechanisms, f_c = 1 Hz, ratio 10): FP32 8 runs and FP64 one run
+    rng = np.random.default_rng(5)
+    qp = rng.uniform(50, 200, m.n_el)
+    fl, fn_ = visco_flops(args.order, 3)
+    for runs, prec in ((8, 32), (1, 64)):
+        s = ViscoSolver(m, args.order, runs, prec, rho, lam, mu, qp, qp / 2, n_rms=3, freq_cen=1.0, freq_rat=10.0)
+        d = torch.from_numpy(q0)[..., None].repeat(1, 1, 1, runs).to(s.dtype).cuda().contiguous()
+        t, a = torch.zeros_like(d), s.alloc_a()
+        loc = timed(lambda st: s.local(d, t, dt, dofs_a=a, stream=st), 2)
+        nei = timed(lambda st: s.neigh(d, t, dofs_a=a, stream=st), 2)
+        peak = PEAK["fp32_mfma"] if prec == 32 else PEAK["fp64"]
+        print(json.dumps({"row": "a12 viscoelastic update", "kernel": "generic (LDS tiles, VALU)", "order": args.order,
+                          "runs": runs, "precision": prec, "n_rms": 3, "elements": m.n_el,
+                          "local_ms": loc * 1e3, "neigh_ms": nei * 1e3,
+                          "dof_updates_per_s": m.n_el * 27 * n * runs / (loc + nei),
+                          "tflops": (fl + fn_) * runs * m.n_el / (loc + nei) / 1e12,
+                          "roofline_frac": (fl + fn_) * runs * m.n_el / (loc + nei) / peak}), flush=True)
+        s.close()
+        del d, t, a
+    # ---- local time stepping: two groups (halves of the mesh), FP32 16 runs
+    s = ElasticSolver(m, args.order, 16, 32, rho, lam, mu)
+    d = (torch.from_numpy(q0)[..., None].repeat(1, 1, 1, 16)).to(torch.float32).cuda().contiguous()
+    t = torch.zeros_like(d)
+    half = m.n_el // 2
+    gts = timed(lambda st: s.step(d, t, dt / 2, 2, stream=st), 2)
+    lts = Rate2Lts(s, [(0, half), (half, m.n_el - half)], dt / 2)
+    cyc = timed(lambda st: lts.cycle(d, 1, stream=st), 2)
+    print(json.dumps({"row": "a7 local time stepping (rate 2)", "order": args.order, "runs": 16, "precision": 32,
+                      "elements": m.n_el, "interface_elements": int((lts.el_lts != 0).sum()),
+                      "gts_two_fine_steps_ms": gts * 1e3, "lts_cycle_ms": cyc * 1e3,
+                      "element_updates_per_cycle": int(half * 2 + (m.n_el - half)),
+                      "ms_per_element_update_ratio_lts_over_gts": (cyc / (half * 2 + m.n_el - half)) /
+                                                                   (gts / (2 * m.n_el))}), flush=True)
+    # ---- receivers (64) and sources (16) per step
+    s.set_lts(None, None)
+    src, recv = point_setup(s, m, 16)
+    out = torch.empty((recv.n, 1, 9, 16), dtype=torch.float32, device="cuda")
+    tau = np.zeros((recv.n, 1))
+    r_ms = timed(lambda st: recv.eval_into(d, tau, out, st), 5)
+    s_ms = timed(lambda st: src.apply(0.0, dt, d, st), 5)
+    print(json.dumps({"row": "a3 / (f) receivers and point sources", "receivers": recv.n, "sources": src.n,
+                      "receiver_eval_ms": r_ms * 1e3, "source_apply_ms": s_ms * 1e3}), flush=True)
+    src.close()
+    recv.close()
+    s.close()
+    del d, t
+    # ---- rupture net updates: 4,608 fault faces, 49 sub-faces (degree 3), 16 runs, FP64
+    nf, nsf, R = 4608, 49, 16
+    rng = np.random.default_rng(7)
+    nrm = np.tile([0.0, 0.0, 1.0], (nf, 1))
+    tan = np.tile([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]], (nf, 1, 1))
+    mat = np.tile([2.67, 32.0, 32.0], (nf, 1))
+    st0 = np.zeros((nf, nsf, R, 14))
+    st0[..., 0] = -120.0
+    st0[..., 2] = rng.uniform(70, 81.24, (nf, nsf, R))
+    st0[..., 5] = 0.677
+    glob = np.stack([np.full(R, 0.677), np.full(R, 0.525), np.full(R, 1 / 0.4)])
+    rup = Rupture(nsf, 343, R, nrm, tan, np.full(nf, 0.01), np.eye(3)[[2, 0, 1]], mat, mat, np.full(nf, 1e-3),
+                  np.full(nf, 1e-3), glob, st0)
+    dl = torch.tensor(rng.normal(size=rup.dof_shape()), device="cuda")
+    dr = torch.tensor(rng.normal(size=rup.dof_shape()), device="cuda")
+    nl, nr_ = rup.alloc(), rup.alloc()
+    r_s = timed(lambda st: rup.net_updates(dl, dr, nl, nr_, 1e-3, stream=st), 5)
+    items = nf * nsf * R
+    byts = items * (4 * 9 * 8 + 2 * 13 * 8 + 1) + nf * 324 * 8
+    print(json.dumps({"row": "a13/a14 rupture net updates", "faces": nf, "sub_faces": nsf, "runs": R,
+                      "precision": 64, "ms": r_s * 1e3, "bytes_per_launch": byts, "gb_per_s": byts / r_s / 1e9,
+                      "roofline_frac": byts / r_s / PEAK["hbm"]}), flush=True)
+    rup.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +319,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 MFMA-tiled")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--rows", action="store_true",
+                    help="measure the other rows (viscoelastic, LTS, receivers/sources, rupture) instead")
     ap.add_argument("--no-points", action="store_true",
                     help="time the element update alone (no point sources and receivers in the step)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -221,6 +333,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.rows:
+        rows(args)
         return
     import torch
     from edge_amd.solver import PATH_PROJECTIONS, ElasticSolver

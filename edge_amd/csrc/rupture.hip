@@ -30,7 +30,7 @@ struct edge_rupture {
   double* solv = nullptr;   // [nF][4][81]: T^-1, jump, flux left, flux right
   double* face = nullptr;   // [nF][3]: l_eq_m, cs/mu minus, cs/mu plus
   double* glob = nullptr;   // [3][R]: mus, mud, 1/Dc
-  double* state = nullptr;  // [nF][n_sf][R][14]
+  double* state = nullptr;  // [nF][14][n_sf][R] (the C ABI's [nF][n_sf][R][14], per face transposed)
   uint8_t* per = nullptr;   // [nF][n_sf][R]
 };
 
@@ -154,77 +154,85 @@ __global__ void k_rupture_setup(int64_t n_fa, const double* normals, const doubl
   face[fa * 3 + 2] = l_eq_m ? cdR : cdL;  // plus side
 }
 
-// One thread per (face, sub-face, run).
-__global__ void k_rupture_net_updates(int64_t first, int64_t n, int n_sf, int R, double dt, const double* solv,
-                                      const double* face, const double* glob, const double* dofs_l,
-                                      const double* dofs_r, double* net_l, double* net_r, double* state,
-                                      uint8_t* per_out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * n_sf * R) return;
-  const int r = (int)(i % R), sf = (int)((i / R) % n_sf);
-  const int64_t fa = first + i / ((int64_t)n_sf * R);
-  const double* s = solv + fa * 324;
-  const int64_t base = fa * 9 * n_sf * R + (int64_t)sf * R + r, qs = (int64_t)n_sf * R;
-  double qL[9], qR[9];
-  for (int q = 0; q < 9; q++) {  // rotate to face coordinates
-    double a = 0, b = 0;
-    for (int p = 0; p < 9; p++) {
-      a += s[q * 9 + p] * dofs_l[base + p * qs];
-      b += s[q * 9 + p] * dofs_r[base + p * qs];
+typedef const __attribute__((address_space(4))) double cdouble;
+
+// One workgroup per face, threads over its (sub-face, run) pairs: the face's
+// solvers and data are wave-uniform and come in by scalar loads (the tables
+// are read-only here).
+__global__ __launch_bounds__(256) void k_rupture_net_updates(int64_t first, int n_sf, int R, double dt,
+                                                             const double* solv_, const double* face_,
+                                                             const double* glob, const double* dofs_l,
+                                                             const double* dofs_r, double* net_l, double* net_r,
+                                                             double* state, uint8_t* per_out) {
+  const int64_t fa = first + blockIdx.x;
+  const cdouble* s = (const cdouble*)(solv_ + fa * 324);
+  const cdouble* face = (const cdouble*)(face_ + fa * 3);
+  for (int i = threadIdx.x; i < n_sf * R; i += blockDim.x) {
+    const int r = i % R, sf = i / R;
+    const int64_t base = fa * 9 * n_sf * R + (int64_t)sf * R + r, qs = (int64_t)n_sf * R;
+    double qL[9], qR[9];
+    for (int q = 0; q < 9; q++) {  // rotate to face coordinates
+      double a = 0, b = 0;
+      for (int p = 0; p < 9; p++) {
+        a += s[q * 9 + p] * dofs_l[base + p * qs];
+        b += s[q * 9 + p] * dofs_r[base + p * qs];
+      }
+      qL[q] = a;
+      qR[q] = b;
     }
-    qL[q] = a;
-    qR[q] = b;
-  }
-  double ms[9];
-  for (int q = 0; q < 9; q++) {
-    double v = qL[q];
-    for (int p = 0; p < 9; p++) v += s[81 + q * 9 + p] * (qR[p] - qL[p]);
-    ms[q] = v;
-  }
-  // linear slip weakening: state sn0, co0, ss0[2], ss0A, muf, st, tr[2], sr[2], dd[2]
-  double* st = state + ((fa * n_sf + sf) * R + r) * kStateVals;
-  const double mus = glob[r], mud = glob[R + r], dcInv = glob[2 * R + r];
-  const bool l_eq_m = face[fa * 3] != 0.0;
-  const double cdM = face[fa * 3 + 1], cdP = face[fa * 3 + 2];
-  double msM[9], msP[9];
-  for (int q = 0; q < 9; q++) msM[q] = msP[q] = ms[q];
-  double strength = st[5] * (st[0] + ms[0]) + st[1];
-  strength = strength < 0 ? -strength : 0.0;
-  const double s1 = st[2] + ms[3], s2 = st[3] + ms[5];
-  const double shear = sqrt(s1 * s1 + s2 * s2);
-  const bool failing = shear > strength;
-  const double scale = strength / shear;
-  const double tr0 = failing ? s1 * scale - st[2] : ms[3];
-  const double tr1 = failing ? s2 * scale - st[3] : ms[5];
-  msM[3] = msP[3] = tr0;
-  msM[5] = msP[5] = tr1;
-  const double d1 = tr0 - ms[3], d2 = tr1 - ms[5];
-  msM[7] -= cdM * d1;
-  msM[8] -= cdM * d2;
-  msP[7] += cdP * d1;
-  msP[8] += cdP * d2;
-  const double sr0 = msP[7] - msM[7], sr1 = msP[8] - msM[8];
-  const double dd0 = st[11] + fabs(sr0) * dt, dd1 = st[12] + fabs(sr1) * dt;
-  const double arg = mus - (mus - mud) * dcInv * sqrt(dd0 * dd0 + dd1 * dd1);
-  st[5] = arg > mud ? arg : mud;
-  st[6] = strength;
-  st[7] = tr0 + st[2];
-  st[8] = tr1 + st[3];
-  st[9] = sr0;
-  st[10] = sr1;
-  st[11] = dd0;
-  st[12] = dd1;
-  per_out[(fa * n_sf + sf) * R + r] = failing ? 1 : 0;
-  const double* mL = l_eq_m ? msM : msP;
-  const double* mR = l_eq_m ? msP : msM;
-  for (int q = 0; q < 9; q++) {  // left-going updates subtract, right-going add
-    double a = 0, b = 0;
-    for (int p = 0; p < 9; p++) {
-      a += s[162 + q * 9 + p] * mL[p];
-      b += s[243 + q * 9 + p] * mR[p];
+    double ms[9];
+    for (int q = 0; q < 9; q++) {
+      double v = qL[q];
+      for (int p = 0; p < 9; p++) v += s[81 + q * 9 + p] * (qR[p] - qL[p]);
+      ms[q] = v;
     }
-    net_l[base + q * qs] = -a * dt;
-    net_r[base + q * qs] = b * dt;
+    // linear slip weakening: state sn0, co0, ss0[2], ss0A, muf, st, tr[2], sr[2], dd[2]
+    // the face's state is [14][n_sf][R] on the device: value k of this sub-face and run at st[k * sst]
+    double* st = state + (fa * kStateVals * n_sf + sf) * R + r;
+    const int64_t sst = (int64_t)n_sf * R;
+    const double mus = glob[r], mud = glob[R + r], dcInv = glob[2 * R + r];
+    const bool l_eq_m = face[0] != 0.0;
+    const double cdM = face[1], cdP = face[2];
+    double msM[9], msP[9];
+    for (int q = 0; q < 9; q++) msM[q] = msP[q] = ms[q];
+    double strength = st[5 * sst] * (st[0 * sst] + ms[0]) + st[1 * sst];
+    strength = strength < 0 ? -strength : 0.0;
+    const double s1 = st[2 * sst] + ms[3], s2 = st[3 * sst] + ms[5];
+    const double shear = sqrt(s1 * s1 + s2 * s2);
+    const bool failing = shear > strength;
+    const double scale = strength / shear;
+    const double tr0 = failing ? s1 * scale - st[2 * sst] : ms[3];
+    const double tr1 = failing ? s2 * scale - st[3 * sst] : ms[5];
+    msM[3] = msP[3] = tr0;
+    msM[5] = msP[5] = tr1;
+    const double d1 = tr0 - ms[3], d2 = tr1 - ms[5];
+    msM[7] -= cdM * d1;
+    msM[8] -= cdM * d2;
+    msP[7] += cdP * d1;
+    msP[8] += cdP * d2;
+    const double sr0 = msP[7] - msM[7], sr1 = msP[8] - msM[8];
+    const double dd0 = st[11 * sst] + fabs(sr0) * dt, dd1 = st[12 * sst] + fabs(sr1) * dt;
+    const double arg = mus - (mus - mud) * dcInv * sqrt(dd0 * dd0 + dd1 * dd1);
+    st[5 * sst] = arg > mud ? arg : mud;
+    st[6 * sst] = strength;
+    st[7 * sst] = tr0 + st[2 * sst];
+    st[8 * sst] = tr1 + st[3 * sst];
+    st[9 * sst] = sr0;
+    st[10 * sst] = sr1;
+    st[11 * sst] = dd0;
+    st[12 * sst] = dd1;
+    per_out[(fa * n_sf + sf) * R + r] = failing ? 1 : 0;
+    const double* mL = l_eq_m ? msM : msP;
+    const double* mR = l_eq_m ? msP : msM;
+    for (int q = 0; q < 9; q++) {  // left-going updates subtract, right-going add
+      double a = 0, b = 0;
+      for (int p = 0; p < 9; p++) {
+        a += s[162 + q * 9 + p] * mL[p];
+        b += s[243 + q * 9 + p] * mR[p];
+      }
+      net_l[base + q * qs] = -a * dt;
+      net_r[base + q * qs] = b * dt;
+    }
   }
 }
 
@@ -287,7 +295,14 @@ extern "C" int edge_rupture_create(const edge_rupture_config* cfg, int64_t n_fac
   al((void**)&d_vr, sizeof(double) * n_faces);
   al((void**)&d_bad, sizeof(int));
   cp(s->glob, glob, sizeof(double) * 3 * R);
-  cp(s->state, state0, sizeof(double) * 14 * ns);
+  {  // [nF][n_sf][R][14] -> [nF][14][n_sf][R]
+    const int64_t per = (int64_t)cfg->n_sub_faces * R;
+    std::vector<double> t((size_t)14 * ns);
+    for (int64_t f = 0; f < n_faces; f++)
+      for (int64_t i = 0; i < per; i++)
+        for (int k = 0; k < 14; k++) t[(f * 14 + k) * per + i] = state0[(f * per + i) * 14 + k];
+    cp(s->state, t.data(), sizeof(double) * 14 * ns);
+  }
   cp(d_nrm, normals, sizeof(double) * 3 * n_faces);
   cp(d_tan, tangents, sizeof(double) * 6 * n_faces);
   cp(d_area, areas, sizeof(double) * n_faces);
@@ -339,11 +354,9 @@ extern "C" int edge_rupture_net_updates(edge_rupture* s, int64_t first, int64_t 
   if (first < 0 || n_faces < 0 || first + n_faces > s->n_fa) return fail(EDGE_ERR_ARG, "face range out of bounds");
   if (!dofs_l || !dofs_r || !net_l || !net_r) return fail(EDGE_ERR_ARG, "device arrays missing");
   if (n_faces == 0) return EDGE_OK;
-  const int64_t nt = n_faces * s->cfg.n_sub_faces * s->cfg.n_runs;
-  const int bs = 128;
-  hipLaunchKernelGGL(edge::k_rupture_net_updates, dim3((unsigned)((nt + bs - 1) / bs)), dim3(bs), 0,
-                     (hipStream_t)stream, first, n_faces, s->cfg.n_sub_faces, s->cfg.n_runs, dt, s->solv, s->face,
-                     s->glob, dofs_l, dofs_r, net_l, net_r, s->state, s->per);
+  hipLaunchKernelGGL(edge::k_rupture_net_updates, dim3((unsigned)n_faces), dim3(256), 0, (hipStream_t)stream, first,
+                     s->cfg.n_sub_faces, s->cfg.n_runs, dt, s->solv, s->face, s->glob, dofs_l, dofs_r, net_l, net_r,
+                     s->state, s->per);
   EDGE_HIP_TRY(hipGetLastError());
   return EDGE_OK;
 }
@@ -354,7 +367,14 @@ extern "C" int edge_rupture_get(const edge_rupture* s, double* solvers, double* 
   const size_t ns = (size_t)s->n_fa * s->cfg.n_sub_faces * s->cfg.n_runs;
   EDGE_HIP_TRY(hipDeviceSynchronize());
   if (solvers) EDGE_HIP_TRY(hipMemcpy(solvers, s->solv, sizeof(double) * 324 * s->n_fa, hipMemcpyDeviceToHost));
-  if (state) EDGE_HIP_TRY(hipMemcpy(state, s->state, sizeof(double) * 14 * ns, hipMemcpyDeviceToHost));
+  if (state) {  // [nF][14][n_sf][R] -> [nF][n_sf][R][14]
+    std::vector<double> t((size_t)14 * ns);
+    EDGE_HIP_TRY(hipMemcpy(t.data(), s->state, sizeof(double) * 14 * ns, hipMemcpyDeviceToHost));
+    const int64_t per = (int64_t)s->cfg.n_sub_faces * s->cfg.n_runs;
+    for (int64_t f = 0; f < s->n_fa; f++)
+      for (int64_t i = 0; i < per; i++)
+        for (int k = 0; k < 14; k++) state[(f * per + i) * 14 + k] = t[(f * 14 + k) * per + i];
+  }
   if (per) EDGE_HIP_TRY(hipMemcpy(per, s->per, ns, hipMemcpyDeviceToHost));
   return EDGE_OK;
 }
