@@ -249,7 +249,9 @@ def rows(args):
         loc = timed(lambda st: s.local(d, t, dt, dofs_a=a, stream=st), 2)
         nei = timed(lambda st: s.neigh(d, t, dofs_a=a, stream=st), 2)
         peak = PEAK["fp32_mfma"] if prec == 32 else PEAK["fp64"]
-        print(json.dumps({"row": "a12 viscoelastic update", "kernel": "generic (LDS tiles, VALU)", "order": args.order,
+        kern = {1: "generic (LDS tiles, VALU)", 2: "MFMA (paired products, elementwise anelastic terms)"}
+        print(json.dumps({"row": "a12 viscoelastic update", "kernel": kern.get(s.kernel_info()[0], "?"),
+                          "order": args.order,
                           "runs": runs, "precision": prec, "n_rms": 3, "elements": m.n_el,
                           "local_ms": loc * 1e3, "neigh_ms": nei * 1e3,
                           "dof_updates_per_s": m.n_el * 27 * n * runs / (loc + nei),

@@ -48,6 +48,10 @@ struct SolverDev {
   const T* star_a = nullptr;  // [nEl][3][6][3] anelastic star matrices
   const T* src_a = nullptr;   // [nEl][n_rms][6][6] anelastic source matrices
   const T* fs_a = nullptr;    // [2][nEl][4][6][9] anelastic flux solvers
+  // factors U_A [2][nEl][4][6][3] of the anelastic flux solvers over the
+  // elastic factors' V^T (fs_a = U_A V^T); null unless the MFMA viscoelastic
+  // kernels run
+  const T* fsfa = nullptr;
 };
 
 // Non-zero pattern of the elastic star matrices: the union of the patterns
@@ -72,6 +76,14 @@ template <typename T>
 int launch_neigh_visco(const SolverDev<T>& s, int64_t first, int64_t n, const T* t_int, T* dofs, T* dofs_a,
                        hipStream_t st);
 bool visco_supported(int order, int n_runs, int prec_bytes, int n_rms);
+
+// MFMA viscoelastic kernels (kernels_visco_mfma.hip): FP32, 8 fused runs,
+// 1 to 3 mechanisms, elastic star pattern and rank-3 factored flux solvers.
+bool visco_mfma_supported(int order, int n_runs, int precision, int n_rms);
+int launch_local_visco_mfma(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs,
+                            float* dofs_a, float* t_int, hipStream_t st);
+int launch_neigh_visco_mfma(const SolverDev<float>& s, int64_t first, int64_t n, const float* t_int, float* dofs,
+                            float* dofs_a, hipStream_t st);
 
 // FP64 single-run kernels on FP64 MFMA (kernels_plain.hip).
 bool plain64_supported(int order, int n_runs, int prec_bytes);

@@ -1,0 +1,587 @@
+// MFMA viscoelastic element kernels for 8 fused runs in FP32 (gfx950).
+//
+// One workgroup per element.  The products with the reference-element
+// matrices (stiffness and flux matrices) run on v_mfma_f32_16x16x4f32 with
+// the runs on the N dimension.  Eight runs fill half of the 16 columns, so
+// every MFMA carries two products that share its A operand (the same
+// reference matrix) in its column halves: lanes with c = lane % 16 < 8 one
+// product, lanes with c >= 8 another, both for run c % 8.
+//   A operand (16 x 4)  lane (g, c) = M[k(g)][16 t + c]
+//   B operand (4 x 16)  lane (g, c) = X_h[k(g)][c % 8],  h = c / 8
+//   D (16 x 16)         lane (g, c), register v: row 16 t + 4 g + v, product h, run c % 8
+// with g = lane / 16.  Everything that mixes quantities at one (mode, run)
+// position (star matrices, anelastic source and relaxation terms, flux-solver
+// factors) is elementwise: the positions are spread over all threads of the
+// workgroup, with the anelastic state in registers, and the raw products
+// move between the MFMA and the elementwise phases through LDS.
+//
+// Local step, three waves; wave d takes direction d in the CK and volume
+// products (src/impl/seismic/solvers/AderDg.hpp:362-533):
+//  CK (kernels/TimePredVanilla.hpp:200-306; with attenuation every
+//  derivative keeps all N modes): per stage de the raw products of
+//  der_{de-1} with the negated transposed stiffness matrix of direction d,
+//    P_{d,i} = der_{de-1}[6 + i] (-K_d^T M^-1)                  (i = 0..2)
+//    V_{d,r} = (sum_p S_d[6 + r][p] der_{de-1}[p]) (-K_d^T M^-1)  (r = 0..2)
+//  (the elastic star pattern: stress rows read only velocities, velocity
+//  rows only stresses), then per position
+//    der_de[q < 6] = sum_d sum_i S_d[q][6 + i] P_{d,i} + sum_rm E_rm derA_rm,
+//    der_de[6 + r] = sum_d V_{d,r},   A = sum_d S^A_d P_d,
+//    derA_rm <- w_rm (A + derA_rm),   tInt += c_de der_de,   tIntA_rm += c_de derA_rm;
+//  volume (kernels/VolIntVanilla.hpp:189-241): the same raw products of tInt
+//  with K_d M^-1, Q[q < 6] += sum_d S_d P_d + sum_rm E_rm tIntA_rm,
+//  Q[6 + r] += sum_d V_d, QA_rm += w_rm (sum_d S^A_d P_d - tIntA_rm);
+//  local surface (kernels/SurfIntVanilla.hpp:212-262, SurfInt.hpp:139-156),
+//  waves by face, with the rank-3 factors fs = U V^T and fsA = U_A V^T:
+//    w_j = (V^T tInt)_j F^L_fa,  L_j = w_j F^T_fa,
+//    Q += sum_j U[.][j] L_j,  QA_rm += w_rm sum_j U_A[.][j] L_j.
+// Neighbour step (kernels/SurfIntVanilla.hpp:274-357), four waves, wave f
+// projects the tInt of face f's neighbour with F^N and the neighbour-side
+// factors; the lifted tiles of the four faces are combined per position.
+// Summation order differs from the reference's loops; results agree to
+// rounding.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace edge {
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) float cfloat;
+typedef const __attribute__((address_space(4))) int32_t cint32;
+typedef const __attribute__((address_space(4))) uint8_t cuint8;
+constexpr int R = 8;
+constexpr int kLocalThreads = 192, kNeighThreads = 256;
+
+__device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f zero4() { return v4f{0.f, 0.f, 0.f, 0.f}; }
+constexpr int tiles(int rows) { return (rows + 15) / 16; }
+constexpr int ksteps(int rows) { return (rows + 3) / 4; }
+
+// Per-element tables that no kernel writes come in through the constant
+// address space: wave-uniform values by scalar loads.
+__device__ __forceinline__ const cfloat* as_const(const float* p) { return (const cfloat*)p; }
+
+// blockIdx -> element offset: each of the 8 XCDs (workgroups dispatched round
+// robin) gets a contiguous element range, so neighbours share its L2.  The
+// grid holds 8 ceil(n / 8) workgroups.
+__device__ __forceinline__ int64_t xcd_element(int64_t n) {
+  const int64_t chunk = (n + 7) / 8;
+  const int64_t b = blockIdx.x;
+  return (b & 7) * chunk + (b >> 3);
+}
+
+// Guarded load without a branch: the index is clamped into the valid range
+// and the value masked afterwards.
+__device__ __forceinline__ float ldg(const float* __restrict__ p, int idx, bool ok) {
+  const float v = p[ok ? idx : 0];
+  return ok ? v : 0.f;
+}
+
+// Elastic star pattern (src/impl/seismic/setups/Elasticity.cpp:172-250):
+// velocity row 6 + r reads the stresses kVelCols[r], stress row q the
+// velocities 6 + kStrVel[q][0 .. kStrNnz[q] - 1].
+constexpr int kVelCols[3][3] = {{0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
+constexpr int kStrNnz[6] = {3, 3, 3, 2, 2, 2};
+constexpr int kStrVel[6][3] = {{0, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 1, 0}, {1, 2, 0}, {0, 2, 0}};
+// Anelastic star pattern S^A_d [6][3] (src/impl/seismic/setups/
+// ViscoElasticity.cpp:233-290): row q reads the velocities kAnCol[q][..].
+constexpr int kAnNnz[6] = {1, 1, 1, 2, 2, 2};
+constexpr int kAnCol[6][2] = {{0, 0}, {1, 0}, {2, 0}, {0, 1}, {1, 2}, {0, 2}};
+// Anelastic source pattern E_rm [6][6] (src/impl/seismic/setups/
+// ViscoElasticity.h:379-458): the normal-stress block rows 0..2 x columns
+// 0..2 and the diagonal of rows 3..5.
+
+template <int O>
+struct VDims {
+  static constexpr int N = n_modes(O), NV = n_modes(O - 1), NF = n_face_modes(O);
+  static constexpr int NP = N * R;  // positions (mode, run) of one quantity
+  static constexpr int SZ = 9 * NP, SZA = 6 * NP;
+};
+
+// Raw products of one direction (wave): rows k < NK of the tile x (LDS,
+// [9][N][R]) with the matrix m ([N][N], rows k, columns n < NN), written to
+// out (LDS, [6][N][R]) as P_0, P_1, P_2 (velocity tiles) and V_0, V_1, V_2
+// (velocity rows of the star-combined tile).  Column-half pairs: (P_0 | P_1),
+// (P_2 | V_0), (V_1 | V_2), so product 2 p + h sits in half h of pair p.
+template <int O, int NK, int NN>
+__device__ __forceinline__ void raw_products(const float* __restrict__ m, const cfloat* __restrict__ sd,
+                                             const float* __restrict__ x, float* __restrict__ out, int g, int c) {
+  using D = VDims<O>;
+  constexpr int N = D::N, NP = D::NP, TT = tiles(NN), KS = ksteps(NK);
+  const int h = c >> 3, r = c & 7;
+  float sv[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) sv[i * 3 + j] = sd[(6 + i) * 9 + kVelCols[i][j]];
+  v4f acc[TT][3];
+#pragma unroll
+  for (int t = 0; t < TT; t++)
+#pragma unroll
+    for (int p = 0; p < 3; p++) acc[t][p] = zero4();
+  auto load = [&](int s, float (&a)[TT], float (&b)[9]) {
+    const int k = 4 * s + g;
+#pragma unroll
+    for (int t = 0; t < TT; t++) a[t] = ldg(m, k * N + 16 * t + c, k < NK && 16 * t + c < NN);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      if (4 * s + 3 < NK)  // constant after unrolling
+        b[q] = x[(q * N + k) * R + r];
+      else
+        b[q] = k < NK ? x[(q * N + (k < NK ? k : 0)) * R + r] : 0.f;
+    }
+  };
+  float an[TT], bn[9];
+  load(0, an, bn);
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    float a[TT], b[9];
+#pragma unroll
+    for (int t = 0; t < TT; t++) a[t] = an[t];
+#pragma unroll
+    for (int q = 0; q < 9; q++) b[q] = bn[q];
+    if (s + 1 < KS) load(s + 1, an, bn);
+    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+    float z[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+      z[i] = sv[i * 3] * b[kVelCols[i][0]] + sv[i * 3 + 1] * b[kVelCols[i][1]] + sv[i * 3 + 2] * b[kVelCols[i][2]];
+    const float b0 = h ? b[7] : b[6], b1 = h ? z[0] : b[8], b2 = h ? z[2] : z[1];
+#pragma unroll
+    for (int t = 0; t < TT; t++) {
+      acc[t][0] = mfma(a[t], b0, acc[t][0]);
+      acc[t][1] = mfma(a[t], b1, acc[t][1]);
+      acc[t][2] = mfma(a[t], b2, acc[t][2]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < TT; t++)
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int n = 16 * t + 4 * g + v;
+        if (n < NN) out[(2 * p + h) * NP + n * R + r] = acc[t][p][v];
+      }
+}
+
+// Lifted face tiles L_j = ((V^T x)_j F) F^T for one face: x ([9][N][R], LDS),
+// the projection matrix F ([N][NF]), the lift F^T ([NF][N]) and the rows V^T
+// ([3][9], wave-uniform) of the face's rank-3 solver factors; out (LDS,
+// [3][N][R]) may alias x (every read of x precedes the first write).
+// Column-half pairs: (w_0 | w_1) and (w_2 | w_2); the lift takes its B
+// operands straight from the projection's D tiles (face mode 4 g + v).
+template <int O>
+__device__ __forceinline__ void face_lifts(const float* __restrict__ fl, const float* __restrict__ ft,
+                                           const cfloat* __restrict__ vt, const float* x, float* out, int g,
+                                           int c) {
+  using D = VDims<O>;
+  constexpr int N = D::N, NF = D::NF, NP = D::NP, NT = tiles(N), KS = ksteps(N);
+  const int h = c >> 3, r = c & 7;
+  float cv[27];
+#pragma unroll
+  for (int i = 0; i < 27; i++) cv[i] = vt[i];
+  auto load = [&](int s, float& a, float (&b)[9]) {
+    const int k = 4 * s + g;
+    a = ldg(fl, k * NF + c, k < N && c < NF);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      if (4 * s + 3 < N)  // constant after unrolling
+        b[q] = x[(q * N + k) * R + r];
+      else
+        b[q] = k < N ? x[(q * N + (k < N ? k : 0)) * R + r] : 0.f;
+    }
+  };
+  v4f w01 = zero4(), w2 = zero4();
+  float an, bn[9];
+  load(0, an, bn);
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    const float a = an;
+    float b[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) b[q] = bn[q];
+    if (s + 1 < KS) load(s + 1, an, bn);
+    __builtin_amdgcn_sched_barrier(0);
+    float y[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      float z = 0.f;
+#pragma unroll
+      for (int q = 0; q < 9; q++) z += cv[j * 9 + q] * b[q];
+      y[j] = z;
+    }
+    w01 = mfma(a, h ? y[1] : y[0], w01);
+    w2 = mfma(a, y[2], w2);
+  }
+  v4f l01[NT], l2[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    l01[t] = zero4();
+    l2[t] = zero4();
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int fm = 4 * g + v;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const float at = ldg(ft, fm * N + 16 * t + c, fm < NF && 16 * t + c < N);
+      l01[t] = mfma(at, w01[v], l01[t]);
+      l2[t] = mfma(at, w2[v], l2[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int n = 16 * t + 4 * g + v;
+      if (n < N) {
+        out[h * NP + n * R + r] = l01[t][v];
+        if (h == 0) out[2 * NP + n * R + r] = l2[t][v];
+      }
+    }
+}
+
+// Per position: stress rows (6), velocity rows (3) and the anelastic
+// combination A (6) from the raw products of the three directions
+// (l_p: [3][6][N][R]).
+template <int NP>
+__device__ __forceinline__ void combine_raw(const float* __restrict__ l_p, const cfloat* __restrict__ star,
+                                            const cfloat* __restrict__ star_a, int p, float (&y)[9], float (&a)[6]) {
+#pragma unroll
+  for (int q = 0; q < 9; q++) y[q] = 0.f;
+#pragma unroll
+  for (int q = 0; q < 6; q++) a[q] = 0.f;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    float pv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) pv[i] = l_p[(d * 6 + i) * NP + p];
+#pragma unroll
+    for (int r = 0; r < 3; r++) y[6 + r] += l_p[(d * 6 + 3 + r) * NP + p];
+    const cfloat* sd = star + d * 81;
+    const cfloat* sa = star_a + d * 18;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+#pragma unroll
+      for (int j = 0; j < kStrNnz[q]; j++) y[q] += sd[q * 9 + 6 + kStrVel[q][j]] * pv[kStrVel[q][j]];
+#pragma unroll
+      for (int j = 0; j < kAnNnz[q]; j++) a[q] += sa[q * 3 + kAnCol[q][j]] * pv[kAnCol[q][j]];
+    }
+  }
+}
+
+// y[0..5] += E x for an anelastic source matrix E [6][6] of the source pattern
+__device__ __forceinline__ void add_source(const cfloat* __restrict__ e, const float (&x)[6], float (&y)[9]) {
+#pragma unroll
+  for (int q = 0; q < 3; q++) y[q] += e[q * 6] * x[0] + e[q * 6 + 1] * x[1] + e[q * 6 + 2] * x[2];
+#pragma unroll
+  for (int q = 3; q < 6; q++) y[q] += e[q * 6 + q] * x[q];
+}
+
+template <int O, int NRM>
+__global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt,
+                                                            float* __restrict__ dofs, float* __restrict__ dofs_a,
+                                                            float* __restrict__ t_int) {
+  using D = VDims<O>;
+  constexpr int N = D::N, NV = D::NV, NP = D::NP, SZ = D::SZ, SZA = D::SZA;
+  constexpr int PJ = (NP + kLocalThreads - 1) / kLocalThreads;  // positions per thread
+  __shared__ __align__(16) float l_x[SZ];      // Q, then der_{de-1}, then tInt
+  __shared__ __align__(16) float l_p[18 * NP];  // raw products [3][6][N][R]; lifted tiles [4][3][N][R]
+  const int64_t off = xcd_element(n);
+  if (off >= n) return;
+  const int64_t el = first + off;
+  const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
+  const cfloat* star = as_const(s.star + el * 243);
+  const cfloat* star_a = as_const(s.star_a + el * 54);
+  const cfloat* src = as_const(s.src_a + el * NRM * 36);
+  const cfloat* rfs = as_const(s.rfs);
+  const float* q0 = dofs + el * SZ;
+  float* qa0 = dofs_a + el * NRM * SZA;
+
+  {
+    const float4* gq = reinterpret_cast<const float4*>(q0);
+    float4* lq = reinterpret_cast<float4*>(l_x);
+    for (int i = tid; i < SZ / 4; i += kLocalThreads) lq[i] = gq[i];
+  }
+  // anelastic state per position: derA (CK), tIntA
+  float da[NRM][6][PJ], ta[NRM][6][PJ], ti[9][PJ];
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kLocalThreads;
+    if (p < NP) {
+#pragma unroll
+      for (int rm = 0; rm < NRM; rm++)
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          const float v = qa0[(rm * 6 + q) * NP + p];
+          da[rm][q][j] = v;
+          ta[rm][q][j] = dt * v;
+        }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kLocalThreads;
+#pragma unroll
+    for (int q = 0; q < 9; q++) ti[q][j] = p < NP ? dt * l_x[q * NP + p] : 0.f;
+  }
+
+  // ---- CK
+  float sca = dt;
+#pragma unroll 1
+  for (int de = 1; de < O; de++) {
+    raw_products<O, N, NV>(s.dg.stiffTn + wv * N * N, star + wv * 81, l_x, l_p + wv * 6 * NP, g, c);
+    __syncthreads();
+    sca *= dt / float(de + 1);
+#pragma unroll
+    for (int j = 0; j < PJ; j++) {
+      const int p = tid + j * kLocalThreads;
+      if (p < NP) {
+        float y[9], a[6];
+        if (p < NV * R) {
+          combine_raw<NP>(l_p, star, star_a, p, y, a);
+        } else {  // top-degree modes: no stiffness contribution
+#pragma unroll
+          for (int q = 0; q < 9; q++) y[q] = 0.f;
+#pragma unroll
+          for (int q = 0; q < 6; q++) a[q] = 0.f;
+        }
+#pragma unroll
+        for (int rm = 0; rm < NRM; rm++) {
+          float x[6];
+#pragma unroll
+          for (int q = 0; q < 6; q++) x[q] = da[rm][q][j];
+          add_source(src + rm * 36, x, y);
+          const float w = rfs[rm];
+#pragma unroll
+          for (int q = 0; q < 6; q++) {
+            const float v = w * (a[q] + x[q]);
+            da[rm][q][j] = v;
+            ta[rm][q][j] += sca * v;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+          l_x[q * NP + p] = y[q];
+          ti[q][j] += sca * y[q];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- tInt to LDS and global memory
+  float* gti = t_int + el * SZ;
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kLocalThreads;
+    if (p < NP) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        l_x[q * NP + p] = ti[q][j];
+        gti[q * NP + p] = ti[q][j];
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- volume
+  raw_products<O, NV, N>(s.dg.stiffV + wv * N * N, star + wv * 81, l_x, l_p + wv * 6 * NP, g, c);
+  __syncthreads();
+  float aq[9][PJ], aa[6][PJ];
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kLocalThreads;
+    float y[9], a[6];
+    if (p < NP) {
+      combine_raw<NP>(l_p, star, star_a, p, y, a);
+#pragma unroll
+      for (int rm = 0; rm < NRM; rm++) {
+        float x[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) x[q] = ta[rm][q][j];
+        add_source(src + rm * 36, x, y);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) y[q] = 0.f;
+#pragma unroll
+      for (int q = 0; q < 6; q++) a[q] = 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) aq[q][j] = y[q];
+#pragma unroll
+    for (int q = 0; q < 6; q++) aa[q][j] = a[q];
+  }
+  __syncthreads();
+
+  // ---- local surface: lifted tiles of face f in l_p[f][3][N][R]; wave 0
+  // also takes face 3
+  const cfloat* fsf = as_const(s.fsf + el * 4 * 54);
+  const cfloat* fsfa = as_const(s.fsfa + el * 4 * 18);
+  face_lifts<O>(s.dg.fluxLN + wv * N * D::NF, s.dg.fluxT + wv * D::NF * N, fsf + wv * 54 + 27, l_x,
+                l_p + wv * 3 * NP, g, c);
+  if (wv == 0)
+    face_lifts<O>(s.dg.fluxLN + 3 * N * D::NF, s.dg.fluxT + 3 * D::NF * N, fsf + 3 * 54 + 27, l_x, l_p + 9 * NP, g,
+                  c);
+  __syncthreads();
+
+  // ---- Q += volume + source + surface; QA_rm += w_rm (A - tIntA_rm + surface)
+  float* gq = dofs + el * SZ;
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kLocalThreads;
+    if (p < NP) {
+      float y[9], ua[6];
+#pragma unroll
+      for (int q = 0; q < 9; q++) y[q] = aq[q][j];
+#pragma unroll
+      for (int q = 0; q < 6; q++) ua[q] = aa[q][j];
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        const float l0 = l_p[(f * 3) * NP + p], l1 = l_p[(f * 3 + 1) * NP + p], l2 = l_p[(f * 3 + 2) * NP + p];
+        const cfloat* u = fsf + f * 54;
+        const cfloat* uan = fsfa + f * 18;
+#pragma unroll
+        for (int q = 0; q < 9; q++) y[q] += u[q * 3] * l0 + u[q * 3 + 1] * l1 + u[q * 3 + 2] * l2;
+#pragma unroll
+        for (int q = 0; q < 6; q++) ua[q] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
+      }
+#pragma unroll
+      for (int q = 0; q < 9; q++) gq[q * NP + p] += y[q];
+#pragma unroll
+      for (int rm = 0; rm < NRM; rm++) {
+        const float w = rfs[rm];
+#pragma unroll
+        for (int q = 0; q < 6; q++) qa0[(rm * 6 + q) * NP + p] += w * (ua[q] - ta[rm][q][j]);
+      }
+    }
+  }
+}
+
+template <int O, int NRM>
+__global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, int64_t first, int64_t n,
+                                                            const float* __restrict__ t_int,
+                                                            float* __restrict__ dofs, float* __restrict__ dofs_a) {
+  using D = VDims<O>;
+  constexpr int N = D::N, NF = D::NF, NP = D::NP, SZ = D::SZ, SZA = D::SZA;
+  constexpr int PJ = (NP + kNeighThreads - 1) / kNeighThreads;
+  // per face: the neighbour's tInt, then (aliased) its lifted tiles [3][N][R]
+  __shared__ __align__(16) float l_n[4][SZ];
+  const int64_t off = xcd_element(n);
+  if (off >= n) return;
+  const int64_t el = first + off;
+  const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
+  const cint32* nes = (const cint32*)(s.nb.ne + el * 4);
+  const cfloat* fsf = as_const(s.fsf + (s.n_el + el) * 4 * 54);
+  const cfloat* fsfa = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
+  const int32_t ne = nes[wv];
+  if (ne >= 0) {  // uniform across the wave
+    const float4* gt = reinterpret_cast<const float4*>(t_int + (int64_t)ne * SZ);
+    float4* lt = reinterpret_cast<float4*>(l_n[wv]);
+    for (int i = lane; i < SZ / 4; i += 64) lt[i] = gt[i];
+  }
+  __syncthreads();
+  if (ne >= 0) {
+    const int fm = ((const cuint8*)s.nb.fmat)[el * 4 + wv];
+    face_lifts<O>(s.dg.fluxLN + fm * N * NF, s.dg.fluxT + wv * NF * N, fsf + wv * 54 + 27, l_n[wv], l_n[wv], g, c);
+  }
+  __syncthreads();
+  float* gq = dofs + el * SZ;
+  float* gqa = dofs_a + el * NRM * SZA;
+  const cfloat* rfs = as_const(s.rfs);
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kNeighThreads;
+    if (p < NP) {
+      float y[9], ua[6];
+#pragma unroll
+      for (int q = 0; q < 9; q++) y[q] = 0.f;
+#pragma unroll
+      for (int q = 0; q < 6; q++) ua[q] = 0.f;
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        if (nes[f] < 0) continue;  // outflow (uniform)
+        const float l0 = l_n[f][p], l1 = l_n[f][NP + p], l2 = l_n[f][2 * NP + p];
+        const cfloat* u = fsf + f * 54;
+        const cfloat* uan = fsfa + f * 18;
+#pragma unroll
+        for (int q = 0; q < 9; q++) y[q] += u[q * 3] * l0 + u[q * 3 + 1] * l1 + u[q * 3 + 2] * l2;
+#pragma unroll
+        for (int q = 0; q < 6; q++) ua[q] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
+      }
+#pragma unroll
+      for (int q = 0; q < 9; q++) gq[q * NP + p] += y[q];
+#pragma unroll
+      for (int rm = 0; rm < NRM; rm++) {
+        const float w = rfs[rm];
+#pragma unroll
+        for (int q = 0; q < 6; q++) gqa[(rm * 6 + q) * NP + p] += w * ua[q];
+      }
+    }
+  }
+}
+
+unsigned grid_of(int64_t n) { return (unsigned)(8 * ((n + 7) / 8)); }
+
+template <int O, int NRM>
+void launch_o(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs, float* dofs_a,
+              float* t_int, const float* tin, bool local, hipStream_t st) {
+  if (local)
+    hipLaunchKernelGGL((k_local_vm<O, NRM>), dim3(grid_of(n)), dim3(kLocalThreads), 0, st, s, first, n, dt, dofs,
+                       dofs_a, t_int);
+  else
+    hipLaunchKernelGGL((k_neigh_vm<O, NRM>), dim3(grid_of(n)), dim3(kNeighThreads), 0, st, s, first, n, tin, dofs,
+                       dofs_a);
+}
+
+template <int O>
+int launch_rm(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs, float* dofs_a,
+              float* t_int, const float* tin, bool local, hipStream_t st) {
+  switch (s.n_rms) {
+    case 1: launch_o<O, 1>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st); break;
+    case 2: launch_o<O, 2>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st); break;
+    case 3: launch_o<O, 3>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st); break;
+    default: return fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: 1 to 3 mechanisms");
+  }
+  EDGE_HIP_TRY(hipGetLastError());
+  return EDGE_OK;
+}
+
+int launch_vm(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs, float* dofs_a,
+              float* t_int, const float* tin, bool local, hipStream_t st) {
+  if (n <= 0) return EDGE_OK;
+  if (!s.fsf || !s.fsfa || !s.fs_rank3 || !s.sparse_star)
+    return fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels need the factored flux solvers");
+  switch (s.order) {
+    case 3: return launch_rm<3>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+    case 4: return launch_rm<4>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+    case 5: return launch_rm<5>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+  }
+  return fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: orders 3 to 5");
+}
+
+}  // namespace
+
+bool visco_mfma_supported(int order, int n_runs, int precision, int n_rms) {
+  return precision == 32 && n_runs == R && order >= 3 && order <= 5 && n_rms >= 1 && n_rms <= 3;
+}
+
+int launch_local_visco_mfma(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs,
+                            float* dofs_a, float* t_int, hipStream_t st) {
+  return launch_vm(s, first, n, dt, dofs, dofs_a, t_int, nullptr, true, st);
+}
+
+int launch_neigh_visco_mfma(const SolverDev<float>& s, int64_t first, int64_t n, const float* t_int, float* dofs,
+                            float* dofs_a, hipStream_t st) {
+  return launch_vm(s, first, n, 0.f, dofs, dofs_a, nullptr, t_int, false, st);
+}
+
+}  // namespace edge

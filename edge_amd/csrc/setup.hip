@@ -253,6 +253,56 @@ __device__ bool rank3_factors(const double* F, T* out) {
   return rem <= 1e-20 * fn2;
 }
 
+// Factor U_A [6][3] of an anelastic flux solver FA [6][9] over the rows V^T
+// [3][9] of the elastic solver's rank-3 factors: FA = U_A V^T, by least
+// squares through the Gram matrix of V^T's rows.  The anelastic middle state
+// reads each side's state through the same three wave combinations as the
+// elastic one (sigma_nn - Z_p v_n, sigma_ns - Z_s v_s, sigma_nt - Z_s v_t in
+// face coordinates, src/impl/seismic/setups/ViscoElasticity.cpp:376-441),
+// so FA lies in that row space.  Returns false when the remainder exceeds
+// 1e-10 of FA (Frobenius norms) or the Gram matrix is singular.
+template <typename T>
+__device__ bool coupled_factor(const double* FA, const double* vt, T* out) {
+  double G[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = 0;
+      for (int p = 0; p < 9; p++) v += vt[i * 9 + p] * vt[j * 9 + p];
+      G[i * 3 + j] = v;
+    }
+  const double det = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
+                     G[2] * (G[3] * G[7] - G[4] * G[6]);
+  const double tr = G[0] + G[4] + G[8];
+  if (!(det > 1e-24 * tr * tr * tr)) return false;
+  double Gi[9];
+  Gi[0] = (G[4] * G[8] - G[5] * G[7]) / det;
+  Gi[1] = (G[2] * G[7] - G[1] * G[8]) / det;
+  Gi[2] = (G[1] * G[5] - G[2] * G[4]) / det;
+  Gi[3] = (G[5] * G[6] - G[3] * G[8]) / det;
+  Gi[4] = (G[0] * G[8] - G[2] * G[6]) / det;
+  Gi[5] = (G[2] * G[3] - G[0] * G[5]) / det;
+  Gi[6] = (G[3] * G[7] - G[4] * G[6]) / det;
+  Gi[7] = (G[1] * G[6] - G[0] * G[7]) / det;
+  Gi[8] = (G[0] * G[4] - G[1] * G[3]) / det;
+  double U[18], fn2 = 0, rem = 0;
+  for (int q = 0; q < 6; q++) {
+    double h[3];
+    for (int j = 0; j < 3; j++) {
+      double v = 0;
+      for (int p = 0; p < 9; p++) v += FA[q * 9 + p] * vt[j * 9 + p];
+      h[j] = v;
+    }
+    for (int j = 0; j < 3; j++) U[q * 3 + j] = h[0] * Gi[0 * 3 + j] + h[1] * Gi[1 * 3 + j] + h[2] * Gi[2 * 3 + j];
+    for (int p = 0; p < 9; p++) {
+      const double d = FA[q * 9 + p] - (U[q * 3] * vt[p] + U[q * 3 + 1] * vt[9 + p] + U[q * 3 + 2] * vt[18 + p]);
+      rem += d * d;
+      fn2 += FA[q * 9 + p] * FA[q * 9 + p];
+    }
+  }
+  for (int i = 0; i < 18; i++) out[i] = (T)U[i];
+  return rem <= 1e-20 * fn2;
+}
+
 template <typename T>
 __global__ void k_fs_factor(int64_t n_mats, const double* fs, T* fsf, int* bad) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,7 +314,7 @@ template <typename T>
 __global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el, const int64_t* el_fa,
                                const int32_t* fa_type, const double* normals, const double* tangents,
                                const double* areas, const double* jdet, const double* rho, const double* lam,
-                               const double* mu, T* fs, T* fsf, T* fs_a, int* bad) {
+                               const double* mu, T* fs, T* fsf, T* fs_a, T* fsfa, int* bad) {
   const int64_t fa = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (fa >= n_fa) return;
   const int64_t eL = fa_el[fa * 2], eR = fa_el[fa * 2 + 1];
@@ -288,8 +338,9 @@ __global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el,
   const double* n = normals + fa * 3;
   const double* s = tangents + fa * 6;
   const double* t = tangents + fa * 6 + 3;
-  double a[81], b[81];
-  face_solvers(rL, rR, lL, lR, mL, mR, n, s, t, fsurf, a, b);
+  double a[81], b[81], al[54], ar[54];
+  // both sides' solvers of element e's face f, scaled by sc; with the
+  // anelastic solvers al / ar (unscaled) when fs_a is given
   auto emit = [&](int64_t e, int f, double sc) {
     for (int i = 0; i < 81; i++) {
       a[i] *= sc;
@@ -297,35 +348,38 @@ __global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el,
       fs[(e * 4 + f) * 81 + i] = (T)a[i];
       fs[((n_el + e) * 4 + f) * 81 + i] = (T)b[i];
     }
+    double fac[2][54];
     if (fsf) {
-      bool ok = rank3_factors(a, fsf + (e * 4 + f) * 54);
-      ok = rank3_factors(b, fsf + ((n_el + e) * 4 + f) * 54) && ok;
+      bool ok = rank3_factors(a, fac[0]);
+      ok = rank3_factors(b, fac[1]) && ok;
+      for (int i = 0; i < 54; i++) {
+        fsf[(e * 4 + f) * 54 + i] = (T)fac[0][i];
+        fsf[((n_el + e) * 4 + f) * 54 + i] = (T)fac[1][i];
+      }
       if (!ok) atomicOr(bad, 8);
     }
-  };
-  emit(eL, fL, -2.0 * areas[fa] / jdet[eL]);
-  if (fs_a) {
-    double al[54], ar[54];
-    anelastic_solvers(rL, rR, lL, lR, mL, mR, n, s, t, fsurf, al, ar);
-    const double sc = -2.0 * areas[fa] / jdet[eL];
-    for (int i = 0; i < 54; i++) {
-      fs_a[(eL * 4 + fL) * 54 + i] = (T)(al[i] * sc);
-      fs_a[((n_el + eL) * 4 + fL) * 54 + i] = (T)(ar[i] * sc);
+    if (fs_a) {
+      for (int i = 0; i < 54; i++) {
+        al[i] *= sc;
+        ar[i] *= sc;
+        fs_a[(e * 4 + f) * 54 + i] = (T)al[i];
+        fs_a[((n_el + e) * 4 + f) * 54 + i] = (T)ar[i];
+      }
+      if (fsf && fsfa) {
+        bool ok = coupled_factor(al, fac[0] + 27, fsfa + (e * 4 + f) * 18);
+        ok = coupled_factor(ar, fac[1] + 27, fsfa + ((n_el + e) * 4 + f) * 18) && ok;
+        if (!ok) atomicOr(bad, 8);
+      }
     }
-  }
+  };
+  face_solvers(rL, rR, lL, lR, mL, mR, n, s, t, fsurf, a, b);
+  if (fs_a) anelastic_solvers(rL, rR, lL, lR, mL, mR, n, s, t, fsurf, al, ar);
+  emit(eL, fL, -2.0 * areas[fa] / jdet[eL]);
   if (eR >= 0) {
     const double nm[3] = {-n[0], -n[1], -n[2]};
     face_solvers(rR, rL, lR, lL, mR, mL, nm, s, t, false, a, b);
+    if (fs_a) anelastic_solvers(rR, rL, lR, lL, mR, mL, nm, s, t, false, al, ar);
     emit(eR, fR, -2.0 * areas[fa] / jdet[eR]);
-    if (fs_a) {
-      double al[54], ar[54];
-      anelastic_solvers(rR, rL, lR, lL, mR, mL, nm, s, t, false, al, ar);
-      const double sc = -2.0 * areas[fa] / jdet[eR];
-      for (int i = 0; i < 54; i++) {
-        fs_a[(eR * 4 + fR) * 54 + i] = (T)(al[i] * sc);
-        fs_a[((n_el + eR) * 4 + fR) * 54 + i] = (T)(ar[i] * sc);
-      }
-    }
   }
 }
 
@@ -333,7 +387,7 @@ __global__ void k_flux_solvers(int64_t n_fa, int64_t n_el, const int64_t* fa_el,
 
 template <typename T>
 int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_fsf, bool* rank3, T* d_star_a,
-                T* d_fs_a, hipStream_t st) {
+                T* d_fs_a, T* d_fsfa, hipStream_t st) {
   double *d_crds = nullptr, *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr, *d_nrm = nullptr,
          *d_tan = nullptr, *d_area = nullptr;
   int64_t *d_elve = nullptr, *d_fael = nullptr, *d_elfa = nullptr;
@@ -386,7 +440,8 @@ int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_f
                      d_lam, d_mu, d_star, d_jdet, d_star_a, d_bad);
   EDGE_SETUP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_flux_solvers<T>, dim3((unsigned)((nf + bs - 1) / bs)), dim3(bs), 0, st, nf, ne, d_fael,
-                     d_elfa, d_type, d_nrm, d_tan, d_area, d_jdet, d_rho, d_lam, d_mu, d_fs, d_fsf, d_fs_a, d_bad);
+                     d_elfa, d_type, d_nrm, d_tan, d_area, d_jdet, d_rho, d_lam, d_mu, d_fs, d_fsf, d_fs_a, d_fsfa,
+                     d_bad);
   EDGE_SETUP_TRY(hipGetLastError());
   int bad = 0;
   EDGE_SETUP_TRY(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -401,8 +456,8 @@ int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_f
 }
 
 template int build_setup<double>(const SetupInput&, double*, double*, double*, double*, bool*, double*, double*,
-                                  hipStream_t);
-template int build_setup<float>(const SetupInput&, float*, float*, double*, float*, bool*, float*, float*,
+                                  double*, hipStream_t);
+template int build_setup<float>(const SetupInput&, float*, float*, double*, float*, bool*, float*, float*, float*,
                                  hipStream_t);
 
 template <typename T>

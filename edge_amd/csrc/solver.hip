@@ -47,6 +47,7 @@ struct edge_el_solver {
   void* star_a = nullptr;    // [nEl][54]
   void* src_a = nullptr;     // [nEl][n_rms][36]
   void* fs_a = nullptr;      // [2][nEl][4][54]
+  void* fsfa = nullptr;      // [2][nEl][4][18] anelastic factors (MFMA viscoelastic kernels)
   std::vector<double> lam_e, mu_e, h_rfs;
   // rate-2 local time stepping
   uint8_t* fa_lts = nullptr;   // [nEl][4]: 0 equal steps, 1 this element's step greater, 2 smaller
@@ -96,6 +97,7 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.star_a = static_cast<const T*>(s->star_a);
   d.src_a = static_cast<const T*>(s->src_a);
   d.fs_a = static_cast<const T*>(s->fs_a);
+  d.fsfa = static_cast<const T*>(s->fsfa);
   return d;
 }
 
@@ -375,9 +377,9 @@ extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mes
     else
       rc = cfg->precision == 64
                ? edge::build_setup<double>(in, (double*)s->star, (double*)s->fs, s->jdet, (double*)s->fsf,
-                                           &s->fs_rank3, nullptr, nullptr, nullptr)
+                                           &s->fs_rank3, nullptr, nullptr, nullptr, nullptr)
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
-                                          &s->fs_rank3, nullptr, nullptr, nullptr);
+                                          &s->fs_rank3, nullptr, nullptr, nullptr, nullptr);
     if (rc == EDGE_OK) rc = finish_mfma(s);
   }
   if (rc != EDGE_OK) {
@@ -446,6 +448,7 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   hipFree(s->star_a);
   hipFree(s->src_a);
   hipFree(s->fs_a);
+  hipFree(s->fsfa);
   drop_lts(s);
   delete s;
   return EDGE_OK;
@@ -505,9 +508,12 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
   if (!mesh || !mesh->ve_crds || !mesh->el_ve || !mesh->normals || !mesh->tangents || !mesh->areas || !rho ||
       !lam || !mu || !qp || !qs)
     return edge::fail(EDGE_ERR_ARG, "mesh geometry, material or quality factors missing");
-  if (cfg->kernel > 1) return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic solvers run the generic kernels");
+  if (cfg->kernel > 2) return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic solvers: kernel 0, 1 or 2");
   if (!edge::visco_supported(cfg->order, cfg->n_runs, cfg->precision / 8, vc->n_rms))
     return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic kernels: 1 or 8 runs whose tiles fit LDS");
+  const bool vm_ok = edge::visco_mfma_supported(cfg->order, cfg->n_runs, cfg->precision, vc->n_rms);
+  if (cfg->kernel == 2 && !vm_ok)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: FP32, 8 runs, orders 3-5, 1-3 mechanisms");
   const int64_t ne = mesh->n_el;
   for (int64_t el = 0; el < ne; el++)
     if (!(rho[el] > 0) || !(mu[el] > 0) || !(lam[el] + 2 * mu[el] > 0) || !(qp[el] > 0) || !(qs[el] > 0))
@@ -524,6 +530,17 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
     if (!(mu_e[el] > 0) || !(lam_e[el] + 2 * mu_e[el] > 0))
       return edge::fail(EDGE_ERR_ARG, "constant-Q fit gives non-physical unrelaxed Lame parameters");
   }
+  // the MFMA kernels read the source matrices by their pattern: the
+  // normal-stress block and the diagonal of the shear rows
+  bool src_pattern = true;
+  for (size_t i = 0; i < src.size(); i++) {
+    const int row = (int)(i % 36) / 6, col = (int)(i % 6);
+    const bool nz = row < 3 ? col < 3 : col == row;
+    if (!nz && src[i] != 0) src_pattern = false;
+  }
+  const bool want_vm = vm_ok && cfg->kernel != 1 && src_pattern;
+  if (cfg->kernel == 2 && !want_vm)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: source matrices off the pattern");
   edge_el_config c = *cfg;
   c.kernel = 1;
   edge_el_solver* s = nullptr;
@@ -542,16 +559,33 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
     if (hipMalloc(&s->jdet, sizeof(double) * ne) != hipSuccess || hipMalloc(&s->star_a, rb * 54 * ne) != hipSuccess ||
         hipMalloc(&s->fs_a, rb * 2 * 4 * 54 * ne) != hipSuccess)
       rc = edge::fail(EDGE_ERR_HIP, "allocation of the viscoelastic set-up");
+    if (rc == EDGE_OK && want_vm &&
+        (hipMalloc(&s->fsf, rb * 2 * 4 * 54 * ne) != hipSuccess || hipMalloc(&s->fsfa, rb * 2 * 4 * 18 * ne) != hipSuccess))
+      rc = edge::fail(EDGE_ERR_HIP, "allocation of the factored flux solvers");
     if (rc == EDGE_OK)
       rc = cfg->precision == 64 ? upload_as<double>(&s->src_a, src) : upload_as<float>(&s->src_a, src);
     if (rc == EDGE_OK)
       rc = cfg->precision == 64 ? upload_as<double>(&s->rfs, rfs) : upload_as<float>(&s->rfs, rfs);
     if (rc == EDGE_OK)
       rc = cfg->precision == 64
-               ? edge::build_setup<double>(in, (double*)s->star, (double*)s->fs, s->jdet, nullptr, nullptr,
-                                           (double*)s->star_a, (double*)s->fs_a, nullptr)
-               : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, nullptr, nullptr,
-                                          (float*)s->star_a, (float*)s->fs_a, nullptr);
+               ? edge::build_setup<double>(in, (double*)s->star, (double*)s->fs, s->jdet, (double*)s->fsf,
+                                           &s->fs_rank3, (double*)s->star_a, (double*)s->fs_a, (double*)s->fsfa,
+                                           nullptr)
+               : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
+                                          &s->fs_rank3, (float*)s->star_a, (float*)s->fs_a, (float*)s->fsfa,
+                                          nullptr);
+    if (rc == EDGE_OK && want_vm) {
+      if (s->fs_rank3) {
+        s->kernel = 2;
+      } else if (cfg->kernel == 2) {
+        rc = edge::fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: flux solvers not of rank 3");
+      } else {  // the generic kernels
+        hipFree(s->fsf);
+        hipFree(s->fsfa);
+        s->fsf = s->fsfa = nullptr;
+        s->fs_rank3 = false;
+      }
+    }
   }
   if (rc != EDGE_OK) {
     if (s) edge_el_destroy(s);
@@ -568,6 +602,9 @@ extern "C" int edge_el_local_a(edge_el_solver* s, int64_t first, int64_t n_els, 
   if (s->n_rms == 0) return edge_el_local(s, first, n_els, dt, dofs, t_int, stream);
   if (!dofs || !dofs_a || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
   hipStream_t st = (hipStream_t)stream;
+  if (s->kernel == 2)
+    return edge::launch_local_visco_mfma(dev_view<float>(s), first, n_els, (float)dt, (float*)dofs, (float*)dofs_a,
+                                         (float*)t_int, st);
   if (s->cfg.precision == 64)
     return edge::launch_local_visco<double>(dev_view<double>(s), first, n_els, dt, (double*)dofs, (double*)dofs_a,
                                             (double*)t_int, st);
@@ -582,6 +619,9 @@ extern "C" int edge_el_neigh_a(edge_el_solver* s, int64_t first, int64_t n_els, 
   if (s->n_rms == 0) return edge_el_neigh(s, first, n_els, t_int, dofs, stream);
   if (!dofs || !dofs_a || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
   hipStream_t st = (hipStream_t)stream;
+  if (s->kernel == 2)
+    return edge::launch_neigh_visco_mfma(dev_view<float>(s), first, n_els, (const float*)t_int, (float*)dofs,
+                                         (float*)dofs_a, st);
   if (s->cfg.precision == 64)
     return edge::launch_neigh_visco<double>(dev_view<double>(s), first, n_els, (const double*)t_int, (double*)dofs,
                                             (double*)dofs_a, st);

@@ -13,7 +13,7 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import PATH_VISCO, ViscoSolver
+from edge_amd.solver import KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, PATH_VISCO, ViscoSolver
 
 pytestmark = pytest.mark.gpu
 
@@ -53,17 +53,22 @@ def test_visco_setup_matches_oracle(gpu):
     assert rel(fs, ref["fsE"]) <= 1e-12
 
 
-@pytest.mark.parametrize("prec,order,runs", [(64, 2, 1), (64, 3, 1), (64, 4, 1), (64, 5, 1), (32, 4, 8),
-                                             (32, 5, 8)])
-def test_visco_steps_match_oracle(gpu, prec, order, runs):
+@pytest.mark.parametrize("prec,order,runs,kernel,nr", [
+    (64, 2, 1, KERNEL_AUTO, NR), (64, 3, 1, KERNEL_AUTO, NR), (64, 4, 1, KERNEL_AUTO, NR),
+    (64, 5, 1, KERNEL_AUTO, NR), (32, 4, 8, KERNEL_GENERIC, NR), (32, 5, 8, KERNEL_GENERIC, NR),
+    (32, 3, 8, KERNEL_MFMA, NR), (32, 4, 8, KERNEL_MFMA, NR), (32, 5, 8, KERNEL_AUTO, NR),
+    (32, 4, 8, KERNEL_MFMA, 1), (32, 5, 8, KERNEL_MFMA, 2)])
+def test_visco_steps_match_oracle(gpu, prec, order, runs, kernel, nr):
     m, rho, lam, mu, qp, qs = _problem(order, seed=40 + order)
-    s = ViscoSolver(m, order, runs, prec, rho, lam, mu, qp, qs, NR, F_CEN, F_RAT)
-    visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qs, NR, F_CEN, F_RAT)
+    s = ViscoSolver(m, order, runs, prec, rho, lam, mu, qp, qs, nr, F_CEN, F_RAT, kernel=kernel)
+    if prec == 32 and kernel != KERNEL_GENERIC:  # auto picks the MFMA kernels for FP32 with 8 runs
+        assert s.kernel_info()[0] == KERNEL_MFMA
+    visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qs, nr, F_CEN, F_RAT)
     mo = orc.MeshOracle(m, order, runs, visco=visco)
     rng = np.random.default_rng(order)
     n = S.n_modes(order)
     d0 = rng.normal(size=(m.n_el, 9, n, runs))
-    a0 = 0.1 * rng.normal(size=(m.n_el, NR, 6, n, runs))
+    a0 = 0.1 * rng.normal(size=(m.n_el, nr, 6, n, runs))
     dt = S.dt_cfl(m, rho, visco["lam_e"], visco["mu_e"], order)
     tol = 1e-12 if prec == 64 else 1e-5
     d, t, a = d0.copy(), np.zeros_like(d0), a0.copy()
