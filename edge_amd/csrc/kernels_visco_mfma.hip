@@ -15,8 +15,10 @@
 // workgroup, with the anelastic state in registers, and the raw products
 // move between the MFMA and the elementwise phases through LDS.
 //
-// Local step, three waves; wave d takes direction d in the CK and volume
-// products (src/impl/seismic/solvers/AderDg.hpp:362-533):
+// Local step, six waves (one position per thread at O5); wave w takes
+// direction w % 3 and half w / 3 of the output row tiles in the CK and volume
+// products, and waves 0..3 a face each in the surface integral
+// (src/impl/seismic/solvers/AderDg.hpp:362-533):
 //  CK (kernels/TimePredVanilla.hpp:200-306; with attenuation every
 //  derivative keeps all N modes): per stage de the raw products of
 //  der_{de-1} with the negated transposed stiffness matrix of direction d,
@@ -30,8 +32,8 @@
 //  volume (kernels/VolIntVanilla.hpp:189-241): the same raw products of tInt
 //  with K_d M^-1, Q[q < 6] += sum_d S_d P_d + sum_rm E_rm tIntA_rm,
 //  Q[6 + r] += sum_d V_d, QA_rm += w_rm (sum_d S^A_d P_d - tIntA_rm);
-//  local surface (kernels/SurfIntVanilla.hpp:212-262, SurfInt.hpp:139-156),
-//  waves by face, with the rank-3 factors fs = U V^T and fsA = U_A V^T:
+//  local surface (kernels/SurfIntVanilla.hpp:212-262, SurfInt.hpp:139-156)
+//  with the rank-3 factors fs = U V^T and fsA = U_A V^T:
 //    w_j = (V^T tInt)_j F^L_fa,  L_j = w_j F^T_fa,
 //    Q += sum_j U[.][j] L_j,  QA_rm += w_rm sum_j U_A[.][j] L_j.
 // Neighbour step (kernels/SurfIntVanilla.hpp:274-357), four waves, wave f
@@ -51,7 +53,7 @@ typedef const __attribute__((address_space(4))) float cfloat;
 typedef const __attribute__((address_space(4))) int32_t cint32;
 typedef const __attribute__((address_space(4))) uint8_t cuint8;
 constexpr int R = 8;
-constexpr int kLocalThreads = 192, kNeighThreads = 256;
+constexpr int kLocalThreads = 384, kNeighThreads = 256;
 
 __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -101,16 +103,18 @@ struct VDims {
   static constexpr int SZ = 9 * NP, SZA = 6 * NP;
 };
 
-// Raw products of one direction (wave): rows k < NK of the tile x (LDS,
-// [9][N][R]) with the matrix m ([N][N], rows k, columns n < NN), written to
-// out (LDS, [6][N][R]) as P_0, P_1, P_2 (velocity tiles) and V_0, V_1, V_2
-// (velocity rows of the star-combined tile).  Column-half pairs: (P_0 | P_1),
-// (P_2 | V_0), (V_1 | V_2), so product 2 p + h sits in half h of pair p.
-template <int O, int NK, int NN>
+// Raw products of one direction: rows k < NK of the tile x (LDS, [9][N][R])
+// with the matrix m ([N][N], rows k, columns n < NN), output row tiles t0 ..
+// t0 + TT - 1, written to out (LDS, [6][N][R]) as P_0, P_1, P_2 (velocity
+// tiles) and V_0, V_1, V_2 (velocity rows of the star-combined tile).
+// Column-half pairs: (P_0 | P_1), (P_2 | V_0), (V_1 | V_2), so product
+// 2 p + h sits in half h of pair p.
+template <int O, int NK, int NN, int TT>
 __device__ __forceinline__ void raw_products(const float* __restrict__ m, const cfloat* __restrict__ sd,
-                                             const float* __restrict__ x, float* __restrict__ out, int g, int c) {
+                                             const float* __restrict__ x, float* __restrict__ out, int t0, int g,
+                                             int c) {
   using D = VDims<O>;
-  constexpr int N = D::N, NP = D::NP, TT = tiles(NN), KS = ksteps(NK);
+  constexpr int N = D::N, NP = D::NP, KS = ksteps(NK);
   const int h = c >> 3, r = c & 7;
   float sv[9];
 #pragma unroll
@@ -125,7 +129,7 @@ __device__ __forceinline__ void raw_products(const float* __restrict__ m, const 
   auto load = [&](int s, float (&a)[TT], float (&b)[9]) {
     const int k = 4 * s + g;
 #pragma unroll
-    for (int t = 0; t < TT; t++) a[t] = ldg(m, k * N + 16 * t + c, k < NK && 16 * t + c < NN);
+    for (int t = 0; t < TT; t++) a[t] = ldg(m, k * N + 16 * (t0 + t) + c, k < NK && 16 * (t0 + t) + c < NN);
 #pragma unroll
     for (int q = 0; q < 9; q++) {
       if (4 * s + 3 < NK)  // constant after unrolling
@@ -163,7 +167,7 @@ __device__ __forceinline__ void raw_products(const float* __restrict__ m, const 
     for (int p = 0; p < 3; p++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int n = 16 * t + 4 * g + v;
+        const int n = 16 * (t0 + t) + 4 * g + v;
         if (n < NN) out[(2 * p + h) * NP + n * R + r] = acc[t][p][v];
       }
 }
@@ -296,6 +300,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   const int64_t el = first + off;
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
+  const int dir = wv % 3, t_ck = wv / 3;  // CK / volume products: direction, half of the row tiles
   const cfloat* star = as_const(s.star + el * 243);
   const cfloat* star_a = as_const(s.star_a + el * 54);
   const cfloat* src = as_const(s.src_a + el * NRM * 36);
@@ -336,7 +341,8 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   float sca = dt;
 #pragma unroll 1
   for (int de = 1; de < O; de++) {
-    raw_products<O, N, NV>(s.dg.stiffTn + wv * N * N, star + wv * 81, l_x, l_p + wv * 6 * NP, g, c);
+    if (t_ck < tiles(NV))
+      raw_products<O, N, NV, 1>(s.dg.stiffTn + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, t_ck, g, c);
     __syncthreads();
     sca *= dt / float(de + 1);
 #pragma unroll
@@ -392,7 +398,15 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   __syncthreads();
 
   // ---- volume
-  raw_products<O, NV, N>(s.dg.stiffV + wv * N * N, star + wv * 81, l_x, l_p + wv * 6 * NP, g, c);
+  {
+    // output tiles: the first half of the waves [0, TV0), the second [TV0, tiles(N))
+    constexpr int TN = tiles(N), TV0 = (TN + 1) / 2;
+    if (t_ck == 0)
+      raw_products<O, NV, N, TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, 0, g, c);
+    else if constexpr (TN > TV0)
+      raw_products<O, NV, N, TN - TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, TV0, g,
+                                       c);
+  }
   __syncthreads();
   float aq[9][PJ], aa[6][PJ];
 #pragma unroll
@@ -421,15 +435,12 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   }
   __syncthreads();
 
-  // ---- local surface: lifted tiles of face f in l_p[f][3][N][R]; wave 0
-  // also takes face 3
+  // ---- local surface: waves 0..3 the lifted tiles of face wv in l_p[wv][3][N][R]
   const cfloat* fsf = as_const(s.fsf + el * 4 * 54);
   const cfloat* fsfa = as_const(s.fsfa + el * 4 * 18);
-  face_lifts<O>(s.dg.fluxLN + wv * N * D::NF, s.dg.fluxT + wv * D::NF * N, fsf + wv * 54 + 27, l_x,
-                l_p + wv * 3 * NP, g, c);
-  if (wv == 0)
-    face_lifts<O>(s.dg.fluxLN + 3 * N * D::NF, s.dg.fluxT + 3 * D::NF * N, fsf + 3 * 54 + 27, l_x, l_p + 9 * NP, g,
-                  c);
+  if (wv < 4)
+    face_lifts<O>(s.dg.fluxLN + wv * N * D::NF, s.dg.fluxT + wv * D::NF * N, fsf + wv * 54 + 27, l_x,
+                  l_p + wv * 3 * NP, g, c);
   __syncthreads();
 
   // ---- Q += volume + source + surface; QA_rm += w_rm (A - tIntA_rm + surface)
