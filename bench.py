@@ -213,6 +213,49 @@ def visco_flops(order, n_rms):
     return ck + vol + 4 * face + n_rms * 2 * 6 * n, 4 * face + n_rms * 2 * 6 * n
 
 
+def ensemble_row(args, timed, flops_el_run):
+    """24 independent viscoelastic meshes (3 mechanisms, Q_p in U[50, 200],
+    Q_s = Q_p / 2, f_c = 1 Hz, ratio 10) advanced as one mesh: FP32 with 8
+    runs on the MFMA kernels, beside FP64 with one run."""
+    import torch
+    from edge_amd import mesh as M
+    from edge_amd import setups as S
+    from edge_amd.solver import ViscoSolver
+    rng = np.random.default_rng(3343)
+    sizes = [int(x) for x in rng.integers(8, 33, 24)]
+    meshes, mats = [], []
+    for i, n in enumerate(sizes):
+        m = M.box(n, jitter=0.1, seed=3343 + i)
+        r = np.random.default_rng(10_000 + i)
+        rho = r.uniform(2.0, 3.0, m.n_el)
+        vp = r.uniform(4.0, 6.0, m.n_el)
+        mu = rho * (vp / np.sqrt(3.0)) ** 2
+        qp = np.full(m.n_el, r.uniform(50, 200))
+        meshes.append(m)
+        mats.append((rho, rho * vp ** 2 - 2 * mu, mu, qp))
+    u, ranges = M.union(meshes)
+    rho, lam, mu, qp = [np.concatenate([x[k] for x in mats]) for k in range(4)]
+    for runs, prec in ((8, 32), (1, 64)):
+        t0 = time.time()
+        s = ViscoSolver(u, args.order, runs, prec, rho, lam, mu, qp, qp / 2, 3, 1.0, 10.0)
+        t_setup = time.time() - t0
+        dt = S.dt_cfl(u, rho, s.setup_a()["lam_e"], s.setup_a()["mu_e"], args.order)
+        d = torch.randn(s.dof_shape(), dtype=s.dtype, device="cuda")
+        a = 0.01 * torch.randn(s.dof_a_shape(), dtype=s.dtype, device="cuda")
+        t = torch.zeros_like(d)
+        ms = timed(lambda st: s.step(d, t, dt, 1, a, graph=True, stream=st), 3)
+        peak = PEAK["fp32_mfma"] if prec == 32 else PEAK["fp64"]
+        n = s.n_modes
+        print(json.dumps({"row": "batched viscoelastic ensemble", "meshes": len(sizes), "hexes_per_axis": sizes,
+                          "elements": u.n_el, "order": args.order, "runs": runs, "precision": prec, "n_rms": 3,
+                          "kernel": s.kernel_info()[0], "graph": True, "setup_s": round(t_setup, 1),
+                          "ms_per_step": ms * 1e3, "dof_updates_per_s": u.n_el * 27 * n * runs / ms,
+                          "roofline_frac": flops_el_run * runs * u.n_el / ms / peak}), flush=True)
+        s.close()
+        del d, a, t
+        torch.cuda.empty_cache()
+
+
 def rows(args):
     """Per-row measurements beside the headline (one JSON line per row):
     viscoelastic updates, local time stepping, receivers and sources, and the
@@ -259,6 +302,10 @@ def rows(args):
                           "roofline_frac": (fl + fn_) * runs * m.n_el / (loc + nei) / peak}), flush=True)
         s.close()
         del d, t, a
+    # ---- batched viscoelastic ensemble: 24 meshes of seeded sizes (6 n^3 tets,
+    # n in [8, 32]) as one disjoint union, one graph-captured step per time step
+    if not args.no_ensemble:
+        ensemble_row(args, timed, fl + fn_)
     # ---- local time stepping: two groups (halves of the mesh), FP32 16 runs
     s = ElasticSolver(m, args.order, 16, 32, rho, lam, mu)
     d = (torch.from_numpy(q0)[..., None].repeat(1, 1, 1, 16)).to(torch.float32).cuda().contiguous()
@@ -321,6 +368,7 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 generic, 2 MFMA-tiled")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-ensemble", action="store_true", help="--rows without the 24-mesh ensemble")
     ap.add_argument("--rows", action="store_true",
                     help="measure the other rows (viscoelastic, LTS, receivers/sources, rupture) instead")
     ap.add_argument("--no-points", action="store_true",

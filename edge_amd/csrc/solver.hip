@@ -61,6 +61,7 @@ struct edge_el_solver {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   void* graph_dofs = nullptr;
+  void* graph_dofs_a = nullptr;
   void* graph_tint = nullptr;
   double graph_dt = 0;
   hipStream_t graph_stream = nullptr;
@@ -337,7 +338,11 @@ int neigh_lts_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, const
                                    (const T*)t0, (const T*)t1, (const T*)t2, (T*)dofs, st);
 }
 
-int one_step(edge_el_solver* s, double dt, void* dofs, void* t_int, hipStream_t st) {
+int one_step(edge_el_solver* s, double dt, void* dofs, void* dofs_a, void* t_int, hipStream_t st) {
+  if (s->n_rms > 0) {
+    const int rc = edge_el_local_a(s, 0, s->n_el, dt, dofs, dofs_a, t_int, st);
+    return rc ? rc : edge_el_neigh_a(s, 0, s->n_el, t_int, dofs, dofs_a, st);
+  }
   int rc = s->cfg.precision == 64 ? local_t<double>(s, 0, s->n_el, dt, dofs, t_int, st)
                                   : local_t<float>(s, 0, s->n_el, dt, dofs, t_int, st);
   if (rc) return rc;
@@ -780,20 +785,18 @@ extern "C" int edge_el_neigh_lts(edge_el_solver* s, int64_t first, int64_t n_els
                                 : neigh_lts_t<float>(s, first, n_els, first_ts, t_int0, t_int1, t_int2, dofs, st);
 }
 
-extern "C" int edge_el_step(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* t_int, int use_graph,
-                            void* stream) {
-  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
-  if (!dofs || !t_int || n_steps < 0) return edge::fail(EDGE_ERR_ARG, "step arguments");
-  hipStream_t st = (hipStream_t)stream;
+namespace {
+int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* dofs_a, void* t_int, int use_graph,
+              hipStream_t st) {
   if (!use_graph) {
     for (int64_t i = 0; i < n_steps; i++) {
-      int rc = one_step(s, dt, dofs, t_int, st);
+      int rc = one_step(s, dt, dofs, dofs_a, t_int, st);
       if (rc) return rc;
     }
     return EDGE_OK;
   }
-  if (s->graph_exec && (s->graph_dofs != dofs || s->graph_tint != t_int || s->graph_dt != dt ||
-                        s->graph_stream != st))
+  if (s->graph_exec && (s->graph_dofs != dofs || s->graph_dofs_a != dofs_a || s->graph_tint != t_int ||
+                        s->graph_dt != dt || s->graph_stream != st))
     drop_graph(s);
   if (!s->graph_exec) {
     // capture needs a non-default stream
@@ -804,7 +807,7 @@ extern "C" int edge_el_step(edge_el_solver* s, int64_t n_steps, double dt, void*
       own = true;
     }
     EDGE_HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    int rc = one_step(s, dt, dofs, t_int, cap);
+    int rc = one_step(s, dt, dofs, dofs_a, t_int, cap);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cap, &g);
     if (rc == EDGE_OK && e != hipSuccess) rc = edge::fail(EDGE_ERR_HIP, hipGetErrorString(e));
@@ -819,12 +822,30 @@ extern "C" int edge_el_step(edge_el_solver* s, int64_t n_steps, double dt, void*
       return rc;
     }
     s->graph_dofs = dofs;
+    s->graph_dofs_a = dofs_a;
     s->graph_tint = t_int;
     s->graph_dt = dt;
     s->graph_stream = st;
   }
   for (int64_t i = 0; i < n_steps; i++) EDGE_HIP_TRY(hipGraphLaunch(s->graph_exec, st));
   return EDGE_OK;
+}
+}  // namespace
+
+extern "C" int edge_el_step(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* t_int, int use_graph,
+                            void* stream) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (!dofs || !t_int || n_steps < 0) return edge::fail(EDGE_ERR_ARG, "step arguments");
+  if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_step_a)");
+  return step_impl(s, n_steps, dt, dofs, nullptr, t_int, use_graph, (hipStream_t)stream);
+}
+
+extern "C" int edge_el_step_a(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* dofs_a, void* t_int,
+                              int use_graph, void* stream) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (s->n_rms == 0) return edge_el_step(s, n_steps, dt, dofs, t_int, use_graph, stream);
+  if (!dofs || !dofs_a || !t_int || n_steps < 0) return edge::fail(EDGE_ERR_ARG, "step arguments");
+  return step_impl(s, n_steps, dt, dofs, dofs_a, t_int, use_graph, (hipStream_t)stream);
 }
 
 extern "C" int edge_el_info(const edge_el_solver* s, int64_t* n_el, int32_t* n_modes, int32_t* n_runs,

@@ -94,3 +94,25 @@ def from_tets(ve_crds, el_ve, bf_ve=None, bf_type=None, default_bc=FA_OUTFLOW):
         setattr(m, k, v.ctypes.data)
     _lib.check(L.edge_mesh_tets(C.byref(t), C.byref(m)))
     return Mesh(**a)
+
+
+def union(meshes):
+    """Disjoint union of meshes as one mesh (element, face and vertex ids of
+    member i follow those of members 0..i-1; missing neighbours stay -1), and
+    the element range (first, n) of each member.  One solver over the union
+    advances every member in one launch per kernel."""
+    ve, el, fa = 0, 0, 0
+    parts = {k: [] for k in ("ve_crds", "el_ve", "el_fa", "fa_el", "el_fa_el", "f_id", "v_id", "fa_type", "normals",
+                             "tangents", "areas", "in_dia", "volume")}
+    ranges = []
+    for m in meshes:
+        parts["ve_crds"].append(m.ve_crds)
+        parts["el_ve"].append(m.el_ve + ve)
+        parts["el_fa"].append(m.el_fa + fa)
+        parts["fa_el"].append(np.where(m.fa_el >= 0, m.fa_el + el, m.fa_el))
+        parts["el_fa_el"].append(np.where(m.el_fa_el >= 0, m.el_fa_el + el, m.el_fa_el))
+        for k in ("f_id", "v_id", "fa_type", "normals", "tangents", "areas", "in_dia", "volume"):
+            parts[k].append(getattr(m, k))
+        ranges.append((el, m.n_el))
+        ve, el, fa = ve + m.n_ve, el + m.n_el, fa + m.n_fa
+    return Mesh(**{k: np.ascontiguousarray(np.concatenate(v)) for k, v in parts.items()}), ranges

@@ -215,7 +215,11 @@ class ViscoSolver(ElasticSolver):
         _lib.check(_lib.lib().edge_el_neigh_a(self._h, first, n, t_int.data_ptr(), dofs.data_ptr(),
                                               dofs_a.data_ptr(), _stream_ptr(stream)))
 
-    def step(self, dofs, t_int, dt, n_steps=1, dofs_a=None, stream=None):
-        for _ in range(n_steps):
-            self.local(dofs, t_int, dt, dofs_a, stream=stream)
-            self.neigh(dofs, t_int, dofs_a, stream=stream)
+    def step(self, dofs, t_int, dt, n_steps=1, dofs_a=None, graph=False, stream=None):
+        """n_steps global time steps with the anelastic DOFs; graph=True
+        captures one step into a HIP graph on first use and replays it."""
+        self._check_arr(dofs)
+        self._check_arr(t_int)
+        self._check_a(dofs_a)
+        _lib.check(_lib.lib().edge_el_step_a(self._h, int(n_steps), float(dt), dofs.data_ptr(), dofs_a.data_ptr(),
+                                             t_int.data_ptr(), 1 if graph else 0, _stream_ptr(stream)))

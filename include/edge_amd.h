@@ -235,6 +235,13 @@ int edge_el_local_a(edge_el_solver *s, int64_t first, int64_t n_els, double dt, 
 int edge_el_neigh_a(edge_el_solver *s, int64_t first, int64_t n_els, const void *t_int, void *dofs, void *dofs_a,
                     void *stream);
 
+/* edge_el_step with the anelastic DOFs (time loop of the reference's
+ * TimeGroupStatic::computeStep over AderDg::local / neigh with TL_N_RMS > 0);
+ * use_graph != 0 captures one step into a HIP graph on first use and
+ * replays it.  As edge_el_step for an elastic solver (dofs_a ignored). */
+int edge_el_step_a(edge_el_solver *s, int64_t n_steps, double dt, void *dofs, void *dofs_a, void *t_int,
+                   int use_graph, void *stream);
+
 /* Copies a viscoelastic solver's set-up to host FP64 arrays (any may be
  * NULL): anelastic star matrices [nEl][3][6][3], source matrices
  * [nEl][n_rms][6][6], anelastic flux solvers [2][nEl][4][6][9], relaxation

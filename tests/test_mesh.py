@@ -106,3 +106,26 @@ def test_traces_match_through_flux_matrices(order):
             np.testing.assert_allclose(nb, own, rtol=1e-9, atol=1e-9)
             checked += 1
     assert checked > 0
+
+
+def test_union_keeps_each_member():
+    """The disjoint union holds every member's entities with shifted ids;
+    each member's elements see only neighbours of the same member."""
+    parts = [M.box(3, jitter=0.1, seed=1), M.box((2, 3, 2), bc=(FA_OUTFLOW, FA_PERIODIC, FA_FREE_SURFACE), seed=2),
+             M.box(1, bc=(FA_FREE_SURFACE,) * 3)]
+    u, ranges = M.union(parts)
+    assert u.n_el == sum(p.n_el for p in parts) and u.n_fa == sum(p.n_fa for p in parts)
+    assert ranges == [(0, 162), (162, 72), (234, 6)]
+    ve = fa = 0
+    for p, (first, n) in zip(parts, ranges):
+        sl = slice(first, first + n)
+        np.testing.assert_array_equal(u.el_ve[sl], p.el_ve + ve)
+        np.testing.assert_array_equal(u.el_fa[sl], p.el_fa + fa)
+        ne = u.el_fa_el[sl]
+        assert np.all((ne == -1) | ((ne >= first) & (ne < first + n)))
+        np.testing.assert_array_equal(np.where(ne >= 0, ne - first, -1), np.where(p.el_fa_el >= 0, p.el_fa_el, -1))
+        np.testing.assert_array_equal(u.el_crds()[sl], p.el_crds())
+        np.testing.assert_array_equal(u.fa_type[fa:fa + p.n_fa], p.fa_type)
+        ve, fa = ve + p.n_ve, fa + p.n_fa
+    fe = u.fa_el
+    assert (u.el_fa[fe[:, 0]] == np.arange(u.n_fa)[:, None]).any(axis=1).all()
