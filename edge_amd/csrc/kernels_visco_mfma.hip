@@ -37,8 +37,11 @@
 //    w_j = (V^T tInt)_j F^L_fa,  L_j = w_j F^T_fa,
 //    Q += sum_j U[.][j] L_j,  QA_rm += w_rm sum_j U_A[.][j] L_j.
 // Neighbour step (kernels/SurfIntVanilla.hpp:274-357), four waves, wave f
-// projects the tInt of face f's neighbour with F^N and the neighbour-side
-// factors; the lifted tiles of the four faces are combined per position.
+// lifts face f's projection of the neighbour's tInt (F^N and the
+// neighbour-side factors): the projection the neighbour's local step wrote
+// (waves 4 and 5 of the local kernel compute them during the surface
+// integral), or one made here from the neighbour's tInt tile; the lifted
+// tiles of the four faces are combined per position.
 // Summation order differs from the reference's loops; results agree to
 // rounding.
 #include <hip/hip_runtime.h>
@@ -172,22 +175,24 @@ __device__ __forceinline__ void raw_products(const float* __restrict__ m, const 
       }
 }
 
-// Lifted face tiles L_j = ((V^T x)_j F) F^T for one face: x ([9][N][R], LDS),
-// the projection matrix F ([N][NF]), the lift F^T ([NF][N]) and the rows V^T
-// ([3][9], wave-uniform) of the face's rank-3 solver factors; out (LDS,
-// [3][N][R]) may alias x (every read of x precedes the first write).
-// Column-half pairs: (w_0 | w_1) and (w_2 | w_2); the lift takes its B
-// operands straight from the projection's D tiles (face mode 4 g + v).
+// Face projections w_j = (V^T x)_j F for one face: x ([9][N][R], LDS or
+// global), the projection matrix F ([N][NF]) and the rows V^T ([3][9],
+// wave-uniform) of the face's rank-3 solver factors.  Column-half pairs:
+// w01 = (w_0 | w_1) and w2 = (w_2 | w_2) as D tiles (face mode 4 g + v).
 template <int O>
-__device__ __forceinline__ void face_lifts(const float* __restrict__ fl, const float* __restrict__ ft,
-                                           const cfloat* __restrict__ vt, const float* x, float* out, int g,
-                                           int c) {
+__device__ __forceinline__ void face_project(const float* __restrict__ fl, const cfloat* __restrict__ vt,
+                                             const float* x, v4f& w01, v4f& w2, int g, int c) {
   using D = VDims<O>;
-  constexpr int N = D::N, NF = D::NF, NP = D::NP, NT = tiles(N), KS = ksteps(N);
+  constexpr int N = D::N, NF = D::NF, KS = ksteps(N);
   const int h = c >> 3, r = c & 7;
-  float cv[27];
+  // per lane the row of its column half (V^T row h) and row 2
+  float ch[9], c2[9];
 #pragma unroll
-  for (int i = 0; i < 27; i++) cv[i] = vt[i];
+  for (int p = 0; p < 9; p++) {
+    const float v0 = vt[p], v1 = vt[9 + p];
+    ch[p] = h ? v1 : v0;
+    c2[p] = vt[18 + p];
+  }
   auto load = [&](int s, float& a, float (&b)[9]) {
     const int k = 4 * s + g;
     a = ldg(fl, k * NF + c, k < N && c < NF);
@@ -199,7 +204,8 @@ __device__ __forceinline__ void face_lifts(const float* __restrict__ fl, const f
         b[q] = k < N ? x[(q * N + (k < N ? k : 0)) * R + r] : 0.f;
     }
   };
-  v4f w01 = zero4(), w2 = zero4();
+  w01 = zero4();
+  w2 = zero4();
   float an, bn[9];
   load(0, an, bn);
 #pragma unroll
@@ -210,17 +216,25 @@ __device__ __forceinline__ void face_lifts(const float* __restrict__ fl, const f
     for (int q = 0; q < 9; q++) b[q] = bn[q];
     if (s + 1 < KS) load(s + 1, an, bn);
     __builtin_amdgcn_sched_barrier(0);
-    float y[3];
+    float yh = 0.f, y2 = 0.f;
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      float z = 0.f;
-#pragma unroll
-      for (int q = 0; q < 9; q++) z += cv[j * 9 + q] * b[q];
-      y[j] = z;
+    for (int q = 0; q < 9; q++) {
+      yh += ch[q] * b[q];
+      y2 += c2[q] * b[q];
     }
-    w01 = mfma(a, h ? y[1] : y[0], w01);
-    w2 = mfma(a, y[2], w2);
+    w01 = mfma(a, yh, w01);
+    w2 = mfma(a, y2, w2);
   }
+}
+
+// Lifted face tiles L_j = w_j F^T ([3][N][R] into out, LDS) from the paired
+// projections (face_project) and the lift F^T ([NF][N]); the B operands come
+// straight from the projections' D tiles.
+template <int O>
+__device__ __forceinline__ void face_lift(const float* __restrict__ ft, v4f w01, v4f w2, float* out, int g, int c) {
+  using D = VDims<O>;
+  constexpr int N = D::N, NF = D::NF, NP = D::NP, NT = tiles(N);
+  const int h = c >> 3, r = c & 7;
   v4f l01[NT], l2[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) {
@@ -247,6 +261,16 @@ __device__ __forceinline__ void face_lifts(const float* __restrict__ fl, const f
         if (h == 0) out[2 * NP + n * R + r] = l2[t][v];
       }
     }
+}
+
+// Face projections exchanged between the local and the neighbour step: per
+// element face the two paired D tiles, lane-major (lane l holds its four
+// values of w01 at [l][0..3] and of w2 at [64 + l][0..3]).
+constexpr int kProjFace = 512;
+
+__device__ __forceinline__ void store_proj(float* __restrict__ p, v4f w01, v4f w2, int lane) {
+  reinterpret_cast<v4f*>(p)[lane] = w01;
+  reinterpret_cast<v4f*>(p)[64 + lane] = w2;
 }
 
 // Per position: stress rows (6), velocity rows (3) and the anelastic
@@ -307,6 +331,8 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   const cfloat* rfs = as_const(s.rfs);
   const float* q0 = dofs + el * SZ;
   float* qa0 = dofs_a + el * NRM * SZA;
+  const cfloat* fsf = as_const(s.fsf + el * 4 * 54);
+  const cfloat* fsfa = as_const(s.fsfa + el * 4 * 18);
 
   {
     const float4* gq = reinterpret_cast<const float4*>(q0);
@@ -397,15 +423,32 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   }
   __syncthreads();
 
-  // ---- volume
+  // this element's tInt as the neighbour through face f sees it (its flux
+  // matrix and neighbour-side solver), for the neighbour step
+  auto nb_proj = [&](int f) {
+    const int32_t rv = ((const cint32*)s.snd_v)[el * 4 + f];
+    if (rv >= 0) {  // somebody receives through this face (uniform)
+      const int fm = ((const cuint8*)s.snd_fm)[el * 4 + f];
+      v4f w01, w2;
+      face_project<O>(s.dg.fluxLN + fm * N * D::NF, as_const(s.fsf + ((int64_t)s.n_el * 4 + rv) * 54 + 27), l_x,
+                      w01, w2, g, c);
+      store_proj(s.proj + (el * 4 + f) * kProjFace, w01, w2, lane);
+    }
+  };
+
+  // ---- volume; the waves with fewer output tiles also project faces 0..2
+  // for the neighbours
   {
     // output tiles: the first half of the waves [0, TV0), the second [TV0, tiles(N))
     constexpr int TN = tiles(N), TV0 = (TN + 1) / 2;
-    if (t_ck == 0)
+    if (t_ck == 0) {
       raw_products<O, NV, N, TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, 0, g, c);
-    else if constexpr (TN > TV0)
-      raw_products<O, NV, N, TN - TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, TV0, g,
-                                       c);
+    } else {
+      if constexpr (TN > TV0)
+        raw_products<O, NV, N, TN - TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, TV0,
+                                         g, c);
+      if (s.proj) nb_proj(dir);
+    }
   }
   __syncthreads();
   float aq[9][PJ], aa[6][PJ];
@@ -435,12 +478,15 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   }
   __syncthreads();
 
-  // ---- local surface: waves 0..3 the lifted tiles of face wv in l_p[wv][3][N][R]
-  const cfloat* fsf = as_const(s.fsf + el * 4 * 54);
-  const cfloat* fsfa = as_const(s.fsfa + el * 4 * 18);
-  if (wv < 4)
-    face_lifts<O>(s.dg.fluxLN + wv * N * D::NF, s.dg.fluxT + wv * D::NF * N, fsf + wv * 54 + 27, l_x,
-                  l_p + wv * 3 * NP, g, c);
+  // ---- local surface: waves 0..3 the lifted tiles of face wv in
+  // l_p[wv][3][N][R]; wave 4 the neighbours' projection through face 3
+  if (wv < 4) {
+    v4f w01, w2;
+    face_project<O>(s.dg.fluxLN + wv * N * D::NF, fsf + wv * 54 + 27, l_x, w01, w2, g, c);
+    face_lift<O>(s.dg.fluxT + wv * D::NF * N, w01, w2, l_p + wv * 3 * NP, g, c);
+  } else if (wv == 4 && s.proj) {
+    nb_proj(3);
+  }
   __syncthreads();
 
   // ---- Q += volume + source + surface; QA_rm += w_rm (A - tIntA_rm + surface)
@@ -476,33 +522,47 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   }
 }
 
-template <int O, int NRM>
+// Neighbour step.  PROJ: the face projections the local step wrote (no tInt
+// reads); else the neighbours' tInt tiles, projected here.
+template <int O, int NRM, bool PROJ>
 __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, int64_t first, int64_t n,
                                                             const float* __restrict__ t_int,
                                                             float* __restrict__ dofs, float* __restrict__ dofs_a) {
   using D = VDims<O>;
   constexpr int N = D::N, NF = D::NF, NP = D::NP, SZ = D::SZ, SZA = D::SZA;
   constexpr int PJ = (NP + kNeighThreads - 1) / kNeighThreads;
-  // per face: the neighbour's tInt, then (aliased) its lifted tiles [3][N][R]
-  __shared__ __align__(16) float l_n[4][SZ];
+  // per face: (without projections) the neighbour's tInt, then (aliased) its
+  // lifted tiles [3][N][R]
+  __shared__ __align__(16) float l_n[4][PROJ ? 3 * NP : SZ];
   const int64_t off = xcd_element(n);
   if (off >= n) return;
   const int64_t el = first + off;
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
-  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index = face, uniform
   const cint32* nes = (const cint32*)(s.nb.ne + el * 4);
+  const cint32* rcv = (const cint32*)(s.rcv + el * 4);
   const cfloat* fsf = as_const(s.fsf + (s.n_el + el) * 4 * 54);
   const cfloat* fsfa = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
-  const int32_t ne = nes[wv];
-  if (ne >= 0) {  // uniform across the wave
-    const float4* gt = reinterpret_cast<const float4*>(t_int + (int64_t)ne * SZ);
-    float4* lt = reinterpret_cast<float4*>(l_n[wv]);
-    for (int i = lane; i < SZ / 4; i += 64) lt[i] = gt[i];
-  }
-  __syncthreads();
-  if (ne >= 0) {
-    const int fm = ((const cuint8*)s.nb.fmat)[el * 4 + wv];
-    face_lifts<O>(s.dg.fluxLN + fm * N * NF, s.dg.fluxT + wv * NF * N, fsf + wv * 54 + 27, l_n[wv], l_n[wv], g, c);
+  if constexpr (PROJ) {
+    const int32_t slot = rcv[wv];
+    if (slot >= 0) {  // else outflow (uniform)
+      const v4f* p = reinterpret_cast<const v4f*>(s.proj + (int64_t)slot * kProjFace);
+      face_lift<O>(s.dg.fluxT + wv * NF * N, p[lane], p[64 + lane], l_n[wv], g, c);
+    }
+  } else {
+    const int32_t ne = nes[wv];
+    if (ne >= 0) {  // uniform across the wave
+      const float4* gt = reinterpret_cast<const float4*>(t_int + (int64_t)ne * SZ);
+      float4* lt = reinterpret_cast<float4*>(l_n[wv]);
+      for (int i = lane; i < SZ / 4; i += 64) lt[i] = gt[i];
+    }
+    __syncthreads();
+    if (ne >= 0) {
+      const int fm = ((const cuint8*)s.nb.fmat)[el * 4 + wv];
+      v4f w01, w2;
+      face_project<O>(s.dg.fluxLN + fm * N * NF, fsf + wv * 54 + 27, l_n[wv], w01, w2, g, c);
+      face_lift<O>(s.dg.fluxT + wv * NF * N, w01, w2, l_n[wv], g, c);
+    }
   }
   __syncthreads();
   float* gq = dofs + el * SZ;
@@ -519,7 +579,7 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
       for (int q = 0; q < 6; q++) ua[q] = 0.f;
 #pragma unroll
       for (int f = 0; f < 4; f++) {
-        if (nes[f] < 0) continue;  // outflow (uniform)
+        if ((PROJ ? rcv[f] : nes[f]) < 0) continue;  // outflow (uniform)
         const float l0 = l_n[f][p], l1 = l_n[f][NP + p], l2 = l_n[f][2 * NP + p];
         const cfloat* u = fsf + f * 54;
         const cfloat* uan = fsfa + f * 18;
@@ -548,9 +608,12 @@ void launch_o(const SolverDev<float>& s, int64_t first, int64_t n, float dt, flo
   if (local)
     hipLaunchKernelGGL((k_local_vm<O, NRM>), dim3(grid_of(n)), dim3(kLocalThreads), 0, st, s, first, n, dt, dofs,
                        dofs_a, t_int);
+  else if (s.proj)
+    hipLaunchKernelGGL((k_neigh_vm<O, NRM, true>), dim3(grid_of(n)), dim3(kNeighThreads), 0, st, s, first, n, tin,
+                       dofs, dofs_a);
   else
-    hipLaunchKernelGGL((k_neigh_vm<O, NRM>), dim3(grid_of(n)), dim3(kNeighThreads), 0, st, s, first, n, tin, dofs,
-                       dofs_a);
+    hipLaunchKernelGGL((k_neigh_vm<O, NRM, false>), dim3(grid_of(n)), dim3(kNeighThreads), 0, st, s, first, n, tin,
+                       dofs, dofs_a);
 }
 
 template <int O>

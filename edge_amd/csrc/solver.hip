@@ -36,6 +36,7 @@ struct edge_el_solver {
   void* fsf = nullptr;       // [2][nEl][4][54]
   bool fs_rank3 = false;
   bool want_proj = false;    // kernel selector 2: exchange face projections when possible
+  bool proj_tables = false;  // build the projection slot tables whatever the kernel (MFMA viscoelastic)
   void* proj = nullptr;      // [nEl][4][768]
   int32_t* snd_v = nullptr;  // [nEl][4]
   uint8_t* snd_fm = nullptr; // [nEl][4]
@@ -172,7 +173,7 @@ int build_neigh(edge_el_solver* s, const edge_mesh_in* m) {
   EDGE_HIP_TRY(hipMalloc(&s->fmat, 4 * ne));
   EDGE_HIP_TRY(hipMemcpy(s->ne, nb.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice));
   EDGE_HIP_TRY(hipMemcpy(s->fmat, fm.data(), 4 * ne, hipMemcpyHostToDevice));
-  if (s->kernel != 2) return EDGE_OK;
+  if (s->kernel != 2 && !s->proj_tables) return EDGE_OK;
   // Face projections: element el's face fa receives the projection that its
   // neighbour n computes through the shared face (n's face fId, or fa itself
   // on a free surface) with el's flux matrix and neighbour-side solver.
@@ -233,7 +234,7 @@ int pick_kernel(edge_el_solver* s) {
 }
 
 int create_common(const edge_el_config* cfg, const edge_mesh_in* mesh, edge_el_solver** out,
-                  edge_el_solver** made) {
+                  edge_el_solver** made, bool proj_tables = false) {
   int rc = check_cfg(cfg);
   if (rc) return rc;
   if ((rc = check_mesh(mesh))) return rc;
@@ -248,6 +249,7 @@ int create_common(const edge_el_config* cfg, const edge_mesh_in* mesh, edge_el_s
   s->n_modes = t.n;
   s->n_face_modes = t.nf;
   s->real_bytes = cfg->precision == 64 ? 8 : 4;
+  s->proj_tables = proj_tables;
   if ((rc = pick_kernel(s))) return rc;
   rc = cfg->precision == 64 ? upload_mats<double>(s, t) : upload_mats<float>(s, t);
   if (rc) return rc;
@@ -513,11 +515,11 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
   if (!mesh || !mesh->ve_crds || !mesh->el_ve || !mesh->normals || !mesh->tangents || !mesh->areas || !rho ||
       !lam || !mu || !qp || !qs)
     return edge::fail(EDGE_ERR_ARG, "mesh geometry, material or quality factors missing");
-  if (cfg->kernel > 2) return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic solvers: kernel 0, 1 or 2");
+  if (cfg->kernel > 3) return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic solvers: kernel 0, 1, 2 or 3");
   if (!edge::visco_supported(cfg->order, cfg->n_runs, cfg->precision / 8, vc->n_rms))
     return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic kernels: 1 or 8 runs whose tiles fit LDS");
   const bool vm_ok = edge::visco_mfma_supported(cfg->order, cfg->n_runs, cfg->precision, vc->n_rms);
-  if (cfg->kernel == 2 && !vm_ok)
+  if ((cfg->kernel == 2 || cfg->kernel == 3) && !vm_ok)
     return edge::fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: FP32, 8 runs, orders 3-5, 1-3 mechanisms");
   const int64_t ne = mesh->n_el;
   for (int64_t el = 0; el < ne; el++)
@@ -544,12 +546,12 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
     if (!nz && src[i] != 0) src_pattern = false;
   }
   const bool want_vm = vm_ok && cfg->kernel != 1 && src_pattern;
-  if (cfg->kernel == 2 && !want_vm)
+  if ((cfg->kernel == 2 || cfg->kernel == 3) && !want_vm)
     return edge::fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: source matrices off the pattern");
   edge_el_config c = *cfg;
   c.kernel = 1;
   edge_el_solver* s = nullptr;
-  int rc = create_common(&c, mesh, out, &s);
+  int rc = create_common(&c, mesh, out, &s, want_vm);
   if (rc == EDGE_OK) {
     s->n_rms = nr;
     s->lam_e = lam_e;
@@ -582,7 +584,11 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
     if (rc == EDGE_OK && want_vm) {
       if (s->fs_rank3) {
         s->kernel = 2;
-      } else if (cfg->kernel == 2) {
+        // face projections exchanged between the local and neighbour steps
+        // ([nEl][4][512]: the paired projection tiles, lane-major)
+        if (cfg->kernel != 3 && s->proj_ok && hipMalloc(&s->proj, sizeof(float) * 4 * 512 * ne) != hipSuccess)
+          rc = edge::fail(EDGE_ERR_HIP, "allocation of the face projections");
+      } else if (cfg->kernel == 2 || cfg->kernel == 3) {
         rc = edge::fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: flux solvers not of rank 3");
       } else {  // the generic kernels
         hipFree(s->fsf);
