@@ -56,7 +56,9 @@ def local_bytes(order, runs, prec=4, proj=False):
     return 3 * 9 * n * runs * prec + (3 * 81 + 4 * 81) * prec + (4 * 3 * 256 * prec if proj else 0)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_k_local_o5_fp32_r16_196608tets.md")
+# HBM traffic of the local kernel: with the face-projection exchange, and without
+TRAFFIC_PROFILE = {True: os.path.join(ROOT, "profiles", "traffic_k_local_o5_fp32_r16_196608tets.md"),
+                   False: os.path.join(ROOT, "profiles", "traffic_k_local_tiles_o5_fp32_r16_196608tets.md")}
 
 
 def read_traffic(path, n_el, order, runs):
@@ -369,6 +371,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true", help="--rows without the 24-mesh ensemble")
+    ap.add_argument("--no-autotune", action="store_true", help="measure the kernels --kernel selects, untuned")
     ap.add_argument("--rows", action="store_true",
                     help="measure the other rows (viscoelastic, LTS, receivers/sources, rupture) instead")
     ap.add_argument("--no-points", action="store_true",
@@ -399,6 +402,9 @@ def main():
 
     # ---- FP32 fused-16 (headline)
     s32 = ElasticSolver(m, order, runs, 32, rho, lam, mu, kernel=args.kernel)
+    tune = {}
+    if not args.no_autotune:  # untimed set-up: the fastest kernel variant on this mesh
+        tune["fp32"] = [{"kernel": k, "projections": p, "ms_per_step": ms} for k, p, ms in s32.autotune(dt, 3)]
     proj = bool(s32.kernel_info()[1] & PATH_PROJECTIONS)
     d32 = (torch.from_numpy(q0)[..., None] * scale).to(torch.float32).cuda().contiguous()
     t32 = torch.zeros_like(d32)
@@ -416,6 +422,8 @@ def main():
     # ---- FP64 plain single run (the drop-in)
     if not args.no_fp64:
         s64 = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=args.kernel)
+        if not args.no_autotune:
+            tune["fp64"] = [{"kernel": k, "projections": p, "ms_per_step": ms} for k, p, ms in s64.autotune(dt, 2)]
         d64 = torch.from_numpy(q0[..., None].copy()).cuda().contiguous()
         t64 = torch.zeros_like(d64)
         print("[bench] fp64 plain timing", file=sys.stderr, flush=True)
@@ -433,7 +441,7 @@ def main():
         f_nei = fl["neigh"] - (fl["proj"] if proj else 0)
         achieved = f_loc * runs * m.n_el / res["fp32"]["local"]
         step_ach = (fl["local"] + fl["neigh"]) * runs * m.n_el / (res["fp32"]["local"] + res["fp32"]["neigh"])
-        traffic = read_traffic(TRAFFIC_PROFILE, m.n_el, order, runs)
+        traffic = read_traffic(TRAFFIC_PROFILE[proj], m.n_el, order, runs)
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32_mfma"] / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK["fp32_mfma"], "traffic": traffic,
                 "traffic_alg": local_bytes(order, runs, proj=proj) * m.n_el,
@@ -452,6 +460,8 @@ def main():
                    "elements": m.n_el, "order": order, "runs": runs, "dt": dt, "finite": res["fp32"]["finite"]},
         "roofline": roof,
     }
+    if tune:
+        out["autotune"] = tune
     if "fp64" in res:
         v64 = dofs_per_step * res["fp64"]["steps"] / res["fp64"]["wall"]
         out["fp64_plain"] = {"value": v64, "unit": "DOF-updates/s", "ms_per_step": 1e3 * res["fp64"]["wall"] /

@@ -655,7 +655,9 @@ __device__ __forceinline__ void load_proj(const float* __restrict__ p, v4f (&w)[
   for (int j = 0; j < 3; j++) w[j] = reinterpret_cast<const v4f*>(p + j * 256)[lane];
 }
 
-template <int O, bool SP, bool R3>
+// WP: also write the face projections for the neighbour step (s.proj,
+// rank-3 solvers only)
+template <int O, bool SP, bool R3, bool WP = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __restrict__ dofs,
              float* __restrict__ t_int) {
@@ -830,8 +832,8 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
     // loaded one projection ahead.
     // ---- this element's tInt as each face-adjacent element sees it (the
     // neighbour's flux matrix and solver side), for the neighbour step
-    v4f a0 = pprj[(s.proj ? as_const(s.snd_fm)[el * 4] : 0) * NGP * 64 + lane];
-    if (s.proj) {
+    v4f a0 = pprj[(WP ? as_const(s.snd_fm)[el * 4] : 0) * NGP * 64 + lane];
+    if constexpr (WP) {
 #pragma unroll 1
       for (int f = 0; f < 4; f++) {
         const int32_t rv = as_const(s.snd_v)[el * 4 + f];
@@ -1029,8 +1031,12 @@ template <int O>
 void launch_local_o(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs, float* t_int,
                     hipStream_t st) {
   const dim3 grid(grid_of(n)), block(64);
-  if (s.sparse_star && s.fs_rank3)
+  if (s.sparse_star && s.fs_rank3 && s.proj)
+    hipLaunchKernelGGL((k_local<O, true, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+  else if (s.sparse_star && s.fs_rank3)
     hipLaunchKernelGGL((k_local<O, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+  else if (s.fs_rank3 && s.proj)
+    hipLaunchKernelGGL((k_local<O, false, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
   else if (s.sparse_star)
     hipLaunchKernelGGL((k_local<O, true, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
   else if (s.fs_rank3)

@@ -36,6 +36,7 @@ struct edge_el_solver {
   void* fsf = nullptr;       // [2][nEl][4][54]
   bool fs_rank3 = false;
   bool want_proj = false;    // kernel selector 2: exchange face projections when possible
+  bool use_proj = true;      // the projections are exchanged when they exist (the autotuner may turn this off)
   bool proj_tables = false;  // build the projection slot tables whatever the kernel (MFMA viscoelastic)
   void* proj = nullptr;      // [nEl][4][768]
   int32_t* snd_v = nullptr;  // [nEl][4]
@@ -90,7 +91,7 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.sparse_star = s->sparse_star;
   d.fsf = static_cast<const T*>(s->fsf);
   d.fs_rank3 = s->fs_rank3 && s->fsf;
-  d.proj = d.fs_rank3 ? static_cast<T*>(s->proj) : nullptr;
+  d.proj = d.fs_rank3 && s->use_proj ? static_cast<T*>(s->proj) : nullptr;
   d.snd_v = s->snd_v;
   d.snd_fm = s->snd_fm;
   d.rcv = s->rcv;
@@ -684,7 +685,96 @@ extern "C" int edge_el_kernel_info(const edge_el_solver* s, int32_t* kernel, int
   if (kernel) *kernel = s->kernel;
   if (flags) {
     const bool r3 = s->kernel == 2 && s->fs_rank3 && s->fsf;
-    *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (r3 && s->proj ? 4 : 0) | (s->n_rms > 0 ? 8 : 0);
+    *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (r3 && s->proj && s->use_proj ? 4 : 0) |
+             (s->n_rms > 0 ? 8 : 0);
+  }
+  return EDGE_OK;
+}
+
+namespace {
+struct TuneCand {
+  int kernel;
+  bool proj;
+};
+
+// The kernel variants a solver can switch between with the data it holds.
+std::vector<TuneCand> tune_candidates(const edge_el_solver* s) {
+  std::vector<TuneCand> c;
+  const int prec_bytes = (int)s->real_bytes;
+  const bool gen = s->n_rms > 0 ? edge::visco_supported(s->cfg.order, s->cfg.n_runs, prec_bytes, s->n_rms)
+                                : edge::generic_supported(s->cfg.order, s->cfg.n_runs, prec_bytes);
+  if (s->kernel == 2) {
+    if (s->proj) c.push_back({2, true});
+    c.push_back({2, false});
+  } else if (s->kernel == 4) {
+    c.push_back({4, false});
+  }
+  if (gen || c.empty()) c.push_back({1, false});
+  return c;
+}
+}  // namespace
+
+extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, void* stream, int32_t* n_cands,
+                                int32_t* kernels, int32_t* projs, double* ms) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (n_reps < 1 || !(dt > 0)) return edge::fail(EDGE_ERR_ARG, "autotune: n_reps >= 1 and dt > 0");
+  const std::vector<TuneCand> cand = tune_candidates(s);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t dof_bytes = s->real_bytes * 9 * s->n_modes * s->cfg.n_runs * s->n_el;
+  const size_t a_bytes = dof_bytes / 9 * 6 * s->n_rms;
+  void *dofs = nullptr, *t_int = nullptr, *dofs_a = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const int old_kernel = s->kernel;
+  const bool old_proj = s->use_proj;
+  std::vector<double> t(cand.size(), 0.0);
+  int rc = EDGE_OK;
+  auto hip = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == EDGE_OK) rc = edge::fail(EDGE_ERR_HIP, std::string("autotune: ") + what + ": " +
+                                                                        hipGetErrorString(e));
+    return rc == EDGE_OK;
+  };
+  // scratch DOFs (zeros: the same instruction stream as any data)
+  if (hip(hipMalloc(&dofs, dof_bytes), "allocation") && hip(hipMalloc(&t_int, dof_bytes), "allocation") &&
+      (s->n_rms == 0 || hip(hipMalloc(&dofs_a, a_bytes), "allocation")) &&
+      hip(hipMemsetAsync(dofs, 0, dof_bytes, st), "memset") && hip(hipMemsetAsync(t_int, 0, dof_bytes, st), "memset") &&
+      (s->n_rms == 0 || hip(hipMemsetAsync(dofs_a, 0, a_bytes, st), "memset")) &&
+      hip(hipEventCreate(&e0), "event") && hip(hipEventCreate(&e1), "event")) {
+    drop_graph(s);
+    for (size_t i = 0; i < cand.size() && rc == EDGE_OK; i++) {
+      s->kernel = cand[i].kernel;
+      s->use_proj = cand[i].proj;
+      rc = one_step(s, dt, dofs, dofs_a, t_int, st);  // warm-up
+      if (rc == EDGE_OK) hip(hipEventRecord(e0, st), "event");
+      for (int r = 0; r < n_reps && rc == EDGE_OK; r++) rc = one_step(s, dt, dofs, dofs_a, t_int, st);
+      if (rc == EDGE_OK && hip(hipEventRecord(e1, st), "event") && hip(hipEventSynchronize(e1), "sync")) {
+        float f = 0.f;
+        if (hip(hipEventElapsedTime(&f, e0, e1), "event")) t[i] = f / n_reps;
+      }
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(dofs);
+  hipFree(t_int);
+  hipFree(dofs_a);
+  if (rc != EDGE_OK) {
+    s->kernel = old_kernel;
+    s->use_proj = old_proj;
+    return rc;
+  }
+  size_t best = 0;
+  for (size_t i = 1; i < cand.size(); i++)
+    if (t[i] < t[best]) best = i;
+  s->kernel = cand[best].kernel;
+  s->use_proj = cand[best].proj;
+  if (n_cands) {
+    const int32_t cap = *n_cands;
+    *n_cands = (int32_t)cand.size();
+    for (int32_t i = 0; i < (int32_t)cand.size() && i < cap; i++) {
+      if (kernels) kernels[i] = cand[i].kernel;
+      if (projs) projs[i] = cand[i].proj ? 1 : 0;
+      if (ms) ms[i] = t[i];
+    }
   }
   return EDGE_OK;
 }

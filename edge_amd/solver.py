@@ -58,6 +58,16 @@ class ElasticSolver:
         _lib.check(_lib.lib().edge_el_kernel_info(self._h, C.byref(k), C.byref(f)))
         return k.value, f.value
 
+    def autotune(self, dt, n_reps=3, stream=None):
+        """Times the kernel variants this solver can switch to on its mesh
+        and keeps the fastest; returns [(kernel, projections, ms per step)]."""
+        cap = 8
+        n = C.c_int32(cap)
+        k, p, ms = (C.c_int32 * cap)(), (C.c_int32 * cap)(), (C.c_double * cap)()
+        _lib.check(_lib.lib().edge_el_autotune(self._h, float(dt), int(n_reps), _stream_ptr(stream), C.byref(n), k,
+                                               p, ms))
+        return [(k[i], bool(p[i]), ms[i]) for i in range(min(n.value, cap))]
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib.lib().edge_el_destroy(self._h)

@@ -1,0 +1,77 @@
+"""The autotuner: it times every kernel variant a solver can switch to on
+its own mesh, keeps the fastest, and the solver still meets the parity bars
+against the FP64 oracle afterwards (FP64 relative L2 <= 1e-12, FP32 <= 1e-5).
+Each candidate is a kernel variant the parity tests check on its own through
+the C ABI's kernel selector (test_gpu_parity.py, test_gpu_visco.py)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from edge_amd import mesh as M
+from edge_amd import setups as S
+from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
+from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, PATH_PROJECTIONS, ElasticSolver,
+                             ViscoSolver)
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def _dev(a, prec):
+    import torch
+    return torch.tensor(a, dtype=torch.float64 if prec == 64 else torch.float32, device="cuda").contiguous()
+
+
+@pytest.mark.parametrize("prec,runs,expect", [(32, 16, {(KERNEL_MFMA, True), (KERNEL_MFMA, False)}),
+                                              (64, 1, {(KERNEL_MFMA64, False), (KERNEL_GENERIC, False)})])
+def test_autotune_elastic(gpu, prec, runs, expect):
+    order = 4
+    m = M.box(4, bc=(FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC), jitter=0.1, seed=4)
+    rho, lam, mu = S.random_materials(m.n_el, seed=4)
+    s = ElasticSolver(m, order, runs, prec, rho, lam, mu)
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    cands = s.autotune(dt, n_reps=2)
+    assert expect <= {(k, p) for k, p, _ in cands}
+    assert all(ms > 0 for *_, ms in cands)
+    k, p, _ = min(cands, key=lambda c: c[2])
+    kernel, flags = s.kernel_info()
+    assert kernel == k and bool(flags & PATH_PROJECTIONS) == p
+    rng = np.random.default_rng(1)
+    d0 = rng.normal(size=(m.n_el, 9, S.n_modes(order), runs))
+    mo = orc.MeshOracle(m, order, runs, rho=rho, lam=lam, mu=mu)
+    d, t = d0.copy(), np.zeros_like(d0)
+    mo.step(d, t, dt, 2)
+    gd, gt = _dev(d0, prec), _dev(np.zeros_like(d0), prec)
+    s.step(gd, gt, dt, 2)
+    assert rel(gd.cpu().numpy(), d) <= (1e-12 if prec == 64 else 1e-5)
+    s.close()
+
+
+def test_autotune_visco(gpu):
+    order, runs, nr = 4, 8, 3
+    m = M.box(3, bc=(FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC), jitter=0.1, seed=6)
+    rho, lam, mu = S.random_materials(m.n_el, seed=6)
+    qp = np.random.default_rng(6).uniform(20, 60, m.n_el)
+    s = ViscoSolver(m, order, runs, 32, rho, lam, mu, qp, qp / 2, nr, 20.0, 10.0)
+    visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qp / 2, nr, 20.0, 10.0)
+    dt = S.dt_cfl(m, rho, visco["lam_e"], visco["mu_e"], order)
+    cands = s.autotune(dt, n_reps=2)
+    assert {(KERNEL_MFMA, True), (KERNEL_MFMA, False), (KERNEL_GENERIC, False)} <= {(k, p) for k, p, _ in cands}
+    k, p, _ = min(cands, key=lambda c: c[2])
+    assert s.kernel_info()[0] == k
+    rng = np.random.default_rng(2)
+    n = S.n_modes(order)
+    d0 = rng.normal(size=(m.n_el, 9, n, runs))
+    a0 = 0.1 * rng.normal(size=(m.n_el, nr, 6, n, runs))
+    mo = orc.MeshOracle(m, order, runs, visco=visco)
+    d, t, a = d0.copy(), np.zeros_like(d0), a0.copy()
+    mo.step(d, t, dt, 2, dofs_a=a)
+    gd, gt, ga = _dev(d0, 32), _dev(np.zeros_like(d0), 32), _dev(a0, 32)
+    s.step(gd, gt, dt, 2, ga)
+    assert rel(gd.cpu().numpy(), d) <= 1e-5
+    assert rel(ga.cpu().numpy(), a) <= 1e-5
+    s.close()
