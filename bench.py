@@ -294,7 +294,8 @@ def rows(args):
         loc = timed(lambda st: s.local(d, t, dt, dofs_a=a, stream=st), 2)
         nei = timed(lambda st: s.neigh(d, t, dofs_a=a, stream=st), 2)
         peak = PEAK["fp32_mfma"] if prec == 32 else PEAK["fp64"]
-        kern = {1: "generic (LDS tiles, VALU)", 2: "MFMA (paired products, elementwise anelastic terms)"}
+        kern = {1: "generic (LDS tiles, VALU)", 2: "MFMA (paired products, elementwise anelastic terms)",
+                4: "FP64 MFMA (anelastic rows in the quantity tiles)"}
         print(json.dumps({"row": "a12 viscoelastic update", "kernel": kern.get(s.kernel_info()[0], "?"),
                           "order": args.order,
                           "runs": runs, "precision": prec, "n_rms": 3, "elements": m.n_el,

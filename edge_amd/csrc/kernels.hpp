@@ -91,6 +91,12 @@ int launch_local_plain64(const SolverDev<double>& s, int64_t first, int64_t n, d
                          double* t_int, hipStream_t st);
 int launch_neigh_plain64(const SolverDev<double>& s, int64_t first, int64_t n, const double* t_int, double* dofs,
                          hipStream_t st);
+// ... and their viscoelastic form (1 to 3 mechanisms; anelastic rows 9..14 of the tiles)
+bool plain64_visco_supported(int order, int n_runs, int prec_bytes, int n_rms);
+int launch_local_plain64_visco(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs,
+                               double* dofs_a, double* t_int, hipStream_t st);
+int launch_neigh_plain64_visco(const SolverDev<double>& s, int64_t first, int64_t n, const double* t_int,
+                               double* dofs, double* dofs_a, hipStream_t st);
 
 // What kernels outside solver.hip need of a solver handle.
 struct SolverRef {

@@ -42,17 +42,17 @@ __device__ __forceinline__ double ld(const double* __restrict__ p, int idx, bool
   return ok ? v : 0.0;
 }
 
-// acc[t] += A B for A [9][NK] (leading dimension lda, LDS or global) and
+// acc[t] += A B for A [MR][NK] (leading dimension lda, LDS or global) and
 // B [NK][NN] (leading dimension ldb, global), output tiles t < NT; the
 // operands of a k-step are loaded one k-step ahead
-template <int NT, int NK, int NN>
+template <int NT, int NK, int NN, int MR = 9>
 __device__ __forceinline__ void mm(const double* __restrict__ a, int lda, const double* __restrict__ b, int ldb,
                                    v4d (&acc)[NT], int g, int c) {
   constexpr int KS = ksteps4(NK);
   double an, bn[NT];
   auto load = [&](int s) {
     const int k = 4 * s + g;
-    an = ld(a, c * lda + k, c < 9 && k < NK);
+    an = ld(a, c * lda + k, c < MR && k < NK);
 #pragma unroll
     for (int t = 0; t < NT; t++) bn[t] = ld(b, k * ldb + 16 * t + c, k < NK && 16 * t + c < NN);
   };
@@ -69,8 +69,8 @@ __device__ __forceinline__ void mm(const double* __restrict__ a, int lda, const 
   }
 }
 
-// X[q][n] = acc (rows q < 9, columns n < nn) into LDS
-template <int NT>
+// X[q][n] = acc (rows q0 <= q < MR, columns n < nn) into LDS
+template <int NT, int MR = 9, int Q0 = 0>
 __device__ __forceinline__ void store_tiles(double* __restrict__ x, int ldx, int nn, const v4d (&acc)[NT], int g,
                                             int c) {
 #pragma unroll
@@ -78,7 +78,7 @@ __device__ __forceinline__ void store_tiles(double* __restrict__ x, int ldx, int
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int q = g + 4 * v, n = 16 * t + c;
-      if (q < 9 && n < nn) x[q * ldx + n] = acc[t][v];
+      if (q >= Q0 && q < MR && n < nn) x[q * ldx + n] = acc[t][v];
     }
 }
 
@@ -238,6 +238,318 @@ __global__ __launch_bounds__(64) void k_neigh_p64(SolverDev<double> s, int64_t f
     }
 }
 
+// ---------------------------------------------------------------- viscoelastic
+// With N_RMS relaxation mechanisms the anelastic star S^A_d (6 x 3, on the
+// velocities) and the anelastic flux solvers fs_A (6 x 9) act on the same
+// products as the elastic ones, so they take rows 9..14 of the A operand and
+// one 16-row tile carries the elastic outputs and the anelastic combinations
+// together (src/impl/seismic/kernels/TimePredVanilla.hpp:200-306,
+// VolIntVanilla.hpp:189-241, SurfIntVanilla.hpp:212-357, SurfInt.hpp:139-156):
+//   CK:      rows 0..8  (S_d der) (-K_d^T M^-1),   rows 9..14 (S^A_d der_vel) (-K_d^T M^-1)
+//            (all N modes per derivative with attenuation)
+//   volume:  rows 0..8 += (S_d tInt) (K_d M^-1),    rows 9..14 A = (S^A_d tInt_vel) (K_d M^-1)
+//   surface: rows 0..8 += (fs (tInt F)) F^T,        rows 9..14 += (fs_A (tInt F)) F^T
+// The relaxation terms are elementwise per mode, one lane per mode with the
+// anelastic state in registers: E_rm derA_rm into the stress rows,
+// derA_rm <- w_rm (A + derA_rm), tIntA_rm += c_de derA_rm; then
+// Q[stress] += sum_rm E_rm tIntA_rm and QA_rm += w_rm (A + upA - tIntA_rm);
+// the neighbour step adds w_rm upA_nb.
+
+// Y rows 0..8 = S X (elastic star pattern or all 81 entries), rows 9..14 =
+// S^A X_vel, for columns k < NK (one lane per k)
+template <bool SP, int NK>
+__device__ __forceinline__ void mix15(const double* __restrict__ sd, const double* __restrict__ sa,
+                                      const double* __restrict__ x, int ldx, double* __restrict__ y, int ldy,
+                                      int lane) {
+  for (int k = lane; k < NK; k += 64) {
+    double xv[9];
+#pragma unroll
+    for (int p = 0; p < 9; p++) xv[p] = x[p * ldx + k];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      double z = 0.0;
+      if constexpr (SP) {
+#pragma unroll
+        for (int j = 0; j < kStarNnzRow[q]; j++) z += sd[q * 9 + kStarCol[q][j]] * xv[kStarCol[q][j]];
+      } else {
+#pragma unroll
+        for (int p = 0; p < 9; p++) z += sd[q * 9 + p] * xv[p];
+      }
+      y[q * ldy + k] = z;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; j++) y[(9 + j) * ldy + k] = sa[j * 3] * xv[6] + sa[j * 3 + 1] * xv[7] + sa[j * 3 + 2] * xv[8];
+  }
+}
+
+// Y rows 0..8 = F X, rows 9..14 = F_A X for a flux solver F (9 x 9) and its
+// anelastic part F_A (6 x 9), columns k < NK
+// (coefficients row by row through scalar loads: all 135 held at once
+// would not fit the registers)
+typedef const __attribute__((address_space(4))) double cdouble;
+template <int NK>
+__device__ __forceinline__ void mixf15(const double* __restrict__ f, const double* __restrict__ fa,
+                                       const double* __restrict__ x, int ldx, double* __restrict__ y, int ldy,
+                                       int lane) {
+  for (int k = lane; k < NK; k += 64) {
+    double xv[9];
+#pragma unroll
+    for (int p = 0; p < 9; p++) xv[p] = x[p * ldx + k];
+#pragma unroll 1
+    for (int q = 0; q < 15; q++) {
+      const cdouble* row = (const cdouble*)(q < 9 ? f + q * 9 : fa + (q - 9) * 9);
+      double z = 0.0;
+#pragma unroll
+      for (int p = 0; p < 9; p++) z += row[p] * xv[p];
+      y[q * ldy + k] = z;
+    }
+  }
+}
+
+// y[0..5] += E x for the anelastic source matrix E (6 x 6)
+__device__ __forceinline__ void add_src(const double* __restrict__ e, const double (&x)[6], double (&y)[9]) {
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    double z = 0.0;
+#pragma unroll
+    for (int p = 0; p < 6; p++) z += e[q * 6 + p] * x[p];
+    y[q] += z;
+  }
+}
+
+template <int O, int NRM, bool SP>
+__global__ __launch_bounds__(64) void k_local_pv64(SolverDev<double> s, int64_t first, double dt,
+                                                   double* __restrict__ dofs, double* __restrict__ dofs_a,
+                                                   double* __restrict__ t_int) {
+  using D = PlainDims<O>;
+  constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ, SZA = 6 * N;
+  constexpr int NV = n_modes(O - 1), TV = tiles16(NV);
+  static_assert(N <= 64, "one lane per mode");
+  // l_x: der (rows 0..8) and the anelastic combination (rows 9..14); l_a: A operands
+  __shared__ double l_ti[SZ], l_x[15 * N], l_a[15 * N];
+  const int64_t el = first + blockIdx.x;
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const double* star = s.star + el * 243;
+  const double* star_a = s.star_a + el * 54;
+  const double* src = s.src_a + el * NRM * 36;
+  double* qa = dofs_a + el * NRM * SZA;
+  for (int i = lane; i < SZ; i += 64) {
+    const double v = dofs[el * SZ + i];
+    l_x[i] = v;
+    l_ti[i] = dt * v;
+  }
+  // anelastic state of mode n = lane
+  double da[NRM][6], ta[NRM][6];
+#pragma unroll
+  for (int rm = 0; rm < NRM; rm++)
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      da[rm][j] = ld(qa, (rm * 6 + j) * N + lane, lane < N);
+      ta[rm][j] = dt * da[rm][j];
+    }
+  __syncthreads();
+
+  // ---- CK
+  double sca = dt;
+#pragma unroll 1
+  for (int de = 1; de < O; de++) {
+    v4d acc[TV];
+#pragma unroll
+    for (int t = 0; t < TV; t++) acc[t] = zero4d();
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      mix15<SP, N>(star + d * 81, star_a + d * 18, l_x, N, l_a, N, lane);
+      __syncthreads();
+      mm<TV, N, NV, 15>(l_a, N, s.dg.stiffTn + d * N * N, N, acc, g, c);
+      __syncthreads();
+    }
+    store_tiles<TV, 15>(l_x, N, NV, acc, g, c);
+    __syncthreads();
+    sca *= dt / double(de + 1);
+    if (lane < N) {
+      const int n = lane;
+      double y[9], a[6];
+#pragma unroll
+      for (int q = 0; q < 9; q++) y[q] = n < NV ? l_x[q * N + n] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; j++) a[j] = n < NV ? l_x[(9 + j) * N + n] : 0.0;
+#pragma unroll
+      for (int rm = 0; rm < NRM; rm++) {
+        add_src(src + rm * 36, da[rm], y);
+        const double w = s.rfs[rm];
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+          da[rm][j] = w * (a[j] + da[rm][j]);
+          ta[rm][j] += sca * da[rm][j];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        l_x[q * N + n] = y[q];
+        l_ti[q * N + n] += sca * y[q];
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[i];
+
+  // ---- volume: rows 0..8 from Q, rows 9..14 the anelastic combination A
+  v4d acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = g + 4 * v, n = 16 * t + c;
+      acc[t][v] = ld(dofs + el * SZ, q * N + n, q < 9 && n < N);
+    }
+  for (int d = 0; d < 3; d++) {
+    mix15<SP, NV>(star + d * 81, star_a + d * 18, l_ti, N, l_a, N, lane);
+    __syncthreads();
+    mm<NT, NV, N, 15>(l_a, N, s.dg.stiffV + d * N * N, N, acc, g, c);
+    __syncthreads();
+  }
+  // sum_rm E_rm tIntA_rm into the stress rows (through LDS)
+  if (lane < N) {
+    double y[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int rm = 0; rm < NRM; rm++) add_src(src + rm * 36, ta[rm], y);
+#pragma unroll
+    for (int j = 0; j < 6; j++) l_x[j * N + lane] = y[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = g + 4 * v, n = 16 * t + c;
+      if (q < 6 && n < N) acc[t][v] += l_x[q * N + n];
+    }
+  __syncthreads();
+  // ---- local surface
+  double* l_s0 = l_x;  // [9][NF]
+  const double* fs0 = s.fs + el * 324;
+  const double* fsa0 = s.fs_a + el * 4 * 54;
+  for (int fa = 0; fa < 4; fa++) {
+    v4d s0[1] = {zero4d()};
+    mm<1, N, NF>(l_ti, N, s.dg.fluxLN + fa * N * NF, NF, s0, g, c);
+    store_tiles<1>(l_s0, NF, NF, s0, g, c);
+    __syncthreads();
+    mixf15<NF>(fs0 + fa * 81, fsa0 + fa * 54, l_s0, NF, l_a, NF, lane);
+    __syncthreads();
+    mm<NT, NF, N, 15>(l_a, NF, s.dg.fluxT + fa * NF * N, N, acc, g, c);
+    __syncthreads();
+  }
+  // ---- DOFs out; rows 9..14 (A + upA) through LDS to the anelastic update
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = g + 4 * v, n = 16 * t + c;
+      if (q < 9 && n < N) dofs[el * SZ + q * N + n] = acc[t][v];
+    }
+  store_tiles<NT, 15, 9>(l_x, N, N, acc, g, c);
+  __syncthreads();
+  if (lane < N) {
+#pragma unroll
+    for (int rm = 0; rm < NRM; rm++) {
+      const double w = s.rfs[rm];
+#pragma unroll
+      for (int j = 0; j < 6; j++) qa[(rm * 6 + j) * N + lane] += w * (l_x[(9 + j) * N + lane] - ta[rm][j]);
+    }
+  }
+}
+
+template <int O, int NRM>
+__global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t first,
+                                                   const double* __restrict__ t_int, double* __restrict__ dofs,
+                                                   double* __restrict__ dofs_a) {
+  using D = PlainDims<O>;
+  constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ, SZA = 6 * N;
+  __shared__ double l_s0[9 * NF], l_a[15 * N];
+  const int64_t el = first + blockIdx.x;
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  v4d acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = g + 4 * v, n = 16 * t + c;
+      acc[t][v] = ld(dofs + el * SZ, q * N + n, q < 9 && n < N);
+    }
+  for (int fa = 0; fa < 4; fa++) {
+    const int32_t ne = s.nb.ne[el * 4 + fa];
+    if (ne < 0) continue;  // outflow: no neighbouring contribution (uniform per wave)
+    const int fm = s.nb.fmat[el * 4 + fa];
+    v4d s0[1] = {zero4d()};
+    mm<1, N, NF>(t_int + (int64_t)ne * SZ, N, s.dg.fluxLN + fm * N * NF, NF, s0, g, c);
+    store_tiles<1>(l_s0, NF, NF, s0, g, c);
+    __syncthreads();
+    mixf15<NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, s.fs_a + ((s.n_el + el) * 4 + fa) * 54, l_s0, NF, l_a, NF,
+               lane);
+    __syncthreads();
+    mm<NT, NF, N, 15>(l_a, NF, s.dg.fluxT + fa * NF * N, N, acc, g, c);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = g + 4 * v, n = 16 * t + c;
+      if (q < 9 && n < N) dofs[el * SZ + q * N + n] = acc[t][v];
+    }
+  store_tiles<NT, 15, 9>(l_a, N, N, acc, g, c);
+  __syncthreads();
+  if (lane < N) {
+    double* qa = dofs_a + el * NRM * SZA;
+#pragma unroll
+    for (int rm = 0; rm < NRM; rm++) {
+      const double w = s.rfs[rm];
+#pragma unroll
+      for (int j = 0; j < 6; j++) qa[(rm * 6 + j) * N + lane] += w * l_a[(9 + j) * N + lane];
+    }
+  }
+}
+
+template <int O, int NRM>
+int visco_o(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs, double* dofs_a,
+            double* t_int, const double* tin, bool local, hipStream_t st) {
+  if (local) {
+    if (s.sparse_star)
+      hipLaunchKernelGGL((k_local_pv64<O, NRM, true>), dim3((unsigned)n), dim3(64), 0, st, s, first, dt, dofs, dofs_a,
+                         t_int);
+    else
+      hipLaunchKernelGGL((k_local_pv64<O, NRM, false>), dim3((unsigned)n), dim3(64), 0, st, s, first, dt, dofs,
+                         dofs_a, t_int);
+  } else {
+    hipLaunchKernelGGL((k_neigh_pv64<O, NRM>), dim3((unsigned)n), dim3(64), 0, st, s, first, tin, dofs, dofs_a);
+  }
+  EDGE_HIP_TRY(hipGetLastError());
+  return EDGE_OK;
+}
+
+template <int O>
+int visco_rm(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs, double* dofs_a,
+             double* t_int, const double* tin, bool local, hipStream_t st) {
+  switch (s.n_rms) {
+    case 1: return visco_o<O, 1>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+    case 2: return visco_o<O, 2>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+    case 3: return visco_o<O, 3>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+  }
+  return fail(EDGE_ERR_UNSUPPORTED, "FP64 MFMA viscoelastic kernels: 1 to 3 mechanisms");
+}
+
+int visco_dispatch(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs, double* dofs_a,
+                   double* t_int, const double* tin, bool local, hipStream_t st) {
+  if (n <= 0) return EDGE_OK;
+  switch (s.order) {
+    case 2: return visco_rm<2>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+    case 3: return visco_rm<3>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+    case 4: return visco_rm<4>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+    case 5: return visco_rm<5>(s, first, n, dt, dofs, dofs_a, t_int, tin, local, st);
+  }
+  return fail(EDGE_ERR_UNSUPPORTED, "order");
+}
+
 template <int O>
 int local_o(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs, double* t_int,
             hipStream_t st) {
@@ -261,6 +573,20 @@ int neigh_o(const SolverDev<double>& s, int64_t first, int64_t n, const double* 
 
 bool plain64_supported(int order, int n_runs, int prec_bytes) {
   return order >= 2 && order <= 5 && n_runs == 1 && prec_bytes == 8;
+}
+
+bool plain64_visco_supported(int order, int n_runs, int prec_bytes, int n_rms) {
+  return plain64_supported(order, n_runs, prec_bytes) && n_rms >= 1 && n_rms <= 3;
+}
+
+int launch_local_plain64_visco(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs,
+                               double* dofs_a, double* t_int, hipStream_t st) {
+  return visco_dispatch(s, first, n, dt, dofs, dofs_a, t_int, nullptr, true, st);
+}
+
+int launch_neigh_plain64_visco(const SolverDev<double>& s, int64_t first, int64_t n, const double* t_int,
+                               double* dofs, double* dofs_a, hipStream_t st) {
+  return visco_dispatch(s, first, n, 0.0, dofs, dofs_a, nullptr, t_int, false, st);
 }
 
 int launch_local_plain64(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs,

@@ -516,7 +516,10 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
   if (!mesh || !mesh->ve_crds || !mesh->el_ve || !mesh->normals || !mesh->tangents || !mesh->areas || !rho ||
       !lam || !mu || !qp || !qs)
     return edge::fail(EDGE_ERR_ARG, "mesh geometry, material or quality factors missing");
-  if (cfg->kernel > 3) return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic solvers: kernel 0, 1, 2 or 3");
+  if (cfg->kernel > 4) return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic solvers: kernel 0 to 4");
+  const bool p64_ok = edge::plain64_visco_supported(cfg->order, cfg->n_runs, cfg->precision / 8, vc->n_rms);
+  if (cfg->kernel == 4 && !p64_ok)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "FP64 MFMA viscoelastic kernels: FP64, 1 run, 1-3 mechanisms");
   if (!edge::visco_supported(cfg->order, cfg->n_runs, cfg->precision / 8, vc->n_rms))
     return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic kernels: 1 or 8 runs whose tiles fit LDS");
   const bool vm_ok = edge::visco_mfma_supported(cfg->order, cfg->n_runs, cfg->precision, vc->n_rms);
@@ -582,6 +585,7 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
                                           &s->fs_rank3, (float*)s->star_a, (float*)s->fs_a, (float*)s->fsfa,
                                           nullptr);
+    if (rc == EDGE_OK && p64_ok && (cfg->kernel == 0 || cfg->kernel == 4)) s->kernel = 4;
     if (rc == EDGE_OK && want_vm) {
       if (s->fs_rank3) {
         s->kernel = 2;
@@ -614,6 +618,9 @@ extern "C" int edge_el_local_a(edge_el_solver* s, int64_t first, int64_t n_els, 
   if (s->n_rms == 0) return edge_el_local(s, first, n_els, dt, dofs, t_int, stream);
   if (!dofs || !dofs_a || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
   hipStream_t st = (hipStream_t)stream;
+  if (s->kernel == 4)
+    return edge::launch_local_plain64_visco(dev_view<double>(s), first, n_els, dt, (double*)dofs, (double*)dofs_a,
+                                            (double*)t_int, st);
   if (s->kernel == 2)
     return edge::launch_local_visco_mfma(dev_view<float>(s), first, n_els, (float)dt, (float*)dofs, (float*)dofs_a,
                                          (float*)t_int, st);
@@ -631,6 +638,9 @@ extern "C" int edge_el_neigh_a(edge_el_solver* s, int64_t first, int64_t n_els, 
   if (s->n_rms == 0) return edge_el_neigh(s, first, n_els, t_int, dofs, stream);
   if (!dofs || !dofs_a || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
   hipStream_t st = (hipStream_t)stream;
+  if (s->kernel == 4)
+    return edge::launch_neigh_plain64_visco(dev_view<double>(s), first, n_els, (const double*)t_int, (double*)dofs,
+                                            (double*)dofs_a, st);
   if (s->kernel == 2)
     return edge::launch_neigh_visco_mfma(dev_view<float>(s), first, n_els, (const float*)t_int, (float*)dofs,
                                          (float*)dofs_a, st);

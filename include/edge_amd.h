@@ -223,9 +223,9 @@ typedef struct {
  * solvers, EDGE_ERR_UNSUPPORTED otherwise), whose local step also writes
  * the face projections the neighbour step reads (as with edge_el_neigh,
  * t_int must then be the array the neighbours' local steps wrote); 3 the
- * MFMA kernels with the neighbour step reading t_int; 0 the MFMA kernels
- * where they apply, else the generic ones.  edge_el_kernel_info reports the
- * choice. */
+ * MFMA kernels with the neighbour step reading t_int; 4 the FP64 MFMA
+ * kernels (FP64, 1 run, 1-3 mechanisms); 0 the MFMA kernels where they
+ * apply, else the generic ones.  edge_el_kernel_info reports the choice. */
 int edge_el_create_visco(const edge_el_config *cfg, const edge_visco_config *vc, const edge_mesh_in *mesh,
                          const double *rho, const double *lam, const double *mu, const double *qp, const double *qs,
                          edge_el_solver **out);

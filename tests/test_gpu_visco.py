@@ -13,8 +13,8 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import (KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES, PATH_PROJECTIONS, PATH_VISCO,
-                             ViscoSolver)
+from edge_amd.solver import (KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES,
+                             PATH_PROJECTIONS, PATH_VISCO, ViscoSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -56,12 +56,16 @@ def test_visco_setup_matches_oracle(gpu):
 
 @pytest.mark.parametrize("prec,order,runs,kernel,nr", [
     (64, 2, 1, KERNEL_AUTO, NR), (64, 3, 1, KERNEL_AUTO, NR), (64, 4, 1, KERNEL_AUTO, NR),
-    (64, 5, 1, KERNEL_AUTO, NR), (32, 4, 8, KERNEL_GENERIC, NR), (32, 5, 8, KERNEL_GENERIC, NR),
+    (64, 5, 1, KERNEL_AUTO, NR), (64, 4, 1, KERNEL_GENERIC, NR), (64, 5, 1, KERNEL_GENERIC, NR),
+    (64, 5, 1, KERNEL_MFMA64, 1), (64, 3, 1, KERNEL_MFMA64, 2),
+    (32, 4, 8, KERNEL_GENERIC, NR), (32, 5, 8, KERNEL_GENERIC, NR),
     (32, 3, 8, KERNEL_MFMA, NR), (32, 4, 8, KERNEL_MFMA, NR), (32, 5, 8, KERNEL_AUTO, NR),
     (32, 4, 8, KERNEL_MFMA, 1), (32, 5, 8, KERNEL_MFMA, 2), (32, 5, 8, KERNEL_MFMA_TILES, NR)])
 def test_visco_steps_match_oracle(gpu, prec, order, runs, kernel, nr):
     m, rho, lam, mu, qp, qs = _problem(order, seed=40 + order)
     s = ViscoSolver(m, order, runs, prec, rho, lam, mu, qp, qs, nr, F_CEN, F_RAT, kernel=kernel)
+    if prec == 64 and kernel != KERNEL_GENERIC:  # auto picks the FP64 MFMA kernels for one run
+        assert s.kernel_info()[0] == KERNEL_MFMA64
     if prec == 32 and kernel != KERNEL_GENERIC:  # auto picks the MFMA kernels for FP32 with 8 runs
         k, flags = s.kernel_info()
         assert k == KERNEL_MFMA
