@@ -306,27 +306,35 @@ __device__ __forceinline__ void mixf15(const double* __restrict__ f, const doubl
   }
 }
 
-// y[0..5] += E x for the anelastic source matrix E (6 x 6)
+// The wave-uniform value x, opaque to the optimiser: per-element
+// coefficients addressed with it are loaded where they are used instead of
+// being hoisted out of the time-derivative loop (and held in registers).
+__device__ __forceinline__ int opaque_u(int x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
+// y[0..5] += E x for an anelastic source matrix E (6 x 6) of the source
+// pattern (src/impl/seismic/setups/ViscoElasticity.h:379-458): the
+// normal-stress block rows 0..2 x columns 0..2 and the shear diagonal
 __device__ __forceinline__ void add_src(const double* __restrict__ e, const double (&x)[6], double (&y)[9]) {
 #pragma unroll
-  for (int q = 0; q < 6; q++) {
-    double z = 0.0;
+  for (int q = 0; q < 3; q++) y[q] += e[q * 6] * x[0] + e[q * 6 + 1] * x[1] + e[q * 6 + 2] * x[2];
 #pragma unroll
-    for (int p = 0; p < 6; p++) z += e[q * 6 + p] * x[p];
-    y[q] += z;
-  }
+  for (int q = 3; q < 6; q++) y[q] += e[q * 6 + q] * x[q];
 }
 
 template <int O, int NRM, bool SP>
-__global__ __launch_bounds__(64) void k_local_pv64(SolverDev<double> s, int64_t first, double dt,
-                                                   double* __restrict__ dofs, double* __restrict__ dofs_a,
-                                                   double* __restrict__ t_int) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restrict__ dofs,
+                  double* __restrict__ dofs_a, double* __restrict__ t_int) {
   using D = PlainDims<O>;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ, SZA = 6 * N;
   constexpr int NV = n_modes(O - 1), TV = tiles16(NV);
   static_assert(N <= 64, "one lane per mode");
-  // l_x: der (rows 0..8) and the anelastic combination (rows 9..14); l_a: A operands
-  __shared__ double l_ti[SZ], l_x[15 * N], l_a[15 * N];
+  // l_x: der (rows 0..8) and the anelastic combination (rows 9..14); l_a: A
+  // operands; l_ta: tIntA [rm][6][mode]
+  __shared__ double l_ti[SZ], l_x[15 * N], l_a[15 * N], l_ta[NRM * 6 * N];
   const int64_t el = first + blockIdx.x;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* star = s.star + el * 243;
@@ -338,14 +346,14 @@ __global__ __launch_bounds__(64) void k_local_pv64(SolverDev<double> s, int64_t 
     l_x[i] = v;
     l_ti[i] = dt * v;
   }
-  // anelastic state of mode n = lane
-  double da[NRM][6], ta[NRM][6];
+  // anelastic state of mode n = lane: derA in registers, tIntA in LDS
+  double da[NRM][6];
 #pragma unroll
   for (int rm = 0; rm < NRM; rm++)
 #pragma unroll
     for (int j = 0; j < 6; j++) {
       da[rm][j] = ld(qa, (rm * 6 + j) * N + lane, lane < N);
-      ta[rm][j] = dt * da[rm][j];
+      if (lane < N) l_ta[(rm * 6 + j) * N + lane] = dt * da[rm][j];
     }
   __syncthreads();
 
@@ -358,7 +366,7 @@ __global__ __launch_bounds__(64) void k_local_pv64(SolverDev<double> s, int64_t 
     for (int t = 0; t < TV; t++) acc[t] = zero4d();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      mix15<SP, N>(star + d * 81, star_a + d * 18, l_x, N, l_a, N, lane);
+      mix15<SP, N>(star + opaque_u(d * 81), star_a + opaque_u(d * 18), l_x, N, l_a, N, lane);
       __syncthreads();
       mm<TV, N, NV, 15>(l_a, N, s.dg.stiffTn + d * N * N, N, acc, g, c);
       __syncthreads();
@@ -375,12 +383,12 @@ __global__ __launch_bounds__(64) void k_local_pv64(SolverDev<double> s, int64_t 
       for (int j = 0; j < 6; j++) a[j] = n < NV ? l_x[(9 + j) * N + n] : 0.0;
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++) {
-        add_src(src + rm * 36, da[rm], y);
+        add_src(src + opaque_u(rm * 36), da[rm], y);
         const double w = s.rfs[rm];
 #pragma unroll
         for (int j = 0; j < 6; j++) {
           da[rm][j] = w * (a[j] + da[rm][j]);
-          ta[rm][j] += sca * da[rm][j];
+          l_ta[(rm * 6 + j) * N + n] += sca * da[rm][j];
         }
       }
 #pragma unroll
@@ -412,7 +420,12 @@ __global__ __launch_bounds__(64) void k_local_pv64(SolverDev<double> s, int64_t 
   if (lane < N) {
     double y[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int rm = 0; rm < NRM; rm++) add_src(src + rm * 36, ta[rm], y);
+    for (int rm = 0; rm < NRM; rm++) {
+      double ta[6];
+#pragma unroll
+      for (int j = 0; j < 6; j++) ta[j] = l_ta[(rm * 6 + j) * N + lane];
+      add_src(src + rm * 36, ta, y);
+    }
 #pragma unroll
     for (int j = 0; j < 6; j++) l_x[j * N + lane] = y[j];
   }
@@ -454,7 +467,8 @@ __global__ __launch_bounds__(64) void k_local_pv64(SolverDev<double> s, int64_t 
     for (int rm = 0; rm < NRM; rm++) {
       const double w = s.rfs[rm];
 #pragma unroll
-      for (int j = 0; j < 6; j++) qa[(rm * 6 + j) * N + lane] += w * (l_x[(9 + j) * N + lane] - ta[rm][j]);
+      for (int j = 0; j < 6; j++)
+        qa[(rm * 6 + j) * N + lane] += w * (l_x[(9 + j) * N + lane] - l_ta[(rm * 6 + j) * N + lane]);
     }
   }
 }

@@ -585,7 +585,7 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
                                           &s->fs_rank3, (float*)s->star_a, (float*)s->fs_a, (float*)s->fsfa,
                                           nullptr);
-    if (rc == EDGE_OK && p64_ok && (cfg->kernel == 0 || cfg->kernel == 4)) s->kernel = 4;
+    if (rc == EDGE_OK && p64_ok && src_pattern && (cfg->kernel == 0 || cfg->kernel == 4)) s->kernel = 4;
     if (rc == EDGE_OK && want_vm) {
       if (s->fs_rank3) {
         s->kernel = 2;
