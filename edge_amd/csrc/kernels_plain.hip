@@ -111,13 +111,13 @@ struct PlainDims {
   static constexpr int N = n_modes(O), NF = n_face_modes(O), NT = tiles16(N), SZ = 9 * N;
 };
 
-// CK stage DE (and the following ones): prev holds der_{DE-1} (leading
-// dimension N); the stage writes der_DE into the l_der buffer of its parity
-// and adds sca dt / (DE + 1) der_DE to tInt
+// CK stage DE (and the following ones): l_der holds der_{DE-1} (leading
+// dimension N); the stage overwrites it with der_DE once the last direction
+// has read it, and adds sca dt / (DE + 1) der_DE to tInt
 template <int O, int DE, bool SP>
 __device__ __forceinline__ void ck_stage(const SolverDev<double>& s, const double* __restrict__ star, double dt,
-                                         double sca, const double* prev, double (*l_der)[9 * n_modes(O)],
-                                         double* l_a, double* l_ti, int lane, int g, int c) {
+                                         double sca, double* l_der, double* l_a, double* l_ti, int lane, int g,
+                                         int c) {
   if constexpr (DE < O) {
     constexpr int N = n_modes(O), NK = n_modes(O - DE + 1), NN = n_modes(O - DE), NT = tiles16(NN);
     v4d acc[NT];
@@ -125,12 +125,12 @@ __device__ __forceinline__ void ck_stage(const SolverDev<double>& s, const doubl
     for (int t = 0; t < NT; t++) acc[t] = zero4d();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      mix<SP, NK>(star + d * 81, prev, N, l_a, N, lane);
+      mix<SP, NK>(star + d * 81, l_der, N, l_a, N, lane);
       __syncthreads();
       mm<NT, NK, NN>(l_a, N, s.dg.stiffTn + d * N * N, N, acc, g, c);
       __syncthreads();
     }
-    double* cur = l_der[DE & 1];
+    double* cur = l_der;
     store_tiles<NT>(cur, N, NN, acc, g, c);
     __syncthreads();
     sca *= dt / double(DE + 1);
@@ -139,37 +139,38 @@ __device__ __forceinline__ void ck_stage(const SolverDev<double>& s, const doubl
       l_ti[q * N + n] += sca * cur[q * N + n];
     }
     __syncthreads();
-    ck_stage<O, DE + 1, SP>(s, star, dt, sca, cur, l_der, l_a, l_ti, lane, g, c);
+    ck_stage<O, DE + 1, SP>(s, star, dt, sca, l_der, l_a, l_ti, lane, g, c);
   }
 }
 
 template <int O, bool SP>
-__global__ __launch_bounds__(64) void k_local_p64(SolverDev<double> s, int64_t first, double dt,
-                                                  double* __restrict__ dofs, double* __restrict__ t_int) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+void k_local_p64(SolverDev<double> s, int64_t first, double dt, double* __restrict__ dofs,
+                 double* __restrict__ t_int) {
   using D = PlainDims<O>;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ;
-  __shared__ double l_q[SZ], l_ti[SZ], l_der[2][SZ], l_a[SZ];
+  __shared__ double l_ti[SZ], l_der[SZ], l_a[SZ];
   const int64_t el = first + blockIdx.x;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* star = s.star + el * 243;
   for (int i = lane; i < SZ; i += 64) {
     const double v = dofs[el * SZ + i];
-    l_q[i] = v;
+    l_der[i] = v;
     l_ti[i] = dt * v;
   }
   __syncthreads();
   // ---- CK: der_de = sum_d (S_d der_{de-1}) (-K_d^T M^-1), tInt += dt^{de+1}/(de+1)! der_de
-  ck_stage<O, 1, SP>(s, star, dt, dt, l_q, l_der, l_a, l_ti, lane, g, c);
+  ck_stage<O, 1, SP>(s, star, dt, dt, l_der, l_a, l_ti, lane, g, c);
   for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[i];
 
-  // ---- volume: dofs += sum_d (S_d tInt) (K_d M^-1)
+  // ---- volume: dofs += sum_d (S_d tInt) (K_d M^-1), from Q read again
   v4d acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++)
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int q = g + 4 * v, n = 16 * t + c;
-      acc[t][v] = (q < 9 && n < N) ? l_q[q * N + n] : 0.0;
+      acc[t][v] = ld(dofs + el * SZ, q * N + n, q < 9 && n < N);
     }
   constexpr int NV = n_modes(O - 1);
   for (int d = 0; d < 3; d++) {
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(64) void k_local_p64(SolverDev<double> s, int64_t f
     __syncthreads();
   }
   // ---- local surface: dofs += ((fs0_fa (tInt F^L_fa)) F^T_fa)
-  double* l_s0 = l_der[0];  // [9][NF]
+  double* l_s0 = l_der;  // [9][NF]
   const double* fs0 = s.fs + el * 324;
   for (int fa = 0; fa < 4; fa++) {
     v4d s0[1] = {zero4d()};
