@@ -32,6 +32,7 @@ struct edge_el_solver {
   double* jdet = nullptr;  // [nEl] Jacobian determinants (geometry-built solvers only)
   bool sparse_star = false;  // all star matrices have the elastic 24-entry pattern
   double* mass = nullptr;  // [N] diagonal reference mass matrix
+  double* mat = nullptr;   // [3][nEl] rho, lambda, mu (FP64; solvers built from materials)
   // MFMA-tiled path: rank-3 flux-solver factors and face projections
   void* fsf = nullptr;       // [2][nEl][4][54]
   bool fs_rank3 = false;
@@ -362,6 +363,18 @@ void drop_graph(edge_el_solver* s) {
 
 }  // namespace
 
+namespace {
+// rho, lambda, mu per element to the device (for the energy reduction)
+int upload_materials(edge_el_solver* s, const double* rho, const double* lam, const double* mu) {
+  const size_t ne = s->n_el;
+  EDGE_HIP_TRY(hipMalloc(&s->mat, sizeof(double) * 3 * ne));
+  EDGE_HIP_TRY(hipMemcpy(s->mat, rho, sizeof(double) * ne, hipMemcpyHostToDevice));
+  EDGE_HIP_TRY(hipMemcpy(s->mat + ne, lam, sizeof(double) * ne, hipMemcpyHostToDevice));
+  EDGE_HIP_TRY(hipMemcpy(s->mat + 2 * ne, mu, sizeof(double) * ne, hipMemcpyHostToDevice));
+  return EDGE_OK;
+}
+}  // namespace
+
 extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mesh, const double* rho,
                               const double* lam, const double* mu, edge_el_solver** out) {
   if (out) *out = nullptr;
@@ -389,6 +402,7 @@ extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mes
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
                                           &s->fs_rank3, nullptr, nullptr, nullptr, nullptr);
     if (rc == EDGE_OK) rc = finish_mfma(s);
+    if (rc == EDGE_OK) rc = upload_materials(s, rho, lam, mu);
   }
   if (rc != EDGE_OK) {
     if (s) edge_el_destroy(s);
@@ -447,6 +461,7 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   hipFree(s->fmat);
   hipFree(s->jdet);
   hipFree(s->mass);
+  hipFree(s->mat);
   hipFree(s->fsf);
   hipFree(s->proj);
   hipFree(s->snd_v);
@@ -585,6 +600,7 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
                                           &s->fs_rank3, (float*)s->star_a, (float*)s->fs_a, (float*)s->fsfa,
                                           nullptr);
+    if (rc == EDGE_OK) rc = upload_materials(s, rho, s->lam_e.data(), s->mu_e.data());
     if (rc == EDGE_OK && p64_ok && src_pattern && (cfg->kernel == 0 || cfg->kernel == 4)) s->kernel = 4;
     if (rc == EDGE_OK && want_vm) {
       if (s->fs_rank3) {
@@ -1039,4 +1055,84 @@ extern "C" int edge_el_l2_norms(edge_el_solver* s, const void* a, int prec_a, co
   }
   hipFree(d_part);
   return rc;
+}
+
+namespace {
+// Per-block partial sums per run of the kinetic energy 1/2 rho |v|^2 and the
+// strain energy 1/2 sigma : C^-1 : sigma (isotropic compliance
+// C^-1 sigma = sigma / (2 mu) - lambda / (2 mu (3 lambda + 2 mu)) tr(sigma) I),
+// integrated exactly through |J| and the diagonal reference mass matrix.
+template <typename T>
+__global__ __launch_bounds__(256) void k_energy(int64_t n_el, int n_md, int n_crs, const double* __restrict__ jdet,
+                                                const double* __restrict__ mass, const double* __restrict__ mat,
+                                                const T* __restrict__ dofs, double* __restrict__ part) {
+  __shared__ double sk[256], ss[256];
+  const int64_t per = (n_el + gridDim.x - 1) / gridDim.x;
+  const int64_t e0 = blockIdx.x * per, e1 = min(n_el, e0 + per);
+  const int t = threadIdx.x, r = t % n_crs, m0 = t / n_crs, mstep = blockDim.x / n_crs;
+  double ek = 0, es = 0;
+  if (m0 < mstep) {
+    for (int64_t el = e0; el < e1; el++) {
+      const double j = fabs(jdet[el]), rho = mat[el], lam = mat[n_el + el], mu = mat[2 * n_el + el];
+      const double c1 = 0.5 / mu, c2 = lam / (2 * mu * (3 * lam + 2 * mu));
+      for (int md = m0; md < n_md; md += mstep) {
+        double x[9];
+        for (int q = 0; q < 9; q++) x[q] = (double)dofs[(((size_t)el * 9 + q) * n_md + md) * n_crs + r];
+        const double w = j * mass[md];
+        const double tr = x[0] + x[1] + x[2];
+        const double ss2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + 2 * (x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+        ek += w * 0.5 * rho * (x[6] * x[6] + x[7] * x[7] + x[8] * x[8]);
+        es += w * 0.5 * (c1 * ss2 - c2 * tr * tr);
+      }
+    }
+  }
+  sk[t] = ek;
+  ss[t] = es;
+  __syncthreads();
+  if (t < n_crs) {
+    double k = 0, s2 = 0;
+    for (int i = 0; i < mstep; i++) {
+      k += sk[i * n_crs + t];
+      s2 += ss[i * n_crs + t];
+    }
+    part[(size_t)blockIdx.x * 2 * n_crs + t] = k;
+    part[(size_t)blockIdx.x * 2 * n_crs + n_crs + t] = s2;
+  }
+}
+}  // namespace
+
+extern "C" int edge_el_energy(edge_el_solver* s, const void* dofs, double* kinetic, double* strain, void* stream) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (!s->jdet || !s->mat) return edge::fail(EDGE_ERR_UNSUPPORTED, "the energy needs a solver built from materials");
+  if (!dofs) return edge::fail(EDGE_ERR_ARG, "device DOFs missing");
+  const int R = s->cfg.n_runs;
+  if (256 % R) return edge::fail(EDGE_ERR_UNSUPPORTED, "energy: runs must divide 256");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nblk = 1024;
+  double* d_part = nullptr;
+  EDGE_HIP_TRY(hipMalloc(&d_part, sizeof(double) * 2 * R * nblk));
+  if (s->cfg.precision == 64)
+    hipLaunchKernelGGL(k_energy<double>, dim3(nblk), dim3(256), 0, st, s->n_el, s->n_modes, R, s->jdet, s->mass,
+                       s->mat, (const double*)dofs, d_part);
+  else
+    hipLaunchKernelGGL(k_energy<float>, dim3(nblk), dim3(256), 0, st, s->n_el, s->n_modes, R, s->jdet, s->mass,
+                       s->mat, (const float*)dofs, d_part);
+  int rc = EDGE_OK;
+  std::vector<double> h(2 * R * nblk);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    rc = edge::fail(EDGE_ERR_HIP, "energy reduction");
+  hipFree(d_part);
+  if (rc) return rc;
+  for (int r = 0; r < R; r++) {
+    double k = 0, e = 0;
+    for (unsigned b = 0; b < nblk; b++) {
+      k += h[(size_t)b * 2 * R + r];
+      e += h[(size_t)b * 2 * R + R + r];
+    }
+    if (kinetic) kinetic[r] = k;
+    if (strain) strain[r] = e;
+  }
+  return EDGE_OK;
 }

@@ -58,6 +58,14 @@ class ElasticSolver:
         _lib.check(_lib.lib().edge_el_kernel_info(self._h, C.byref(k), C.byref(f)))
         return k.value, f.value
 
+    def energy(self, dofs, stream=None):
+        """(kinetic [R], strain [R]) elastic energies of the DOFs, FP64 on the GPU."""
+        self._check_arr(dofs)
+        k, e = np.zeros(self.n_runs), np.zeros(self.n_runs)
+        _lib.check(_lib.lib().edge_el_energy(self._h, dofs.data_ptr(), k.ctypes.data, e.ctypes.data,
+                                             _stream_ptr(stream)))
+        return k, e
+
     def autotune(self, dt, n_reps=3, stream=None):
         """Times the kernel variants this solver can switch to on its mesh
         and keeps the fastest; returns [(kernel, projections, ms per step)]."""

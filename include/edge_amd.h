@@ -394,6 +394,16 @@ int edge_el_autotune(edge_el_solver *s, double dt, int32_t n_reps, void *stream,
 int edge_el_l2_norms(edge_el_solver *s, const void *a, int prec_a, const void *b, int prec_b, double *diff_sq,
                      double *ref_sq, void *stream);
 
+/* Elastic energy per run of a solver's device DOFs (its precision), FP64 on
+ * the GPU: kinetic 1/2 rho |v|^2 and strain 1/2 sigma : C^-1 : sigma with the
+ * element's rho, lambda, mu (the unrelaxed ones for viscoelastic solvers),
+ * integrated exactly through |J| and the diagonal mass matrix; kinetic[R],
+ * strain[R] on the host (either may be NULL).  Solvers built from materials
+ * only (EDGE_ERR_UNSUPPORTED for edge_el_create_raw).  No counterpart in the
+ * reference's kernels (its energy and error output lives in the setups,
+ * InitialDofs.hpp:277-370). */
+int edge_el_energy(edge_el_solver *s, const void *dofs, double *kinetic, double *strain, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
