@@ -33,6 +33,7 @@ struct edge_el_solver {
   bool sparse_star = false;  // all star matrices have the elastic 24-entry pattern
   double* mass = nullptr;  // [N] diagonal reference mass matrix
   double* mat = nullptr;   // [3][nEl] rho, lambda, mu (FP64; solvers built from materials)
+  double* geo = nullptr;   // [nEl][12] v0, v1 - v0, v2 - v0, v3 - v0 (FP64; solvers built from geometry)
   // MFMA-tiled path: rank-3 flux-solver factors and face projections
   void* fsf = nullptr;       // [2][nEl][4][54]
   bool fs_rank3 = false;
@@ -364,13 +365,26 @@ void drop_graph(edge_el_solver* s) {
 }  // namespace
 
 namespace {
-// rho, lambda, mu per element to the device (for the energy reduction)
-int upload_materials(edge_el_solver* s, const double* rho, const double* lam, const double* mu) {
+// rho, lambda, mu per element (energy reduction) and the element maps
+// (error norms) to the device
+int upload_element_data(edge_el_solver* s, const edge_mesh_in* m, const double* rho, const double* lam,
+                        const double* mu) {
   const size_t ne = s->n_el;
   EDGE_HIP_TRY(hipMalloc(&s->mat, sizeof(double) * 3 * ne));
   EDGE_HIP_TRY(hipMemcpy(s->mat, rho, sizeof(double) * ne, hipMemcpyHostToDevice));
   EDGE_HIP_TRY(hipMemcpy(s->mat + ne, lam, sizeof(double) * ne, hipMemcpyHostToDevice));
   EDGE_HIP_TRY(hipMemcpy(s->mat + 2 * ne, mu, sizeof(double) * ne, hipMemcpyHostToDevice));
+  std::vector<double> geo(12 * ne);
+  for (size_t el = 0; el < ne; el++) {
+    const double* v0 = m->ve_crds + 3 * m->el_ve[el * 4];
+    for (int d = 0; d < 3; d++) geo[el * 12 + d] = v0[d];
+    for (int i = 1; i < 4; i++) {
+      const double* vi = m->ve_crds + 3 * m->el_ve[el * 4 + i];
+      for (int d = 0; d < 3; d++) geo[el * 12 + 3 * i + d] = vi[d] - v0[d];
+    }
+  }
+  EDGE_HIP_TRY(hipMalloc(&s->geo, sizeof(double) * 12 * ne));
+  EDGE_HIP_TRY(hipMemcpy(s->geo, geo.data(), sizeof(double) * 12 * ne, hipMemcpyHostToDevice));
   return EDGE_OK;
 }
 }  // namespace
@@ -402,7 +416,7 @@ extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mes
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
                                           &s->fs_rank3, nullptr, nullptr, nullptr, nullptr);
     if (rc == EDGE_OK) rc = finish_mfma(s);
-    if (rc == EDGE_OK) rc = upload_materials(s, rho, lam, mu);
+    if (rc == EDGE_OK) rc = upload_element_data(s, mesh, rho, lam, mu);
   }
   if (rc != EDGE_OK) {
     if (s) edge_el_destroy(s);
@@ -462,6 +476,7 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   hipFree(s->jdet);
   hipFree(s->mass);
   hipFree(s->mat);
+  hipFree(s->geo);
   hipFree(s->fsf);
   hipFree(s->proj);
   hipFree(s->snd_v);
@@ -600,7 +615,7 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
                                           &s->fs_rank3, (float*)s->star_a, (float*)s->fs_a, (float*)s->fsfa,
                                           nullptr);
-    if (rc == EDGE_OK) rc = upload_materials(s, rho, s->lam_e.data(), s->mu_e.data());
+    if (rc == EDGE_OK) rc = upload_element_data(s, mesh, rho, s->lam_e.data(), s->mu_e.data());
     if (rc == EDGE_OK && p64_ok && src_pattern && (cfg->kernel == 0 || cfg->kernel == 4)) s->kernel = 4;
     if (rc == EDGE_OK && want_vm) {
       if (s->fs_rank3) {
@@ -1133,6 +1148,126 @@ extern "C" int edge_el_energy(edge_el_solver* s, const void* dofs, double* kinet
     }
     if (kinetic) kinetic[r] = k;
     if (strain) strain[r] = e;
+  }
+  return EDGE_OK;
+}
+
+namespace {
+// Per-block L1, squared L2 and Linf errors per (quantity, run) at the
+// quadrature points of every element against a sum of plane waves
+// q(x, t) = sum_w r_w sin(k_w . x - omega_w t + phase_w) (per run:
+// waves [R][n_waves][14] = r[9], k[3], omega, phase).  Thread t keeps entry
+// t % (9 R) for elements t / (9 R), t / (9 R) + lanes, ... of the block's
+// range; partials summed in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_err(int64_t n_el, int n_md, int n_crs, int n_pts, const double* __restrict__ geo,
+                                             const double* __restrict__ jdet, const double* __restrict__ pts,
+                                             const double* __restrict__ wts, const double* __restrict__ basis,
+                                             int n_waves, const double* __restrict__ waves, double time,
+                                             const T* __restrict__ dofs, double* __restrict__ part) {
+  __shared__ double s1[256], s2[256], si[256];
+  const int nqr = 9 * n_crs, lanes = blockDim.x / nqr;
+  const int tid = threadIdx.x, qr = tid % nqr, sub = tid / nqr, q = qr / n_crs, r = qr % n_crs;
+  const int64_t per = (n_el + gridDim.x - 1) / gridDim.x;
+  const int64_t e0 = blockIdx.x * per, e1 = min(n_el, e0 + per);
+  double l1 = 0, l2 = 0, li = 0;
+  if (sub < lanes) {
+    for (int64_t el = e0 + sub; el < e1; el += lanes) {
+      const double* gm = geo + el * 12;
+      const double j = fabs(jdet[el]);
+      const T* a = dofs + ((size_t)el * 9 + q) * n_md * n_crs + r;
+      for (int p = 0; p < n_pts; p++) {
+        double num = 0;
+        for (int n = 0; n < n_md; n++) num += basis[p * n_md + n] * (double)a[(size_t)n * n_crs];
+        double x[3];
+        for (int d = 0; d < 3; d++)
+          x[d] = gm[d] + pts[3 * p] * gm[3 + d] + pts[3 * p + 1] * gm[6 + d] + pts[3 * p + 2] * gm[9 + d];
+        double ref = 0;
+        for (int w = 0; w < n_waves; w++) {
+          const double* wv = waves + ((size_t)r * n_waves + w) * 14;
+          ref += wv[q] * sin(wv[9] * x[0] + wv[10] * x[1] + wv[11] * x[2] - wv[12] * time + wv[13]);
+        }
+        const double d = fabs(num - ref), wj = wts[p] * j;
+        l1 += wj * d;
+        l2 += wj * d * d;
+        li = fmax(li, d);
+      }
+    }
+  }
+  s1[tid] = l1;
+  s2[tid] = l2;
+  si[tid] = li;
+  __syncthreads();
+  if (tid < nqr) {
+    double a1 = 0, a2 = 0, ai = 0;
+    for (int k = 0; k < lanes; k++) {
+      a1 += s1[k * nqr + tid];
+      a2 += s2[k * nqr + tid];
+      ai = fmax(ai, si[k * nqr + tid]);
+    }
+    part[((size_t)blockIdx.x * 3 + 0) * nqr + tid] = a1;
+    part[((size_t)blockIdx.x * 3 + 1) * nqr + tid] = a2;
+    part[((size_t)blockIdx.x * 3 + 2) * nqr + tid] = ai;
+  }
+}
+}  // namespace
+
+extern "C" int edge_el_error_norms(edge_el_solver* s, const void* dofs, int32_t n1d, int32_t n_waves,
+                                   const double* waves, double time, double* l1, double* l2sq, double* linf,
+                                   void* stream) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (!s->jdet || !s->geo) return edge::fail(EDGE_ERR_UNSUPPORTED, "error norms need a solver built from geometry");
+  if (!dofs || n_waves < 0 || (n_waves > 0 && !waves) || n1d < 0 || n1d > 32)
+    return edge::fail(EDGE_ERR_ARG, "error-norm arguments");
+  const int R = s->cfg.n_runs, nqr = 9 * R, N = s->n_modes;
+  if (nqr > 256) return edge::fail(EDGE_ERR_UNSUPPORTED, "error norms: at most 28 runs");
+  const int n1 = n1d ? n1d : s->cfg.order + 1, np = n1 * n1 * n1;
+  std::vector<double> pts(3 * np), w(np), basis((size_t)np * N);
+  int rc = edge_dg_quad_tet(n1, pts.data(), w.data());
+  if (rc == EDGE_OK) rc = edge_dg_eval_basis(s->cfg.order, np, pts.data(), basis.data());
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nblk = 1024;
+  double *d_pts = nullptr, *d_w = nullptr, *d_b = nullptr, *d_wv = nullptr, *d_part = nullptr;
+  const size_t nwv = (size_t)R * n_waves * 14;
+  auto ok = [](hipError_t e) { return e == hipSuccess; };
+  std::vector<double> h((size_t)3 * nqr * nblk);
+  if (!ok(hipMalloc(&d_pts, sizeof(double) * 3 * np)) || !ok(hipMalloc(&d_w, sizeof(double) * np)) ||
+      !ok(hipMalloc(&d_b, sizeof(double) * np * N)) || !ok(hipMalloc(&d_wv, sizeof(double) * (nwv ? nwv : 1))) ||
+      !ok(hipMalloc(&d_part, sizeof(double) * h.size())) ||
+      !ok(hipMemcpyAsync(d_pts, pts.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice, st)) ||
+      !ok(hipMemcpyAsync(d_w, w.data(), sizeof(double) * np, hipMemcpyHostToDevice, st)) ||
+      !ok(hipMemcpyAsync(d_b, basis.data(), sizeof(double) * np * N, hipMemcpyHostToDevice, st)) ||
+      (nwv && !ok(hipMemcpyAsync(d_wv, waves, sizeof(double) * nwv, hipMemcpyHostToDevice, st)))) {
+    rc = edge::fail(EDGE_ERR_HIP, "error norms: buffers");
+  } else {
+    if (s->cfg.precision == 64)
+      hipLaunchKernelGGL(k_err<double>, dim3(nblk), dim3(256), 0, st, s->n_el, N, R, np, s->geo, s->jdet, d_pts, d_w,
+                         d_b, n_waves, d_wv, time, (const double*)dofs, d_part);
+    else
+      hipLaunchKernelGGL(k_err<float>, dim3(nblk), dim3(256), 0, st, s->n_el, N, R, np, s->geo, s->jdet, d_pts, d_w,
+                         d_b, n_waves, d_wv, time, (const float*)dofs, d_part);
+    if (!ok(hipGetLastError()) ||
+        !ok(hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st)) ||
+        !ok(hipStreamSynchronize(st)))
+      rc = edge::fail(EDGE_ERR_HIP, "error-norm reduction");
+  }
+  hipFree(d_pts);
+  hipFree(d_w);
+  hipFree(d_b);
+  hipFree(d_wv);
+  hipFree(d_part);
+  if (rc) return rc;
+  for (int i = 0; i < nqr; i++) {
+    double a1 = 0, a2 = 0, ai = 0;
+    for (unsigned b = 0; b < nblk; b++) {
+      a1 += h[((size_t)b * 3 + 0) * nqr + i];
+      a2 += h[((size_t)b * 3 + 1) * nqr + i];
+      ai = std::max(ai, h[((size_t)b * 3 + 2) * nqr + i]);
+    }
+    if (l1) l1[i] = a1;
+    if (l2sq) l2sq[i] = a2;
+    if (linf) linf[i] = ai;
   }
   return EDGE_OK;
 }

@@ -66,6 +66,25 @@ class ElasticSolver:
                                              _stream_ptr(stream)))
         return k, e
 
+    def error_norms(self, dofs, waves, t=0.0, n1d=0, stream=None):
+        """(L1, squared L2, Linf) [9][R] errors, FP64 on the GPU, against plane
+        waves: waves[r] lists (r[9], k[3], omega, phase) per run, or objects with
+        .r, .k, .omega (phase 0)."""
+        self._check_arr(dofs)
+        nw = max(len(w) for w in waves) if len(waves) else 0
+        arr = np.zeros((self.n_runs, nw, 14))
+        for r in range(self.n_runs):
+            for i, w in enumerate(waves[r]):
+                if hasattr(w, "omega"):
+                    arr[r, i, :9], arr[r, i, 9:12], arr[r, i, 12] = w.r, w.k, w.omega
+                else:
+                    rv, kv, om, ph = w
+                    arr[r, i, :9], arr[r, i, 9:12], arr[r, i, 12], arr[r, i, 13] = rv, kv, om, ph
+        out = [np.zeros((9, self.n_runs)) for _ in range(3)]
+        _lib.check(_lib.lib().edge_el_error_norms(self._h, dofs.data_ptr(), int(n1d), nw, arr.ctypes.data, float(t),
+                                                  *[o.ctypes.data for o in out], _stream_ptr(stream)))
+        return tuple(out)
+
     def autotune(self, dt, n_reps=3, stream=None):
         """Times the kernel variants this solver can switch to on its mesh
         and keeps the fastest; returns [(kernel, projections, ms per step)]."""

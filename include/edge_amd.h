@@ -404,6 +404,17 @@ int edge_el_l2_norms(edge_el_solver *s, const void *a, int prec_a, const void *b
  * InitialDofs.hpp:277-370). */
 int edge_el_energy(edge_el_solver *s, const void *dofs, double *kinetic, double *strain, void *stream);
 
+/* L1, squared L2 and Linf errors per quantity and run (each [9][R], host),
+ * FP64 on the GPU, of a solver's device DOFs against the sum of n_waves
+ * plane waves q(x, t) = sum_w r_w sin(k_w . x - omega_w t + phase_w) per run
+ * (waves [R][n_waves][14]: r[9], k[3], omega, phase; host), evaluated at
+ * time t on the collapsed Gauss rule with n1d points per direction (0:
+ * order + 1) of every element.  Replaces InitialDofs::err
+ * (src/setups/InitialDofs.hpp:277-370), whose exact solutions are exprtk
+ * expressions, for plane-wave solutions.  Solvers built from geometry only. */
+int edge_el_error_norms(edge_el_solver *s, const void *dofs, int32_t n1d, int32_t n_waves, const double *waves,
+                        double t, double *l1, double *l2sq, double *linf, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
