@@ -405,7 +405,8 @@ def main():
     s32 = ElasticSolver(m, order, runs, 32, rho, lam, mu, kernel=args.kernel)
     tune = {}
     if not args.no_autotune:  # untimed set-up: the fastest kernel variant on this mesh
-        tune["fp32"] = [{"kernel": k, "projections": p, "ms_per_step": ms} for k, p, ms in s32.autotune(dt, 3)]
+        tune["fp32"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "ms_per_step": ms}
+                        for k, p, x, ms in s32.autotune(dt, 3)]
     proj = bool(s32.kernel_info()[1] & PATH_PROJECTIONS)
     d32 = (torch.from_numpy(q0)[..., None] * scale).to(torch.float32).cuda().contiguous()
     t32 = torch.zeros_like(d32)
@@ -424,7 +425,8 @@ def main():
     if not args.no_fp64:
         s64 = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=args.kernel)
         if not args.no_autotune:
-            tune["fp64"] = [{"kernel": k, "projections": p, "ms_per_step": ms} for k, p, ms in s64.autotune(dt, 2)]
+            tune["fp64"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "ms_per_step": ms}
+                            for k, p, x, ms in s64.autotune(dt, 2)]
         d64 = torch.from_numpy(q0[..., None].copy()).cuda().contiguous()
         t64 = torch.zeros_like(d64)
         print("[bench] fp64 plain timing", file=sys.stderr, flush=True)

@@ -32,6 +32,9 @@ struct SolverDev {
   const T* star;  // [nEl][3][81]
   const T* fs;    // [2][nEl][4][81]
   bool sparse_star;  // every star matrix has the elastic 24-entry pattern
+  // MFMA kernels' workgroup -> element schedule: contiguous element ranges
+  // per XCD (true) or workgroup i -> element i (false)
+  bool xcd_map = true;
   // rank-3 factors of the flux solvers [2][nEl][4][54] (U [9][3], then V^T
   // [3][9]); null unless the MFMA-tiled kernels run
   const T* fsf = nullptr;

@@ -56,7 +56,8 @@ struct Dims {
 };
 
 // blockIdx -> element offset: each of the 8 XCDs (blocks dispatched round
-// robin) gets a contiguous element range, so neighbours share its L2.
+// robin) gets a contiguous element range, so neighbours share its L2 (the
+// autotuner may switch this schedule off: block i -> element i).
 __device__ __forceinline__ int64_t xcd_element(int64_t n) {
   const int64_t chunk = (n + 7) / 8;
   const int64_t b = blockIdx.x;
@@ -666,7 +667,7 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   constexpr int NV = n_modes(O - 1);  // rows of tInt the volume integral reads
   // Q, then the time-integrated DOFs, then the output
   __shared__ __align__(16) float l_ti[D::SZ + kLdsPad > D::CHUNKS * 256 ? D::SZ + kLdsPad : D::CHUNKS * 256];
-  const int64_t off = xcd_element(n);
+  const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
@@ -924,7 +925,7 @@ void k_neigh(SolverDev<float> s, int64_t first, int64_t n, const float* __restri
   using D = Dims<O>;
   constexpr int N = D::N, NF = D::NF, NT = D::NT;
   __shared__ float l_nb[D::CHUNKS * 256];  // the neighbour's tInt tile
-  const int64_t off = xcd_element(n);
+  const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
@@ -971,7 +972,7 @@ void k_neigh_proj(SolverDev<float> s, int64_t first, int64_t n, float* __restric
   using D = Dims<O>;
   constexpr int N = D::N, NT = D::NT, NF = D::NF;
   __shared__ __align__(16) float l_out[D::SZ + kLdsPad];
-  const int64_t off = xcd_element(n);
+  const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;

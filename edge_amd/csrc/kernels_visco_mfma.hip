@@ -70,7 +70,8 @@ constexpr int ksteps(int rows) { return (rows + 3) / 4; }
 __device__ __forceinline__ const cfloat* as_const(const float* p) { return (const cfloat*)p; }
 
 // blockIdx -> element offset: each of the 8 XCDs (workgroups dispatched round
-// robin) gets a contiguous element range, so neighbours share its L2.  The
+// robin) gets a contiguous element range, so neighbours share its L2 (the
+// autotuner may switch this schedule off: workgroup i -> element i).  The
 // grid holds 8 ceil(n / 8) workgroups.
 __device__ __forceinline__ int64_t xcd_element(int64_t n) {
   const int64_t chunk = (n + 7) / 8;
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   constexpr int PJ = (NP + kLocalThreads - 1) / kLocalThreads;  // positions per thread
   __shared__ __align__(16) float l_x[SZ];      // Q, then der_{de-1}, then tInt
   __shared__ __align__(16) float l_p[18 * NP];  // raw products [3][6][N][R]; lifted tiles [4][3][N][R]
-  const int64_t off = xcd_element(n);
+  const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
@@ -534,7 +535,7 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
   // per face: (without projections) the neighbour's tInt, then (aliased) its
   // lifted tiles [3][N][R]
   __shared__ __align__(16) float l_n[4][PROJ ? 3 * NP : SZ];
-  const int64_t off = xcd_element(n);
+  const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;

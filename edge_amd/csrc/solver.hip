@@ -39,6 +39,7 @@ struct edge_el_solver {
   bool fs_rank3 = false;
   bool want_proj = false;    // kernel selector 2: exchange face projections when possible
   bool use_proj = true;      // the projections are exchanged when they exist (the autotuner may turn this off)
+  bool xcd_map = true;       // MFMA kernels: contiguous element ranges per XCD (the autotuner may turn this off)
   bool proj_tables = false;  // build the projection slot tables whatever the kernel (MFMA viscoelastic)
   void* proj = nullptr;      // [nEl][4][768]
   int32_t* snd_v = nullptr;  // [nEl][4]
@@ -91,6 +92,7 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.star = static_cast<const T*>(s->star);
   d.fs = static_cast<const T*>(s->fs);
   d.sparse_star = s->sparse_star;
+  d.xcd_map = s->xcd_map;
   d.fsf = static_cast<const T*>(s->fsf);
   d.fs_rank3 = s->fs_rank3 && s->fsf;
   d.proj = d.fs_rank3 && s->use_proj ? static_cast<T*>(s->proj) : nullptr;
@@ -736,6 +738,7 @@ namespace {
 struct TuneCand {
   int kernel;
   bool proj;
+  bool xcd = true;
 };
 
 // The kernel variants a solver can switch between with the data it holds.
@@ -745,8 +748,10 @@ std::vector<TuneCand> tune_candidates(const edge_el_solver* s) {
   const bool gen = s->n_rms > 0 ? edge::visco_supported(s->cfg.order, s->cfg.n_runs, prec_bytes, s->n_rms)
                                 : edge::generic_supported(s->cfg.order, s->cfg.n_runs, prec_bytes);
   if (s->kernel == 2) {
-    if (s->proj) c.push_back({2, true});
-    c.push_back({2, false});
+    for (int x = 1; x >= 0; x--) {  // XCD-aware element schedule, then the linear one
+      if (s->proj) c.push_back({2, true, x == 1});
+      c.push_back({2, false, x == 1});
+    }
   } else if (s->kernel == 4) {
     c.push_back({4, false});
   }
@@ -756,7 +761,7 @@ std::vector<TuneCand> tune_candidates(const edge_el_solver* s) {
 }  // namespace
 
 extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, void* stream, int32_t* n_cands,
-                                int32_t* kernels, int32_t* projs, double* ms) {
+                                int32_t* kernels, int32_t* flags, double* ms) {
   if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
   if (n_reps < 1 || !(dt > 0)) return edge::fail(EDGE_ERR_ARG, "autotune: n_reps >= 1 and dt > 0");
   const std::vector<TuneCand> cand = tune_candidates(s);
@@ -766,7 +771,7 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
   void *dofs = nullptr, *t_int = nullptr, *dofs_a = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const int old_kernel = s->kernel;
-  const bool old_proj = s->use_proj;
+  const bool old_proj = s->use_proj, old_xcd = s->xcd_map;
   std::vector<double> t(cand.size(), 0.0);
   int rc = EDGE_OK;
   auto hip = [&](hipError_t e, const char* what) {
@@ -784,6 +789,7 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
     for (size_t i = 0; i < cand.size() && rc == EDGE_OK; i++) {
       s->kernel = cand[i].kernel;
       s->use_proj = cand[i].proj;
+      s->xcd_map = cand[i].xcd;
       rc = one_step(s, dt, dofs, dofs_a, t_int, st);  // warm-up
       if (rc == EDGE_OK) hip(hipEventRecord(e0, st), "event");
       for (int r = 0; r < n_reps && rc == EDGE_OK; r++) rc = one_step(s, dt, dofs, dofs_a, t_int, st);
@@ -801,6 +807,7 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
   if (rc != EDGE_OK) {
     s->kernel = old_kernel;
     s->use_proj = old_proj;
+    s->xcd_map = old_xcd;
     return rc;
   }
   size_t best = 0;
@@ -808,12 +815,13 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
     if (t[i] < t[best]) best = i;
   s->kernel = cand[best].kernel;
   s->use_proj = cand[best].proj;
+  s->xcd_map = cand[best].xcd;
   if (n_cands) {
     const int32_t cap = *n_cands;
     *n_cands = (int32_t)cand.size();
     for (int32_t i = 0; i < (int32_t)cand.size() && i < cap; i++) {
       if (kernels) kernels[i] = cand[i].kernel;
-      if (projs) projs[i] = cand[i].proj ? 1 : 0;
+      if (flags) flags[i] = (cand[i].proj ? 1 : 0) | (cand[i].xcd ? 2 : 0);
       if (ms) ms[i] = t[i];
     }
   }

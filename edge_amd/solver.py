@@ -86,14 +86,15 @@ class ElasticSolver:
         return tuple(out)
 
     def autotune(self, dt, n_reps=3, stream=None):
-        """Times the kernel variants this solver can switch to on its mesh
-        and keeps the fastest; returns [(kernel, projections, ms per step)]."""
-        cap = 8
+        """Times the kernel variants and schedules this solver can switch to on
+        its mesh and keeps the fastest; returns [(kernel, projections,
+        xcd_schedule, ms per step)]."""
+        cap = 16
         n = C.c_int32(cap)
-        k, p, ms = (C.c_int32 * cap)(), (C.c_int32 * cap)(), (C.c_double * cap)()
+        k, f, ms = (C.c_int32 * cap)(), (C.c_int32 * cap)(), (C.c_double * cap)()
         _lib.check(_lib.lib().edge_el_autotune(self._h, float(dt), int(n_reps), _stream_ptr(stream), C.byref(n), k,
-                                               p, ms))
-        return [(k[i], bool(p[i]), ms[i]) for i in range(min(n.value, cap))]
+                                               f, ms))
+        return [(k[i], bool(f[i] & 1), bool(f[i] & 2), ms[i]) for i in range(min(n.value, cap))]
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -372,17 +372,19 @@ int edge_el_neigh_lts(edge_el_solver *s, int64_t first, int64_t n_els, int32_t f
  * Diagnostic with no counterpart in the reference (for tests and tools). */
 int edge_el_kernel_info(const edge_el_solver *s, int32_t *kernel, int32_t *flags);
 
-/* Autotuner: times every kernel variant the solver can switch to (the MFMA
- * kernels with and without the face-projection exchange, the FP64 MFMA
- * kernels, the generic kernels; all parity-tested against the oracle) over
+/* Autotuner: times every kernel variant and schedule the solver can switch
+ * to (the MFMA kernels with and without the face-projection exchange, each
+ * with the XCD-aware element schedule or the linear one; the FP64 MFMA
+ * kernels; the generic kernels; all parity-tested against the oracle) over
  * n_reps full steps of time step dt on scratch DOFs on its own mesh, and
  * keeps the fastest for the solver's later calls.  In: *n_cands = capacity
  * of the arrays (may be NULL); out: *n_cands = number of candidates,
- * kernels[i] (as edge_el_kernel_info), projs[i] (1: projections exchanged)
- * and ms[i] (per step).  Drops a captured step graph.  No counterpart in the
- * reference (its back-end is a compile-time choice, Kernels.hpp:30-44). */
+ * kernels[i] (as edge_el_kernel_info), flags[i] (bit 0: projections
+ * exchanged, bit 1: XCD-aware schedule) and ms[i] (per step).  Drops a
+ * captured step graph.  No counterpart in the reference (its back-end is a
+ * compile-time choice, Kernels.hpp:30-44). */
 int edge_el_autotune(edge_el_solver *s, double dt, int32_t n_reps, void *stream, int32_t *n_cands,
-                     int32_t *kernels, int32_t *projs, double *ms);
+                     int32_t *kernels, int32_t *flags, double *ms);
 
 /* FP64 L2 norms per quantity and run, exact through the diagonal mass
  * matrix: for device DOF arrays a (element precision prec_a: 32 or 64) and

@@ -35,9 +35,9 @@ def test_autotune_elastic(gpu, prec, runs, expect):
     s = ElasticSolver(m, order, runs, prec, rho, lam, mu)
     dt = S.dt_cfl(m, rho, lam, mu, order)
     cands = s.autotune(dt, n_reps=2)
-    assert expect <= {(k, p) for k, p, _ in cands}
+    assert expect <= {(k, p) for k, p, _, _ in cands}
     assert all(ms > 0 for *_, ms in cands)
-    k, p, _ = min(cands, key=lambda c: c[2])
+    k, p, _, _ = min(cands, key=lambda c: c[3])
     kernel, flags = s.kernel_info()
     assert kernel == k and bool(flags & PATH_PROJECTIONS) == p
     rng = np.random.default_rng(1)
@@ -60,8 +60,9 @@ def test_autotune_visco(gpu):
     visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qp / 2, nr, 20.0, 10.0)
     dt = S.dt_cfl(m, rho, visco["lam_e"], visco["mu_e"], order)
     cands = s.autotune(dt, n_reps=2)
-    assert {(KERNEL_MFMA, True), (KERNEL_MFMA, False), (KERNEL_GENERIC, False)} <= {(k, p) for k, p, _ in cands}
-    k, p, _ = min(cands, key=lambda c: c[2])
+    assert {(KERNEL_MFMA, True), (KERNEL_MFMA, False), (KERNEL_GENERIC, False)} <= {(k, p) for k, p, _, _ in cands}
+    assert {x for k, _, x, _ in cands if k == KERNEL_MFMA} == {True, False}  # both element schedules
+    k, p, _, _ = min(cands, key=lambda c: c[3])
     assert s.kernel_info()[0] == k
     rng = np.random.default_rng(2)
     n = S.n_modes(order)
