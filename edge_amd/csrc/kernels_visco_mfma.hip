@@ -479,6 +479,22 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   }
   __syncthreads();
 
+  // ---- Q and QA for the output, requested before the surface integral
+  float* gq = dofs + el * SZ;
+  float qin[9][PJ], qain[NRM][6][PJ];
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kLocalThreads;
+    if (p < NP) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) qin[q][j] = gq[q * NP + p];
+#pragma unroll
+      for (int rm = 0; rm < NRM; rm++)
+#pragma unroll
+        for (int q = 0; q < 6; q++) qain[rm][q][j] = qa0[(rm * 6 + q) * NP + p];
+    }
+  }
+
   // ---- local surface: waves 0..3 the lifted tiles of face wv in
   // l_p[wv][3][N][R]; wave 4 the neighbours' projection through face 3
   if (wv < 4) {
@@ -491,7 +507,6 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
   __syncthreads();
 
   // ---- Q += volume + source + surface; QA_rm += w_rm (A - tIntA_rm + surface)
-  float* gq = dofs + el * SZ;
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
     const int p = tid + j * kLocalThreads;
@@ -512,12 +527,12 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
         for (int q = 0; q < 6; q++) ua[q] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
       }
 #pragma unroll
-      for (int q = 0; q < 9; q++) gq[q * NP + p] += y[q];
+      for (int q = 0; q < 9; q++) gq[q * NP + p] = qin[q][j] + y[q];
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++) {
         const float w = rfs[rm];
 #pragma unroll
-        for (int q = 0; q < 6; q++) qa0[(rm * 6 + q) * NP + p] += w * (ua[q] - ta[rm][q][j]);
+        for (int q = 0; q < 6; q++) qa0[(rm * 6 + q) * NP + p] = qain[rm][q][j] + w * (ua[q] - ta[rm][q][j]);
       }
     }
   }
@@ -544,6 +559,22 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
   const cint32* rcv = (const cint32*)(s.rcv + el * 4);
   const cfloat* fsf = as_const(s.fsf + (s.n_el + el) * 4 * 54);
   const cfloat* fsfa = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
+  float* gq = dofs + el * SZ;
+  float* gqa = dofs_a + el * NRM * SZA;
+  // this element's Q and QA, requested before the lifts
+  float qin[9][PJ], qain[NRM][6][PJ];
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kNeighThreads;
+    if (p < NP) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) qin[q][j] = gq[q * NP + p];
+#pragma unroll
+      for (int rm = 0; rm < NRM; rm++)
+#pragma unroll
+        for (int q = 0; q < 6; q++) qain[rm][q][j] = gqa[(rm * 6 + q) * NP + p];
+    }
+  }
   if constexpr (PROJ) {
     const int32_t slot = rcv[wv];
     if (slot >= 0) {  // else outflow (uniform)
@@ -566,8 +597,6 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
     }
   }
   __syncthreads();
-  float* gq = dofs + el * SZ;
-  float* gqa = dofs_a + el * NRM * SZA;
   const cfloat* rfs = as_const(s.rfs);
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
@@ -590,12 +619,12 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
         for (int q = 0; q < 6; q++) ua[q] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
       }
 #pragma unroll
-      for (int q = 0; q < 9; q++) gq[q * NP + p] += y[q];
+      for (int q = 0; q < 9; q++) gq[q * NP + p] = qin[q][j] + y[q];
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++) {
         const float w = rfs[rm];
 #pragma unroll
-        for (int q = 0; q < 6; q++) gqa[(rm * 6 + q) * NP + p] += w * ua[q];
+        for (int q = 0; q < 6; q++) gqa[(rm * 6 + q) * NP + p] = qain[rm][q][j] + w * ua[q];
       }
     }
   }
