@@ -113,10 +113,12 @@ struct VDims {
 // tiles) and V_0, V_1, V_2 (velocity rows of the star-combined tile).
 // Column-half pairs: (P_0 | P_1), (P_2 | V_0), (V_1 | V_2), so product
 // 2 p + h sits in half h of pair p.
+// apre (optional): the A operands of every k-step of one tile (TT = 1),
+// loaded by the caller ahead of time.
 template <int O, int NK, int NN, int TT>
 __device__ __forceinline__ void raw_products(const float* __restrict__ m, const cfloat* __restrict__ sd,
                                              const float* __restrict__ x, float* __restrict__ out, int t0, int g,
-                                             int c) {
+                                             int c, const float* apre = nullptr) {
   using D = VDims<O>;
   constexpr int N = D::N, NP = D::NP, KS = ksteps(NK);
   const int h = c >> 3, r = c & 7;
@@ -132,8 +134,12 @@ __device__ __forceinline__ void raw_products(const float* __restrict__ m, const 
     for (int p = 0; p < 3; p++) acc[t][p] = zero4();
   auto load = [&](int s, float (&a)[TT], float (&b)[9]) {
     const int k = 4 * s + g;
+    if (apre) {
+      a[0] = apre[s];
+    } else {
 #pragma unroll
-    for (int t = 0; t < TT; t++) a[t] = ldg(m, k * N + 16 * (t0 + t) + c, k < NK && 16 * (t0 + t) + c < NN);
+      for (int t = 0; t < TT; t++) a[t] = ldg(m, k * N + 16 * (t0 + t) + c, k < NK && 16 * (t0 + t) + c < NN);
+    }
 #pragma unroll
     for (int q = 0; q < 9; q++) {
       if (4 * s + 3 < NK)  // constant after unrolling
@@ -364,12 +370,20 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
     for (int q = 0; q < 9; q++) ti[q][j] = p < NP ? dt * l_x[q * NP + p] : 0.f;
   }
 
-  // ---- CK
+  // ---- CK; the wave's stiffness A operands (one row tile, every k-step)
+  // are the same in every stage
+  float a_ck[ksteps(N)];
+#pragma unroll
+  for (int st = 0; st < ksteps(N); st++) {
+    const int k = 4 * st + g;
+    a_ck[st] = ldg(s.dg.stiffTn + dir * N * N, k * N + 16 * t_ck + c, k < N && 16 * t_ck + c < NV);
+  }
   float sca = dt;
 #pragma unroll 1
   for (int de = 1; de < O; de++) {
     if (t_ck < tiles(NV))
-      raw_products<O, N, NV, 1>(s.dg.stiffTn + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, t_ck, g, c);
+      raw_products<O, N, NV, 1>(s.dg.stiffTn + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, t_ck, g, c,
+                                a_ck);
     __syncthreads();
     sca *= dt / float(de + 1);
 #pragma unroll
