@@ -114,3 +114,24 @@ def test_rupture_create_rejects_bad_arguments():
     assert rc == 1 and "area" in msg
     assert L.edge_rupture_net_updates(None, 0, 1, 0.1, None, None, None, None, None) == 1
     assert L.edge_rupture_destroy(None) == 0
+
+
+def test_null_handles_are_rejected():
+    """Entry points return EDGE_ERR_ARG for a missing solver handle before any
+    device work."""
+    L = _lib.lib()
+    n = C.c_int32(4)
+    assert L.edge_el_autotune(None, 1e-3, 3, None, C.byref(n), None, None, None) == 1
+    assert L.edge_el_energy(None, None, None, None, None) == 1
+    assert L.edge_el_error_norms(None, None, 0, 0, None, 0.0, None, None, None, None) == 1
+    assert L.edge_el_step_a(None, 1, 1e-3, None, None, None, 0, None) == 1
+    assert b"handle" in L.edge_last_error()
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """Without the built HIP library every entry point raises; nothing falls
+    back to host code."""
+    monkeypatch.setattr(_lib, "LIB_PATH", _lib.LIB_PATH + ".absent")
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.EdgeError, match="missing"):
+        _lib.lib()
