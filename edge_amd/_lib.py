@@ -103,6 +103,8 @@ _SIGS = {
     "edge_recv_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "edge_recv_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
+    "edge_recv_eval_a": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
     "edge_src_create": (C.c_int, [C.c_void_p, C.c_int64] + [C.c_void_p] * 8 + [C.POINTER(C.c_void_p)]),
     "edge_src_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "edge_points_destroy": (C.c_int, [C.c_void_p]),

@@ -117,6 +117,11 @@ template <typename T>
 int launch_recv_eval(const SolverDev<T>& s, int64_t n, const int64_t* el, const double* basis, int n_tau,
                      const double* tau, const T* dofs, T* out, hipStream_t st);
 bool recv_supported(int order, int n_runs, int prec_bytes);
+// ... with attenuation (the anelastic DOFs of the receivers' elements)
+template <typename T>
+int launch_recv_eval_visco(const SolverDev<T>& s, int64_t n, const int64_t* el, const double* basis, int n_tau,
+                           const double* tau, const T* dofs, const T* dofs_a, T* out, hipStream_t st);
+bool recv_visco_supported(int order, int n_runs, int prec_bytes, int n_rms);
 
 // Rate-2 local time stepping (lts.hip): tDofs[2] of the listed elements,
 // tDofs[1] sums, and the interface faces of the listed elements.

@@ -171,6 +171,157 @@ __global__ __launch_bounds__(kThreads) void k_src_apply(int n_modes, int n_runs,
 
 }  // namespace
 
+namespace {
+
+// ---------------------------------------------------------------- receivers with attenuation
+// Viscoelastic solvers: the receiver's elastic derivatives come from the CK
+// with the anelastic terms (TimePredVanilla.hpp:200-306 with TL_N_RMS > 0,
+// all N modes per derivative): der_de = sum_d S_d (der_{de-1} (-K_d^T M^-1))
+// + sum_rm E_rm derA_rm, derA_rm <- w_rm (sum_d S^A_d (der_{de-1} K)_vel +
+// derA_rm).  Dynamic LDS: the previous and current derivative, the stiffness
+// product, the anelastic combination, derA of every mechanism, the star /
+// anelastic star / source matrices, the basis values and the accumulators.
+struct RecvViscoLds {
+  size_t prev, cur, scr, scra, da, star, star_a, src, phi, acc, size;
+  __host__ __device__ RecvViscoLds(int order, int runs, int nr, int tb) {
+    const size_t N = n_modes(order), sz = 9 * N * runs, sza = 6 * N * runs;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+      const size_t at = o;
+      o += (bytes + 15) / 16 * 16;
+      return at;
+    };
+    prev = take(tb * sz);
+    cur = take(tb * sz);
+    scr = take(tb * sz);
+    scra = take(tb * sza);
+    da = take(tb * sza * nr);
+    star = take(tb * 243);
+    star_a = take(tb * 54);
+    src = take(tb * 36 * nr);
+    phi = take(8 * N);
+    acc = take(8 * kMaxTau * 9 * runs);
+    size = o;
+  }
+};
+
+template <typename T, int O, int R>
+__global__ __launch_bounds__(kThreads) void k_recv_visco(SolverDev<T> s, const int64_t* __restrict__ el_ids,
+                                                         const double* __restrict__ basis, int n_tau,
+                                                         const double* __restrict__ tau, const T* __restrict__ dofs,
+                                                         const T* __restrict__ dofs_a, T* __restrict__ out) {
+  constexpr int N = n_modes(O), NV = n_modes(O - 1), SZ = 9 * N * R, SZA = 6 * N * R;
+  const int nr = s.n_rms;
+  const RecvViscoLds L(O, R, nr, sizeof(T));
+  extern __shared__ __align__(16) unsigned char lds[];
+  T* l_prev = reinterpret_cast<T*>(lds + L.prev);
+  T* l_cur = reinterpret_cast<T*>(lds + L.cur);
+  T* l_scr = reinterpret_cast<T*>(lds + L.scr);
+  T* l_scra = reinterpret_cast<T*>(lds + L.scra);
+  T* l_da = reinterpret_cast<T*>(lds + L.da);
+  T* l_star = reinterpret_cast<T*>(lds + L.star);
+  T* l_star_a = reinterpret_cast<T*>(lds + L.star_a);
+  T* l_src = reinterpret_cast<T*>(lds + L.src);
+  double* l_phi = reinterpret_cast<double*>(lds + L.phi);
+  double* l_acc = reinterpret_cast<double*>(lds + L.acc);
+  const int64_t rc = blockIdx.x, el = el_ids[rc];
+  for (int i = threadIdx.x; i < SZ; i += kThreads) l_prev[i] = dofs[el * SZ + i];
+  for (int i = threadIdx.x; i < nr * SZA; i += kThreads) l_da[i] = dofs_a[el * nr * SZA + i];
+  for (int i = threadIdx.x; i < 243; i += kThreads) l_star[i] = s.star[el * 243 + i];
+  for (int i = threadIdx.x; i < 54; i += kThreads) l_star_a[i] = s.star_a[el * 54 + i];
+  for (int i = threadIdx.x; i < 36 * nr; i += kThreads) l_src[i] = s.src_a[el * 36 * nr + i];
+  for (int i = threadIdx.x; i < N; i += kThreads) l_phi[i] = basis[rc * N + i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < 9 * R; i += kThreads) {
+    const int q = i / R, r = i % R;
+    double v = 0;
+    for (int md = 0; md < N; md++) v += l_phi[md] * double(l_prev[(q * N + md) * R + r]);
+    for (int k = 0; k < n_tau; k++) l_acc[k * 9 * R + i] = v;
+  }
+  for (int de = 1; de < O; de++) {
+    for (int i = threadIdx.x; i < SZ; i += kThreads) l_cur[i] = T(0);
+    for (int i = threadIdx.x; i < SZA; i += kThreads) l_scra[i] = T(0);
+    for (int d = 0; d < 3; d++) {
+      __syncthreads();
+      gemm_ac<T, R>(9, NV, N, N, N, N, false, l_prev, s.dg.stiffTn + d * N * N, l_scr);
+      __syncthreads();
+      gemm_bc<T, R>(9, NV, 9, 9, N, N, true, l_star + d * 81, l_scr, l_cur);
+      gemm_bc<T, R>(6, NV, 3, 3, N, N, true, l_star_a + d * 18, l_scr + 6 * N * R, l_scra);
+    }
+    __syncthreads();
+    for (int rm = 0; rm < nr; rm++) {
+      gemm_bc<T, R>(6, N, 6, 6, N, N, true, l_src + rm * 36, l_da + rm * SZA, l_cur);
+      __syncthreads();
+    }
+    for (int i = threadIdx.x; i < nr * SZA; i += kThreads) {
+      const int rm = i / SZA;
+      l_da[i] = s.rfs[rm] * (l_scra[i % SZA] + l_da[i]);
+    }
+    for (int i = threadIdx.x; i < 9 * R; i += kThreads) {
+      const int q = i / R, r = i % R;
+      double v = 0;
+      for (int md = 0; md < N; md++) v += l_phi[md] * double(l_cur[(q * N + md) * R + r]);
+      double fac = 1.0;  // tau^de / de!
+      for (int j = 1; j <= de; j++) fac /= j;
+      for (int k = 0; k < n_tau; k++) {
+        double c = fac;
+        for (int j = 0; j < de; j++) c *= tau[rc * n_tau + k];
+        l_acc[k * 9 * R + i] += c * v;
+      }
+    }
+    __syncthreads();
+    T* t = l_prev;
+    l_prev = l_cur;
+    l_cur = t;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_tau * 9 * R; i += kThreads) out[rc * n_tau * 9 * R + i] = T(l_acc[i]);
+}
+
+template <typename T, int R>
+int dispatch_recv_visco(const SolverDev<T>& s, int64_t n, const int64_t* el, const double* basis, int n_tau,
+                        const double* tau, const T* dofs, const T* dofs_a, T* out, hipStream_t st) {
+  auto go = [&](auto kern, int order) {
+    const size_t lds = RecvViscoLds(order, R, s.n_rms, sizeof(T)).size;
+    EDGE_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(kThreads), lds, st, s, el, basis, n_tau, tau, dofs, dofs_a,
+                       out);
+    EDGE_HIP_TRY(hipGetLastError());
+    return EDGE_OK;
+  };
+  switch (s.order) {
+    case 2: return go(k_recv_visco<T, 2, R>, 2);
+    case 3: return go(k_recv_visco<T, 3, R>, 3);
+    case 4: return go(k_recv_visco<T, 4, R>, 4);
+    case 5: return go(k_recv_visco<T, 5, R>, 5);
+  }
+  return fail(EDGE_ERR_UNSUPPORTED, "receivers: order");
+}
+
+}  // namespace
+
+bool recv_visco_supported(int order, int n_runs, int prec_bytes, int n_rms) {
+  if (order < 2 || order > 5 || (n_runs != 1 && n_runs != 8) || n_rms < 1) return false;
+  return RecvViscoLds(order, n_runs, n_rms, prec_bytes).size <= 160 * 1024;
+}
+
+template <typename T>
+int launch_recv_eval_visco(const SolverDev<T>& s, int64_t n, const int64_t* el, const double* basis, int n_tau,
+                           const double* tau, const T* dofs, const T* dofs_a, T* out, hipStream_t st) {
+  if (n <= 0 || n_tau <= 0) return EDGE_OK;
+  if (n_tau > kMaxTau) return fail(EDGE_ERR_ARG, "receivers: at most 8 output times per call");
+  if (!recv_visco_supported(s.order, s.n_runs, sizeof(T), s.n_rms))
+    return fail(EDGE_ERR_UNSUPPORTED, "receivers: configuration");
+  if (s.n_runs == 1) return dispatch_recv_visco<T, 1>(s, n, el, basis, n_tau, tau, dofs, dofs_a, out, st);
+  return dispatch_recv_visco<T, 8>(s, n, el, basis, n_tau, tau, dofs, dofs_a, out, st);
+}
+
+template int launch_recv_eval_visco<float>(const SolverDev<float>&, int64_t, const int64_t*, const double*, int,
+                                           const double*, const float*, const float*, float*, hipStream_t);
+template int launch_recv_eval_visco<double>(const SolverDev<double>&, int64_t, const int64_t*, const double*, int,
+                                            const double*, const double*, const double*, double*, hipStream_t);
+
 bool recv_supported(int order, int n_runs, int prec_bytes) {
   if (order < 2 || order > 5) return false;
   if (n_runs != 1 && n_runs != 8 && n_runs != 16) return false;
@@ -286,8 +437,8 @@ extern "C" int edge_recv_create(const edge_el_solver* s, int64_t n, const int64_
   int rc = edge::solver_ref(s, &r);
   if (rc != EDGE_OK) return rc;
   if (n < 0) return fail(EDGE_ERR_ARG, "receivers: negative count");
-  if (r.n_rms > 0) return fail(EDGE_ERR_UNSUPPORTED, "receivers: viscoelastic solvers");
-  if (!edge::recv_supported(r.order, r.n_runs, r.precision / 8))
+  if (r.n_rms > 0 ? !edge::recv_visco_supported(r.order, r.n_runs, r.precision / 8, r.n_rms)
+                  : !edge::recv_supported(r.order, r.n_runs, r.precision / 8))
     return fail(EDGE_ERR_UNSUPPORTED, "receivers: order, runs and precision do not fit one workgroup's LDS");
   for (int64_t i = 0; i < n; i++)
     if (el[i] < 0 || el[i] >= r.n_el) return fail(EDGE_ERR_ARG, "receivers: element out of range");
@@ -318,6 +469,7 @@ extern "C" int edge_recv_eval(edge_points* p, const edge_el_solver* s, const voi
   int rc = check_solver(p, s, &r);
   if (rc != EDGE_OK) return rc;
   if (p->kind != 1) return fail(EDGE_ERR_ARG, "points: not receivers");
+  if (r.n_rms > 0) return fail(EDGE_ERR_ARG, "receivers: a viscoelastic solver needs dofs_a (edge_recv_eval_a)");
   if (!dofs || !out || (n_tau > 0 && !tau)) return fail(EDGE_ERR_ARG, "receivers: argument missing");
   if (n_tau < 0 || n_tau > 8) return fail(EDGE_ERR_ARG, "receivers: 0 to 8 output times per call");
   if (p->n == 0 || n_tau == 0) return EDGE_OK;
@@ -328,6 +480,26 @@ extern "C" int edge_recv_eval(edge_points* p, const edge_el_solver* s, const voi
                                           (double*)out, st);
   return edge::launch_recv_eval<float>(r.f, p->n, p->el, p->basis, n_tau, p->tau, (const float*)dofs, (float*)out,
                                        st);
+}
+
+extern "C" int edge_recv_eval_a(edge_points* p, const edge_el_solver* s, const void* dofs, const void* dofs_a,
+                                int32_t n_tau, const double* tau, void* out, void* stream) {
+  using edge::fail;
+  edge::SolverRef r;
+  int rc = check_solver(p, s, &r);
+  if (rc != EDGE_OK) return rc;
+  if (r.n_rms == 0) return edge_recv_eval(p, s, dofs, n_tau, tau, out, stream);
+  if (p->kind != 1) return fail(EDGE_ERR_ARG, "points: not receivers");
+  if (!dofs || !dofs_a || !out || (n_tau > 0 && !tau)) return fail(EDGE_ERR_ARG, "receivers: argument missing");
+  if (n_tau < 0 || n_tau > 8) return fail(EDGE_ERR_ARG, "receivers: 0 to 8 output times per call");
+  if (p->n == 0 || n_tau == 0) return EDGE_OK;
+  hipStream_t st = (hipStream_t)stream;
+  EDGE_HIP_TRY(hipMemcpyAsync(p->tau, tau, sizeof(double) * p->n * n_tau, hipMemcpyHostToDevice, st));
+  if (r.precision == 64)
+    return edge::launch_recv_eval_visco<double>(r.d, p->n, p->el, p->basis, n_tau, p->tau, (const double*)dofs,
+                                                (const double*)dofs_a, (double*)out, st);
+  return edge::launch_recv_eval_visco<float>(r.f, p->n, p->el, p->basis, n_tau, p->tau, (const float*)dofs,
+                                             (const float*)dofs_a, (float*)out, st);
 }
 
 extern "C" int edge_src_create(const edge_el_solver* s, int64_t n, const int64_t* el, const int32_t* run,

@@ -107,9 +107,10 @@ class Receivers:
         except Exception:
             pass
 
-    def sample(self, dofs, time, dt, stream=None):
+    def sample(self, dofs, time, dt, stream=None, dofs_a=None):
         """Records the outputs of the step [time, time + dt) from the DOFs at
-        its start; returns the number of output times evaluated."""
+        its start (and the anelastic DOFs for a viscoelastic solver); returns
+        the number of output times evaluated."""
         import torch
         tau = output_times(self.next, time, dt, self.freq, self.tol)
         k = tau.shape[1]
@@ -118,8 +119,9 @@ class Receivers:
         r = self.solver.n_runs
         out = torch.empty((self.n, k, 9, r), dtype=dofs.dtype, device=dofs.device)
         tau = np.ascontiguousarray(tau)
-        _lib.check(_lib.lib().edge_recv_eval(self._h, self.solver._h, dofs.data_ptr(), k, tau.ctypes.data,
-                                             out.data_ptr(), _stream_ptr(stream)))
+        _lib.check(_lib.lib().edge_recv_eval_a(self._h, self.solver._h, dofs.data_ptr(),
+                                               None if dofs_a is None else dofs_a.data_ptr(), k, tau.ctypes.data,
+                                               out.data_ptr(), _stream_ptr(stream)))
         vals = out.double().cpu().numpy()
         for i in range(self.n):
             for j in range(k):

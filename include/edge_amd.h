@@ -280,6 +280,15 @@ int edge_recv_create(const edge_el_solver *s, int64_t n, const int64_t *el, cons
 int edge_recv_eval(edge_points *p, const edge_el_solver *s, const void *dofs, int32_t n_tau, const double *tau,
                    void *out, void *stream);
 
+/* edge_recv_eval for a viscoelastic solver: the time prediction comes from
+ * the CK with the anelastic terms, which needs the anelastic DOFs dofs_a
+ * (device, [nEl][n_rms][6][mode][run]) beside the elastic ones
+ * (TimePredVanilla.hpp:200-306 with TL_N_RMS > 0, as the receiver branch of
+ * AderDg::local uses it, AderDg.hpp:475-496).  1 or 8 runs.  As
+ * edge_recv_eval for an elastic solver (dofs_a ignored). */
+int edge_recv_eval_a(edge_points *p, const edge_el_solver *s, const void *dofs, const void *dofs_a, int32_t n_tau,
+                     const double *tau, void *out, void *stream);
+
 /* Point sources (PointSources::t_ps, PointSources.hpp:82-107, one run per
  * source): element el[n], fused run run[n], series start times t0[n] and
  * sampling steps dts[n], basis evaluations bev[n][N] (edge_el_point_basis,

@@ -115,3 +115,40 @@ def test_sources_match_oracle(gpu, prec, runs):
     assert err <= (1e-13 if prec == 64 else 1e-6), err
     src.close()
     s.close()
+
+
+@pytest.mark.parametrize("prec,order,runs", [(64, 5, 1), (64, 3, 1), (32, 5, 8), (32, 4, 8)])
+def test_receivers_with_attenuation_match_oracle(gpu, prec, order, runs):
+    """Viscoelastic solvers: the receivers' time prediction comes from the CK
+    with the anelastic terms (orc_ck with the anelastic star, source and
+    relaxation frequencies), FP64 relative 1e-12, FP32 1e-5."""
+    from edge_amd.solver import ViscoSolver
+    nr, fc, fr = 3, 20.0, 10.0
+    m = M.box(3, jitter=0.1, seed=5)
+    rho, lam, mu = S.random_materials(m.n_el, seed=5)
+    qp = np.random.default_rng(5).uniform(20, 60, m.n_el)
+    s = ViscoSolver(m, order, runs, prec, rho, lam, mu, qp, qp / 2, nr, fc, fr)
+    visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qp / 2, nr, fc, fr)
+    dt = S.dt_cfl(m, rho, visco["lam_e"], visco["mu_e"], order)
+    rng = np.random.default_rng(order + runs)
+    n = s.n_modes
+    d = _dev(rng.normal(size=(m.n_el, 9, n, runs)), prec)
+    a = _dev(0.1 * rng.normal(size=(m.n_el, nr, 6, n, runs)), prec)
+    pts = rng.uniform(0.05, 0.95, size=(10, 3))
+    rec = Receivers(s, pts, freq=dt / 3, t0=0.0)
+    assert rec.sample(d, 0.0, dt, dofs_a=a) == 3
+    dg = orc.Dg(order, runs, rfs=visco["rfs"])
+    basis = point_basis(s, rec.el, rec.xi)
+    tol = 1e-12 if prec == 64 else 1e-5
+    for i in range(rec.n):
+        e = rec.el[i]
+        der = orc.ck(dg, dt, visco["starE"][e], d[e].double().cpu().numpy(), visco["starA"][e], visco["srcA"][e],
+                     a[e].double().cpu().numpy())[0]
+        t, v = rec.seismogram(i)
+        exp = np.array([np.einsum("qmr,m->qr", orc.eval_time_pred(dg, tau, der), basis[i]) for tau in t])
+        err = np.linalg.norm(v - exp) / np.linalg.norm(exp)
+        assert err <= tol, (i, err)
+    with pytest.raises(Exception):
+        rec.sample(d, dt, dt)  # a viscoelastic solver's receivers need the anelastic DOFs
+    rec.close()
+    s.close()
