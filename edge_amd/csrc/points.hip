@@ -397,7 +397,7 @@ extern "C" int edge_points_destroy(edge_points* p) {
   for (void* d : {(void*)p->el, (void*)p->basis, (void*)p->tau, (void*)p->sel_el, (void*)p->sel_ptr, (void*)p->run,
                   (void*)p->t0, (void*)p->dts, (void*)p->bev, (void*)p->sca, (void*)p->ts_ptr, (void*)p->ts,
                   (void*)p->ints})
-    if (d) hipFree(d);
+    if (d) (void)hipFree(d);
   delete p;
   return EDGE_OK;
 }

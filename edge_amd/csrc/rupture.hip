@@ -323,7 +323,7 @@ extern "C" int edge_rupture_create(const edge_rupture_config* cfg, int64_t n_fac
   if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost);
   for (void* p : {(void*)d_nrm, (void*)d_tan, (void*)d_area, (void*)d_bnd, (void*)d_ml, (void*)d_mr, (void*)d_vl,
                   (void*)d_vr, (void*)d_bad})
-    hipFree(p);
+    (void)hipFree(p);
   int rc = EDGE_OK;
   if (e != hipSuccess) rc = fail(EDGE_ERR_HIP, std::string("rupture set-up: ") + hipGetErrorString(e));
   else if (bad) rc = fail(EDGE_ERR_MESH, "rupture: a fault face's normal is perpendicular to the fault normal");
@@ -337,11 +337,11 @@ extern "C" int edge_rupture_create(const edge_rupture_config* cfg, int64_t n_fac
 
 extern "C" int edge_rupture_destroy(edge_rupture* s) {
   if (!s) return EDGE_OK;
-  hipFree(s->solv);
-  hipFree(s->face);
-  hipFree(s->glob);
-  hipFree(s->state);
-  hipFree(s->per);
+  (void)hipFree(s->solv);
+  (void)hipFree(s->face);
+  (void)hipFree(s->glob);
+  (void)hipFree(s->state);
+  (void)hipFree(s->per);
   delete s;
   return EDGE_OK;
 }

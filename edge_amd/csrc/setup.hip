@@ -396,9 +396,9 @@ int build_setup(const SetupInput& in, T* d_star, T* d_fs, double* d_jdet, T* d_f
   int rc = EDGE_OK;
   const int64_t ne = in.n_el, nf = in.n_fa;
   auto cleanup = [&]() {
-    hipFree(d_crds); hipFree(d_rho); hipFree(d_lam); hipFree(d_mu); hipFree(d_nrm); hipFree(d_tan);
-    hipFree(d_area); hipFree(d_elve); hipFree(d_fael); hipFree(d_elfa); hipFree(d_type);
-    hipFree(d_bad);
+    (void)hipFree(d_crds); (void)hipFree(d_rho); (void)hipFree(d_lam); (void)hipFree(d_mu); (void)hipFree(d_nrm); (void)hipFree(d_tan);
+    (void)hipFree(d_area); (void)hipFree(d_elve); (void)hipFree(d_fael); (void)hipFree(d_elfa); (void)hipFree(d_type);
+    (void)hipFree(d_bad);
   };
 #define EDGE_SETUP_TRY(x)                                                                         \
   do {                                                                                            \
@@ -476,8 +476,8 @@ int factor_flux_solvers(const double* h_fs, int64_t n_mats, T* d_fsf, bool* rank
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost);
-  hipFree(d_fs);
-  hipFree(d_bad);
+  (void)hipFree(d_fs);
+  (void)hipFree(d_bad);
   if (e != hipSuccess) return fail(EDGE_ERR_HIP, std::string("flux-solver factors: ") + hipGetErrorString(e));
   *rank3 = !(bad & 8);
   return EDGE_OK;

@@ -290,7 +290,7 @@ int neigh_t(edge_el_solver* s, int64_t first, int64_t n, const void* t_int, void
 void drop_lts(edge_el_solver* s) {
   for (void* p : {(void*)s->fa_lts, (void*)s->ne_lts, (void*)s->rcv_lts, (void*)s->lts_lt, (void*)s->lts_gt,
                   (void*)s->lts_if})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   s->fa_lts = nullptr;
   s->ne_lts = s->rcv_lts = nullptr;
   s->lts_lt = s->lts_gt = s->lts_if = nullptr;
@@ -358,8 +358,8 @@ int one_step(edge_el_solver* s, double dt, void* dofs, void* dofs_a, void* t_int
 }
 
 void drop_graph(edge_el_solver* s) {
-  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
-  if (s->graph) hipGraphDestroy(s->graph);
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
   s->graph_exec = nullptr;
   s->graph = nullptr;
 }
@@ -469,26 +469,26 @@ extern "C" int edge_el_create_raw(const edge_el_config* cfg, const edge_mesh_in*
 extern "C" int edge_el_destroy(edge_el_solver* s) {
   if (!s) return EDGE_OK;
   drop_graph(s);
-  hipFree(s->mats);
-  hipFree(s->pk);
-  hipFree(s->star);
-  hipFree(s->fs);
-  hipFree(s->ne);
-  hipFree(s->fmat);
-  hipFree(s->jdet);
-  hipFree(s->mass);
-  hipFree(s->mat);
-  hipFree(s->geo);
-  hipFree(s->fsf);
-  hipFree(s->proj);
-  hipFree(s->snd_v);
-  hipFree(s->snd_fm);
-  hipFree(s->rcv);
-  hipFree(s->rfs);
-  hipFree(s->star_a);
-  hipFree(s->src_a);
-  hipFree(s->fs_a);
-  hipFree(s->fsfa);
+  (void)hipFree(s->mats);
+  (void)hipFree(s->pk);
+  (void)hipFree(s->star);
+  (void)hipFree(s->fs);
+  (void)hipFree(s->ne);
+  (void)hipFree(s->fmat);
+  (void)hipFree(s->jdet);
+  (void)hipFree(s->mass);
+  (void)hipFree(s->mat);
+  (void)hipFree(s->geo);
+  (void)hipFree(s->fsf);
+  (void)hipFree(s->proj);
+  (void)hipFree(s->snd_v);
+  (void)hipFree(s->snd_fm);
+  (void)hipFree(s->rcv);
+  (void)hipFree(s->rfs);
+  (void)hipFree(s->star_a);
+  (void)hipFree(s->src_a);
+  (void)hipFree(s->fs_a);
+  (void)hipFree(s->fsfa);
   drop_lts(s);
   delete s;
   return EDGE_OK;
@@ -629,8 +629,8 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
       } else if (cfg->kernel == 2 || cfg->kernel == 3) {
         rc = edge::fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: flux solvers not of rank 3");
       } else {  // the generic kernels
-        hipFree(s->fsf);
-        hipFree(s->fsfa);
+        (void)hipFree(s->fsf);
+        (void)hipFree(s->fsfa);
         s->fsf = s->fsfa = nullptr;
         s->fs_rank3 = false;
       }
@@ -799,11 +799,11 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
       }
     }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  hipFree(dofs);
-  hipFree(t_int);
-  hipFree(dofs_a);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(dofs);
+  (void)hipFree(t_int);
+  (void)hipFree(dofs_a);
   if (rc != EDGE_OK) {
     s->kernel = old_kernel;
     s->use_proj = old_proj;
@@ -961,7 +961,7 @@ int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* d
       if (e != hipSuccess) rc = edge::fail(EDGE_ERR_HIP, hipGetErrorString(e));
     }
     s->graph = g;
-    if (own) hipStreamDestroy(cap);
+    if (own) (void)hipStreamDestroy(cap);
     if (rc) {
       drop_graph(s);
       return rc;
@@ -1076,7 +1076,7 @@ extern "C" int edge_el_l2_norms(edge_el_solver* s, const void* a, int prec_a, co
   } else if (rc == EDGE_OK) {
     rc = edge::fail(EDGE_ERR_HIP, "norm reduction");
   }
-  hipFree(d_part);
+  (void)hipFree(d_part);
   return rc;
 }
 
@@ -1146,7 +1146,7 @@ extern "C" int edge_el_energy(edge_el_solver* s, const void* dofs, double* kinet
       hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     rc = edge::fail(EDGE_ERR_HIP, "energy reduction");
-  hipFree(d_part);
+  (void)hipFree(d_part);
   if (rc) return rc;
   for (int r = 0; r < R; r++) {
     double k = 0, e = 0;
@@ -1260,11 +1260,11 @@ extern "C" int edge_el_error_norms(edge_el_solver* s, const void* dofs, int32_t 
         !ok(hipStreamSynchronize(st)))
       rc = edge::fail(EDGE_ERR_HIP, "error-norm reduction");
   }
-  hipFree(d_pts);
-  hipFree(d_w);
-  hipFree(d_b);
-  hipFree(d_wv);
-  hipFree(d_part);
+  (void)hipFree(d_pts);
+  (void)hipFree(d_w);
+  (void)hipFree(d_b);
+  (void)hipFree(d_wv);
+  (void)hipFree(d_part);
   if (rc) return rc;
   for (int i = 0; i < nqr; i++) {
     double a1 = 0, a2 = 0, ai = 0;
