@@ -152,3 +152,37 @@ def test_receivers_with_attenuation_match_oracle(gpu, prec, order, runs):
         rec.sample(d, dt, dt)  # a viscoelastic solver's receivers need the anelastic DOFs
     rec.close()
     s.close()
+
+
+@pytest.mark.parametrize("prec,runs", [(64, 1), (32, 8)])
+def test_sources_on_viscoelastic_solvers(gpu, prec, runs):
+    """Point sources act on the elastic DOFs of a viscoelastic solver as on an
+    elastic one (PointSources::apply adds to dofsE only): the update matches
+    orc_src_apply with the solver's own basis scaling."""
+    import torch
+    from edge_amd.solver import ViscoSolver
+    m = M.box(3, jitter=0.1, seed=13)
+    rho, lam, mu = S.random_materials(m.n_el, seed=13)
+    qp = np.full(m.n_el, 40.0)
+    s = ViscoSolver(m, 4, runs, prec, rho, lam, mu, qp, qp / 2, 3, 20.0, 10.0)
+    rng = np.random.default_rng(runs)
+    ns = 6
+    pts = rng.uniform(0.05, 0.95, size=(ns, 3))
+    src_runs = rng.integers(0, runs, ns)
+    scas = np.array([moment_tensor_scalings(rng.normal(size=(3, 3)) + rng.normal(size=(3, 3)).T) for _ in range(ns)])
+    t0 = rng.uniform(0, 0.05, ns)
+    dts = rng.uniform(0.004, 0.02, ns)
+    series = [ricker(rng.uniform(1, 5), 0.2, dts[i], int(rng.integers(20, 80))) for i in range(ns)]
+    src = PointSources(s, pts, src_runs, scas, t0, dts, series)
+    d = _dev(rng.normal(size=(m.n_el, 9, s.n_modes, runs)), prec)
+    dofs0 = d.double().cpu().numpy()
+    ref = dofs0.copy()
+    for t1, t2 in [(0.0, 0.13), (0.13, 0.5)]:
+        src.apply(t1, t2, d)
+        orc.src_apply(ref, src.el, src.runs, src.t0, src.dts, src.bev, src.scas, series, t1, t2)
+    torch.cuda.synchronize()
+    upd, upd_ref = d.double().cpu().numpy() - dofs0, ref - dofs0
+    assert np.abs(upd_ref).max() > 1e-3
+    assert np.abs(upd - upd_ref).max() / np.abs(upd_ref).max() <= (1e-13 if prec == 64 else 1e-6)
+    src.close()
+    s.close()
