@@ -15,6 +15,24 @@ struct DgDev {
   const T* fluxLN;   // [16][N][NF] local (0..3) then neighbouring (4 + v*4 + f)
   const T* fluxT;    // [4][NF][N]
   const T* pk = nullptr;  // MFMA A operands packed in lane order (mfma_pack.hpp), or null
+  // FP64 single-run kernels: the same matrices zero-padded to whole MFMA
+  // operand tiles (layout PadLayout below), or null
+  const T* pad = nullptr;
+};
+
+// Zero-padded reference-element matrices of the FP64 single-run kernels:
+// rows padded to whole k-steps of 4, columns to whole 16-wide tiles, so that
+// no B operand load needs a bounds guard. Offsets in elements:
+//   stiffTn [3][P4(N)][P16(N)] | stiffV [3][P4(N)][P16(N)] |
+//   fluxLN [16][P4(N)][P16(NF)] | fluxT [4][P4(NF)][P16(N)]
+struct PadLayout {
+  static constexpr int p4(int n) { return (n + 3) / 4 * 4; }
+  static constexpr int p16(int n) { return (n + 15) / 16 * 16; }
+  static constexpr size_t stiff(int n) { return (size_t)3 * p4(n) * p16(n); }
+  static constexpr size_t stiff_v(int n) { return stiff(n); }
+  static constexpr size_t flux_ln(int n) { return 2 * stiff(n); }
+  static constexpr size_t flux_t(int n, int nf) { return flux_ln(n) + (size_t)16 * p4(n) * p16(nf); }
+  static constexpr size_t total(int n, int nf) { return flux_t(n, nf) + (size_t)4 * p4(nf) * p16(n); }
 };
 
 // Per element-face neighbour data, precomputed from the mesh.

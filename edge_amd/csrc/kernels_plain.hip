@@ -82,15 +82,80 @@ __device__ __forceinline__ void store_tiles(double* __restrict__ x, int ldx, int
     }
 }
 
-// Y[q][k] = sum_p S[q][p] X[p][k] for k < NK (one lane per k): the elastic
-// star's non-zeros (SP) or all 81 entries; S wave-uniform
+template <int O>
+struct PlainDims {
+  static constexpr int N = n_modes(O), NF = n_face_modes(O), NT = tiles16(N), SZ = 9 * N;
+};
+
+// ---------------------------------------------------------------- elastic
+// The elastic kernels keep their LDS tiles with padded leading dimensions
+// (P4(N) for mode rows, P4(NF) for face-mode rows) and read B operands from
+// the zero-padded matrix copies (PadLayout), so no operand load needs a
+// guard: A rows 9..15 (lanes c >= 9) repeat row 8, and the D rows they
+// produce are never stored; the k-step tail of every LDS A operand (columns
+// NK to P4(NK) - 1) holds zeros, written by the operand's producer.
+
+// acc[t] += A B for A [9][4 KS] in LDS (leading dimension LDA) and B
+// [4 KS][16 NT] (leading dimension LDB, padded copy); operands loaded one
+// k-step ahead
+template <int NT, int KS, int LDA, int LDB>
+__device__ __forceinline__ void mmp(const double* __restrict__ a, const double* __restrict__ b, v4d (&acc)[NT],
+                                    int g, int c) {
+  const double* ap = a + (c < 9 ? c : 8) * LDA + g;
+  const double* bp = b + g * LDB + c;
+  double an, bn[NT];
+  auto load = [&](int s) {
+    an = ap[4 * s];
+#pragma unroll
+    for (int t = 0; t < NT; t++) bn[t] = bp[4 * s * LDB + 16 * t];
+  };
+  load(0);
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    const double av = an;
+    double bv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) bv[t] = bn[t];
+    if (s + 1 < KS) load(s + 1);
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = mfma64(av, bv[t], acc[t]);
+  }
+}
+
+// acc += A B for A [9][NK] in global memory (the reference layout, leading
+// dimension NK) and B [P4(NK)][16] (leading dimension LDB, padded copy); only
+// the last k-step's A load is guarded
+template <int NK, int LDB>
+__device__ __forceinline__ void mmg(const double* __restrict__ a, const double* __restrict__ b, v4d& acc, int g,
+                                    int c) {
+  constexpr int KS = ksteps4(NK);
+  const double* ap = a + (c < 9 ? c : 8) * NK + g;
+  const double* bp = b + g * LDB + c;
+  double an, bn;
+  auto load = [&](int s) {
+    an = (4 * s + 3 < NK) ? ap[4 * s] : ld(ap, 4 * s, 4 * s + g < NK);
+    bn = bp[4 * s * LDB];
+  };
+  load(0);
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    const double av = an, bv = bn;
+    if (s + 1 < KS) load(s + 1);
+    acc = mfma64(av, bv, acc);
+  }
+}
+
+// Y[q][k] = sum_p S[q][p] X[p][k] for k < NK, and 0 for NK <= k < P4(NK) (one
+// lane per k): the elastic star's non-zeros (SP) or all 81 entries; S
+// wave-uniform; X and Y leading dimensions at least P4(NK)
 template <bool SP, int NK>
-__device__ __forceinline__ void mix(const double* __restrict__ sd, const double* __restrict__ x, int ldx,
-                                    double* __restrict__ y, int ldy, int lane) {
-  for (int k = lane; k < NK; k += 64) {
+__device__ __forceinline__ void mixp(const double* __restrict__ sd, const double* __restrict__ x, int ldx,
+                                     double* __restrict__ y, int ldy, int lane) {
+  for (int k = lane; k < PadLayout::p4(NK); k += 64) {
+    const bool in = k < NK;
     double xv[9];
 #pragma unroll
-    for (int p = 0; p < 9; p++) xv[p] = x[p * ldx + k];
+    for (int p = 0; p < 9; p++) xv[p] = in ? x[p * ldx + k] : 0.0;
 #pragma unroll
     for (int q = 0; q < 9; q++) {
       double z = 0.0;
@@ -106,40 +171,48 @@ __device__ __forceinline__ void mix(const double* __restrict__ sd, const double*
   }
 }
 
-template <int O>
-struct PlainDims {
-  static constexpr int N = n_modes(O), NF = n_face_modes(O), NT = tiles16(N), SZ = 9 * N;
-};
+// der_DE = acc into x (columns n < NN, leading dimension LD) and tInt += sca
+// der_DE, each position by the lane that holds it
+template <int NT, int NN, int LD>
+__device__ __forceinline__ void store_der(double* __restrict__ x, double* __restrict__ ti, double sca,
+                                          const v4d (&acc)[NT], int g, int c) {
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = g + 4 * v, n = 16 * t + c;
+      if (q < 9 && n < NN) {
+        x[q * LD + n] = acc[t][v];
+        ti[q * LD + n] += sca * acc[t][v];
+      }
+    }
+}
 
-// CK stage DE (and the following ones): l_der holds der_{DE-1} (leading
-// dimension N); the stage overwrites it with der_DE once the last direction
-// has read it, and adds sca dt / (DE + 1) der_DE to tInt
+// CK stage DE (and the following ones): l_der holds der_{DE-1}; the stage
+// overwrites it with der_DE once the last direction has read it, and adds
+// sca dt / (DE + 1) der_DE to tInt (TimePredVanilla.hpp:242-306)
 template <int O, int DE, bool SP>
-__device__ __forceinline__ void ck_stage(const SolverDev<double>& s, const double* __restrict__ star, double dt,
-                                         double sca, double* l_der, double* l_a, double* l_ti, int lane, int g,
-                                         int c) {
+__device__ __forceinline__ void ck_stage(const double* __restrict__ stiff, const double* __restrict__ star,
+                                         double dt, double sca, double* l_der, double* l_a, double* l_ti, int lane,
+                                         int g, int c) {
   if constexpr (DE < O) {
-    constexpr int N = n_modes(O), NK = n_modes(O - DE + 1), NN = n_modes(O - DE), NT = tiles16(NN);
+    using P = PadLayout;
+    constexpr int N = n_modes(O), LD = P::p4(N), LQ = P::p16(N);
+    constexpr int NK = n_modes(O - DE + 1), NN = n_modes(O - DE), NT = tiles16(NN);
     v4d acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = zero4d();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      mix<SP, NK>(star + d * 81, l_der, N, l_a, N, lane);
+      mixp<SP, NK>(star + d * 81, l_der, LD, l_a, LD, lane);
       __syncthreads();
-      mm<NT, NK, NN>(l_a, N, s.dg.stiffTn + d * N * N, N, acc, g, c);
+      mmp<NT, ksteps4(NK), LD, LQ>(l_a, stiff + d * LD * LQ, acc, g, c);
       __syncthreads();
     }
-    double* cur = l_der;
-    store_tiles<NT>(cur, N, NN, acc, g, c);
-    __syncthreads();
     sca *= dt / double(DE + 1);
-    for (int i = lane; i < 9 * NN; i += 64) {
-      const int q = i / NN, n = i % NN;
-      l_ti[q * N + n] += sca * cur[q * N + n];
-    }
+    store_der<NT, NN, LD>(l_der, l_ti, sca, acc, g, c);
     __syncthreads();
-    ck_stage<O, DE + 1, SP>(s, star, dt, sca, l_der, l_a, l_ti, lane, g, c);
+    ck_stage<O, DE + 1, SP>(stiff, star, dt, sca, l_der, l_a, l_ti, lane, g, c);
   }
 }
 
@@ -148,20 +221,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 void k_local_p64(SolverDev<double> s, int64_t first, double dt, double* __restrict__ dofs,
                  double* __restrict__ t_int) {
   using D = PlainDims<O>;
+  using P = PadLayout;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ;
-  __shared__ double l_ti[SZ], l_der[SZ], l_a[SZ];
+  constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
+  __shared__ double l_ti[9 * LD], l_der[9 * LD], l_a[9 * LD];
   const int64_t el = first + blockIdx.x;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* star = s.star + el * 243;
-  for (int i = lane; i < SZ; i += 64) {
-    const double v = dofs[el * SZ + i];
-    l_der[i] = v;
-    l_ti[i] = dt * v;
+  const double* pad = s.dg.pad;
+  {
+    constexpr int IT = (SZ + 63) / 64;
+    double v[IT];
+#pragma unroll
+    for (int j = 0; j < IT; j++) v[j] = ld(dofs + el * SZ, lane + 64 * j, lane + 64 * j < SZ);
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const int i = lane + 64 * j, q = i / N, n = i - q * N;
+      if (i < SZ) {
+        l_der[q * LD + n] = v[j];
+        l_ti[q * LD + n] = dt * v[j];
+      }
+    }
+    // tInt's k-step tail: the face projections read it as an A operand
+    if constexpr (LD > N)
+      for (int i = lane; i < 9 * (LD - N); i += 64) l_ti[(i / (LD - N)) * LD + N + i % (LD - N)] = 0.0;
   }
   __syncthreads();
   // ---- CK: der_de = sum_d (S_d der_{de-1}) (-K_d^T M^-1), tInt += dt^{de+1}/(de+1)! der_de
-  ck_stage<O, 1, SP>(s, star, dt, dt, l_der, l_a, l_ti, lane, g, c);
-  for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[i];
+  ck_stage<O, 1, SP>(pad, star, dt, dt, l_der, l_a, l_ti, lane, g, c);
+  for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[(i / N) * LD + i % N];
 
   // ---- volume: dofs += sum_d (S_d tInt) (K_d M^-1), from Q read again
   v4d acc[NT];
@@ -174,22 +262,22 @@ void k_local_p64(SolverDev<double> s, int64_t first, double dt, double* __restri
     }
   constexpr int NV = n_modes(O - 1);
   for (int d = 0; d < 3; d++) {
-    mix<SP, NV>(star + d * 81, l_ti, N, l_a, N, lane);
+    mixp<SP, NV>(star + d * 81, l_ti, LD, l_a, LD, lane);
     __syncthreads();
-    mm<NT, NV, N>(l_a, N, s.dg.stiffV + d * N * N, N, acc, g, c);
+    mmp<NT, ksteps4(NV), LD, LQ>(l_a, pad + P::stiff_v(N) + d * LD * LQ, acc, g, c);
     __syncthreads();
   }
   // ---- local surface: dofs += ((fs0_fa (tInt F^L_fa)) F^T_fa)
-  double* l_s0 = l_der;  // [9][NF]
+  double* l_s0 = l_der;  // [9][LDF]
   const double* fs0 = s.fs + el * 324;
   for (int fa = 0; fa < 4; fa++) {
     v4d s0[1] = {zero4d()};
-    mm<1, N, NF>(l_ti, N, s.dg.fluxLN + fa * N * NF, NF, s0, g, c);
-    store_tiles<1>(l_s0, NF, NF, s0, g, c);
+    mmp<1, ksteps4(N), LD, LQF>(l_ti, pad + P::flux_ln(N) + fa * LD * LQF, s0, g, c);
+    store_tiles<1>(l_s0, LDF, NF, s0, g, c);
     __syncthreads();
-    mix<false, NF>(fs0 + fa * 81, l_s0, NF, l_a, NF, lane);
+    mixp<false, NF>(fs0 + fa * 81, l_s0, LDF, l_a, LDF, lane);
     __syncthreads();
-    mm<NT, NF, N>(l_a, NF, s.dg.fluxT + fa * NF * N, N, acc, g, c);
+    mmp<NT, ksteps4(NF), LDF, LQ>(l_a, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
     __syncthreads();
   }
 #pragma unroll
@@ -205,10 +293,13 @@ template <int O>
 __global__ __launch_bounds__(64) void k_neigh_p64(SolverDev<double> s, int64_t first,
                                                   const double* __restrict__ t_int, double* __restrict__ dofs) {
   using D = PlainDims<O>;
+  using P = PadLayout;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ;
-  __shared__ double l_s0[9 * NF], l_s1[9 * NF];
+  constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
+  __shared__ double l_s0[9 * LDF], l_s1[9 * LDF];
   const int64_t el = first + blockIdx.x;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const double* pad = s.dg.pad;
   v4d acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++)
@@ -222,12 +313,12 @@ __global__ __launch_bounds__(64) void k_neigh_p64(SolverDev<double> s, int64_t f
     if (ne < 0) continue;  // outflow: no neighbouring contribution (uniform per wave)
     const int fm = s.nb.fmat[el * 4 + fa];
     v4d s0[1] = {zero4d()};
-    mm<1, N, NF>(t_int + (int64_t)ne * SZ, N, s.dg.fluxLN + fm * N * NF, NF, s0, g, c);
-    store_tiles<1>(l_s0, NF, NF, s0, g, c);
+    mmg<N, LQF>(t_int + (int64_t)ne * SZ, pad + P::flux_ln(N) + fm * LD * LQF, s0[0], g, c);
+    store_tiles<1>(l_s0, LDF, NF, s0, g, c);
     __syncthreads();
-    mix<false, NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, l_s0, NF, l_s1, NF, lane);
+    mixp<false, NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, l_s0, LDF, l_s1, LDF, lane);
     __syncthreads();
-    mm<NT, NF, N>(l_s1, NF, s.dg.fluxT + fa * NF * N, N, acc, g, c);
+    mmp<NT, ksteps4(NF), LDF, LQ>(l_s1, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
     __syncthreads();
   }
 #pragma unroll

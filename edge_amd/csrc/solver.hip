@@ -23,7 +23,7 @@ struct edge_el_solver {
   int kernel = 1;  // 1 generic, 2 MFMA-tiled
   size_t real_bytes = 8;
   // device allocations
-  void* mats = nullptr;  // stiffTn | stiffV | fluxLN | fluxT  (precision T)
+  void* mats = nullptr;  // stiffTn | stiffV | fluxLN | fluxT  (precision T); FP64: then edge::PadLayout
   float* pk = nullptr;   // MFMA-tiled path: A operands packed in lane order
   void* star = nullptr;  // [nEl][3][81]
   void* fs = nullptr;    // [2][nEl][4][81]
@@ -87,6 +87,7 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.dg.fluxLN = m + 6 * N * N;
   d.dg.fluxT = m + 6 * N * N + 16 * N * NF;
   if constexpr (sizeof(T) == 4) d.dg.pk = s->pk;
+  if constexpr (sizeof(T) == 8) d.dg.pad = m + 6 * N * N + 16 * N * NF + 4 * NF * N;
   d.nb.ne = s->ne;
   d.nb.fmat = s->fmat;
   d.star = static_cast<const T*>(s->star);
@@ -111,14 +112,32 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
 template <typename T>
 int upload_mats(edge_el_solver* s, const edge::DgTables& t) {
   const size_t N = t.n, NF = t.nf;
-  const size_t total = 6 * N * N + 16 * N * NF + 4 * NF * N;
-  std::vector<T> h(total);
+  const size_t base = 6 * N * N + 16 * N * NF + 4 * NF * N;
+  // FP64: the zero-padded copies of the FP64 single-run kernels follow
+  const size_t total = base + (sizeof(T) == 8 ? edge::PadLayout::total((int)N, (int)NF) : 0);
+  std::vector<T> h(total, T(0));
   size_t o = 0;
   for (size_t i = 0; i < 3 * N * N; i++) h[o++] = (T)(-t.stiffT[i]);
   for (size_t i = 0; i < 3 * N * N; i++) h[o++] = (T)t.stiffV[i];
   for (size_t i = 0; i < 4 * N * NF; i++) h[o++] = (T)t.fluxL[i];
   for (size_t i = 0; i < 12 * N * NF; i++) h[o++] = (T)t.fluxN[i];
   for (size_t i = 0; i < 4 * NF * N; i++) h[o++] = (T)t.fluxT[i];
+  if (sizeof(T) == 8) {
+    using P = edge::PadLayout;
+    const int n = (int)N, nf = (int)NF;
+    // mats [nm][rows][cols] (dense, starting at src) into [nm][P4(rows)][P16(cols)] at dst
+    auto pad = [&](size_t src, size_t dst, int nm, int rows, int cols) {
+      for (int m = 0; m < nm; m++)
+        for (int r = 0; r < rows; r++)
+          for (int c = 0; c < cols; c++)
+            h[base + dst + ((size_t)m * P::p4(rows) + r) * P::p16(cols) + c] =
+                h[src + ((size_t)m * rows + r) * cols + c];
+    };
+    pad(0, 0, 3, n, n);
+    pad(3 * N * N, P::stiff_v(n), 3, n, n);
+    pad(6 * N * N, P::flux_ln(n), 16, n, nf);
+    pad(6 * N * N + 16 * N * NF, P::flux_t(n, nf), 4, nf, n);
+  }
   EDGE_HIP_TRY(hipMalloc(&s->mats, sizeof(T) * total));
   EDGE_HIP_TRY(hipMemcpy(s->mats, h.data(), sizeof(T) * total, hipMemcpyHostToDevice));
   if (s->kernel == 2) {
