@@ -42,33 +42,6 @@ __device__ __forceinline__ double ld(const double* __restrict__ p, int idx, bool
   return ok ? v : 0.0;
 }
 
-// acc[t] += A B for A [MR][NK] (leading dimension lda, LDS or global) and
-// B [NK][NN] (leading dimension ldb, global), output tiles t < NT; the
-// operands of a k-step are loaded one k-step ahead
-template <int NT, int NK, int NN, int MR = 9>
-__device__ __forceinline__ void mm(const double* __restrict__ a, int lda, const double* __restrict__ b, int ldb,
-                                   v4d (&acc)[NT], int g, int c) {
-  constexpr int KS = ksteps4(NK);
-  double an, bn[NT];
-  auto load = [&](int s) {
-    const int k = 4 * s + g;
-    an = ld(a, c * lda + k, c < MR && k < NK);
-#pragma unroll
-    for (int t = 0; t < NT; t++) bn[t] = ld(b, k * ldb + 16 * t + c, k < NK && 16 * t + c < NN);
-  };
-  load(0);
-#pragma unroll
-  for (int s = 0; s < KS; s++) {
-    const double av = an;
-    double bv[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) bv[t] = bn[t];
-    if (s + 1 < KS) load(s + 1);
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = mfma64(av, bv[t], acc[t]);
-  }
-}
-
 // X[q][n] = acc (rows q0 <= q < MR, columns n < nn) into LDS
 template <int NT, int MR = 9, int Q0 = 0>
 __device__ __forceinline__ void store_tiles(double* __restrict__ x, int ldx, int nn, const v4d (&acc)[NT], int g,
@@ -88,20 +61,20 @@ struct PlainDims {
 };
 
 // ---------------------------------------------------------------- elastic
-// The elastic kernels keep their LDS tiles with padded leading dimensions
-// (P4(N) for mode rows, P4(NF) for face-mode rows) and read B operands from
-// the zero-padded matrix copies (PadLayout), so no operand load needs a
-// guard: A rows 9..15 (lanes c >= 9) repeat row 8, and the D rows they
-// produce are never stored; the k-step tail of every LDS A operand (columns
+// The kernels keep their LDS tiles with padded leading dimensions (P4(N) for
+// mode rows, P4(NF) for face-mode rows) and read B operands from the
+// zero-padded matrix copies (PadLayout), so no operand load needs a guard:
+// A rows past the operand's last (lanes c >= 9, or 15 with attenuation)
+// repeat that row, and the D rows they produce are never stored; the k-step tail of every LDS A operand (columns
 // NK to P4(NK) - 1) holds zeros, written by the operand's producer.
 
-// acc[t] += A B for A [9][4 KS] in LDS (leading dimension LDA) and B
+// acc[t] += A B for A [MR][4 KS] in LDS (leading dimension LDA) and B
 // [4 KS][16 NT] (leading dimension LDB, padded copy); operands loaded one
 // k-step ahead
-template <int NT, int KS, int LDA, int LDB>
+template <int NT, int KS, int LDA, int LDB, int MR = 9>
 __device__ __forceinline__ void mmp(const double* __restrict__ a, const double* __restrict__ b, v4d (&acc)[NT],
                                     int g, int c) {
-  const double* ap = a + (c < 9 ? c : 8) * LDA + g;
+  const double* ap = a + (c < MR ? c : MR - 1) * LDA + g;
   const double* bp = b + g * LDB + c;
   double an, bn[NT];
   auto load = [&](int s) {
@@ -348,15 +321,16 @@ __global__ __launch_bounds__(64) void k_neigh_p64(SolverDev<double> s, int64_t f
 // the neighbour step adds w_rm upA_nb.
 
 // Y rows 0..8 = S X (elastic star pattern or all 81 entries), rows 9..14 =
-// S^A X_vel, for columns k < NK (one lane per k)
+// S^A X_vel, for columns k < NK, and 0 for NK <= k < P4(NK) (one lane per k)
 template <bool SP, int NK>
 __device__ __forceinline__ void mix15(const double* __restrict__ sd, const double* __restrict__ sa,
                                       const double* __restrict__ x, int ldx, double* __restrict__ y, int ldy,
                                       int lane) {
-  for (int k = lane; k < NK; k += 64) {
+  for (int k = lane; k < PadLayout::p4(NK); k += 64) {
+    const bool in = k < NK;
     double xv[9];
 #pragma unroll
-    for (int p = 0; p < 9; p++) xv[p] = x[p * ldx + k];
+    for (int p = 0; p < 9; p++) xv[p] = in ? x[p * ldx + k] : 0.0;
 #pragma unroll
     for (int q = 0; q < 9; q++) {
       double z = 0.0;
@@ -375,7 +349,7 @@ __device__ __forceinline__ void mix15(const double* __restrict__ sd, const doubl
 }
 
 // Y rows 0..8 = F X, rows 9..14 = F_A X for a flux solver F (9 x 9) and its
-// anelastic part F_A (6 x 9), columns k < NK
+// anelastic part F_A (6 x 9), columns k < NK, and 0 for NK <= k < P4(NK)
 // (coefficients row by row through scalar loads: all 135 held at once
 // would not fit the registers)
 typedef const __attribute__((address_space(4))) double cdouble;
@@ -383,10 +357,11 @@ template <int NK>
 __device__ __forceinline__ void mixf15(const double* __restrict__ f, const double* __restrict__ fa,
                                        const double* __restrict__ x, int ldx, double* __restrict__ y, int ldy,
                                        int lane) {
-  for (int k = lane; k < NK; k += 64) {
+  for (int k = lane; k < PadLayout::p4(NK); k += 64) {
+    const bool in = k < NK;
     double xv[9];
 #pragma unroll
-    for (int p = 0; p < 9; p++) xv[p] = x[p * ldx + k];
+    for (int p = 0; p < 9; p++) xv[p] = in ? x[p * ldx + k] : 0.0;
 #pragma unroll 1
     for (int q = 0; q < 15; q++) {
       const cdouble* row = (const cdouble*)(q < 9 ? f + q * 9 : fa + (q - 9) * 9);
@@ -421,22 +396,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restrict__ dofs,
                   double* __restrict__ dofs_a, double* __restrict__ t_int) {
   using D = PlainDims<O>;
+  using P = PadLayout;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ, SZA = 6 * N;
   constexpr int NV = n_modes(O - 1), TV = tiles16(NV);
+  constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
   static_assert(N <= 64, "one lane per mode");
   // l_x: der (rows 0..8) and the anelastic combination (rows 9..14); l_a: A
   // operands; l_ta: tIntA [rm][6][mode]
-  __shared__ double l_ti[SZ], l_x[15 * N], l_a[15 * N], l_ta[NRM * 6 * N];
+  __shared__ double l_ti[9 * LD], l_x[15 * LD], l_a[15 * LD], l_ta[NRM * 6 * N];
   const int64_t el = first + blockIdx.x;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* star = s.star + el * 243;
   const double* star_a = s.star_a + el * 54;
   const double* src = s.src_a + el * NRM * 36;
+  const double* pad = s.dg.pad;
   double* qa = dofs_a + el * NRM * SZA;
-  for (int i = lane; i < SZ; i += 64) {
-    const double v = dofs[el * SZ + i];
-    l_x[i] = v;
-    l_ti[i] = dt * v;
+  {
+    constexpr int IT = (SZ + 63) / 64;
+    double v[IT];
+#pragma unroll
+    for (int j = 0; j < IT; j++) v[j] = ld(dofs + el * SZ, lane + 64 * j, lane + 64 * j < SZ);
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const int i = lane + 64 * j, q = i / N, n = i - q * N;
+      if (i < SZ) {
+        l_x[q * LD + n] = v[j];
+        l_ti[q * LD + n] = dt * v[j];
+      }
+    }
+    // tInt's k-step tail: the face projections read it as an A operand
+    if constexpr (LD > N)
+      for (int i = lane; i < 9 * (LD - N); i += 64) l_ti[(i / (LD - N)) * LD + N + i % (LD - N)] = 0.0;
   }
   // anelastic state of mode n = lane: derA in registers, tIntA in LDS
   double da[NRM][6];
@@ -458,21 +448,21 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
     for (int t = 0; t < TV; t++) acc[t] = zero4d();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      mix15<SP, N>(star + opaque_u(d * 81), star_a + opaque_u(d * 18), l_x, N, l_a, N, lane);
+      mix15<SP, N>(star + opaque_u(d * 81), star_a + opaque_u(d * 18), l_x, LD, l_a, LD, lane);
       __syncthreads();
-      mm<TV, N, NV, 15>(l_a, N, s.dg.stiffTn + d * N * N, N, acc, g, c);
+      mmp<TV, ksteps4(N), LD, LQ, 15>(l_a, pad + d * LD * LQ, acc, g, c);
       __syncthreads();
     }
-    store_tiles<TV, 15>(l_x, N, NV, acc, g, c);
+    store_tiles<TV, 15>(l_x, LD, NV, acc, g, c);
     __syncthreads();
     sca *= dt / double(de + 1);
     if (lane < N) {
       const int n = lane;
       double y[9], a[6];
 #pragma unroll
-      for (int q = 0; q < 9; q++) y[q] = n < NV ? l_x[q * N + n] : 0.0;
+      for (int q = 0; q < 9; q++) y[q] = n < NV ? l_x[q * LD + n] : 0.0;
 #pragma unroll
-      for (int j = 0; j < 6; j++) a[j] = n < NV ? l_x[(9 + j) * N + n] : 0.0;
+      for (int j = 0; j < 6; j++) a[j] = n < NV ? l_x[(9 + j) * LD + n] : 0.0;
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++) {
         add_src(src + opaque_u(rm * 36), da[rm], y);
@@ -485,13 +475,13 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        l_x[q * N + n] = y[q];
-        l_ti[q * N + n] += sca * y[q];
+        l_x[q * LD + n] = y[q];
+        l_ti[q * LD + n] += sca * y[q];
       }
     }
     __syncthreads();
   }
-  for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[i];
+  for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[(i / N) * LD + i % N];
 
   // ---- volume: rows 0..8 from Q, rows 9..14 the anelastic combination A
   v4d acc[NT];
@@ -503,9 +493,9 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
       acc[t][v] = ld(dofs + el * SZ, q * N + n, q < 9 && n < N);
     }
   for (int d = 0; d < 3; d++) {
-    mix15<SP, NV>(star + d * 81, star_a + d * 18, l_ti, N, l_a, N, lane);
+    mix15<SP, NV>(star + d * 81, star_a + d * 18, l_ti, LD, l_a, LD, lane);
     __syncthreads();
-    mm<NT, NV, N, 15>(l_a, N, s.dg.stiffV + d * N * N, N, acc, g, c);
+    mmp<NT, ksteps4(NV), LD, LQ, 15>(l_a, pad + P::stiff_v(N) + d * LD * LQ, acc, g, c);
     __syncthreads();
   }
   // sum_rm E_rm tIntA_rm into the stress rows (through LDS)
@@ -519,7 +509,7 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
       add_src(src + rm * 36, ta, y);
     }
 #pragma unroll
-    for (int j = 0; j < 6; j++) l_x[j * N + lane] = y[j];
+    for (int j = 0; j < 6; j++) l_x[j * LD + lane] = y[j];
   }
   __syncthreads();
 #pragma unroll
@@ -527,21 +517,21 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int q = g + 4 * v, n = 16 * t + c;
-      if (q < 6 && n < N) acc[t][v] += l_x[q * N + n];
+      if (q < 6 && n < N) acc[t][v] += l_x[q * LD + n];
     }
   __syncthreads();
   // ---- local surface
-  double* l_s0 = l_x;  // [9][NF]
+  double* l_s0 = l_x;  // [9][LDF]
   const double* fs0 = s.fs + el * 324;
   const double* fsa0 = s.fs_a + el * 4 * 54;
   for (int fa = 0; fa < 4; fa++) {
     v4d s0[1] = {zero4d()};
-    mm<1, N, NF>(l_ti, N, s.dg.fluxLN + fa * N * NF, NF, s0, g, c);
-    store_tiles<1>(l_s0, NF, NF, s0, g, c);
+    mmp<1, ksteps4(N), LD, LQF>(l_ti, pad + P::flux_ln(N) + fa * LD * LQF, s0, g, c);
+    store_tiles<1>(l_s0, LDF, NF, s0, g, c);
     __syncthreads();
-    mixf15<NF>(fs0 + fa * 81, fsa0 + fa * 54, l_s0, NF, l_a, NF, lane);
+    mixf15<NF>(fs0 + fa * 81, fsa0 + fa * 54, l_s0, LDF, l_a, LDF, lane);
     __syncthreads();
-    mm<NT, NF, N, 15>(l_a, NF, s.dg.fluxT + fa * NF * N, N, acc, g, c);
+    mmp<NT, ksteps4(NF), LDF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
     __syncthreads();
   }
   // ---- DOFs out; rows 9..14 (A + upA) through LDS to the anelastic update
@@ -552,7 +542,7 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
       const int q = g + 4 * v, n = 16 * t + c;
       if (q < 9 && n < N) dofs[el * SZ + q * N + n] = acc[t][v];
     }
-  store_tiles<NT, 15, 9>(l_x, N, N, acc, g, c);
+  store_tiles<NT, 15, 9>(l_x, LD, N, acc, g, c);
   __syncthreads();
   if (lane < N) {
 #pragma unroll
@@ -560,7 +550,7 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
       const double w = s.rfs[rm];
 #pragma unroll
       for (int j = 0; j < 6; j++)
-        qa[(rm * 6 + j) * N + lane] += w * (l_x[(9 + j) * N + lane] - l_ta[(rm * 6 + j) * N + lane]);
+        qa[(rm * 6 + j) * N + lane] += w * (l_x[(9 + j) * LD + lane] - l_ta[(rm * 6 + j) * N + lane]);
     }
   }
 }
@@ -570,10 +560,14 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
                                                    const double* __restrict__ t_int, double* __restrict__ dofs,
                                                    double* __restrict__ dofs_a) {
   using D = PlainDims<O>;
+  using P = PadLayout;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ, SZA = 6 * N;
-  __shared__ double l_s0[9 * NF], l_a[15 * N];
+  constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
+  static_assert(LDF <= N, "l_a holds the [15][LDF] operand");
+  __shared__ double l_s0[9 * LDF], l_a[15 * N];
   const int64_t el = first + blockIdx.x;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const double* pad = s.dg.pad;
   v4d acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++)
@@ -587,13 +581,13 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
     if (ne < 0) continue;  // outflow: no neighbouring contribution (uniform per wave)
     const int fm = s.nb.fmat[el * 4 + fa];
     v4d s0[1] = {zero4d()};
-    mm<1, N, NF>(t_int + (int64_t)ne * SZ, N, s.dg.fluxLN + fm * N * NF, NF, s0, g, c);
-    store_tiles<1>(l_s0, NF, NF, s0, g, c);
+    mmg<N, LQF>(t_int + (int64_t)ne * SZ, pad + P::flux_ln(N) + fm * LD * LQF, s0[0], g, c);
+    store_tiles<1>(l_s0, LDF, NF, s0, g, c);
     __syncthreads();
-    mixf15<NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, s.fs_a + ((s.n_el + el) * 4 + fa) * 54, l_s0, NF, l_a, NF,
-               lane);
+    mixf15<NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, s.fs_a + ((s.n_el + el) * 4 + fa) * 54, l_s0, LDF, l_a,
+               LDF, lane);
     __syncthreads();
-    mm<NT, NF, N, 15>(l_a, NF, s.dg.fluxT + fa * NF * N, N, acc, g, c);
+    mmp<NT, ksteps4(NF), LDF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
     __syncthreads();
   }
 #pragma unroll
