@@ -36,6 +36,16 @@ constexpr int ksteps4(int n) { return (n + 3) / 4; }
 
 __device__ __forceinline__ v4d zero4d() { return v4d{0.0, 0.0, 0.0, 0.0}; }
 
+// blockIdx -> element offset: each of the 8 XCDs (workgroups dispatched round
+// robin) gets a contiguous element range, so neighbours share its L2 (the
+// autotuner may switch this schedule off: workgroup i -> element i).  The
+// grid then holds 8 ceil(n / 8) workgroups.
+__device__ __forceinline__ int64_t xcd_element(int64_t n) {
+  const int64_t chunk = (n + 7) / 8;
+  const int64_t b = blockIdx.x;
+  return (b & 7) * chunk + (b >> 3);
+}
+
 // Guarded loads without branches: clamped index, value masked afterwards.
 __device__ __forceinline__ double ld(const double* __restrict__ p, int idx, bool ok) {
   const double v = p[ok ? idx : 0];
@@ -191,14 +201,16 @@ __device__ __forceinline__ void ck_stage(const double* __restrict__ stiff, const
 
 template <int O, bool SP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
-void k_local_p64(SolverDev<double> s, int64_t first, double dt, double* __restrict__ dofs,
+void k_local_p64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, double* __restrict__ dofs,
                  double* __restrict__ t_int) {
   using D = PlainDims<O>;
   using P = PadLayout;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ;
   constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
   __shared__ double l_ti[9 * LD], l_der[9 * LD], l_a[9 * LD];
-  const int64_t el = first + blockIdx.x;
+  const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
+  if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
+  const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* star = s.star + el * 243;
   const double* pad = s.dg.pad;
@@ -263,14 +275,16 @@ void k_local_p64(SolverDev<double> s, int64_t first, double dt, double* __restri
 }
 
 template <int O>
-__global__ __launch_bounds__(64) void k_neigh_p64(SolverDev<double> s, int64_t first,
+__global__ __launch_bounds__(64) void k_neigh_p64(SolverDev<double> s, int64_t first, int64_t n_el,
                                                   const double* __restrict__ t_int, double* __restrict__ dofs) {
   using D = PlainDims<O>;
   using P = PadLayout;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ;
   constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
   __shared__ double l_s0[9 * LDF], l_s1[9 * LDF];
-  const int64_t el = first + blockIdx.x;
+  const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
+  if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
+  const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* pad = s.dg.pad;
   v4d acc[NT];
@@ -393,7 +407,7 @@ __device__ __forceinline__ void add_src(const double* __restrict__ e, const doub
 
 template <int O, int NRM, bool SP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restrict__ dofs,
+void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, double* __restrict__ dofs,
                   double* __restrict__ dofs_a, double* __restrict__ t_int) {
   using D = PlainDims<O>;
   using P = PadLayout;
@@ -404,7 +418,9 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
   // l_x: der (rows 0..8) and the anelastic combination (rows 9..14); l_a: A
   // operands; l_ta: tIntA [rm][6][mode]
   __shared__ double l_ti[9 * LD], l_x[15 * LD], l_a[15 * LD], l_ta[NRM * 6 * N];
-  const int64_t el = first + blockIdx.x;
+  const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
+  if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
+  const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* star = s.star + el * 243;
   const double* star_a = s.star_a + el * 54;
@@ -556,7 +572,7 @@ void k_local_pv64(SolverDev<double> s, int64_t first, double dt, double* __restr
 }
 
 template <int O, int NRM>
-__global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t first,
+__global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t first, int64_t n_el,
                                                    const double* __restrict__ t_int, double* __restrict__ dofs,
                                                    double* __restrict__ dofs_a) {
   using D = PlainDims<O>;
@@ -565,7 +581,9 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
   constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
   static_assert(LDF <= N, "l_a holds the [15][LDF] operand");
   __shared__ double l_s0[9 * LDF], l_a[15 * N];
-  const int64_t el = first + blockIdx.x;
+  const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
+  if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
+  const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* pad = s.dg.pad;
   v4d acc[NT];
@@ -610,18 +628,21 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
   }
 }
 
+// one workgroup per element, 8 ceil(n / 8) under the XCD schedule
+dim3 grid(const SolverDev<double>& s, int64_t n) { return dim3((unsigned)(s.xcd_map ? (n + 7) / 8 * 8 : n)); }
+
 template <int O, int NRM>
 int visco_o(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs, double* dofs_a,
             double* t_int, const double* tin, bool local, hipStream_t st) {
   if (local) {
     if (s.sparse_star)
-      hipLaunchKernelGGL((k_local_pv64<O, NRM, true>), dim3((unsigned)n), dim3(64), 0, st, s, first, dt, dofs, dofs_a,
+      hipLaunchKernelGGL((k_local_pv64<O, NRM, true>), grid(s, n), dim3(64), 0, st, s, first, n, dt, dofs, dofs_a,
                          t_int);
     else
-      hipLaunchKernelGGL((k_local_pv64<O, NRM, false>), dim3((unsigned)n), dim3(64), 0, st, s, first, dt, dofs,
-                         dofs_a, t_int);
+      hipLaunchKernelGGL((k_local_pv64<O, NRM, false>), grid(s, n), dim3(64), 0, st, s, first, n, dt, dofs, dofs_a,
+                         t_int);
   } else {
-    hipLaunchKernelGGL((k_neigh_pv64<O, NRM>), dim3((unsigned)n), dim3(64), 0, st, s, first, tin, dofs, dofs_a);
+    hipLaunchKernelGGL((k_neigh_pv64<O, NRM>), grid(s, n), dim3(64), 0, st, s, first, n, tin, dofs, dofs_a);
   }
   EDGE_HIP_TRY(hipGetLastError());
   return EDGE_OK;
@@ -654,9 +675,9 @@ template <int O>
 int local_o(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs, double* t_int,
             hipStream_t st) {
   if (s.sparse_star)
-    hipLaunchKernelGGL((k_local_p64<O, true>), dim3((unsigned)n), dim3(64), 0, st, s, first, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local_p64<O, true>), grid(s, n), dim3(64), 0, st, s, first, n, dt, dofs, t_int);
   else
-    hipLaunchKernelGGL((k_local_p64<O, false>), dim3((unsigned)n), dim3(64), 0, st, s, first, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local_p64<O, false>), grid(s, n), dim3(64), 0, st, s, first, n, dt, dofs, t_int);
   EDGE_HIP_TRY(hipGetLastError());
   return EDGE_OK;
 }
@@ -664,7 +685,7 @@ int local_o(const SolverDev<double>& s, int64_t first, int64_t n, double dt, dou
 template <int O>
 int neigh_o(const SolverDev<double>& s, int64_t first, int64_t n, const double* t_int, double* dofs,
             hipStream_t st) {
-  hipLaunchKernelGGL((k_neigh_p64<O>), dim3((unsigned)n), dim3(64), 0, st, s, first, t_int, dofs);
+  hipLaunchKernelGGL((k_neigh_p64<O>), grid(s, n), dim3(64), 0, st, s, first, n, t_int, dofs);
   EDGE_HIP_TRY(hipGetLastError());
   return EDGE_OK;
 }

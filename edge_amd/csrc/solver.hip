@@ -772,7 +772,7 @@ std::vector<TuneCand> tune_candidates(const edge_el_solver* s) {
       c.push_back({2, false, x == 1});
     }
   } else if (s->kernel == 4) {
-    c.push_back({4, false});
+    for (int x = 1; x >= 0; x--) c.push_back({4, false, x == 1});  // XCD-aware schedule, then the linear one
   }
   if (gen || c.empty()) c.push_back({1, false});
   return c;
