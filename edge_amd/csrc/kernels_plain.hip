@@ -52,6 +52,15 @@ __device__ __forceinline__ double ld(const double* __restrict__ p, int idx, bool
   return ok ? v : 0.0;
 }
 
+// X[q][n] = a (rows q < 9, columns n < nn) for one tile into LDS
+__device__ __forceinline__ void store_tile(double* __restrict__ x, int ldx, int nn, const v4d& a, int g, int c) {
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int q = g + 4 * v;
+    if (q < 9 && c < nn) x[q * ldx + c] = a[v];
+  }
+}
+
 // X[q][n] = acc (rows q0 <= q < MR, columns n < nn) into LDS
 template <int NT, int MR = 9, int Q0 = 0>
 __device__ __forceinline__ void store_tiles(double* __restrict__ x, int ldx, int nn, const v4d (&acc)[NT], int g,
@@ -78,10 +87,11 @@ struct PlainDims {
 // repeat that row, and the D rows they produce are never stored; the k-step tail of every LDS A operand (columns
 // NK to P4(NK) - 1) holds zeros, written by the operand's producer.
 
-// acc[t] += A B for A [MR][4 KS] in LDS (leading dimension LDA) and B
-// [4 KS][16 NT] (leading dimension LDB, padded copy); operands loaded one
-// k-step ahead
-template <int NT, int KS, int LDA, int LDB, int MR = 9>
+// acc[t] += A B_t for A [MR][4 KS] in LDS (leading dimension LDA) and the
+// B_t [4 KS][16] (leading dimension LDB, padded copy) TS elements apart: the
+// 16-column tiles of one matrix (TS = 16) or separate matrices; operands
+// loaded one k-step ahead
+template <int NT, int KS, int LDA, int LDB, int MR = 9, int TS = 16>
 __device__ __forceinline__ void mmp(const double* __restrict__ a, const double* __restrict__ b, v4d (&acc)[NT],
                                     int g, int c) {
   const double* ap = a + (c < MR ? c : MR - 1) * LDA + g;
@@ -90,7 +100,7 @@ __device__ __forceinline__ void mmp(const double* __restrict__ a, const double* 
   auto load = [&](int s) {
     an = ap[4 * s];
 #pragma unroll
-    for (int t = 0; t < NT; t++) bn[t] = bp[4 * s * LDB + 16 * t];
+    for (int t = 0; t < NT; t++) bn[t] = bp[4 * s * LDB + TS * t];
   };
   load(0);
 #pragma unroll
@@ -125,6 +135,32 @@ __device__ __forceinline__ void mmg(const double* __restrict__ a, const double* 
     const double av = an, bv = bn;
     if (s + 1 < KS) load(s + 1);
     acc = mfma64(av, bv, acc);
+  }
+}
+
+// Y_f = F_f X_f in place for the four faces' tiles X_f [9][P4(NF)] at
+// x + 9 P4(NF) f: columns k < NF, zeros for NF <= k < P4(NF); one lane per
+// (face, column), lane = P4(NF) f + k, with the flux solver F_f at f + 81 f
+// (per-lane coefficients, four distinct per wave)
+template <int NF>
+__device__ __forceinline__ void mixf4(const double* __restrict__ f, double* __restrict__ x, int lane) {
+  constexpr int LDF = PadLayout::p4(NF);
+  static_assert(4 * LDF <= 64, "one lane per face and column");
+  if (lane < 4 * LDF) {
+    const int fa = lane / LDF, k = lane - fa * LDF;
+    double* col = x + fa * 9 * LDF + k;
+    const double* fm = f + fa * 81;
+    const bool in = k < NF;
+    double xv[9];
+#pragma unroll
+    for (int p = 0; p < 9; p++) xv[p] = in ? col[p * LDF] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      double z = 0.0;
+#pragma unroll
+      for (int p = 0; p < 9; p++) z += fm[q * 9 + p] * xv[p];
+      col[q * LDF] = z;
+    }
   }
 }
 
@@ -207,7 +243,11 @@ void k_local_p64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, do
   using P = PadLayout;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ;
   constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
-  __shared__ double l_ti[9 * LD], l_der[9 * LD], l_a[9 * LD];
+  constexpr int LDA = 2 * 9 * LD > 4 * 9 * LDF ? 2 * 9 * LD : 4 * 9 * LDF;
+  // l_da: der and the A operands (CK, volume), then the four faces' tiles
+  __shared__ double l_ti[9 * LD], l_da[LDA];
+  double* l_der = l_da;
+  double* l_a = l_da + 9 * LD;
   const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
   if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
   const int64_t el = first + off;
@@ -252,19 +292,23 @@ void k_local_p64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, do
     mmp<NT, ksteps4(NV), LD, LQ>(l_a, pad + P::stiff_v(N) + d * LD * LQ, acc, g, c);
     __syncthreads();
   }
-  // ---- local surface: dofs += ((fs0_fa (tInt F^L_fa)) F^T_fa)
-  double* l_s0 = l_der;  // [9][LDF]
-  const double* fs0 = s.fs + el * 324;
-  for (int fa = 0; fa < 4; fa++) {
-    v4d s0[1] = {zero4d()};
-    mmp<1, ksteps4(N), LD, LQF>(l_ti, pad + P::flux_ln(N) + fa * LD * LQF, s0, g, c);
-    store_tiles<1>(l_s0, LDF, NF, s0, g, c);
-    __syncthreads();
-    mixp<false, NF>(fs0 + fa * 81, l_s0, LDF, l_a, LDF, lane);
-    __syncthreads();
-    mmp<NT, ksteps4(NF), LDF, LQ>(l_a, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
-    __syncthreads();
+  // ---- local surface: dofs += sum_fa ((fs0_fa (tInt F^L_fa)) F^T_fa): the
+  // four projections as independent chains on the shared A operand, one
+  // flux-solver mix for all faces, then the four lifts
+  {
+    v4d s0[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) s0[f] = zero4d();
+    mmp<4, ksteps4(N), LD, LQF, 9, LD * LQF>(l_ti, pad + P::flux_ln(N), s0, g, c);
+#pragma unroll
+    for (int f = 0; f < 4; f++) store_tile(l_da + f * 9 * LDF, LDF, NF, s0[f], g, c);
   }
+  __syncthreads();
+  mixf4<NF>(s.fs + el * 324, l_da, lane);
+  __syncthreads();
+#pragma unroll
+  for (int fa = 0; fa < 4; fa++)
+    mmp<NT, ksteps4(NF), LDF, LQ>(l_da + fa * 9 * LDF, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
 #pragma unroll
   for (int t = 0; t < NT; t++)
 #pragma unroll
