@@ -97,14 +97,17 @@ def test_fp64_steps_match_oracle(gpu, order, bc):
 
 def test_random_star_and_flux_solvers_match_oracle(gpu):
     """Dense random star matrices / flux solvers (no physical structure)
-    catch transposed or mis-indexed 9x9 products."""
+    catch transposed or mis-indexed 9x9 products.  The FP64 MFMA kernels
+    run at orders 3 and 5 too, where the modes do not fill whole k-steps
+    (10 -> 12, 35 -> 36) and the operands' zero tails matter."""
     m = M.box(3, bc=(FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC))
     rng = np.random.default_rng(5)
-    order = 4
     star = rng.normal(size=(m.n_el, 3, 81)) * 0.2
     fs = rng.normal(size=(2, m.n_el, 4, 81)) * 0.2
-    for prec, kern in [(64, KERNEL_GENERIC), (64, KERNEL_MFMA64), (32, KERNEL_GENERIC)]:
+    for prec, kern, order in [(64, KERNEL_GENERIC, 4), (64, KERNEL_MFMA64, 3), (64, KERNEL_MFMA64, 4),
+                              (64, KERNEL_MFMA64, 5), (32, KERNEL_GENERIC, 4)]:
         s = ElasticSolver(m, order, 1, prec, star=star, fs=fs, kernel=kern)
+        assert s.kernel_info()[0] == kern
         mo = orc.MeshOracle(m, order, star=star, fs=fs)
         dofs0 = rng.normal(size=(m.n_el, 9, S.n_modes(order), 1))
         d, t = dofs0.copy(), np.zeros_like(dofs0)
