@@ -191,47 +191,49 @@ __device__ __forceinline__ void mixp(const double* __restrict__ sd, const double
 }
 
 // der_DE = acc into x (columns n < NN, leading dimension LD) and tInt += sca
-// der_DE, each position by the lane that holds it
-template <int NT, int NN, int LD>
-__device__ __forceinline__ void store_der(double* __restrict__ x, double* __restrict__ ti, double sca,
-                                          const v4d (&acc)[NT], int g, int c) {
+// der_DE into the tInt tiles ti held in registers (the same D layout)
+template <int NT, int NN, int LD, int NTN>
+__device__ __forceinline__ void store_der(double* __restrict__ x, v4d (&ti)[NTN], double sca, const v4d (&acc)[NT],
+                                          int g, int c) {
 #pragma unroll
   for (int t = 0; t < NT; t++)
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int q = g + 4 * v, n = 16 * t + c;
-      if (q < 9 && n < NN) {
-        x[q * LD + n] = acc[t][v];
-        ti[q * LD + n] += sca * acc[t][v];
-      }
+      if (q < 9 && n < NN) x[q * LD + n] = acc[t][v];
+      ti[t][v] += n < NN ? sca * acc[t][v] : 0.0;
     }
 }
 
 // CK stage DE (and the following ones): l_der holds der_{DE-1}; the stage
 // overwrites it with der_DE once the last direction has read it, and adds
-// sca dt / (DE + 1) der_DE to tInt (TimePredVanilla.hpp:242-306)
-template <int O, int DE, bool SP>
+// sca dt / (DE + 1) der_DE to the tInt tiles ti (TimePredVanilla.hpp:242-306).
+// The directions' A operands alternate between two LDS buffers, so that one
+// direction's mix runs while the previous direction's products are in flight.
+template <int O, int DE, bool SP, int NTN>
 __device__ __forceinline__ void ck_stage(const double* __restrict__ stiff, const double* __restrict__ star,
-                                         double dt, double sca, double* l_der, double* l_a, double* l_ti, int lane,
-                                         int g, int c) {
+                                         double dt, double sca, double* l_der, double* l_a0, double* l_a1,
+                                         v4d (&ti)[NTN], int lane, int g, int c) {
   if constexpr (DE < O) {
     using P = PadLayout;
     constexpr int N = n_modes(O), LD = P::p4(N), LQ = P::p16(N);
-    constexpr int NK = n_modes(O - DE + 1), NN = n_modes(O - DE), NT = tiles16(NN);
+    constexpr int NK = n_modes(O - DE + 1), NN = n_modes(O - DE), NT = tiles16(NN), KS = ksteps4(NK);
     v4d acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = zero4d();
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      mixp<SP, NK>(star + d * 81, l_der, LD, l_a, LD, lane);
-      __syncthreads();
-      mmp<NT, ksteps4(NK), LD, LQ>(l_a, stiff + d * LD * LQ, acc, g, c);
-      __syncthreads();
-    }
-    sca *= dt / double(DE + 1);
-    store_der<NT, NN, LD>(l_der, l_ti, sca, acc, g, c);
+    mixp<SP, NK>(star, l_der, LD, l_a0, LD, lane);
     __syncthreads();
-    ck_stage<O, DE + 1, SP>(stiff, star, dt, sca, l_der, l_a, l_ti, lane, g, c);
+    mmp<NT, KS, LD, LQ>(l_a0, stiff, acc, g, c);
+    mixp<SP, NK>(star + 81, l_der, LD, l_a1, LD, lane);
+    __syncthreads();
+    mmp<NT, KS, LD, LQ>(l_a1, stiff + LD * LQ, acc, g, c);
+    mixp<SP, NK>(star + 162, l_der, LD, l_a0, LD, lane);
+    __syncthreads();
+    mmp<NT, KS, LD, LQ>(l_a0, stiff + 2 * LD * LQ, acc, g, c);
+    sca *= dt / double(DE + 1);
+    store_der<NT, NN, LD>(l_der, ti, sca, acc, g, c);
+    __syncthreads();
+    ck_stage<O, DE + 1, SP>(stiff, star, dt, sca, l_der, l_a0, l_a1, ti, lane, g, c);
   }
 }
 
@@ -262,19 +264,33 @@ void k_local_p64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, do
 #pragma unroll
     for (int j = 0; j < IT; j++) {
       const int i = lane + 64 * j, q = i / N, n = i - q * N;
-      if (i < SZ) {
-        l_der[q * LD + n] = v[j];
-        l_ti[q * LD + n] = dt * v[j];
-      }
+      if (i < SZ) l_der[q * LD + n] = v[j];
     }
-    // tInt's k-step tail: the face projections read it as an A operand
-    if constexpr (LD > N)
-      for (int i = lane; i < 9 * (LD - N); i += 64) l_ti[(i / (LD - N)) * LD + N + i % (LD - N)] = 0.0;
   }
+  // tInt in registers during the CK (D layout), dt Q to start with
+  v4d ti[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = g + 4 * v, n = 16 * t + c;
+      ti[t][v] = dt * ld(dofs + el * SZ, q * N + n, q < 9 && n < N);
+    }
   __syncthreads();
-  // ---- CK: der_de = sum_d (S_d der_{de-1}) (-K_d^T M^-1), tInt += dt^{de+1}/(de+1)! der_de
-  ck_stage<O, 1, SP>(pad, star, dt, dt, l_der, l_a, l_ti, lane, g, c);
-  for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[(i / N) * LD + i % N];
+  // ---- CK: der_de = sum_d (S_d der_{de-1}) (-K_d^T M^-1), tInt += dt^{de+1}/(de+1)! der_de;
+  // l_ti serves as the second A buffer until tInt is stored
+  ck_stage<O, 1, SP>(pad, star, dt, dt, l_der, l_a, l_ti, ti, lane, g, c);
+  // tInt out, and into LDS with its k-step tail (zeros: the face projections
+  // read it as an A operand)
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q = g + 4 * v, n = 16 * t + c;
+      if (q < 9 && n < N) t_int[el * SZ + q * N + n] = ti[t][v];
+      if (q < 9 && n < LD) l_ti[q * LD + n] = ti[t][v];
+    }
+  __syncthreads();
 
   // ---- volume: dofs += sum_d (S_d tInt) (K_d M^-1), from Q read again
   v4d acc[NT];
