@@ -301,13 +301,16 @@ void k_local_p64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, do
       const int q = g + 4 * v, n = 16 * t + c;
       acc[t][v] = ld(dofs + el * SZ, q * N + n, q < 9 && n < N);
     }
+  // (the directions' A operands alternate between l_a and the free der tile)
   constexpr int NV = n_modes(O - 1);
+#pragma unroll
   for (int d = 0; d < 3; d++) {
-    mixp<SP, NV>(star + d * 81, l_ti, LD, l_a, LD, lane);
+    double* a = d == 1 ? l_der : l_a;
+    mixp<SP, NV>(star + d * 81, l_ti, LD, a, LD, lane);
     __syncthreads();
-    mmp<NT, ksteps4(NV), LD, LQ>(l_a, pad + P::stiff_v(N) + d * LD * LQ, acc, g, c);
-    __syncthreads();
+    mmp<NT, ksteps4(NV), LD, LQ>(a, pad + P::stiff_v(N) + d * LD * LQ, acc, g, c);
   }
+  __syncthreads();
   // ---- local surface: dofs += sum_fa ((fs0_fa (tInt F^L_fa)) F^T_fa): the
   // four projections as independent chains on the shared A operand, one
   // flux-solver mix for all faces, then the four lifts
