@@ -480,7 +480,10 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
   static_assert(N <= 64, "one lane per mode");
   // l_x: der (rows 0..8) and the anelastic combination (rows 9..14); l_a: A
   // operands; l_ta: tIntA [rm][6][mode]
-  __shared__ double l_ti[9 * LD], l_x[15 * LD], l_a[15 * LD], l_ta[NRM * 6 * N];
+  // l_x and l_a also hold two faces' projections and mixed tiles
+  constexpr int LXX = 15 * LD > 2 * 9 * LDF ? 15 * LD : 2 * 9 * LDF;
+  constexpr int LXA = 15 * LD > 2 * 15 * LDF ? 15 * LD : 2 * 15 * LDF;
+  __shared__ double l_ti[9 * LD], l_x[LXX], l_a[LXA], l_ta[NRM * 6 * N];
   const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
   if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
   const int64_t el = first + off;
@@ -600,17 +603,22 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
     }
   __syncthreads();
   // ---- local surface
-  double* l_s0 = l_x;  // [9][LDF]
+  // (faces in pairs: the two projections as independent chains on the
+  // shared tInt operand, both flux-solver mixes, then both lifts)
   const double* fs0 = s.fs + el * 324;
   const double* fsa0 = s.fs_a + el * 4 * 54;
-  for (int fa = 0; fa < 4; fa++) {
-    v4d s0[1] = {zero4d()};
-    mmp<1, ksteps4(N), LD, LQF>(l_ti, pad + P::flux_ln(N) + fa * LD * LQF, s0, g, c);
-    store_tiles<1>(l_s0, LDF, NF, s0, g, c);
+#pragma unroll 1
+  for (int fp = 0; fp < 4; fp += 2) {
+    v4d s0[2] = {zero4d(), zero4d()};
+    mmp<2, ksteps4(N), LD, LQF, 9, LD * LQF>(l_ti, pad + P::flux_ln(N) + fp * LD * LQF, s0, g, c);
+    store_tile(l_x, LDF, NF, s0[0], g, c);
+    store_tile(l_x + 9 * LDF, LDF, NF, s0[1], g, c);
     __syncthreads();
-    mixf15<NF>(fs0 + fa * 81, fsa0 + fa * 54, l_s0, LDF, l_a, LDF, lane);
+    mixf15<NF>(fs0 + fp * 81, fsa0 + fp * 54, l_x, LDF, l_a, LDF, lane);
+    mixf15<NF>(fs0 + (fp + 1) * 81, fsa0 + (fp + 1) * 54, l_x + 9 * LDF, LDF, l_a + 15 * LDF, LDF, lane);
     __syncthreads();
-    mmp<NT, ksteps4(NF), LDF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
+    mmp<NT, ksteps4(NF), LDF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fp * LDF * LQ, acc, g, c);
+    mmp<NT, ksteps4(NF), LDF, LQ, 15>(l_a + 15 * LDF, pad + P::flux_t(N, NF) + (fp + 1) * LDF * LQ, acc, g, c);
     __syncthreads();
   }
   // ---- DOFs out; rows 9..14 (A + upA) through LDS to the anelastic update
