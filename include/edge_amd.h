@@ -382,9 +382,10 @@ int edge_el_neigh_lts(edge_el_solver *s, int64_t first, int64_t n_els, int32_t f
 int edge_el_kernel_info(const edge_el_solver *s, int32_t *kernel, int32_t *flags);
 
 /* Autotuner: times every kernel variant and schedule the solver can switch
- * to (the MFMA kernels with and without the face-projection exchange, each
- * with the XCD-aware element schedule or the linear one; the FP64 MFMA
- * kernels; the generic kernels; all parity-tested against the oracle) over
+ * to (the FP32 MFMA kernels with and without the face-projection exchange
+ * and the FP64 MFMA kernels, each with the XCD-aware element schedule or
+ * the linear one; the generic kernels; all parity-tested against the
+ * oracle) over
  * n_reps full steps of time step dt on scratch DOFs on its own mesh, and
  * keeps the fastest for the solver's later calls.  In: *n_cands = capacity
  * of the arrays (may be NULL); out: *n_cands = number of candidates,
