@@ -113,6 +113,7 @@ _SIGS = {
     "edge_rupture_destroy": (C.c_int, [C.c_void_p]),
     "edge_rupture_net_updates": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double] + [C.c_void_p] * 5),
     "edge_rupture_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "edge_el_set_variant": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "edge_el_kernel_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edge_el_l2_norms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),

@@ -130,6 +130,12 @@ struct SolverRef {
 };
 int solver_ref(const edge_el_solver* s, SolverRef* out);
 
+// Per-block partial sums of the FP64 L2 norms of DOF arrays a (and a - b):
+// part[nblk][2][9 R] = (sum |J| M_n (a - b)^2, sum |J| M_n b^2) over each
+// block's element range; every entry is written.
+int l2_partials(edge_el_solver* s, const void* a, int prec_a, const void* b, int prec_b, double* part, unsigned nblk,
+                hipStream_t st);
+
 // Receivers and point sources (points.hip).
 template <typename T>
 int launch_recv_eval(const SolverDev<T>& s, int64_t n, const int64_t* el, const double* basis, int n_tau,

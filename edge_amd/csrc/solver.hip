@@ -46,6 +46,11 @@ struct edge_el_solver {
   uint8_t* snd_fm = nullptr; // [nEl][4]
   int32_t* rcv = nullptr;    // [nEl][4]
   bool proj_ok = false;      // every projection slot feeds at most one element face
+  // The face projections stand in for the tInt array of the local step that
+  // wrote them: the neighbour step checks that it is handed the same array.
+  const void* proj_tint = nullptr;  // tInt of the local steps since the last neighbour step (null: none valid)
+  bool proj_mixed = false;          // those local steps wrote different tInt arrays
+  bool proj_fresh = false;          // a local step ran since the last neighbour step
   // viscoelastic solvers
   int n_rms = 0;
   void* rfs = nullptr;       // [n_rms]
@@ -286,10 +291,45 @@ int create_common(const edge_el_config* cfg, const edge_mesh_in* mesh, edge_el_s
   return EDGE_OK;
 }
 
+// The MFMA kernels exchange face projections between the local and the
+// neighbour step: the neighbour step then reads them instead of tInt.
+bool proj_active(const edge_el_solver* s) {
+  return s->kernel == 2 && s->fs_rank3 && s->fsf && s->use_proj && s->proj;
+}
+
+// A local step that writes projections records the tInt array it wrote.
+void note_local(edge_el_solver* s, const void* t_int) {
+  if (!proj_active(s)) return;
+  if (!s->proj_fresh) {
+    s->proj_tint = t_int;
+    s->proj_mixed = false;
+    s->proj_fresh = true;
+  } else if (s->proj_tint != t_int) {
+    s->proj_mixed = true;
+  }
+}
+
+// The neighbour step reads the projections in place of t_int: they must come
+// from local steps that wrote this very array (a substituted tInt, such as a
+// receive buffer, would be silently ignored otherwise).
+int check_neigh(edge_el_solver* s, const void* t_int) {
+  if (!proj_active(s)) return EDGE_OK;
+  if (!s->proj_tint)
+    return edge::fail(EDGE_ERR_ARG, "neighbour step: no local step has written the face projections since set-up "
+                                    "or the last autotune (run the local step first)");
+  if (s->proj_mixed || s->proj_tint != t_int)
+    return edge::fail(EDGE_ERR_ARG, "neighbour step: t_int is not the array the local step wrote; this solver "
+                                    "exchanges face projections and ignores t_int (create it with kernel 3 to read "
+                                    "a substituted tInt)");
+  s->proj_fresh = false;
+  return EDGE_OK;
+}
+
 template <typename T>
 int local_t(edge_el_solver* s, int64_t first, int64_t n, double dt, void* dofs, void* t_int, hipStream_t st) {
   if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_local_a)");
   const auto d = dev_view<T>(s);
+  note_local(s, t_int);
   if (s->kernel == 2) return edge::launch_local_mfma<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
   if constexpr (sizeof(T) == 8)
     if (s->kernel == 4) return edge::launch_local_plain64(d, first, n, dt, (double*)dofs, (double*)t_int, st);
@@ -300,6 +340,7 @@ template <typename T>
 int neigh_t(edge_el_solver* s, int64_t first, int64_t n, const void* t_int, void* dofs, hipStream_t st) {
   if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_neigh_a)");
   const auto d = dev_view<T>(s);
+  if (int rc = check_neigh(s, t_int)) return rc;
   if (s->kernel == 2) return edge::launch_neigh_mfma<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
   if constexpr (sizeof(T) == 8)
     if (s->kernel == 4) return edge::launch_neigh_plain64(d, first, n, (const double*)t_int, (double*)dofs, st);
@@ -350,7 +391,8 @@ int neigh_lts_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, const
   auto d = dev_view<T>(s);
   d.nb.ne = s->ne_lts;
   if (d.rcv) d.rcv = s->rcv_lts;
-  int rc;
+  int rc = check_neigh(s, t0);
+  if (rc) return rc;
   if (s->kernel == 2)
     rc = edge::launch_neigh_mfma<T>(d, first, n, (const T*)t0, (T*)dofs, st);
   else if constexpr (sizeof(T) == 8)
@@ -567,7 +609,7 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
   if (!mesh || !mesh->ve_crds || !mesh->el_ve || !mesh->normals || !mesh->tangents || !mesh->areas || !rho ||
       !lam || !mu || !qp || !qs)
     return edge::fail(EDGE_ERR_ARG, "mesh geometry, material or quality factors missing");
-  if (cfg->kernel > 4) return edge::fail(EDGE_ERR_UNSUPPORTED, "viscoelastic solvers: kernel 0 to 4");
+  if (cfg->kernel < 0 || cfg->kernel > 4) return edge::fail(EDGE_ERR_ARG, "kernel selector");
   const bool p64_ok = edge::plain64_visco_supported(cfg->order, cfg->n_runs, cfg->precision / 8, vc->n_rms);
   if (cfg->kernel == 4 && !p64_ok)
     return edge::fail(EDGE_ERR_UNSUPPORTED, "FP64 MFMA viscoelastic kernels: FP64, 1 run, 1-3 mechanisms");
@@ -600,9 +642,11 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
     const bool nz = row < 3 ? col < 3 : col == row;
     if (!nz && src[i] != 0) src_pattern = false;
   }
-  const bool want_vm = vm_ok && cfg->kernel != 1 && src_pattern;
+  const bool want_vm = vm_ok && cfg->kernel != 1 && cfg->kernel != 4 && src_pattern;
   if ((cfg->kernel == 2 || cfg->kernel == 3) && !want_vm)
     return edge::fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: source matrices off the pattern");
+  if (cfg->kernel == 4 && !src_pattern)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "FP64 MFMA viscoelastic kernels: source matrices off the pattern");
   edge_el_config c = *cfg;
   c.kernel = 1;
   edge_el_solver* s = nullptr;
@@ -673,9 +717,11 @@ extern "C" int edge_el_local_a(edge_el_solver* s, int64_t first, int64_t n_els, 
   if (s->kernel == 4)
     return edge::launch_local_plain64_visco(dev_view<double>(s), first, n_els, dt, (double*)dofs, (double*)dofs_a,
                                             (double*)t_int, st);
-  if (s->kernel == 2)
+  if (s->kernel == 2) {
+    note_local(s, t_int);
     return edge::launch_local_visco_mfma(dev_view<float>(s), first, n_els, (float)dt, (float*)dofs, (float*)dofs_a,
                                          (float*)t_int, st);
+  }
   if (s->cfg.precision == 64)
     return edge::launch_local_visco<double>(dev_view<double>(s), first, n_els, dt, (double*)dofs, (double*)dofs_a,
                                             (double*)t_int, st);
@@ -693,9 +739,11 @@ extern "C" int edge_el_neigh_a(edge_el_solver* s, int64_t first, int64_t n_els, 
   if (s->kernel == 4)
     return edge::launch_neigh_plain64_visco(dev_view<double>(s), first, n_els, (const double*)t_int, (double*)dofs,
                                             (double*)dofs_a, st);
-  if (s->kernel == 2)
+  if (s->kernel == 2) {
+    if ((rc = check_neigh(s, t_int))) return rc;
     return edge::launch_neigh_visco_mfma(dev_view<float>(s), first, n_els, (const float*)t_int, (float*)dofs,
                                          (float*)dofs_a, st);
+  }
   if (s->cfg.precision == 64)
     return edge::launch_neigh_visco<double>(dev_view<double>(s), first, n_els, (const double*)t_int, (double*)dofs,
                                             (double*)dofs_a, st);
@@ -823,6 +871,10 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
   (void)hipFree(dofs);
   (void)hipFree(t_int);
   (void)hipFree(dofs_a);
+  // the candidates' steps overwrote the face projections: a neighbour step
+  // pending from before must not read them
+  s->proj_tint = nullptr;
+  s->proj_fresh = false;
   if (rc != EDGE_OK) {
     s->kernel = old_kernel;
     s->use_proj = old_proj;
@@ -845,6 +897,22 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
     }
   }
   return EDGE_OK;
+}
+
+extern "C" int edge_el_set_variant(edge_el_solver* s, int32_t kernel, int32_t flags) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  for (const TuneCand& c : tune_candidates(s)) {
+    if (c.kernel != kernel) continue;
+    if (c.kernel != 1 && (c.proj != ((flags & 1) != 0) || c.xcd != ((flags & 2) != 0))) continue;
+    drop_graph(s);
+    s->kernel = c.kernel;
+    s->use_proj = c.proj;
+    s->xcd_map = c.xcd;
+    s->proj_tint = nullptr;
+    s->proj_fresh = false;
+    return EDGE_OK;
+  }
+  return edge::fail(EDGE_ERR_ARG, "not a kernel variant of this solver (see edge_el_autotune's candidates)");
 }
 
 extern "C" int edge_el_local(edge_el_solver* s, int64_t first, int64_t n_els, double dt, void* dofs, void* t_int,
@@ -970,7 +1038,11 @@ int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* d
       EDGE_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       own = true;
     }
-    EDGE_HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    const hipError_t eb = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (eb != hipSuccess) {
+      if (own) (void)hipStreamDestroy(cap);
+      return edge::fail(EDGE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(eb));
+    }
     int rc = one_step(s, dt, dofs, dofs_a, t_int, cap);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cap, &g);
@@ -1062,6 +1134,18 @@ int l2_launch(edge_el_solver* s, const void* a, const void* b, double* d_part, u
 
 }  // namespace
 
+namespace edge {
+// Per-block partial sums of the L2 norms (k_l2) into part[nblk][2][9 R].
+int l2_partials(edge_el_solver* s, const void* a, int prec_a, const void* b, int prec_b, double* part, unsigned nblk,
+                hipStream_t st) {
+  const int pb = b ? prec_b : prec_a;
+  if (prec_a == 64 && pb == 64) return l2_launch<double, double>(s, a, b, part, nblk, st);
+  if (prec_a == 64) return l2_launch<double, float>(s, a, b, part, nblk, st);
+  if (pb == 64) return l2_launch<float, double>(s, a, b, part, nblk, st);
+  return l2_launch<float, float>(s, a, b, part, nblk, st);
+}
+}  // namespace edge
+
 extern "C" int edge_el_l2_norms(edge_el_solver* s, const void* a, int prec_a, const void* b, int prec_b,
                                 double* diff_sq, double* ref_sq, void* stream) {
   if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
@@ -1073,12 +1157,7 @@ extern "C" int edge_el_l2_norms(edge_el_solver* s, const void* a, int prec_a, co
   const unsigned nblk = 1024;
   double* d_part = nullptr;
   EDGE_HIP_TRY(hipMalloc(&d_part, sizeof(double) * 2 * nqr * nblk));
-  int rc;
-  const int pb = b ? prec_b : prec_a;
-  if (prec_a == 64 && pb == 64) rc = l2_launch<double, double>(s, a, b, d_part, nblk, st);
-  else if (prec_a == 64) rc = l2_launch<double, float>(s, a, b, d_part, nblk, st);
-  else if (pb == 64) rc = l2_launch<float, double>(s, a, b, d_part, nblk, st);
-  else rc = l2_launch<float, float>(s, a, b, d_part, nblk, st);
+  int rc = edge::l2_partials(s, a, prec_a, b, prec_b, d_part, nblk, st);
   std::vector<double> h(2 * nqr * nblk);
   if (rc == EDGE_OK &&
       hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st) == hipSuccess &&

@@ -96,6 +96,11 @@ class ElasticSolver:
                                                f, ms))
         return [(k[i], bool(f[i] & 1), bool(f[i] & 2), ms[i]) for i in range(min(n.value, cap))]
 
+    def set_variant(self, kernel, projections=False, xcd_schedule=True):
+        """Switches to one of the autotuner's candidates without timing it."""
+        flags = (1 if projections else 0) | (2 if xcd_schedule else 0)
+        _lib.check(_lib.lib().edge_el_set_variant(self._h, int(kernel), flags))
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _lib.lib().edge_el_destroy(self._h)

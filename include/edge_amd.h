@@ -188,7 +188,13 @@ int edge_el_local(edge_el_solver *s, int64_t first, int64_t n_els, double dt, vo
  * also writes, per element face, the projection of the element's tInt that
  * its neighbour needs (the reference's neighFluxInt, AderDg.hpp:419-473), and
  * this step reads those instead of t_int: t_int must then be the array the
- * local steps of the neighbouring elements wrote, unchanged since. */
+ * local steps of the neighbouring elements wrote, unchanged since.  The
+ * solver records the tInt array of the local steps since the last neighbour
+ * step and returns EDGE_ERR_ARG when t_int is another array (a substituted
+ * tInt, such as the receive buffers of AderDg.hpp:675-684, needs kernel 3),
+ * when those local steps wrote different arrays, or when no local step has
+ * written the projections since set-up or the last edge_el_autotune /
+ * edge_el_set_variant. */
 int edge_el_neigh(edge_el_solver *s, int64_t first, int64_t n_els, const void *t_int, void *dofs,
                   void *stream);
 
@@ -391,10 +397,18 @@ int edge_el_kernel_info(const edge_el_solver *s, int32_t *kernel, int32_t *flags
  * of the arrays (may be NULL); out: *n_cands = number of candidates,
  * kernels[i] (as edge_el_kernel_info), flags[i] (bit 0: projections
  * exchanged, bit 1: XCD-aware schedule) and ms[i] (per step).  Drops a
- * captured step graph.  No counterpart in the reference (its back-end is a
+ * captured step graph.  Its steps overwrite the face projections: a
+ * neighbour step pending from before the call returns EDGE_ERR_ARG.  No counterpart in the reference (its back-end is a
  * compile-time choice, Kernels.hpp:30-44). */
 int edge_el_autotune(edge_el_solver *s, double dt, int32_t n_reps, void *stream, int32_t *n_cands,
                      int32_t *kernels, int32_t *flags, double *ms);
+
+/* Switches the solver to one of the candidates edge_el_autotune times
+ * (kernel and flags as it reports them; flags are ignored for the generic
+ * kernel), without timing: EDGE_ERR_ARG if the solver has no such variant.
+ * Drops a captured step graph.  For parity tests of every candidate; no
+ * counterpart in the reference. */
+int edge_el_set_variant(edge_el_solver *s, int32_t kernel, int32_t flags);
 
 /* FP64 L2 norms per quantity and run, exact through the diagonal mass
  * matrix: for device DOF arrays a (element precision prec_a: 32 or 64) and

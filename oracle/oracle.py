@@ -35,7 +35,7 @@ def lib():
             subprocess.run(["make", "-C", HERE, "oracle"], check=True, capture_output=True)
         _lib = C.CDLL(LIB)
         _lib.orc_dt_cfl.restype = C.c_double
-        _lib.orc_dt_cfl.argtypes = [C.c_double] * 5 + [C.c_int]
+        _lib.orc_dt_cfl.argtypes = [C.c_double] * 4 + [C.c_int]
         for f in ["orc_ck", "orc_vol", "orc_surf_local", "orc_surf_neigh", "orc_neigh_flux_int", "orc_local",
                   "orc_neigh", "orc_star", "orc_fs", "orc_jac", "orc_trafo", "orc_fs_mid", "orc_init_fs",
                   "orc_integrate", "orc_eval_time_pred", "orc_scatter_update_a", "orc_visco_frequencies",

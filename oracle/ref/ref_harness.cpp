@@ -5,6 +5,7 @@
 //   edge::seismic::setups::Elasticity::star  (Elasticity.cpp:167-252)
 //   edge::seismic::setups::Elasticity::fs    (Elasticity.cpp:372-441)
 //   edge::seismic::common::setupTrafo3d / setupTrafoInv3d (common.hpp:314-344)
+//   edge::seismic::common::getTimeStepCFL (common.hpp:84-98)
 //   the generated middle-state jump / flux solvers of the internal boundary
 // composed the way AderDgInit::setUpFs composes them (AderDgInit.hpp:300-370).
 #include <algorithm>
@@ -65,4 +66,11 @@ extern "C" void ref_ms_flux(const double* n, const double* s, const double* t, d
   double o_msFlux[9][9];
 #include "impl/seismic/generated/MiddleStateFlux3D.inc"
   for (int i = 0; i < 81; i++) out[i] = o_msFlux[i / 9][i % 9];
+}
+
+// The reference's CFL time step (common.hpp:84-98) for the order this
+// harness is compiled with (PP_ORDER); i_scale is SCALE_CFL = 1/2
+// (const.inc:135-137) in the reference's set-up.
+extern "C" double ref_dt_cfl(double rho, double lam, double mu, double in_dia, double scale) {
+  return edge::seismic::common::getTimeStepCFL(rho, lam, mu, in_dia, scale);
 }

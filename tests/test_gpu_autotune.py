@@ -76,3 +76,104 @@ def test_autotune_visco(gpu):
     assert rel(gd.cpu().numpy(), d) <= 1e-5
     assert rel(ga.cpu().numpy(), a) <= 1e-5
     s.close()
+
+
+def _cases():
+    return [("elastic", 32, 16), ("elastic", 64, 1), ("visco", 32, 8), ("visco", 64, 1)]
+
+
+@pytest.mark.parametrize("kind,prec,runs", _cases())
+def test_every_candidate_matches_oracle(gpu, kind, prec, runs):
+    """Each variant the autotuner may keep, pinned through edge_el_set_variant
+    and stepped against the oracle (FP64 1e-12, FP32 1e-5)."""
+    order, nr = 4, 3
+    m = M.box(3, bc=(FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC), jitter=0.1, seed=8)
+    rho, lam, mu = S.random_materials(m.n_el, seed=8)
+    n = S.n_modes(order)
+    rng = np.random.default_rng(9)
+    d0 = rng.normal(size=(m.n_el, 9, n, runs))
+    a0 = 0.1 * rng.normal(size=(m.n_el, nr, 6, n, runs)) if kind == "visco" else None
+    if kind == "visco":
+        qp = rng.uniform(20, 60, m.n_el)
+        s = ViscoSolver(m, order, runs, prec, rho, lam, mu, qp, qp / 2, nr, 20.0, 10.0)
+        visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qp / 2, nr, 20.0, 10.0)
+        dt = S.dt_cfl(m, rho, visco["lam_e"], visco["mu_e"], order)
+        mo = orc.MeshOracle(m, order, runs, visco=visco)
+    else:
+        s = ElasticSolver(m, order, runs, prec, rho, lam, mu)
+        dt = S.dt_cfl(m, rho, lam, mu, order)
+        mo = orc.MeshOracle(m, order, runs, rho=rho, lam=lam, mu=mu)
+    d, t = d0.copy(), np.zeros_like(d0)
+    a = a0.copy() if a0 is not None else None
+    mo.step(d, t, dt, 2, dofs_a=a) if a is not None else mo.step(d, t, dt, 2)
+    cands = s.autotune(dt, n_reps=1)
+    assert len(cands) >= 2
+    tol = 1e-12 if prec == 64 else 1e-5
+    for k, p, x, _ in cands:
+        s.set_variant(k, p, x)
+        assert s.kernel_info()[0] == k
+        gd, gt = _dev(d0, prec), _dev(np.zeros_like(d0), prec)
+        if a is not None:
+            ga = _dev(a0, prec)
+            s.step(gd, gt, dt, 2, ga)
+            assert rel(ga.cpu().numpy(), a) <= tol, (k, p, x)
+        else:
+            s.step(gd, gt, dt, 2)
+        assert rel(gd.cpu().numpy(), d) <= tol, (k, p, x)
+    with pytest.raises(Exception):
+        s.set_variant(7)
+    s.close()
+
+
+@pytest.mark.parametrize("kind", ["elastic", "visco"])
+def test_neigh_checks_the_tint_the_projections_stand_for(gpu, kind):
+    """With face projections exchanged the neighbour step reads them instead
+    of t_int, so it refuses a t_int other than the one the local step wrote
+    (and a step after the autotuner overwrote them); the tInt-reading variant
+    takes a substituted copy and gives the same DOFs."""
+    import torch
+    from edge_amd._lib import EdgeError
+    order, runs, nr = 4, (16 if kind == "elastic" else 8), 3
+    m = M.box(3, jitter=0.1, seed=5)
+    rho, lam, mu = S.random_materials(m.n_el, seed=5)
+    if kind == "visco":
+        qp = np.full(m.n_el, 40.0)
+        s = ViscoSolver(m, order, runs, 32, rho, lam, mu, qp, qp / 2, nr, 20.0, 10.0)
+    else:
+        s = ElasticSolver(m, order, runs, 32, rho, lam, mu)
+    dt = 0.5 * S.dt_cfl(m, rho, lam, mu, order)
+    s.set_variant(KERNEL_MFMA, projections=True)
+    assert s.kernel_info()[1] & PATH_PROJECTIONS
+    rng = np.random.default_rng(3)
+    d0 = rng.normal(size=(m.n_el, 9, S.n_modes(order), runs))
+    ga = _dev(np.zeros((m.n_el, nr, 6, S.n_modes(order), runs)), 32) if kind == "visco" else None
+
+    def local(d, t):
+        s.local(d, t, dt, dofs_a=ga) if ga is not None else s.local(d, t, dt)
+
+    def neigh(d, t):
+        s.neigh(d, t, dofs_a=ga) if ga is not None else s.neigh(d, t)
+
+    gd, gt = _dev(d0, 32), _dev(np.zeros_like(d0), 32)
+    with pytest.raises(EdgeError):  # no local step yet
+        neigh(gd, gt)
+    local(gd, gt)
+    with pytest.raises(EdgeError):  # a substituted tInt
+        neigh(gd, gt.clone())
+    neigh(gd, gt)
+    ref = gd.cpu().numpy()
+    local(gd, gt)
+    s.autotune(dt, n_reps=1)
+    s.set_variant(KERNEL_MFMA, projections=True)
+    with pytest.raises(EdgeError):  # the autotuner overwrote the projections
+        neigh(gd, gt)
+    # the tInt-reading variant takes the substituted copy
+    s.set_variant(KERNEL_MFMA, projections=False)
+    gd2, gt2 = _dev(d0, 32), _dev(np.zeros_like(d0), 32)
+    if ga is not None:
+        ga.zero_()
+    local(gd2, gt2)
+    neigh(gd2, gt2.clone())
+    torch.cuda.synchronize()
+    assert rel(gd2.cpu().numpy(), ref) <= 1e-6
+    s.close()
