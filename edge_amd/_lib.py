@@ -16,6 +16,7 @@ FA_INTERIOR = 0
 FA_FREE_SURFACE = 101
 FA_OUTFLOW = 105
 FA_PERIODIC = 106
+FA_RUPTURE = 201
 
 
 class EdgeError(RuntimeError):
@@ -113,6 +114,14 @@ _SIGS = {
     "edge_rupture_destroy": (C.c_int, [C.c_void_p]),
     "edge_rupture_net_updates": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double] + [C.c_void_p] * 5),
     "edge_rupture_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "edge_dg_fault_tables": (C.c_int, [C.c_int, C.POINTER(C.c_int32), C.c_void_p, C.c_void_p]),
+    "edge_fault_create": (C.c_int, [C.c_void_p, C.POINTER(MeshIn)] + [C.c_void_p] * 6 + [C.POINTER(C.c_void_p)]),
+    "edge_fault_destroy": (C.c_int, [C.c_void_p]),
+    "edge_fault_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "edge_fault_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "edge_fault_get": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "edge_el_set_fault": (C.c_int, [C.c_void_p, C.c_void_p]),
     "edge_el_set_variant": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "edge_el_kernel_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "edge_el_l2_norms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,

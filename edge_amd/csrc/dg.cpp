@@ -78,6 +78,9 @@ bool dg_tables(int order, DgTables* t) {
     t->fluxL = g_tet4_d##D##_fluxL;         \
     t->fluxN = g_tet4_d##D##_fluxN;         \
     t->fluxT = g_tet4_d##D##_fluxT;         \
+    t->massF = g_tet4_d##D##_massF;         \
+    t->sfMean = g_tet4_d##D##_sfMean;       \
+    t->sfGather = g_tet4_d##D##_sfGather;   \
     break;
     EDGE_CASE(1)
     EDGE_CASE(2)
@@ -90,6 +93,7 @@ bool dg_tables(int order, DgTables* t) {
   t->order = order;
   t->n = n_modes(order);
   t->nf = n_face_modes(order);
+  t->nsf = n_sub_faces(order);
   return true;
 }
 }  // namespace edge

@@ -133,6 +133,11 @@ int solver_ref(const edge_el_solver* s, SolverRef* out);
 // Per-block partial sums of the FP64 L2 norms of DOF arrays a (and a - b):
 // part[nblk][2][9 R] = (sum |J| M_n (a - b)^2, sum |J| M_n b^2) over each
 // block's element range; every entry is written.
+// The fault phase of the coupled step (fault.hip): sub-face states from
+// tInt, friction and net updates, DOF updates of both sides.
+int fault_apply(edge_fault* f, double dt, const void* t_int, void* dofs, hipStream_t st);
+const edge_el_solver* fault_solver(const edge_fault* f);
+
 int l2_partials(edge_el_solver* s, const void* a, int prec_a, const void* b, int prec_b, double* part, unsigned nblk,
                 hipStream_t st);
 

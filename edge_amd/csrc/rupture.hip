@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "face_frame.hpp"
+#include "rupture_core.hpp"
 
 struct edge_rupture {
   edge_rupture_config cfg;
@@ -37,7 +38,7 @@ struct edge_rupture {
 namespace edge {
 namespace {
 
-constexpr int kStateVals = 14;
+constexpr int kStateVals = kRuptureStateVals;
 
 __host__ __device__ void mm9(const double* a, const double* b, double* c) {
   for (int i = 0; i < 9; i++)
@@ -170,73 +171,32 @@ __global__ __launch_bounds__(256) void k_rupture_net_updates(int64_t first, int 
   for (int i = threadIdx.x; i < n_sf * R; i += blockDim.x) {
     const int r = i % R, sf = i / R;
     const int64_t base = fa * 9 * n_sf * R + (int64_t)sf * R + r, qs = (int64_t)n_sf * R;
-    double qL[9], qR[9];
-    for (int q = 0; q < 9; q++) {  // rotate to face coordinates
-      double a = 0, b = 0;
-      for (int p = 0; p < 9; p++) {
-        a += s[q * 9 + p] * dofs_l[base + p * qs];
-        b += s[q * 9 + p] * dofs_r[base + p * qs];
-      }
-      qL[q] = a;
-      qR[q] = b;
-    }
-    double ms[9];
+    double qL[9], qR[9], nL[9], nR[9];
     for (int q = 0; q < 9; q++) {
-      double v = qL[q];
-      for (int p = 0; p < 9; p++) v += s[81 + q * 9 + p] * (qR[p] - qL[p]);
-      ms[q] = v;
+      qL[q] = dofs_l[base + q * qs];
+      qR[q] = dofs_r[base + q * qs];
     }
-    // linear slip weakening: state sn0, co0, ss0[2], ss0A, muf, st, tr[2], sr[2], dd[2]
     // the face's state is [14][n_sf][R] on the device: value k of this sub-face and run at st[k * sst]
     double* st = state + (fa * kStateVals * n_sf + sf) * R + r;
-    const int64_t sst = (int64_t)n_sf * R;
-    const double mus = glob[r], mud = glob[R + r], dcInv = glob[2 * R + r];
-    const bool l_eq_m = face[0] != 0.0;
-    const double cdM = face[1], cdP = face[2];
-    double msM[9], msP[9];
-    for (int q = 0; q < 9; q++) msM[q] = msP[q] = ms[q];
-    double strength = st[5 * sst] * (st[0 * sst] + ms[0]) + st[1 * sst];
-    strength = strength < 0 ? -strength : 0.0;
-    const double s1 = st[2 * sst] + ms[3], s2 = st[3 * sst] + ms[5];
-    const double shear = sqrt(s1 * s1 + s2 * s2);
-    const bool failing = shear > strength;
-    const double scale = strength / shear;
-    const double tr0 = failing ? s1 * scale - st[2 * sst] : ms[3];
-    const double tr1 = failing ? s2 * scale - st[3 * sst] : ms[5];
-    msM[3] = msP[3] = tr0;
-    msM[5] = msP[5] = tr1;
-    const double d1 = tr0 - ms[3], d2 = tr1 - ms[5];
-    msM[7] -= cdM * d1;
-    msM[8] -= cdM * d2;
-    msP[7] += cdP * d1;
-    msP[8] += cdP * d2;
-    const double sr0 = msP[7] - msM[7], sr1 = msP[8] - msM[8];
-    const double dd0 = st[11 * sst] + fabs(sr0) * dt, dd1 = st[12 * sst] + fabs(sr1) * dt;
-    const double arg = mus - (mus - mud) * dcInv * sqrt(dd0 * dd0 + dd1 * dd1);
-    st[5 * sst] = arg > mud ? arg : mud;
-    st[6 * sst] = strength;
-    st[7 * sst] = tr0 + st[2 * sst];
-    st[8 * sst] = tr1 + st[3 * sst];
-    st[9 * sst] = sr0;
-    st[10 * sst] = sr1;
-    st[11 * sst] = dd0;
-    st[12 * sst] = dd1;
+    const bool failing = lsw_net_updates(s, face, glob[r], glob[R + r], glob[2 * R + r], dt, qL, qR, st,
+                                         (int64_t)n_sf * R, nL, nR);
     per_out[(fa * n_sf + sf) * R + r] = failing ? 1 : 0;
-    const double* mL = l_eq_m ? msM : msP;
-    const double* mR = l_eq_m ? msP : msM;
-    for (int q = 0; q < 9; q++) {  // left-going updates subtract, right-going add
-      double a = 0, b = 0;
-      for (int p = 0; p < 9; p++) {
-        a += s[162 + q * 9 + p] * mL[p];
-        b += s[243 + q * 9 + p] * mR[p];
-      }
-      net_l[base + q * qs] = -a * dt;
-      net_r[base + q * qs] = b * dt;
+    for (int q = 0; q < 9; q++) {
+      net_l[base + q * qs] = nL[q];
+      net_r[base + q * qs] = nR[q];
     }
   }
 }
 
 }  // namespace
+}  // namespace edge
+
+namespace edge {
+int rupture_dev(edge_rupture* r, RuptureDev* out) {
+  if (!r) return fail(EDGE_ERR_ARG, "rupture handle missing");
+  *out = RuptureDev{r->solv, r->face, r->glob, r->state, r->per, r->cfg.n_sub_faces, r->cfg.n_runs, r->n_fa};
+  return EDGE_OK;
+}
 }  // namespace edge
 
 extern "C" int edge_rupture_create(const edge_rupture_config* cfg, int64_t n_faces, const double* normals,

@@ -59,6 +59,9 @@ struct edge_el_solver {
   void* fs_a = nullptr;      // [2][nEl][4][54]
   void* fsfa = nullptr;      // [2][nEl][4][18] anelastic factors (MFMA viscoelastic kernels)
   std::vector<double> lam_e, mu_e, h_rfs;
+  // element faces on a fault (face type EDGE_FA_RUPTURE), el * 4 + fa
+  std::vector<int64_t> h_fault_ef;
+  edge_fault* fault = nullptr;  // attached fault: the step runs its phase between local and neighbour
   // rate-2 local time stepping
   uint8_t* fa_lts = nullptr;   // [nEl][4]: 0 equal steps, 1 this element's step greater, 2 smaller
   int32_t* ne_lts = nullptr;   // ne with the interface faces skipped
@@ -183,7 +186,13 @@ int build_neigh(edge_el_solver* s, const edge_mesh_in* m) {
       const int64_t f = m->el_fa[el * 4 + fa];
       if (f < 0 || f >= m->n_fa) return edge::fail(EDGE_ERR_MESH, "element face id out of range");
       const int32_t ty = m->fa_type[f];
-      if ((ty & EDGE_FA_OUTFLOW) == EDGE_FA_OUTFLOW) {
+      if ((ty & EDGE_FA_RUPTURE) == EDGE_FA_RUPTURE) {
+        // a fault face: its flux comes from the fault phase (sub-face net
+        // updates), neither from the Godunov solvers nor the neighbour step
+        nb[el * 4 + fa] = -1;
+        fm[el * 4 + fa] = 0;
+        s->h_fault_ef.push_back(el * 4 + fa);
+      } else if ((ty & EDGE_FA_OUTFLOW) == EDGE_FA_OUTFLOW) {
         nb[el * 4 + fa] = -1;
         fm[el * 4 + fa] = 0;
       } else if ((ty & EDGE_FA_FREE_SURFACE) == EDGE_FA_FREE_SURFACE) {
@@ -234,6 +243,43 @@ int build_neigh(edge_el_solver* s, const edge_mesh_in* m) {
 }
 
 // after the flux solvers (and their factors) exist: projection buffer
+// Zeroes both sides' Godunov solvers (and their factors) of the element
+// faces listed in ef ([n], el * 4 + fa): arr [2][nEl][4][per].
+template <typename T>
+__global__ void k_zero_faces(const int64_t* ef, int64_t n, int64_t n_el, int per, T* arr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * 2 * per) return;
+  const int64_t f = i / (2 * per), side = (i / per) % 2, k = i % per;
+  arr[(side * n_el * 4 + ef[f]) * per + k] = T(0);
+}
+
+template <typename T>
+int zero_fault_faces_t(edge_el_solver* s) {
+  const int64_t n = (int64_t)s->h_fault_ef.size();
+  int64_t* d = nullptr;
+  EDGE_HIP_TRY(hipMalloc(&d, sizeof(int64_t) * n));
+  hipError_t e = hipMemcpy(d, s->h_fault_ef.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice);
+  const std::pair<void*, int> arrs[] = {{s->fs, 81}, {s->fsf, 54}, {s->fs_a, 54}, {s->fsfa, 18}};
+  for (const auto& a : arrs) {
+    if (!a.first || e != hipSuccess) continue;
+    const int64_t tot = n * 2 * a.second;
+    hipLaunchKernelGGL(k_zero_faces<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, d, n, s->n_el,
+                       a.second, (T*)a.first);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  (void)hipFree(d);
+  if (e != hipSuccess) return edge::fail(EDGE_ERR_HIP, std::string("fault faces: ") + hipGetErrorString(e));
+  return EDGE_OK;
+}
+
+// Fault faces carry no Godunov flux: their local and neighbour solvers are
+// zero (the fault phase adds their flux instead).
+int zero_fault_faces(edge_el_solver* s) {
+  if (s->h_fault_ef.empty()) return EDGE_OK;
+  return s->cfg.precision == 64 ? zero_fault_faces_t<double>(s) : zero_fault_faces_t<float>(s);
+}
+
 int finish_mfma(edge_el_solver* s) {
   if (s->kernel == 2 && s->want_proj && s->fs_rank3 && s->proj_ok)
     EDGE_HIP_TRY(hipMalloc(&s->proj, sizeof(float) * 4 * 768 * s->n_el));
@@ -414,6 +460,7 @@ int one_step(edge_el_solver* s, double dt, void* dofs, void* dofs_a, void* t_int
   int rc = s->cfg.precision == 64 ? local_t<double>(s, 0, s->n_el, dt, dofs, t_int, st)
                                   : local_t<float>(s, 0, s->n_el, dt, dofs, t_int, st);
   if (rc) return rc;
+  if (s->fault && (rc = edge::fault_apply(s->fault, dt, t_int, dofs, st))) return rc;
   return s->cfg.precision == 64 ? neigh_t<double>(s, 0, s->n_el, t_int, dofs, st)
                                 : neigh_t<float>(s, 0, s->n_el, t_int, dofs, st);
 }
@@ -478,6 +525,7 @@ extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mes
                                            &s->fs_rank3, nullptr, nullptr, nullptr, nullptr)
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
                                           &s->fs_rank3, nullptr, nullptr, nullptr, nullptr);
+    if (rc == EDGE_OK) rc = zero_fault_faces(s);
     if (rc == EDGE_OK) rc = finish_mfma(s);
     if (rc == EDGE_OK) rc = upload_element_data(s, mesh, rho, lam, mu);
   }
@@ -517,6 +565,7 @@ extern "C" int edge_el_create_raw(const edge_el_config* cfg, const edge_mesh_in*
     if (rc == EDGE_OK && s->fsf)
       rc = cfg->precision == 64 ? edge::factor_flux_solvers<double>(fs, 8 * s->n_el, (double*)s->fsf, &s->fs_rank3)
                                 : edge::factor_flux_solvers<float>(fs, 8 * s->n_el, (float*)s->fsf, &s->fs_rank3);
+    if (rc == EDGE_OK) rc = zero_fault_faces(s);
     if (rc == EDGE_OK) rc = finish_mfma(s);
   }
   if (rc != EDGE_OK) {
@@ -680,6 +729,7 @@ extern "C" int edge_el_create_visco(const edge_el_config* cfg, const edge_visco_
                : edge::build_setup<float>(in, (float*)s->star, (float*)s->fs, s->jdet, (float*)s->fsf,
                                           &s->fs_rank3, (float*)s->star_a, (float*)s->fs_a, (float*)s->fsfa,
                                           nullptr);
+    if (rc == EDGE_OK) rc = zero_fault_faces(s);
     if (rc == EDGE_OK) rc = upload_element_data(s, mesh, rho, s->lam_e.data(), s->mu_e.data());
     if (rc == EDGE_OK && p64_ok && src_pattern && (cfg->kernel == 0 || cfg->kernel == 4)) s->kernel = 4;
     if (rc == EDGE_OK && want_vm) {
@@ -896,6 +946,15 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
       if (ms) ms[i] = t[i];
     }
   }
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_set_fault(edge_el_solver* s, edge_fault* f) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (f && edge::fault_solver(f) != s) return edge::fail(EDGE_ERR_ARG, "fault: built for another solver");
+  if (f && s->n_rms > 0) return edge::fail(EDGE_ERR_UNSUPPORTED, "fault: elastic solvers only");
+  drop_graph(s);
+  s->fault = f;
   return EDGE_OK;
 }
 

@@ -28,6 +28,11 @@ Reference behaviour restated here (no code taken from it):
     fluxN_{v*4+fa}  = (fluxL_fa M_F^-1) Ori_v,
                       Ori_v[g][f] = int_face psi_g(faToFa_v(chi)) psi_f(chi)
     fluxT_fa        = (fluxL_fa M_F^-1)^T M^-1
+* Fault sub-faces (the internal boundary's sub-face grid, src/sc/cexprs.inc:
+  51-64, ordered as tools/edge_pre/edge_pre/sc/grid/Tet.py:750-777 orders a
+  face's sub-triangles):
+    sfMean[sf][f]   = mean of psi_f over sub-triangle sf
+    sfGather        = (sfMean^T sfMean)^-1 sfMean^T  (sfGather sfMean = I)
 
 Run: ``python -m edge_amd.dg.gen_tet4 [max_degree]``.
 """
@@ -233,6 +238,63 @@ def fa_to_fa(v):
     return [[c2, c1], [rest, c2], [c1, rest]][v]
 
 
+def sub_faces(deg):
+    """The uniform sub-triangles of the unit face triangle that the fault's
+    friction is evaluated on: n = 2 deg + 1 divisions per edge, n^2
+    sub-faces (CE_N_SUB_FACES, src/sc/cexprs.inc:51-64), ordered by rows
+    along chi_1 and, in a row, upward- then downward-pointing triangle per
+    column (the default face ordering of the sub-grid,
+    tools/edge_pre/edge_pre/sc/grid/Tet.py:750-777).  Vertices [3][2] each."""
+    n = 2 * deg + 1
+    out = []
+    for e2 in range(n):
+        for e1 in range(n):
+            if e1 < n - e2:
+                out.append([(Fraction(e1, n), Fraction(e2, n)), (Fraction(e1 + 1, n), Fraction(e2, n)),
+                            (Fraction(e1, n), Fraction(e2 + 1, n))])
+            if e1 < n - e2 - 1:
+                out.append([(Fraction(e1 + 1, n), Fraction(e2, n)), (Fraction(e1 + 1, n), Fraction(e2 + 1, n)),
+                            (Fraction(e1, n), Fraction(e2 + 1, n))])
+    assert len(out) == n * n
+    return out
+
+
+def sub_face_means(psi, tris):
+    """mean[sf][j]: the mean of psi_j over sub-triangle sf (exact)."""
+    out = []
+    for a, b, c in tris:
+        amap = [p_lin([b[0] - a[0], c[0] - a[0]], a[0]), p_lin([b[1] - a[1], c[1] - a[1]], a[1])]
+        out.append([2 * int_tria(p_compose(pj, amap)) for pj in psi])
+    return out
+
+
+def solve_exact(a, b):
+    """x = a^-1 b for a square Fraction matrix a and a matrix b (Gauss-Jordan)."""
+    n = len(a)
+    m = [list(a[i]) + list(b[i]) for i in range(n)]
+    for c in range(n):
+        piv = next(r for r in range(c, n) if m[r][c] != 0)
+        m[c], m[piv] = m[piv], m[c]
+        inv = 1 / m[c][c]
+        m[c] = [v * inv for v in m[c]]
+        for r in range(n):
+            if r != c and m[r][c] != 0:
+                f = m[r][c]
+                m[r] = [vr - f * vc for vr, vc in zip(m[r], m[c])]
+    return [row[n:] for row in m]
+
+
+def sub_face_gather(mean):
+    """Least-squares inverse of the sub-face means on the face polynomials:
+    gather[j][sf] = ((S^T S)^-1 S^T)[j][sf] with S = mean, so that
+    gather . mean = I (a face polynomial of degree <= deg is recovered from
+    its sub-face means)."""
+    nsf, nf = len(mean), len(mean[0])
+    sts = [[sum(mean[k][i] * mean[k][j] for k in range(nsf)) for j in range(nf)] for i in range(nf)]
+    st = [[mean[k][i] for k in range(nsf)] for i in range(nf)]
+    return solve_exact(sts, st)
+
+
 def generate(deg):
     """Returns a dict of exact Fraction matrices for polynomial degree ``deg``."""
     phi = tet_basis(deg)
@@ -278,8 +340,12 @@ def generate(deg):
     flux_t = [[[flux_l[fa][k][f] * mfinv[f] * minv[k] for k in range(n)]
                for f in range(nf)] for fa in range(4)]
 
+    sf_mean = sub_face_means(psi, sub_faces(deg))
+    sf_gather = sub_face_gather(sf_mean)
+
     return {
         "deg": deg, "n": n, "nf": nf,
+        "sfMean": sf_mean, "sfGather": sf_gather,
         "mass": [mass[i][i] for i in range(n)],
         "mass_face": [mass_f[i][i] for i in range(nf)],
         "stiffT": stiff_t, "stiffV": stiff_v,
@@ -338,6 +404,8 @@ def emit(mats_by_deg, path):
         out.append(_emit_array(f"g_tet4_d{d}_fluxL", m["fluxL"]))
         out.append(_emit_array(f"g_tet4_d{d}_fluxN", m["fluxN"]))
         out.append(_emit_array(f"g_tet4_d{d}_fluxT", m["fluxT"]))
+        out.append(_emit_array(f"g_tet4_d{d}_sfMean", m["sfMean"]))
+        out.append(_emit_array(f"g_tet4_d{d}_sfGather", m["sfGather"]))
         s, nt = _emit_basis(f"g_tet4_d{d}_basis", m["basis"], 3)
         out.append(s)
         out.append(f"static const int g_tet4_d{d}_basis_nterms = {nt};\n")

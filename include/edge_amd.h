@@ -38,6 +38,7 @@ extern "C" {
 #define EDGE_FA_FREE_SURFACE 101
 #define EDGE_FA_OUTFLOW 105
 #define EDGE_FA_PERIODIC 106
+#define EDGE_FA_RUPTURE 201
 
 /* Message of the calling thread's last error ("" if none). */
 const char *edge_last_error(void);
@@ -69,6 +70,16 @@ int edge_dg_eval_basis(int order, int64_t n_pts, const double *pts, double *val)
 /* Collapsed Gauss-Legendre rule with n1d points per direction on the unit
  * tet (src/dg/QuadraturePoints.h:424-530): pts[n1d^3][3], w[n1d^3]. */
 int edge_dg_quad_tet(int n1d, double *pts, double *w);
+
+/* The fault phase's tables of order O (host arrays, either may be NULL):
+ * *n_sub_faces = n_sf = (2 O - 1)^2; scat[16][N][n_sf], the means over each
+ * sub-face of the traces of the N element basis functions (t < 4: on face t
+ * in its own coordinates; t = 4 + 4 v + f: on a neighbour's face f seen from
+ * the element's face with vertex orientation v, SurfInt.hpp:127-130);
+ * lift[16][n_sf][N], the DOF update of a net update of one sub-face (the
+ * least-squares face polynomial of the sub-face values, lifted through the
+ * inverse mass matrix, times n_sf / (3 n_sc) with n_sc = (2 O - 1)^3). */
+int edge_dg_fault_tables(int order, int32_t *n_sub_faces, double *scat, double *lift);
 
 /* ====================================================================== */
 /* Structured box meshes (the conventions of tools/edge_v, see mesh.cpp)   */
@@ -353,6 +364,62 @@ int edge_rupture_net_updates(edge_rupture *r, int64_t first, int64_t n_faces, do
  * the sub-face state [nF][n_sf][R][14] and the failure flags [nF][n_sf][R]
  * to the host (any may be NULL). */
 int edge_rupture_get(const edge_rupture *r, double *solvers, double *state, uint8_t *per);
+
+/* ====================================================================== */
+/* A fault coupled to the element update (dynamic rupture in the time     */
+/* step).  The reference has the pieces (InternalBoundarySolvers<TET4,O,3> */
+/* ::init, InternalBoundary.hpp:743-923; RuptureInit, RuptureInit.hpp:     */
+/* 170-206; netUpdates with linSlipWeak, InternalBoundary.hpp:184-280) but */
+/* no call site in its time loop (src/impl/seismic/steps.inc:24-77); this  */
+/* defines one.  Faces of type EDGE_FA_RUPTURE carry no Godunov flux in    */
+/* the element update (edge_el_create zeroes their solvers and the         */
+/* neighbour step skips them); the fault phase, run between the local and */
+/* the neighbour step, adds their flux:                                     */
+/*  1. per fault face and sub-face (n_sf = (2 O - 1)^2 sub-triangles of the */
+/*     face, in the coordinates of the face's first element L, rows along  */
+/*     its first face edge, upward then downward triangles), the means of  */
+/*     both elements' time-averaged traces tInt / dt;                       */
+/*  2. the net updates (edge_rupture_net_updates with n_sub_faces = n_sf,  */
+/*     n_sub_cells = (2 O - 1)^3), advancing the friction state;          */
+/*  3. the face polynomial of degree O - 1 whose sub-face means are closest */
+/*     to the net updates (least squares), lifted into both elements as a  */
+/*     surface flux.  A fault that does not slip gives the welded Godunov  */
+/*     flux exactly.                                                       */
+/* ====================================================================== */
+typedef struct edge_fault edge_fault;
+
+/* Builds the fault of an elastic solver's mesh (the same mesh and
+ * materials as edge_el_create; faces of type EDGE_FA_RUPTURE, each between
+ * two elements): fault_frame [3][3] (normal, strike, dip; unit vectors),
+ * friction parameters glob [3][R] (static and dynamic coefficient, 1/Dc)
+ * and the initial sub-face state state0 [nF][n_sf][R][14] as
+ * edge_rupture_create, fault faces in mesh-face order.  The precision of
+ * the DOFs is the solver's; the friction and net updates run in FP64. */
+int edge_fault_create(const edge_el_solver *s, const edge_mesh_in *mesh, const double *rho, const double *lam,
+                      const double *mu, const double *fault_frame, const double *glob, const double *state0,
+                      edge_fault **out);
+int edge_fault_destroy(edge_fault *f);
+
+/* The fault phase for the solver's time step dt: reads tInt (after the
+ * local step) and adds the fault faces' fluxes to dofs (device arrays of
+ * the solver's layout and precision). */
+int edge_fault_apply(edge_fault *f, const edge_el_solver *s, double dt, const void *t_int, void *dofs,
+                     void *stream);
+
+/* Number of fault faces and sub-faces; per fault face (any may be NULL) the
+ * mesh face id faces[nF], its elements L, R elements[nF][2] and L's local
+ * face local_faces[nF] (the sub-faces lie in that face's coordinates). */
+int edge_fault_info(const edge_fault *f, int64_t *n_faces, int32_t *n_sub_faces, int64_t *faces, int64_t *elements,
+                    int32_t *local_faces);
+
+/* The friction state [nF][n_sf][R][14] and slipping flags [nF][n_sf][R] of
+ * the last step (either may be NULL). */
+int edge_fault_get(const edge_fault *f, double *state, uint8_t *slipping);
+
+/* Attaches a fault built for this solver (NULL detaches it): edge_el_step
+ * then runs local, the fault phase, neighbour.  Elastic solvers with global
+ * time stepping.  Drops a captured step graph. */
+int edge_el_set_fault(edge_el_solver *s, edge_fault *f);
 
 /* ---------------------------------------------------------------------- */
 /* Rate-2 local time stepping (AderDg.hpp:384-417 and :620-674): elements   */

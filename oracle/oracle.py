@@ -489,3 +489,136 @@ class SampleOracle(MeshOracle):
         self.f_id = np.ascontiguousarray(sample["f_id"], dtype=np.uint16)
         self.v_id = np.ascontiguousarray(sample["v_id"], dtype=np.uint16)
         self.fa_type = np.ascontiguousarray(sample["fa_type"], dtype=np.int32)
+
+
+# ---------------------------------------------------------------- a fault in the time step
+# Restated conventions (tools/edge_pre/edge_pre/types/Tet.py:50-91): the face
+# maps chi -> xi of the reference tet's faces (0: 0-2-1, 1: 0-1-3, 2: 0-3-2,
+# 3: 1-2-3) and the neighbour face's coordinates as a function of the local
+# face's, per position v of local face-vertex 0 in the neighbour's face.
+def _fa_to_el(fa, c1, c2):
+    z = np.zeros_like(c1)
+    return np.stack([(c2, c1, z), (c1, z, c2), (z, c2, c1), (1 - c1 - c2, c1, c2)][fa], axis=-1)
+
+
+def _fa_to_fa(v, c1, c2):
+    return [(c2, c1), (1 - c1 - c2, c2), (c1, 1 - c1 - c2)][v]
+
+
+def fault_sub_faces(order):
+    """Vertices [n_sf][3][2] of the face's sub-triangles, n = 2 O - 1 per edge,
+    ordered by rows along chi_1, up then down triangle per column
+    (tools/edge_pre/edge_pre/sc/grid/Tet.py:750-777)."""
+    n = 2 * order - 1
+    out = []
+    for e2 in range(n):
+        for e1 in range(n):
+            if e1 < n - e2:
+                out.append([(e1, e2), (e1 + 1, e2), (e1, e2 + 1)])
+            if e1 < n - e2 - 1:
+                out.append([(e1 + 1, e2), (e1 + 1, e2 + 1), (e1, e2 + 1)])
+    return np.array(out, dtype=np.float64) / n
+
+
+def _tria_rule(npt):
+    """Collapsed Gauss-Legendre rule on the unit triangle (weights sum to 1/2)."""
+    x, w = np.polynomial.legendre.leggauss(npt)
+    x, w = 0.5 * (x + 1), 0.5 * w
+    a, b = np.meshgrid(x, x, indexing="ij")
+    wa, wb = np.meshgrid(w, w, indexing="ij")
+    return np.stack([a.ravel(), (b * (1 - a)).ravel()], axis=1), (wa * wb * (1 - a)).ravel()
+
+
+def fault_tables(order):
+    """(scat [16][N][n_sf], lift [16][n_sf][N]) by quadrature: the sub-face
+    means of each table's trace (t < 4: face t; t = 4 + 4 v + f: neighbour
+    face f seen with orientation v), and the lift of one sub-face's net
+    update: the least-squares face polynomial (monomial basis) of the
+    sub-face values, integrated against the trace, over the diagonal mass,
+    times n_sf / (3 n_sc), n_sc = (2 O - 1)^3."""
+    from edge_amd.setups import dg_matrices, eval_basis
+    N = order * (order + 1) * (order + 2) // 6
+    tris = fault_sub_faces(order)
+    nsf = len(tris)
+    nsc = (2 * order - 1) ** 3
+    mass = dg_matrices(order)["mass"]
+    qp, qw = _tria_rule(order + 2)
+    mono = [(a, b) for a in range(order) for b in range(order - a)]
+    # sub-face means of the monomials, least-squares inverse
+    smono = np.zeros((nsf, len(mono)))
+    sub_pts = []
+    for sf, (a, b, c) in enumerate(tris):
+        x = a + qp[:, :1] * (b - a) + qp[:, 1:] * (c - a)
+        sub_pts.append(x)
+        for j, (p, q) in enumerate(mono):
+            smono[sf, j] = 2 * np.sum(qw * x[:, 0] ** p * x[:, 1] ** q)
+    g = np.linalg.pinv(smono)
+    scat = np.zeros((16, N, nsf))
+    lift = np.zeros((16, nsf, N))
+    for t in range(16):
+        def to_el(c):
+            if t < 4:
+                return _fa_to_el(t, c[:, 0], c[:, 1])
+            v, f = (t - 4) // 4, (t - 4) % 4
+            return _fa_to_el(f, *_fa_to_fa(v, c[:, 0], c[:, 1]))
+        for sf in range(nsf):
+            scat[t, :, sf] = 2 * qw @ eval_basis(order, to_el(sub_pts[sf]))
+        phi = eval_basis(order, to_el(qp))  # [q][N] on the whole face
+        m = np.stack([qp[:, 0] ** p * qp[:, 1] ** q for p, q in mono], axis=1)
+        inner = (phi * qw[:, None]).T @ m  # [N][mono]
+        lift[t] = (inner @ g).T / mass[None, :] * (nsf / (3.0 * nsc))
+    return scat, lift
+
+
+class FaultOracle(MeshOracle):
+    """The coupled step with a fault (faces of type 201): local, the fault
+    phase, neighbour.  Fault faces' Godunov solvers are zeroed; the fault
+    phase takes both sides' sub-face means of tInt / dt, the net updates of
+    the pinned orc_net_updates, and lifts them back into both elements."""
+
+    def __init__(self, mesh, order, n_crs, rho, lam, mu, fault_frame, glob, state0):
+        super().__init__(mesh, order, n_crs, rho=rho, lam=lam, mu=mu)
+        ft = np.asarray(mesh.fa_type)
+        self.faces = np.nonzero((ft & 201) == 201)[0]
+        self.scat, self.lift = fault_tables(order)
+        self.n_sf = self.scat.shape[2]
+        n_sc = (2 * order - 1) ** 3
+        ve = mesh.el_crds()
+        vol = np.abs(np.linalg.det(ve[:, 1:, :] - ve[:, :1, :])) / 6.0
+        self.sides, self.solv, self.face = [], [], []
+        bnd = np.asarray(fault_frame, dtype=np.float64).reshape(3, 3)
+        mat = np.stack([rho, lam, mu], axis=1)
+        for f in self.faces:
+            eL, eR = int(mesh.fa_el[f][0]), int(mesh.fa_el[f][1])
+            fl = int(np.nonzero(np.asarray(mesh.el_fa[eL]) == f)[0][0])
+            tr = 4 + 4 * int(mesh.v_id[eL][fl]) + int(mesh.f_id[eL][fl])
+            fr = int(np.nonzero(np.asarray(mesh.el_fa[eR]) == f)[0][0])
+            for e, k in ((eL, fl), (eR, fr)):
+                self.fs[0, e, k] = 0.0
+                self.fs[1, e, k] = 0.0
+            s, pref = ib_solvers(mesh.normals[f], mesh.tangents[f][0], mesh.tangents[f][1], bnd, mat[eL], mat[eR],
+                                 mesh.areas[f], vol[eL], vol[eR], self.n_sf, n_sc)
+            l_eq_m = not pref  # RuptureInit.hpp:198-206
+            cdl, cdr = np.sqrt(1.0 / (mu[eL] * rho[eL])), np.sqrt(1.0 / (mu[eR] * rho[eR]))
+            self.sides.append((eL, eR, fl, tr))
+            self.solv.append(s)
+            self.face.append((l_eq_m, cdl if l_eq_m else cdr, cdr if l_eq_m else cdl))
+        self.glob = _c(glob)
+        self.state = np.ascontiguousarray(np.asarray(state0, dtype=np.float64).reshape(len(self.faces), -1, 14))
+        self.per = np.zeros((len(self.faces), self.n_sf, n_crs), dtype=bool)
+
+    def fault(self, dofs, tint, dt):
+        for i, (eL, eR, fl, tr) in enumerate(self.sides):
+            qL = np.einsum("qnr,ns->qsr", tint[eL], self.scat[fl]) / dt
+            qR = np.einsum("qnr,ns->qsr", tint[eR], self.scat[tr]) / dt
+            l_eq_m, cm, cp = self.face[i]
+            nl, nr, per = net_updates(self.solv[i], qL, qR, dt, l_eq_m, cm, cp, self.glob, self.state[i])
+            self.per[i] = per
+            dofs[eL] += np.einsum("qsr,sn->qnr", nl, self.lift[fl])
+            dofs[eR] += np.einsum("qsr,sn->qnr", nr, self.lift[tr])
+
+    def step(self, dofs, tint, dt, n_steps=1, dofs_a=None):
+        for _ in range(n_steps):
+            self.local(dofs, tint, dt)
+            self.fault(dofs, tint, dt)
+            self.neigh(dofs, tint)

@@ -1,0 +1,122 @@
+"""Pins the fault phase of the coupled time step (a fault between the local
+and the neighbour step; its pieces are the reference's internal boundary,
+src/impl/seismic/solvers/InternalBoundary.hpp:184-280 and :743-923, and
+linear slip weakening, FrictionLaws.hpp:309-392):
+
+* the library's tables (exact rationals composed in FP64,
+  edge_dg_fault_tables) against the oracle's (quadrature over each sub-face
+  of the restated face maps, a monomial face basis and numpy's
+  pseudo-inverse): two independent routes;
+* lift after scatter is the surface-flux operator on polynomials: the
+  least-squares face polynomial of a trace's sub-face means is the trace
+  itself, so lift(scat(u)) = c M^-1 F M_F^-1 F^T u with the flux matrices F
+  the reference's fixtures pin;
+* a locked fault (static friction too high to slip) reproduces the welded
+  interface's step (the Godunov fluxes) to rounding;
+* a slipping fault holds the traction magnitude at the strength on every
+  slipping sub-face, and slips only where the shear exceeds it.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from edge_amd import _lib
+from edge_amd import fault as FL
+from edge_amd import mesh as M
+from edge_amd import setups as S
+from edge_amd.mesh import FA_OUTFLOW, FA_PERIODIC
+
+
+def _lib_tables(order):
+    n = C.c_int32()
+    _lib.check(_lib.lib().edge_dg_fault_tables(order, C.byref(n), None, None))
+    N = S.n_modes(order)
+    sc, li = np.zeros((16, N, n.value)), np.zeros((16, n.value, N))
+    _lib.check(_lib.lib().edge_dg_fault_tables(order, None, sc.ctypes.data, li.ctypes.data))
+    return sc, li
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+def test_tables_match_the_oracle(order):
+    sc, li = _lib_tables(order)
+    assert sc.shape[2] == (2 * order - 1) ** 2 == FL.n_sub_faces(order)
+    osc, oli = orc.fault_tables(order)
+    np.testing.assert_allclose(sc, osc, rtol=0, atol=1e-13 * np.abs(osc).max())
+    np.testing.assert_allclose(li, oli, rtol=0, atol=1e-12 * np.abs(oli).max())
+
+
+@pytest.mark.parametrize("order", [2, 4, 5])
+def test_lift_after_scatter_is_the_surface_flux(order):
+    mats = S.dg_matrices(order)
+    N, NF = S.n_modes(order), S.n_face_modes(order)
+    # the face mass from the flux matrices: fluxT_fa[f][k] = fluxL_fa[k][f] / (M_F[f] M[k])
+    ft, fl = mats["fluxT"], mats["fluxL"]
+    k = np.argmax(np.abs(fl[0]), axis=0)
+    mass_f = np.array([fl[0][k[f], f] / (ft[0][f, k[f]] * mats["mass"][k[f]]) for f in range(NF)])
+    nsc = (2 * order - 1) ** 3
+    c = FL.n_sub_faces(order) / (3.0 * nsc)
+    rng = np.random.default_rng(order)
+    for sc, li in (_lib_tables(order), orc.fault_tables(order)):
+        for t in range(16):
+            F = fl[t] if t < 4 else mats["fluxN"][t - 4]
+            u = rng.normal(size=N)
+            want = c * (F @ ((F.T @ u) / mass_f)) / mats["mass"]
+            got = li[t].T @ (sc[t].T @ u)
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-11 * np.abs(want).max())
+
+
+def _fault_problem(order, R, mus, seed=3, n=4, nucleate=False):
+    m = M.box(n, bc=(FA_PERIODIC, FA_PERIODIC, FA_OUTFLOW), seed=seed)
+    faces = FL.tag_plane(m, 2, 0.5)
+    assert len(faces) == 2 * n * n
+    rho, lam, mu = S.random_materials(m.n_el, seed=seed)
+    frame = np.array([[0, 0, 1.0], [1, 0, 0], [0, 1, 0]])
+    glob = np.array([[mus] * R, [0.525] * R, [1 / 0.4] * R])
+    rng = np.random.default_rng(seed)
+    ss0 = np.broadcast_to(rng.uniform(70, 81.24, R)[None, None, :], (len(faces), FL.n_sub_faces(order), R)).copy()
+    if nucleate:
+        pts = FL.sub_face_points(m, order)
+        inside = (np.abs(pts[..., 0] - 0.5) < 0.3) & (np.abs(pts[..., 1] - 0.5) < 0.3)
+        ss0[inside] = 81.6
+    st0 = FL.lsw_state(len(faces), FL.n_sub_faces(order), R, sn0=-120.0, ss0_0=ss0, ss0_1=0.0, mus=glob[0])
+    return m, rho, lam, mu, frame, glob, st0
+
+
+def test_locked_fault_is_the_welded_interface():
+    order, R = 3, 2
+    m, rho, lam, mu, frame, glob, st0 = _fault_problem(order, R, mus=1e6)
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    fo = orc.FaultOracle(m, order, R, rho, lam, mu, frame, glob, st0)
+    mo = orc.MeshOracle(m, order, R, rho=rho, lam=lam, mu=mu)
+    d0 = np.random.default_rng(1).normal(size=(m.n_el, 9, S.n_modes(order), R))
+    a, ta = d0.copy(), np.zeros_like(d0)
+    b, tb = d0.copy(), np.zeros_like(d0)
+    fo.step(a, ta, dt, 3)
+    mo.step(b, tb, dt, 3)
+    assert not fo.per.any()
+    assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(b - d0)
+    # the fault faces do carry flux: without the fault phase the step differs
+    c, tc = d0.copy(), np.zeros_like(d0)
+    for _ in range(3):
+        fo.local(c, tc, dt)
+        fo.neigh(c, tc)
+    assert np.linalg.norm(c - b) > 1e-3 * np.linalg.norm(b - d0)
+
+
+def test_slipping_fault_holds_traction_at_strength():
+    order, R = 3, 2
+    m, rho, lam, mu, frame, glob, st0 = _fault_problem(order, R, mus=0.677, nucleate=True)
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    fo = orc.FaultOracle(m, order, R, rho, lam, mu, frame, glob, st0)
+    d, t = np.zeros((m.n_el, 9, S.n_modes(order), R)), np.zeros((m.n_el, 9, S.n_modes(order), R))
+    fo.step(d, t, dt, 4)
+    st = fo.state.reshape(len(fo.faces), -1, R, 14)
+    per = fo.per
+    assert per.any() and not per.all()
+    trac = np.hypot(st[..., 7], st[..., 8])
+    np.testing.assert_allclose(trac[per], st[..., 6][per], rtol=1e-12)
+    assert np.all(trac[~per] <= st[..., 6][~per] * (1 + 1e-12))
+    assert np.all(st[..., 11][per] > 0)  # slip accumulates where it slips
+    assert np.all(np.isfinite(d)) and np.abs(d).max() > 0
