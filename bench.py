@@ -171,6 +171,75 @@ def time_solver(solver, dofs, tint, dt, steps, warmup, points=None):
     return wall, loc, nei
 
 
+def host_cpu():
+    """(CPU model, logical CPUs this process may run on, physical cores among them)."""
+    model, cores = "?", set()
+    try:
+        allowed = os.sched_getaffinity(0)
+    except AttributeError:
+        allowed = set(range(os.cpu_count() or 1))
+    try:
+        cpu, phys, core = None, None, None
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().splitlines() + [""]:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    cpu = int(v)
+                elif k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and cpu is not None:
+                    if cpu in allowed:
+                        cores.add((phys, core))
+                    cpu, phys, core = None, None, None
+    except OSError:
+        pass
+    return model, len(allowed), len(cores) or len(allowed)
+
+
+def omp_threads(n):
+    """Sets the OpenMP thread count of the oracle's parallel loops."""
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+    except OSError:
+        pass
+
+
+def config1_single_thread(budget_s=8.0):
+    """The oracle on the reference's CPU-runnable case: unit cube of 8^3 hexes
+    x 6 = 3,072 tets, periodic, degree 3 (order 4, 20 modes), FP64, one run,
+    rho = 1, lambda = 2, mu = 1, P plane wave, global time stepping at the
+    CFL step; one thread, as many of its 100 steps as fit the budget."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from edge_amd import mesh as M
+    from edge_amd import setups as S
+    m = M.box(8)
+    one = np.ones(m.n_el)
+    order = 4
+    dt = S.dt_cfl(m, one, 2 * one, one, order)
+    d = S.project(m, order, S.PlaneWave(k=(2 * np.pi,) * 3))
+    t = np.zeros_like(d)
+    mo = orc.MeshOracle(m, order, 1, rho=one, lam=2 * one, mu=one)
+    omp_threads(1)
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < 100 and time.perf_counter() - t0 < budget_s:
+        mo.step(d, t, dt, 1)
+        steps += 1
+    wall = time.perf_counter() - t0
+    omp_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    n = S.n_modes(order)
+    return {"value": m.n_el * 9 * n * steps / wall, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} of the 100 steps of the 3,072-tet O4 FP64 single-run plane-wave case, {wall:.1f} s, "
+                      f"one thread"}
+
+
 def cpu_baseline(m, order, runs, rho, lam, mu, dt, q0, budget_s=15.0):
     """FP64 oracle (vanilla-semantics CPU restatement; LIBXSMM is not
     available offline) on a bounded element sample, all host threads."""
@@ -195,10 +264,14 @@ def cpu_baseline(m, order, runs, rho, lam, mu, dt, q0, budget_s=15.0):
         reps += 1
     wall = time.perf_counter() - t0
     n = order * (order + 1) * (order + 2) // 6
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n_el * 9 * n * runs / wall, "unit": "DOF-updates/s", "cores": cores, "kind": "port",
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model, logical, physical = host_cpu()
+    return {"value": n_el * 9 * n * runs / wall, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "threads": threads, "cpu_model": model, "logical_cpus": logical, "physical_cores": physical,
             "sample": f"{n_el} element-steps (local+neigh) of the {m.n_el}-tet O{order} R={runs} FP64 workload, "
-                      f"{wall:.1f} s, OpenMP over elements; vanilla-semantics restatement, LIBXSMM unavailable"}
+                      f"{wall:.1f} s, OpenMP over elements with {threads} threads on {physical} physical cores "
+                      f"({model}); vanilla-semantics restatement, LIBXSMM unavailable",
+            "config1_single_thread": config1_single_thread()}
 
 
 def visco_flops(order, n_rms):
@@ -258,6 +331,69 @@ def ensemble_row(args, timed, flops_el_run):
         torch.cuda.empty_cache()
 
 
+def fault_problem(order, runs, seed=3343):
+    """48 x 48 x 24 hexes x 6 = 331,776 tets on the unit cube (periodic in x
+    and y, outflow in z) with a planar fault at z = 0.5 (4,608 faces):
+    rho = 2.67, vp = 6, vs = 3.464; linear slip weakening with mu_s = 0.677,
+    mu_d = 0.525, d_c = 0.4, normal stress -120; per run an initial shear
+    stress U[70, 81.24] and a seeded square nucleation patch (half-width 0.15)
+    at 81.6, above the static strength 81.24."""
+    from edge_amd import fault as FL
+    from edge_amd import mesh as M
+    from edge_amd import setups as S
+    from edge_amd.mesh import FA_OUTFLOW, FA_PERIODIC
+    m = M.box((48, 48, 24), bc=(FA_PERIODIC, FA_PERIODIC, FA_OUTFLOW), seed=seed)
+    faces = FL.tag_plane(m, 2, 0.5)
+    rho = np.full(m.n_el, 2.67)
+    mu = rho * 3.464 ** 2
+    lam = rho * 6.0 ** 2 - 2 * mu
+    rng = np.random.default_rng(seed + 4)
+    nsf = FL.n_sub_faces(order)
+    ss0 = np.broadcast_to(rng.uniform(70.0, 81.24, runs)[None, None, :], (len(faces), nsf, runs)).copy()
+    pts = FL.sub_face_points(m, order)
+    ctr = rng.uniform(0.3, 0.7, (runs, 2))
+    for r in range(runs):
+        inside = (np.abs(pts[..., 0] - ctr[r, 0]) < 0.15) & (np.abs(pts[..., 1] - ctr[r, 1]) < 0.15)
+        ss0[..., r][inside] = 81.6
+    glob = np.array([[0.677] * runs, [0.525] * runs, [1 / 0.4] * runs])
+    st0 = FL.lsw_state(len(faces), nsf, runs, sn0=-120.0, ss0_0=ss0, ss0_1=0.0, mus=glob[0])
+    frame = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    return m, rho, lam, mu, frame, glob, st0, dt
+
+
+def fault_row(args, timed):
+    """Elastic + dynamic rupture: 331,776 tets, 4,608 fault faces, degree 3,
+    16 fused runs in FP32 (friction, net updates and the fault's sub-face
+    projections in FP64), one graph-captured coupled step (local, fault,
+    neighbour) per time step; beside it the fault phase alone."""
+    import torch
+    from edge_amd import fault as FL
+    from edge_amd.solver import ElasticSolver
+    order, runs = 4, 16
+    m, rho, lam, mu, frame, glob, st0, dt = fault_problem(order, runs)
+    s = ElasticSolver(m, order, runs, 32, rho, lam, mu)
+    f = FL.Fault(s, m, rho, lam, mu, frame, glob, st0)
+    d = torch.zeros(s.dof_shape(), dtype=torch.float32, device="cuda")
+    t = torch.zeros_like(d)
+    n_steps = 100
+    ms = timed(lambda st: s.step(d, t, dt, 1, graph=True, stream=st), n_steps)
+    st, slip = f.get()
+    fault_ms = timed(lambda st: f.apply(d, t, dt, stream=st), 5)
+    fl = FLOPS[order]
+    n = s.n_modes
+    step_flop = (fl["local"] + fl["neigh"]) * runs * m.n_el
+    print(json.dumps({"row": "config 4: elastic + dynamic rupture (coupled step)", "elements": m.n_el,
+                      "fault_faces": f.n_faces, "sub_faces": f.n_sf, "order": order, "runs": runs, "precision": 32,
+                      "graph": True, "steps": n_steps + 1, "t_end": (n_steps + 1) * dt, "ms_per_step": ms * 1e3,
+                      "dof_updates_per_s": m.n_el * 9 * n * runs / ms, "fault_phase_ms": fault_ms * 1e3,
+                      "step_tflops": step_flop / ms / 1e12, "step_frac": step_flop / ms / PEAK["fp32_mfma"],
+                      "slipping_sub_faces": float(slip.mean()), "max_slip": float(st[..., 11].max()),
+                      "finite": bool(torch.isfinite(d).all())}), flush=True)
+    f.close()
+    s.close()
+
+
 def rows(args):
     """Per-row measurements beside the headline (one JSON line per row):
     viscoelastic updates, local time stepping, receivers and sources, and the
@@ -281,13 +417,20 @@ def rows(args):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) * 1e-3 / reps
 
+    want = set(args.only.split(",")) if args.only else {"visco", "ensemble", "lts", "points", "rupture", "fault"}
+    if args.no_ensemble:
+        want.discard("ensemble")
+    if "fault" in want:
+        fault_row(args, timed)
+    if not want - {"fault"}:
+        return
     m, rho, lam, mu, dt, q0 = build_problem(args.hex, args.order)
     n = q0.shape[2]
     # ---- viscoelastic (3 mechanisms, f_c = 1 Hz, ratio 10): FP32 8 runs and FP64 one run
     rng = np.random.default_rng(5)
     qp = rng.uniform(50, 200, m.n_el)
     fl, fn_ = visco_flops(args.order, 3)
-    for runs, prec in ((8, 32), (1, 64)):
+    for runs, prec in (((8, 32), (1, 64)) if "visco" in want else ()):
         s = ViscoSolver(m, args.order, runs, prec, rho, lam, mu, qp, qp / 2, n_rms=3, freq_cen=1.0, freq_rat=10.0)
         d = torch.from_numpy(q0)[..., None].repeat(1, 1, 1, runs).to(s.dtype).cuda().contiguous()
         t, a = torch.zeros_like(d), s.alloc_a()
@@ -307,8 +450,10 @@ def rows(args):
         del d, t, a
     # ---- batched viscoelastic ensemble: 24 meshes of seeded sizes (6 n^3 tets,
     # n in [8, 32]) as one disjoint union, one graph-captured step per time step
-    if not args.no_ensemble:
+    if "ensemble" in want:
         ensemble_row(args, timed, fl + fn_)
+    if not want & {"lts", "points", "rupture"}:
+        return
     # ---- local time stepping: two groups (halves of the mesh), FP32 16 runs
     s = ElasticSolver(m, args.order, 16, 32, rho, lam, mu)
     d = (torch.from_numpy(q0)[..., None].repeat(1, 1, 1, 16)).to(torch.float32).cuda().contiguous()
@@ -372,6 +517,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true", help="--rows without the 24-mesh ensemble")
+    ap.add_argument("--only", default="", help="--rows: comma list of visco, ensemble, lts, points, rupture, fault")
     ap.add_argument("--no-autotune", action="store_true", help="measure the kernels --kernel selects, untuned")
     ap.add_argument("--rows", action="store_true",
                     help="measure the other rows (viscoelastic, LTS, receivers/sources, rupture) instead")

@@ -100,6 +100,8 @@ _SIGS = {
     "edge_el_set_lts": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "edge_el_local_lts": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_int32] + [C.c_void_p] * 5),
     "edge_el_neigh_lts": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32] + [C.c_void_p] * 5),
+    "edge_el_local_lts_a": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_int32] + [C.c_void_p] * 6),
+    "edge_el_neigh_lts_a": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32] + [C.c_void_p] * 6),
     "edge_el_point_basis": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "edge_recv_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "edge_recv_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,

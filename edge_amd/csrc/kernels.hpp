@@ -163,5 +163,14 @@ template <typename T>
 int launch_lts_faces(const SolverDev<T>& s, const uint8_t* fa_lts, const int64_t* list, int64_t n, int first_ts,
                      const T* t0, const T* t1, const T* t2, T* dofs, hipStream_t st);
 bool lts_supported(int order, int n_runs, int prec_bytes);
+// ... of viscoelastic solvers (lts_visco.hip): tDofs[2] from the CK with the
+// anelastic terms, and interface faces with the anelastic update
+template <typename T>
+int launch_lts_half_visco(const SolverDev<T>& s, const int64_t* list, int64_t n, T h, const T* dofs,
+                          const T* dofs_a, T* t2, hipStream_t st);
+template <typename T>
+int launch_lts_faces_visco(const SolverDev<T>& s, const uint8_t* fa_lts, const int64_t* list, int64_t n, int first_ts,
+                           const T* t0, const T* t1, const T* t2, T* dofs, T* dofs_a, hipStream_t st);
+bool lts_visco_supported(int order, int n_runs, int prec_bytes, int n_rms);
 
 }  // namespace edge

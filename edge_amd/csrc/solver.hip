@@ -452,6 +452,45 @@ int neigh_lts_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, const
                                    (const T*)t0, (const T*)t1, (const T*)t2, (T*)dofs, st);
 }
 
+// AderDg::local / neigh with local time stepping and anelastic DOFs
+template <typename T>
+int local_lts_a_t(edge_el_solver* s, int64_t first, int64_t n, double dt, int first_ts, void* dofs, void* dofs_a,
+                  void* t0, void* t1, void* t2, hipStream_t st) {
+  const auto g = list_range(s->h_lts_gt, first, n);
+  int rc = edge::launch_lts_half_visco<T>(dev_view<T>(s), s->lts_gt + g.first, g.second - g.first, (T)(0.5 * dt),
+                                          (const T*)dofs, (const T*)dofs_a, (T*)t2, st);
+  if (rc) return rc;
+  rc = edge_el_local_a(s, first, n, dt, dofs, dofs_a, t0, st);
+  if (rc) return rc;
+  const auto l = list_range(s->h_lts_lt, first, n);
+  return edge::launch_lts_sum<T>(s->lts_lt + l.first, l.second - l.first, 9 * s->n_modes * s->cfg.n_runs,
+                                 first_ts, (const T*)t0, (T*)t1, st);
+}
+
+template <typename T>
+int neigh_lts_a_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, const void* t0, const void* t1,
+                  const void* t2, void* dofs, void* dofs_a, hipStream_t st) {
+  auto d = dev_view<T>(s);
+  d.nb.ne = s->ne_lts;
+  if (d.rcv) d.rcv = s->rcv_lts;
+  int rc = check_neigh(s, t0);
+  if (rc) return rc;
+  if (s->kernel == 2) {
+    if constexpr (sizeof(T) == 4)
+      rc = edge::launch_neigh_visco_mfma(d, first, n, (const float*)t0, (float*)dofs, (float*)dofs_a, st);
+  } else if (s->kernel == 4) {
+    if constexpr (sizeof(T) == 8)
+      rc = edge::launch_neigh_plain64_visco(d, first, n, (const double*)t0, (double*)dofs, (double*)dofs_a, st);
+  } else {
+    rc = edge::launch_neigh_visco<T>(d, first, n, (const T*)t0, (T*)dofs, (T*)dofs_a, st);
+  }
+  if (rc) return rc;
+  const auto f = list_range(s->h_lts_if, first, n);
+  return edge::launch_lts_faces_visco<T>(dev_view<T>(s), s->fa_lts, s->lts_if + f.first, f.second - f.first,
+                                         first_ts, (const T*)t0, (const T*)t1, (const T*)t2, (T*)dofs, (T*)dofs_a,
+                                         st);
+}
+
 int one_step(edge_el_solver* s, double dt, void* dofs, void* dofs_a, void* t_int, hipStream_t st) {
   if (s->n_rms > 0) {
     const int rc = edge_el_local_a(s, 0, s->n_el, dt, dofs, dofs_a, t_int, st);
@@ -1000,8 +1039,8 @@ extern "C" int edge_el_set_lts(edge_el_solver* s, const uint8_t* el_lts, const u
   drop_lts(s);
   if (!el_lts && !fa_lts) return EDGE_OK;
   if (!el_lts || !fa_lts) return fail(EDGE_ERR_ARG, "local time stepping: element and face characteristics");
-  if (s->n_rms > 0) return fail(EDGE_ERR_UNSUPPORTED, "local time stepping: viscoelastic solvers");
-  if (!edge::lts_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8))
+  if (s->n_rms > 0 ? !edge::lts_visco_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8, s->n_rms)
+                   : !edge::lts_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8))
     return fail(EDGE_ERR_UNSUPPORTED, "local time stepping: order, runs and precision do not fit one workgroup");
   const int64_t ne4 = s->n_el * 4;
   std::vector<int32_t> ne(ne4), rcv;
@@ -1053,10 +1092,38 @@ extern "C" int edge_el_set_lts(edge_el_solver* s, const uint8_t* el_lts, const u
   return EDGE_OK;
 }
 
+extern "C" int edge_el_local_lts_a(edge_el_solver* s, int64_t first, int64_t n_els, double dt, int32_t first_ts,
+                                   void* dofs, void* dofs_a, void* t_int0, void* t_int1, void* t_int2, void* stream) {
+  int rc = check_range(s, first, n_els);
+  if (rc) return rc;
+  if (s->n_rms == 0) return edge_el_local_lts(s, first, n_els, dt, first_ts, dofs, t_int0, t_int1, t_int2, stream);
+  if (!s->fa_lts) return edge::fail(EDGE_ERR_ARG, "local time stepping is not set up (edge_el_set_lts)");
+  if (!dofs || !dofs_a || !t_int0 || !t_int1 || !t_int2) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  hipStream_t st = (hipStream_t)stream;
+  return s->cfg.precision == 64
+             ? local_lts_a_t<double>(s, first, n_els, dt, first_ts, dofs, dofs_a, t_int0, t_int1, t_int2, st)
+             : local_lts_a_t<float>(s, first, n_els, dt, first_ts, dofs, dofs_a, t_int0, t_int1, t_int2, st);
+}
+
+extern "C" int edge_el_neigh_lts_a(edge_el_solver* s, int64_t first, int64_t n_els, int32_t first_ts,
+                                   const void* t_int0, const void* t_int1, const void* t_int2, void* dofs,
+                                   void* dofs_a, void* stream) {
+  int rc = check_range(s, first, n_els);
+  if (rc) return rc;
+  if (s->n_rms == 0) return edge_el_neigh_lts(s, first, n_els, first_ts, t_int0, t_int1, t_int2, dofs, stream);
+  if (!s->fa_lts) return edge::fail(EDGE_ERR_ARG, "local time stepping is not set up (edge_el_set_lts)");
+  if (!dofs || !dofs_a || !t_int0 || !t_int1 || !t_int2) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  hipStream_t st = (hipStream_t)stream;
+  return s->cfg.precision == 64
+             ? neigh_lts_a_t<double>(s, first, n_els, first_ts, t_int0, t_int1, t_int2, dofs, dofs_a, st)
+             : neigh_lts_a_t<float>(s, first, n_els, first_ts, t_int0, t_int1, t_int2, dofs, dofs_a, st);
+}
+
 extern "C" int edge_el_local_lts(edge_el_solver* s, int64_t first, int64_t n_els, double dt, int32_t first_ts,
                                  void* dofs, void* t_int0, void* t_int1, void* t_int2, void* stream) {
   int rc = check_range(s, first, n_els);
   if (rc) return rc;
+  if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_local_lts_a)");
   if (!s->fa_lts) return edge::fail(EDGE_ERR_ARG, "local time stepping is not set up (edge_el_set_lts)");
   if (!dofs || !t_int0 || !t_int1 || !t_int2) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
   hipStream_t st = (hipStream_t)stream;
@@ -1069,6 +1136,7 @@ extern "C" int edge_el_neigh_lts(edge_el_solver* s, int64_t first, int64_t n_els
                                  void* stream) {
   int rc = check_range(s, first, n_els);
   if (rc) return rc;
+  if (s->n_rms > 0) return edge::fail(EDGE_ERR_ARG, "viscoelastic solver: the step needs dofs_a (edge_el_neigh_lts_a)");
   if (!s->fa_lts) return edge::fail(EDGE_ERR_ARG, "local time stepping is not set up (edge_el_set_lts)");
   if (!dofs || !t_int0 || !t_int1 || !t_int2) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
   hipStream_t st = (hipStream_t)stream;

@@ -26,21 +26,24 @@ STATE_VALS = 14   # sn0, co0, ss0[2], ss0A, muf, st, tr[2], sr[2], dd[2]
 _FACE_VES = [(0, 2, 1), (0, 1, 3), (0, 3, 2), (1, 2, 3)]
 
 
+def _face_vertices(mesh, faces):
+    """Vertex coordinates [n][3][3] of faces seen from their first element, in
+    that element's face-map order."""
+    el = mesh.fa_el[faces, 0]
+    loc = np.argmax(mesh.el_fa[el] == faces[:, None], axis=1)
+    idx = np.array(_FACE_VES)[loc]  # [n][3] local vertex ids
+    return mesh.ve_crds[mesh.el_ve[el[:, None], idx]], loc
+
+
 def tag_plane(mesh, axis, value, tol=1e-9):
     """Sets the face type of every interior face lying in the plane
     x[axis] = value to FA_RUPTURE; returns the tagged face ids."""
-    ve = mesh.el_crds()
-    out = []
-    for f in range(mesh.n_fa):
-        eL, eR = mesh.fa_el[f]
-        if eL < 0 or eR < 0:
-            continue
-        fl = int(np.nonzero(mesh.el_fa[eL] == f)[0][0])
-        pts = ve[eL][list(_FACE_VES[fl])]
-        if np.all(np.abs(pts[:, axis] - value) <= tol):
-            mesh.fa_type[f] = FA_RUPTURE
-            out.append(f)
-    return np.array(out, dtype=np.int64)
+    inner = np.nonzero((mesh.fa_el[:, 0] >= 0) & (mesh.fa_el[:, 1] >= 0))[0]
+    v, _ = _face_vertices(mesh, inner)
+    on = np.all(np.abs(v[:, :, axis] - value) <= tol, axis=1)
+    faces = inner[on]
+    mesh.fa_type[faces] = FA_RUPTURE
+    return faces.astype(np.int64)
 
 
 def n_sub_faces(order):
@@ -66,20 +69,17 @@ def fault_faces(mesh):
     mesh's fault faces, in the order the fault holds them."""
     faces = np.nonzero((mesh.fa_type & FA_RUPTURE) == FA_RUPTURE)[0]
     els = mesh.fa_el[faces]
-    loc = np.array([int(np.nonzero(mesh.el_fa[e] == f)[0][0]) for f, e in zip(faces, els[:, 0])], dtype=np.int64)
+    loc = np.argmax(mesh.el_fa[els[:, 0]] == faces[:, None], axis=1).astype(np.int64)
     return faces, els, loc
 
 
 def sub_face_points(mesh, order):
     """Physical centroids [nF][n_sf][3] of the fault faces' sub-faces."""
-    faces, els, loc = fault_faces(mesh)
+    faces, _, _ = fault_faces(mesh)
     c = sub_face_centroids(order)
-    ve = mesh.el_crds()
-    out = np.zeros((len(faces), len(c), 3))
-    for i, (e, fl) in enumerate(zip(els[:, 0], loc)):
-        v = ve[e][list(_FACE_VES[fl])]
-        out[i] = v[0] + c[:, :1] * (v[1] - v[0]) + c[:, 1:] * (v[2] - v[0])
-    return out
+    v, _ = _face_vertices(mesh, faces)
+    return (v[:, None, 0] + c[None, :, :1] * (v[:, None, 1] - v[:, None, 0]) +
+            c[None, :, 1:] * (v[:, None, 2] - v[:, None, 0]))
 
 
 def lsw_state(n_faces, n_sf, n_runs, sn0, ss0_0, ss0_1, mus, co0=0.0):

@@ -64,17 +64,18 @@ class Rate2Lts:
         self.t = [torch.zeros(solver.dof_shape(), dtype=solver.dtype, device="cuda") for _ in range(3)]
         self.counts = [0] * len(self.ranges)
 
-    def _step(self, dofs, g, stream):
+    def _step(self, dofs, g, stream, dofs_a):
         ft = self.counts[g] % 2 == 0
         first, n = self.ranges[g]
-        self.solver.local_lts(dofs, *self.t, self.dt0 * 2 ** g, ft, first, n, stream=stream)
+        self.solver.local_lts(dofs, *self.t, self.dt0 * 2 ** g, ft, first, n, stream=stream, dofs_a=dofs_a)
         if g > 0:
-            self._step(dofs, g - 1, stream)
-            self._step(dofs, g - 1, stream)
-        self.solver.neigh_lts(dofs, *self.t, ft, first, n, stream=stream)
+            self._step(dofs, g - 1, stream, dofs_a)
+            self._step(dofs, g - 1, stream, dofs_a)
+        self.solver.neigh_lts(dofs, *self.t, ft, first, n, stream=stream, dofs_a=dofs_a)
         self.counts[g] += 1
 
-    def cycle(self, dofs, n_cycles=1, stream=None):
-        """n_cycles steps of the coarsest group (2^(G-1) steps of the finest)."""
+    def cycle(self, dofs, n_cycles=1, stream=None, dofs_a=None):
+        """n_cycles steps of the coarsest group (2^(G-1) steps of the finest);
+        dofs_a: the anelastic DOFs of a viscoelastic solver."""
         for _ in range(n_cycles):
-            self._step(dofs, len(self.ranges) - 1, stream)
+            self._step(dofs, len(self.ranges) - 1, stream, dofs_a)

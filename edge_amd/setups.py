@@ -104,7 +104,7 @@ def map_points(mesh, ref_pts, els=None):
     """Physical coordinates of reference points per element: [nEl][nPts][3]."""
     ve = mesh.el_crds() if els is None else mesh.ve_crds[mesh.el_ve[els]]
     d = ve[:, 1:, :] - ve[:, :1, :]          # [e][i][j] = (v_{i+1} - v0)_j
-    return ve[:, None, 0, :] + np.einsum("pi,eij->epj", ref_pts, d)
+    return ve[:, None, 0, :] + np.matmul(ref_pts[None], d)
 
 
 def project(mesh, order, fun, n_runs=1, chunk=8192):
@@ -117,9 +117,14 @@ def project(mesh, order, fun, n_runs=1, chunk=8192):
     proj = (b * w[:, None]) / mass[None, :]    # [qp][N]
     n = n_modes(order)
     out = np.zeros((mesh.n_el, N_QTS, n, n_runs))
+    wave = isinstance(fun, PlaneWave)
     for e0 in range(0, mesh.n_el, chunk):
         els = np.arange(e0, min(mesh.n_el, e0 + chunk))
         x = map_points(mesh, pts, els)
+        if wave:  # q = r sin(k.x): project the scalar phase once, one matrix product
+            coef = np.sin(x @ fun.k) @ proj  # [e][N]
+            out[els] = coef[:, None, :, None] * fun.r[None, :, None, None]
+            continue
         f = fun(x)
         if f.ndim == 3:
             f = np.repeat(f[..., None], n_runs, axis=-1)

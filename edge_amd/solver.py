@@ -171,17 +171,29 @@ class ElasticSolver:
         fa_lts = np.ascontiguousarray(fa_lts, dtype=np.uint8)
         _lib.check(_lib.lib().edge_el_set_lts(self._h, el_lts.ctypes.data, fa_lts.ctypes.data))
 
-    def local_lts(self, dofs, t0, t1, t2, dt, first_ts, first=0, n=None, stream=None):
+    def local_lts(self, dofs, t0, t1, t2, dt, first_ts, first=0, n=None, stream=None, dofs_a=None):
+        """AderDg::local with local time stepping; dofs_a: the anelastic DOFs
+        of a viscoelastic solver."""
         for a in (dofs, t0, t1, t2):
             self._check_arr(a)
         n = self.mesh.n_el - first if n is None else n
+        if dofs_a is not None:
+            _lib.check(_lib.lib().edge_el_local_lts_a(self._h, first, n, float(dt), 1 if first_ts else 0,
+                                                      dofs.data_ptr(), dofs_a.data_ptr(), t0.data_ptr(),
+                                                      t1.data_ptr(), t2.data_ptr(), _stream_ptr(stream)))
+            return
         _lib.check(_lib.lib().edge_el_local_lts(self._h, first, n, float(dt), 1 if first_ts else 0, dofs.data_ptr(),
                                                 t0.data_ptr(), t1.data_ptr(), t2.data_ptr(), _stream_ptr(stream)))
 
-    def neigh_lts(self, dofs, t0, t1, t2, first_ts, first=0, n=None, stream=None):
+    def neigh_lts(self, dofs, t0, t1, t2, first_ts, first=0, n=None, stream=None, dofs_a=None):
         for a in (dofs, t0, t1, t2):
             self._check_arr(a)
         n = self.mesh.n_el - first if n is None else n
+        if dofs_a is not None:
+            _lib.check(_lib.lib().edge_el_neigh_lts_a(self._h, first, n, 1 if first_ts else 0, t0.data_ptr(),
+                                                      t1.data_ptr(), t2.data_ptr(), dofs.data_ptr(),
+                                                      dofs_a.data_ptr(), _stream_ptr(stream)))
+            return
         _lib.check(_lib.lib().edge_el_neigh_lts(self._h, first, n, 1 if first_ts else 0, t0.data_ptr(), t1.data_ptr(),
                                                 t2.data_ptr(), dofs.data_ptr(), _stream_ptr(stream)))
 

@@ -430,7 +430,8 @@ int edge_el_set_fault(edge_el_solver *s, edge_fault *f);
  * bit 1 "an adjacent element has a smaller time step" (tDofs[2]);
  * fa_lts[nEl][4]: 0 equal steps, 1 the element's step is the greater, 2 the
  * smaller (interior faces only).  NULL, NULL switches local time stepping
- * off.  Elastic solvers. */
+ * off.  Elastic and viscoelastic solvers (the latter step with
+ * edge_el_local_lts_a / edge_el_neigh_lts_a). */
 int edge_el_set_lts(edge_el_solver *s, const uint8_t *el_lts, const uint8_t *fa_lts);
 
 /* AderDg::local with local time stepping over elements [first, first+n):
@@ -445,6 +446,18 @@ int edge_el_local_lts(edge_el_solver *s, int64_t first, int64_t n_els, double dt
  * larger step its tDofs[2] (first_ts != 0) or tDofs[0] - tDofs[2]. */
 int edge_el_neigh_lts(edge_el_solver *s, int64_t first, int64_t n_els, int32_t first_ts, const void *t_int0,
                       const void *t_int1, const void *t_int2, void *dofs, void *stream);
+
+/* The same two steps with the anelastic DOFs dofs_a [nEl][n_rms][6][N][R]
+ * of a viscoelastic solver (AderDg::local / neigh with TL_N_RMS > 0 and
+ * local time stepping, AderDg.hpp:384-417, :620-720): tDofs[0..2] hold the
+ * elastic time-integrated DOFs, tDofs[2] the integral over [0, dt/2] of the
+ * CK with the anelastic terms; an interface face adds the anelastic flux
+ * update to every mechanism.  1 or 8 runs.  As the elastic calls for an
+ * elastic solver (dofs_a ignored). */
+int edge_el_local_lts_a(edge_el_solver *s, int64_t first, int64_t n_els, double dt, int32_t first_ts, void *dofs,
+                        void *dofs_a, void *t_int0, void *t_int1, void *t_int2, void *stream);
+int edge_el_neigh_lts_a(edge_el_solver *s, int64_t first, int64_t n_els, int32_t first_ts, const void *t_int0,
+                        const void *t_int1, const void *t_int2, void *dofs, void *dofs_a, void *stream);
 
 /* Which kernels a solver runs: *kernel = 1 (generic), 2 (MFMA-tiled) or 4
  * (FP64 single run on FP64 MFMA);

@@ -332,37 +332,50 @@ void orc_neigh(const orc_dg *dg, int64_t first, int64_t n_els, const int32_t *fa
  *                     first step of a pair, tDofs[0] - tDofs[2] on the second.
  * ---------------------------------------------------------------------- */
 void orc_local_lts(const orc_dg *dg, int64_t first, int64_t n_els, double dt, int first_ts, const double *starE,
-                   const double *fsE0, const uint8_t *el_lts, double *dofsE, double *t0, double *t1, double *t2) {
-  const int O = dg->order, N = nmodes(O), R = dg->n_crs;
-  const size_t sz = (size_t)NQ * N * R;
+                   const double *starA, const double *srcA, const double *fsE0, const double *fsA0,
+                   const uint8_t *el_lts, double *dofsE, double *dofsA, double *t0, double *t1, double *t2) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs, NR = dg->n_rms;
+  const size_t sz = (size_t)NQ * N * R, szA = (size_t)NQA * N * R;
 #pragma omp parallel
   {
     double *der = (double *)malloc(sizeof(double) * O * sz);
+    double *derA = (double *)malloc(sizeof(double) * (NR > 0 ? NR : 1) * O * szA);
+    double *tA = (double *)malloc(sizeof(double) * (NR > 0 ? NR : 1) * szA);
 #pragma omp for schedule(static)
     for (int64_t el = first; el < first + n_els; el++) {
-      orc_ck(dg, dt, starE + el * 243, NULL, NULL, dofsE + el * sz, NULL, der, NULL, t0 + el * sz, NULL);
+      double *dA = (NR > 0) ? dofsA + (size_t)el * NR * szA : NULL;
+      /* AderDg.hpp:371-381: the CK (with the anelastic terms) into tDofs[0] */
+      orc_ck(dg, dt, starE + el * 243, NR ? starA + el * 54 : NULL, NR ? srcA + el * NR * 36 : NULL,
+             dofsE + el * sz, dA, der, derA, t0 + el * sz, tA);
+      /* :384-411: tDofs[1] sums tDofs[0] over a pair of steps */
       if (el_lts[el] & 1) {
         for (size_t i = 0; i < sz; i++) t1[el * sz + i] = (first_ts ? 0.0 : t1[el * sz + i]) + t0[el * sz + i];
       }
+      /* :413-417: tDofs[2] integrates the (elastic) derivatives over [0, dt/2] */
       if (el_lts[el] & 2) orc_integrate(dg, 0.5 * dt, der, t2 + el * sz);
-      orc_vol(dg, starE + el * 243, NULL, NULL, t0 + el * sz, NULL, dofsE + el * sz, NULL);
-      orc_surf_local(dg, fsE0 + el * 324, NULL, t0 + el * sz, dofsE + el * sz, NULL);
+      orc_vol(dg, starE + el * 243, NR ? starA + el * 54 : NULL, NR ? srcA + el * NR * 36 : NULL, t0 + el * sz, tA,
+              dofsE + el * sz, dA);
+      orc_surf_local(dg, fsE0 + el * 324, NR ? fsA0 + el * 216 : NULL, t0 + el * sz, dofsE + el * sz, dA);
     }
     free(der);
+    free(derA);
+    free(tA);
   }
 }
 
 void orc_neigh_lts(const orc_dg *dg, int64_t first, int64_t n_els, int first_ts, const int32_t *fa_type,
                    const int64_t *el_fa, const int64_t *el_fa_el, const uint16_t *f_id, const uint16_t *v_id,
-                   const double *fsE1, const uint8_t *fa_lts, const double *t0, const double *t1,
-                   const double *t2, double *dofsE) {
-  const int O = dg->order, N = nmodes(O), R = dg->n_crs;
-  const size_t sz = (size_t)NQ * N * R;
+                   const double *fsE1, const double *fsA1, const uint8_t *fa_lts, const double *t0, const double *t1,
+                   const double *t2, double *dofsE, double *dofsA) {
+  const int O = dg->order, N = nmodes(O), R = dg->n_crs, NR = dg->n_rms;
+  const size_t sz = (size_t)NQ * N * R, szA = (size_t)NQA * N * R;
 #pragma omp parallel
   {
     double *tn = (double *)malloc(sizeof(double) * sz);
+    double *upA = (double *)malloc(sizeof(double) * szA);
 #pragma omp for schedule(static)
     for (int64_t el = first; el < first + n_els; el++) {
+      for (size_t i = 0; i < szA; i++) upA[i] = 0.0;
       for (int fa = 0; fa < 4; fa++) {
         const int64_t fid = el_fa[el * 4 + fa];
         const int32_t ty = fa_type[fid];
@@ -374,6 +387,7 @@ void orc_neigh_lts(const orc_dg *dg, int64_t first, int64_t n_els, int first_ts,
           vId = v_id[el * 4 + fa];
           fId = f_id[el * 4 + fa];
         }
+        /* AderDg.hpp:620-674: the neighbour's tDofs by the faces' time steps */
         const int mode = fa_lts[el * 4 + fa];
         const double *src = t0 + ne * sz;
         if (mode == 1) src = t1 + ne * sz;
@@ -382,10 +396,14 @@ void orc_neigh_lts(const orc_dg *dg, int64_t first, int64_t n_els, int first_ts,
           for (size_t i = 0; i < sz; i++) tn[i] = t0[ne * sz + i] - t2[ne * sz + i];
           src = tn;
         }
-        orc_surf_neigh(dg, fa, vId, fId, fsE1 + (el * 4 + fa) * 81, NULL, src, dofsE + el * sz, NULL);
+        orc_surf_neigh(dg, fa, vId, fId, fsE1 + (el * 4 + fa) * 81, NR ? fsA1 + (el * 4 + fa) * 54 : NULL, src,
+                       dofsE + el * sz, upA);
       }
+      /* :715-720 */
+      if (NR > 0) orc_scatter_update_a(dg, upA, dofsA + (size_t)el * NR * szA);
     }
     free(tn);
+    free(upA);
   }
 }
 

@@ -415,17 +415,18 @@ class MeshOracle:
         self.el_lts = np.ascontiguousarray(el_lts, dtype=np.uint8)
         self.fa_lts = np.ascontiguousarray(fa_lts, dtype=np.uint8)
 
-    def local_lts(self, dofs, t0, t1, t2, dt, first_ts, first=0, n=None):
+    def local_lts(self, dofs, t0, t1, t2, dt, first_ts, first=0, n=None, dofs_a=None):
         n = self.mesh.n_el - first if n is None else n
         lib().orc_local_lts(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), C.c_double(dt),
-                            C.c_int(int(first_ts)), _p(self.star), _p(self.fs[0]), _p(self.el_lts), _p(dofs),
-                            _p(t0), _p(t1), _p(t2))
+                            C.c_int(int(first_ts)), _p(self.star), _p(self.star_a), _p(self.src_a), _p(self.fs[0]),
+                            _p(self.fs_a[0]), _p(self.el_lts), _p(dofs), _p(dofs_a), _p(t0), _p(t1), _p(t2))
 
-    def neigh_lts(self, dofs, t0, t1, t2, first_ts, first=0, n=None):
+    def neigh_lts(self, dofs, t0, t1, t2, first_ts, first=0, n=None, dofs_a=None):
         n = self.mesh.n_el - first if n is None else n
         lib().orc_neigh_lts(C.byref(self.dg.s), C.c_int64(first), C.c_int64(n), C.c_int(int(first_ts)),
                             _p(self.fa_type), _p(self.el_fa), _p(self.el_fa_el), _p(self.f_id), _p(self.v_id),
-                            _p(self.fs[1]), _p(self.fa_lts), _p(t0), _p(t1), _p(t2), _p(dofs))
+                            _p(self.fs[1]), _p(self.fs_a[1]), _p(self.fa_lts), _p(t0), _p(t1), _p(t2), _p(dofs),
+                            _p(dofs_a))
 
 
 def lts_groups(mesh, group):

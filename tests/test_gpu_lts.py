@@ -10,7 +10,9 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.lts import Rate2Lts
-from edge_amd.solver import PATH_PROJECTIONS, ElasticSolver
+from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
+from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, PATH_PROJECTIONS,
+                             ElasticSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -62,4 +64,54 @@ def test_lts_matches_oracle(gpu, prec, order, runs, n_groups):
     gts = q0.copy()
     o.step(gts, np.zeros_like(gts), dt0, 2 ** (n_groups - 1) * 2)
     assert np.linalg.norm(gts - ref) / np.linalg.norm(ref) > 10 * err
+    s.close()
+
+
+@pytest.mark.parametrize("prec,runs,kernel,tol", [(64, 1, KERNEL_MFMA64, 1e-12), (64, 1, KERNEL_GENERIC, 1e-12),
+                                                  (32, 8, KERNEL_MFMA, 1e-5), (32, 8, KERNEL_MFMA_TILES, 1e-5)])
+def test_visco_rate2_matches_oracle(gpu, prec, runs, kernel, tol):
+    """Viscoelastic solvers with three time groups (AderDg with TL_N_RMS > 0
+    and local time stepping): tDofs[2] from the CK with the anelastic terms,
+    interface faces with the anelastic update."""
+    import torch
+    from edge_amd.solver import ViscoSolver
+    order, nr = 4, 3
+    m = M.box(4, bc=(FA_PERIODIC, FA_FREE_SURFACE, FA_OUTFLOW), jitter=0.1, seed=12)
+    rho, lam, mu = S.random_materials(m.n_el, seed=12)
+    qp = np.random.default_rng(12).uniform(20, 60, m.n_el)
+    s = ViscoSolver(m, order, runs, prec, rho, lam, mu, qp, qp / 2, nr, 2.0, 10.0, kernel=kernel)
+    assert s.kernel_info()[0] == (KERNEL_MFMA if kernel == KERNEL_MFMA_TILES else kernel)
+    visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qp / 2, nr, 2.0, 10.0)
+    mo = orc.MeshOracle(m, order, runs, visco=visco)
+    dt0 = S.dt_cfl(m, rho, visco["lam_e"], visco["mu_e"], order) / 4
+    n3 = m.n_el // 3
+    ranges = [(0, n3), (n3, n3), (2 * n3, m.n_el - 2 * n3)]
+    lts = Rate2Lts(s, ranges, dt0)
+    mo.set_lts(lts.el_lts, lts.fa_lts)
+    rng = np.random.default_rng(4)
+    n = S.n_modes(order)
+    d0 = rng.normal(size=(m.n_el, 9, n, runs))
+    a0 = 0.1 * rng.normal(size=(m.n_el, nr, 6, n, runs))
+    d, a = d0.copy(), a0.copy()
+    bufs = [np.zeros_like(d0) for _ in range(3)]
+    counts = [0] * 3
+
+    def step(g):
+        ft = counts[g] % 2 == 0
+        first, cnt = ranges[g]
+        mo.local_lts(d, *bufs, dt0 * 2 ** g, ft, first, cnt, dofs_a=a)
+        if g > 0:
+            step(g - 1)
+            step(g - 1)
+        mo.neigh_lts(d, *bufs, ft, first, cnt, dofs_a=a)
+        counts[g] += 1
+
+    step(2)
+    dt_t = torch.float64 if prec == 64 else torch.float32
+    gd = torch.tensor(d0, dtype=dt_t, device="cuda")
+    ga = torch.tensor(a0, dtype=dt_t, device="cuda")
+    lts.cycle(gd, 1, dofs_a=ga)
+    for got, ref in ((gd, d), (ga, a)):
+        g = got.cpu().double().numpy()
+        assert np.linalg.norm(g - ref) / np.linalg.norm(ref) <= tol
     s.close()

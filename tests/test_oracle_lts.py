@@ -89,3 +89,105 @@ def test_product_characteristics_match_the_oracle():
         g = np.random.default_rng(1).integers(0, 3, m.n_el)
         for a, b in zip(characteristics(m, g), orc.lts_groups(m, g)):
             np.testing.assert_array_equal(a, b)
+
+
+def _rate2_a(o, dofs, dofs_a, bufs, dt0, groups, n_cycles):
+    counts = [0] * len(groups)
+
+    def step(g):
+        ft = counts[g] % 2 == 0
+        first, n = groups[g]
+        o.local_lts(dofs, *bufs, dt0 * 2 ** g, ft, first, n, dofs_a=dofs_a)
+        if g > 0:
+            step(g - 1)
+            step(g - 1)
+        o.neigh_lts(dofs, *bufs, ft, first, n, dofs_a=dofs_a)
+        counts[g] += 1
+
+    for _ in range(n_cycles):
+        step(len(groups) - 1)
+
+
+def _visco_problem(n, order, nr=3, f_cen=0.5):
+    m = M.box(n)
+    ne = m.n_el
+    rho, lam, mu = np.ones(ne), 2 * np.ones(ne), np.ones(ne)
+    qp = np.full(ne, 20.0)
+    visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qp / 2, nr, f_cen, 10.0)
+    o = orc.MeshOracle(m, order, visco=visco)
+    q0 = S.project(m, order, S.PlaneWave(k=(2 * np.pi, 2 * np.pi, 0.0)))
+    # anelastic state: smooth, so that the anelastic terms act from the first derivative on
+    a0 = np.zeros((ne, nr, 6, S.n_modes(order), 1))
+    for rm in range(nr):
+        a0[:, rm] = 0.2 * (rm + 1) * S.project(m, order, S.PlaneWave(k=(0.0, 2 * np.pi, 2 * np.pi)))[:, :6]
+    dt = S.dt_cfl(m, rho, visco["lam_e"], visco["mu_e"], order)
+    return m, o, q0, a0, dt
+
+
+def test_visco_one_group_is_gts():
+    m, o, q0, a0, dt = _visco_problem(3, 3)
+    o.set_lts(*orc.lts_groups(m, np.zeros(m.n_el, dtype=int)))
+    a, b = q0.copy(), q0.copy()
+    aa, ba = a0.copy(), a0.copy()
+    ta = np.zeros_like(a)
+    bufs = [np.zeros_like(a) for _ in range(3)]
+    for s in range(3):
+        o.step(a, ta, dt, dofs_a=aa)
+        o.local_lts(b, *bufs, dt, s % 2 == 0, dofs_a=ba)
+        o.neigh_lts(b, *bufs, s % 2 == 0, dofs_a=ba)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(aa, ba)
+
+
+@pytest.mark.parametrize("order", [3, 5])
+def test_visco_half_step_is_the_ck_integral(order):
+    """tDofs[2] is the Taylor integral over [0, dt/2] of the derivatives the
+    CK computes with the anelastic terms: the time-integrated DOFs of a
+    local step with dt/2 from the same DOFs (the CK's derivatives do not
+    depend on the step)."""
+    m, o, q0, a0, dt = _visco_problem(3, order, f_cen=5.0)
+    o.set_lts(np.full(m.n_el, 2, dtype=np.uint8), np.zeros((m.n_el, 4), dtype=np.uint8))
+    bufs = [np.zeros_like(q0) for _ in range(3)]
+    o.local_lts(q0.copy(), *bufs, dt, True, dofs_a=a0.copy())
+    ref = np.zeros_like(q0)
+    o.local(q0.copy(), ref, 0.5 * dt, dofs_a=a0.copy())
+    np.testing.assert_allclose(bufs[2], ref, rtol=0, atol=1e-14 * np.abs(ref).max())
+    # the anelastic terms matter: the elastic CK's half-step integral differs
+    oe = orc.MeshOracle(m, order, rho=np.ones(m.n_el), lam=2 * np.ones(m.n_el), mu=np.ones(m.n_el),
+                        star=o.star, fs=o.fs)
+    el = np.zeros_like(q0)
+    oe.local(q0.copy(), el, 0.5 * dt)
+    assert np.linalg.norm(el - ref) > 1e-6 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_visco_two_groups_converge_to_gts(order):
+    """With attenuation (Q_p = 20, Q_s = 10, 3 mechanisms over [0.05, 0.5] x
+    10) the rate-2 scheme and global time stepping at the fine step
+    approximate the same solution: the elastic difference falls at the
+    scheme's order under refinement, and neither difference exceeds three
+    times the time discretisation error of global time stepping itself (its
+    step against half its step; the anelastic memory variables converge at
+    about second order in time under either scheme)."""
+    diffs = []
+    for n in (4, 8):
+        m, o, q0, a0, dt = _visco_problem(n, order)
+        half = m.n_el // 2
+        group = np.r_[np.zeros(half, dtype=int), np.ones(m.n_el - half, dtype=int)]
+        el_lts, fa_lts = orc.lts_groups(m, group)
+        o.set_lts(el_lts, fa_lts)
+        dt = dt / 2
+        cycles = int(round(0.05 / (2 * dt)))
+        lts, lts_a = q0.copy(), a0.copy()
+        bufs = [np.zeros_like(q0) for _ in range(3)]
+        _rate2_a(o, lts, lts_a, bufs, dt, [(0, half), (half, m.n_el - half)], cycles)
+        gts, gts_a, t = q0.copy(), a0.copy(), np.zeros_like(q0)
+        o.step(gts, t, dt, 2 * cycles, dofs_a=gts_a)
+        diffs.append(np.linalg.norm(lts - gts) / np.linalg.norm(gts))
+        half, half_a = q0.copy(), a0.copy()
+        o.step(half, t, dt / 2, 4 * cycles, dofs_a=half_a)
+        e_t = np.linalg.norm(gts - half) / np.linalg.norm(half)
+        e_ta = np.linalg.norm(gts_a - half_a) / np.linalg.norm(half_a)
+        assert diffs[-1] < 3 * e_t, (n, diffs[-1], e_t)
+        assert np.linalg.norm(lts_a - gts_a) / np.linalg.norm(gts_a) < 3 * e_ta
+    assert np.log2(diffs[0] / diffs[1]) > order - 0.6, diffs
