@@ -56,9 +56,48 @@ def local_bytes(order, runs, prec=4, proj=False):
     return 3 * 9 * n * runs * prec + (3 * 81 + 4 * 81) * prec + (4 * 3 * 256 * prec if proj else 0)
 
 
-# HBM traffic of the local kernel: with the face-projection exchange, and without
-TRAFFIC_PROFILE = {True: os.path.join(ROOT, "profiles", "traffic_k_local_o5_fp32_r16_196608tets.md"),
-                   False: os.path.join(ROOT, "profiles", "traffic_k_local_tiles_o5_fp32_r16_196608tets.md")}
+def launch_bytes(order, runs, variant, prec=4):
+    """Compulsory bytes per element of the dominant FP32 launch: the local
+    kernel (local_bytes), or the fused one, which also reads the face
+    projections its neighbours wrote and the neighbour-side flux solvers."""
+    if variant != "fused":
+        return local_bytes(order, runs, prec, proj=variant == "proj")
+    return local_bytes(order, runs, prec, proj=True) + 4 * 3 * 256 * prec + 4 * 81 * prec + 80
+
+
+# FP32 variants: (projections exchanged, XCD-aware schedule, fused multi-step)
+FP32_VARIANTS = {"fused": (True, True, True), "proj": (True, True, False), "tiles": (False, True, False)}
+
+# HBM traffic of the dominant FP32 kernel per variant
+TRAFFIC_PROFILE = {"fused": os.path.join(ROOT, "profiles", "traffic_k_local_fused_o5_fp32_r16_196608tets.md"),
+                   "proj": os.path.join(ROOT, "profiles", "traffic_k_local_o5_fp32_r16_196608tets.md"),
+                   "tiles": os.path.join(ROOT, "profiles", "traffic_k_local_tiles_o5_fp32_r16_196608tets.md")}
+# MFMA utilisation (SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE x SIMDs) per kernel
+MFMA_PROFILE = os.path.join(ROOT, "profiles", "mfma_util_o5_196608tets.md")
+
+
+def dom_kernel_name(variant, order):
+    return {"fused": f"k_local<{order}, true, true, true, true>", "proj": f"k_local<{order}, true, true, true, false>",
+            "tiles": f"k_local<{order}, true, true, false, false>"}[variant]
+
+
+def read_mfma_util(path, n_el, order, runs, kernel):
+    """The MFMA busy fraction of `kernel` from a committed rocprofv3 counter
+    summary (a table row `| kernel | ... | util |`, whose last column is the
+    busy fraction), or None when absent or for another workload."""
+    try:
+        with open(path) as f:
+            text = f.read()
+        m = re.search(r"elements\s*=\s*(\d+)\s*\n\s*order\s*=\s*(\d+)", text)
+        if not m or (int(m.group(1)), int(m.group(2))) != (n_el, order):
+            return None
+        for line in text.splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 3 and cells[0].strip("`") == kernel and cells[1] == str(runs):
+                return float(cells[-1])
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def read_traffic(path, n_el, order, runs):
@@ -126,49 +165,68 @@ def point_setup(solver, m, runs, seed=3343, n_recv=64):
     return src, recv
 
 
-def time_solver(solver, dofs, tint, dt, steps, warmup, points=None):
-    """Times exactly `steps` steps between synchronisations; per-kernel
-    durations from HIP events recorded around each launch on the stream
-    the kernels run on.  With points = (sources, receivers), every step is
-    the reference's: receivers at the step start, local, sources over the
-    step, neigh."""
+def time_solver(solver, dofs, tint, dt, steps, warmup, points=None, fused=False):
+    """Times exactly `steps` steps between synchronisations, after `warmup`
+    untimed ones; per-kernel durations from HIP events recorded around each
+    launch on the stream the kernels run on.  With points = (sources,
+    receivers), every step is the reference's (src/impl/seismic/steps.inc):
+    receivers at the step start, local, sources over the step, neigh.
+    fused: the steps run as local, neigh_local x (steps - 1), neigh (one
+    launch per step, the neighbour step of step k - 1 with the local step of
+    step k; the sources between the launches, the receivers on the
+    step-start DOFs the fused launches write to taps).
+    Returns (wall seconds, {"local" | "fused" | "neigh": mean seconds})."""
     import torch
     stream = torch.cuda.current_stream()
     src, recv = points if points is not None else (None, None)
-    out = None
+    out = taps = tau = None
     if recv is not None:
         out = torch.empty((recv.n, 1, 9, solver.n_runs), dtype=dofs.dtype, device=dofs.device)
         tau = np.zeros((recv.n, 1))
+        if fused:
+            taps = torch.zeros_like(dofs)
+            solver.set_taps(recv.el, taps)
 
-    def one(i, ev=None):
-        t = i * dt
-        if recv is not None:
-            recv.eval_into(dofs, tau, out, stream)
-        if ev is not None:
-            ev[0].record(stream)
-        solver.local(dofs, tint, dt, stream=stream)
-        if ev is not None:
-            ev[1].record(stream)
-        if src is not None:
-            src.apply(t, t + dt, dofs, stream)
-        if ev is not None:
-            ev[2].record(stream)
-        solver.neigh(dofs, tint, stream=stream)
-        if ev is not None:
-            ev[3].record(stream)
+    def run(n, i0, rec):
+        def timed(name, fn):
+            if rec is None:
+                fn()
+                return
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            rec.setdefault(name, []).append((a, b))
 
-    for i in range(warmup):
-        one(i)
+        for k in range(n):
+            t = (i0 + k) * dt
+            if not fused or k == 0:
+                if recv is not None:
+                    recv.eval_into(dofs, tau, out, stream)
+                timed("local", lambda: solver.local(dofs, tint, dt, stream=stream))
+            else:
+                timed("fused", lambda: solver.neigh_local(dofs, tint, dt, stream=stream))
+                if recv is not None:
+                    recv.eval_into(taps, tau, out, stream)
+            if src is not None:
+                src.apply(t, t + dt, dofs, stream)
+            if not fused:
+                timed("neigh", lambda: solver.neigh(dofs, tint, stream=stream))
+        if fused:
+            timed("neigh", lambda: solver.neigh(dofs, tint, stream=stream))
+
+    if warmup > 0:
+        run(warmup, 0, None)
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+    rec = {}
     t0 = time.perf_counter()
-    for i in range(steps):
-        one(warmup + i, ev[i])
+    run(steps, warmup, rec)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    loc = np.mean([e[0].elapsed_time(e[1]) for e in ev]) * 1e-3
-    nei = np.mean([e[2].elapsed_time(e[3]) for e in ev]) * 1e-3
-    return wall, loc, nei
+    if taps is not None:
+        solver.set_taps(None)
+    return wall, {k: float(np.mean([a.elapsed_time(b) for a, b in v])) * 1e-3 for k, v in rec.items()}
 
 
 def host_cpu():
@@ -519,6 +577,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="--rows without the 24-mesh ensemble")
     ap.add_argument("--only", default="", help="--rows: comma list of visco, ensemble, lts, points, rupture, fault")
     ap.add_argument("--no-autotune", action="store_true", help="measure the kernels --kernel selects, untuned")
+    ap.add_argument("--fp32-variant", default="auto", choices=["auto"] + sorted(FP32_VARIANTS),
+                    help="fix the FP32 kernel variant (no autotuning): fused, proj or tiles")
     ap.add_argument("--rows", action="store_true",
                     help="measure the other rows (viscoelastic, LTS, receivers/sources, rupture) instead")
     ap.add_argument("--no-points", action="store_true",
@@ -538,7 +598,7 @@ def main():
         rows(args)
         return
     import torch
-    from edge_amd.solver import PATH_PROJECTIONS, ElasticSolver
+    from edge_amd.solver import KERNEL_MFMA, PATH_FUSED, PATH_PROJECTIONS, ElasticSolver
 
     order, runs = args.order, 16
     print(f"[bench] building the {6 * args.hex ** 3}-tet mesh (order {order})", file=sys.stderr, flush=True)
@@ -550,20 +610,24 @@ def main():
     # ---- FP32 fused-16 (headline)
     s32 = ElasticSolver(m, order, runs, 32, rho, lam, mu, kernel=args.kernel)
     tune = {}
-    if not args.no_autotune:  # untimed set-up: the fastest kernel variant on this mesh
-        tune["fp32"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "ms_per_step": ms}
-                        for k, p, x, ms in s32.autotune(dt, 3)]
-    proj = bool(s32.kernel_info()[1] & PATH_PROJECTIONS)
+    if args.fp32_variant != "auto":  # a fixed variant (profiling): no autotuning
+        s32.set_variant(KERNEL_MFMA, *FP32_VARIANTS[args.fp32_variant])
+    elif not args.no_autotune:  # untimed set-up: the fastest kernel variant on this mesh
+        tune["fp32"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "fused": f, "ms_per_step": ms}
+                        for k, p, x, ms, f in s32.autotune(dt, 3)]
+    flags = s32.kernel_info()[1]
+    proj, fused = bool(flags & PATH_PROJECTIONS), bool(flags & PATH_FUSED)
+    variant = "fused" if fused else ("proj" if proj else "tiles")
     d32 = (torch.from_numpy(q0)[..., None] * scale).to(torch.float32).cuda().contiguous()
     t32 = torch.zeros_like(d32)
     points = None if args.no_points else point_setup(s32, m, runs)
-    print("[bench] fp32 fused-16 timing", file=sys.stderr, flush=True)
-    wall, loc, nei = time_solver(s32, d32, t32, dt, args.steps, args.warmup, points)
+    print(f"[bench] fp32 fused-16 timing ({variant})", file=sys.stderr, flush=True)
+    wall, ks = time_solver(s32, d32, t32, dt, args.steps, args.warmup, points, fused=fused)
     if points is not None:
         for p in points:
             p.close()
     ok = bool(torch.isfinite(d32).all())
-    res["fp32"] = dict(wall=wall, local=loc, neigh=nei, finite=ok)
+    res["fp32"] = dict(wall=wall, k=ks, finite=ok)
     del d32, t32
     s32.close()
 
@@ -572,12 +636,12 @@ def main():
         s64 = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=args.kernel)
         if not args.no_autotune:
             tune["fp64"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "ms_per_step": ms}
-                            for k, p, x, ms in s64.autotune(dt, 2)]
+                            for k, p, x, ms, _ in s64.autotune(dt, 2)]
         d64 = torch.from_numpy(q0[..., None].copy()).cuda().contiguous()
         t64 = torch.zeros_like(d64)
         print("[bench] fp64 plain timing", file=sys.stderr, flush=True)
-        w64, l64, n64 = time_solver(s64, d64, t64, dt, max(1, args.steps // 2), args.warmup)
-        res["fp64"] = dict(wall=w64, local=l64, neigh=n64, steps=max(1, args.steps // 2))
+        w64, k64 = time_solver(s64, d64, t64, dt, max(1, args.steps // 2), args.warmup)
+        res["fp64"] = dict(wall=w64, k=k64, steps=max(1, args.steps // 2))
         del d64, t64
         s64.close()
 
@@ -586,18 +650,32 @@ def main():
     fl = FLOPS.get(order)
     roof = None
     if fl:
-        f_loc = fl["local"] + (fl["proj"] if proj else 0)
-        f_nei = fl["neigh"] - (fl["proj"] if proj else 0)
-        achieved = f_loc * runs * m.n_el / res["fp32"]["local"]
-        step_ach = (fl["local"] + fl["neigh"]) * runs * m.n_el / (res["fp32"]["local"] + res["fp32"]["neigh"])
-        traffic = read_traffic(TRAFFIC_PROFILE[proj], m.n_el, order, runs)
+        k32 = res["fp32"]["k"]
+        f_step = fl["local"] + fl["neigh"]
+        if fused and "fused" in k32:
+            # the fused launch: neighbour step k - 1 (projection lifts) and
+            # local step k with the projections it writes: a whole step's FLOPs
+            dom, f_dom, t_dom = "fused", f_step, k32["fused"]
+            t_step = (k32["local"] + (args.steps - 1) * k32["fused"] + k32["neigh"]) / args.steps
+            desc = ("fused neighbour step k - 1 + local step k (projection lifts; CK + volume + local surface + "
+                    "neighbour-flux projections)")
+        else:
+            f_loc = fl["local"] + (fl["proj"] if proj else 0)
+            dom, f_dom, t_dom = "local", f_loc, k32["local"]
+            t_step = k32["local"] + k32["neigh"]
+            desc = "local (CK + volume + local surface" + (" + neighbour-flux projections)" if proj else ")")
+        achieved = f_dom * runs * m.n_el / t_dom
+        step_ach = f_step * runs * m.n_el / t_step
+        traffic = read_traffic(TRAFFIC_PROFILE[variant], m.n_el, order, runs)
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32_mfma"] / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK["fp32_mfma"], "traffic": traffic,
-                "traffic_alg": local_bytes(order, runs, proj=proj) * m.n_el,
-                "kernel": "local (CK + volume + local surface" + (" + neighbour-flux projections)" if proj else ")"),
-                "flops_per_element_run": f_loc,
-                "neigh_tflops": f_nei * runs * m.n_el / res["fp32"]["neigh"] / 1e12,
+                "traffic_alg": launch_bytes(order, runs, variant) * m.n_el,
+                "kernel": desc, "kernel_ms": t_dom * 1e3, "flops_per_element_run": f_dom,
+                "kernel_ms_all": {k: v * 1e3 for k, v in k32.items()},
                 "step_tflops": step_ach / 1e12, "step_frac": step_ach / PEAK["fp32_mfma"]}
+        util = read_mfma_util(MFMA_PROFILE, m.n_el, order, runs, dom_kernel_name(variant, order))
+        if util is not None:
+            roof["mfma_busy"] = util
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * res["fp32"]["wall"] / args.steps, "higher_is_better": True,
@@ -606,19 +684,23 @@ def main():
                                f"+-10% jitter, random rho/vp), plane-wave initial DOFs scaled per run"
                                + ("" if args.no_points else f", one Ricker moment-tensor point source per run, "
                                   f"64 receivers sampled every step"),
-                   "elements": m.n_el, "order": order, "runs": runs, "dt": dt, "finite": res["fp32"]["finite"]},
+                   "elements": m.n_el, "order": order, "runs": runs, "dt": dt, "variant": variant,
+                   "finite": res["fp32"]["finite"]},
         "roofline": roof,
     }
     if tune:
         out["autotune"] = tune
     if "fp64" in res:
+        k64 = res["fp64"]["k"]
         v64 = dofs_per_step * res["fp64"]["steps"] / res["fp64"]["wall"]
         out["fp64_plain"] = {"value": v64, "unit": "DOF-updates/s", "ms_per_step": 1e3 * res["fp64"]["wall"] /
-                             res["fp64"]["steps"], "local_ms": 1e3 * res["fp64"]["local"],
-                             "neigh_ms": 1e3 * res["fp64"]["neigh"]}
+                             res["fp64"]["steps"], "local_ms": 1e3 * k64["local"], "neigh_ms": 1e3 * k64["neigh"]}
         if fl:
-            a64 = (fl["local"] + fl["neigh"]) * m.n_el / (res["fp64"]["local"] + res["fp64"]["neigh"])
+            a64 = (fl["local"] + fl["neigh"]) * m.n_el / (k64["local"] + k64["neigh"])
             out["fp64_plain"]["roofline_frac"] = a64 / PEAK["fp64"]
+            u64 = read_mfma_util(MFMA_PROFILE, m.n_el, order, 1, f"k_local_p64<{order}, true>")
+            if u64 is not None:
+                out["fp64_plain"]["mfma_busy_local"] = u64
         out["speedup_fp32_fused16_vs_fp64_plain"] = value / v64
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

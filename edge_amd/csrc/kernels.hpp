@@ -63,6 +63,11 @@ struct SolverDev {
   const int32_t* snd_v = nullptr;  // [nEl][4]: receiving element-face (el * 4 + fa) of this face, -1: none
   const uint8_t* snd_fm = nullptr; // [nEl][4]: flux matrix (index into fluxLN) of that projection
   const int32_t* rcv = nullptr;    // [nEl][4]: projection slot (el * 4 + f) feeding this face, -1: none
+  // fused neighbour + local launches: the DOFs each local step starts from
+  // (after the neighbour step) also go to tap (the DOF layout) for the
+  // elements with tap_on[el] != 0; null: none
+  const uint8_t* tap_on = nullptr;
+  T* tap = nullptr;
   // viscoelastic solvers (n_rms > 0)
   int n_rms = 0;
   const T* rfs = nullptr;     // [n_rms] relaxation frequencies
@@ -130,14 +135,17 @@ struct SolverRef {
 };
 int solver_ref(const edge_el_solver* s, SolverRef* out);
 
-// Per-block partial sums of the FP64 L2 norms of DOF arrays a (and a - b):
-// part[nblk][2][9 R] = (sum |J| M_n (a - b)^2, sum |J| M_n b^2) over each
-// block's element range; every entry is written.
 // The fault phase of the coupled step (fault.hip): sub-face states from
 // tInt, friction and net updates, DOF updates of both sides.
 int fault_apply(edge_fault* f, double dt, const void* t_int, void* dofs, hipStream_t st);
 const edge_el_solver* fault_solver(const edge_fault* f);
 
+// Blocks of the L2-norm reduction (their partial sums are added in block
+// order, on the host or on the device).
+constexpr unsigned kL2Blocks = 1024;
+// Per-block partial sums of the FP64 L2 norms of DOF arrays a (and a - b):
+// part[nblk][2][9 R] = (sum |J| M_n (a - b)^2, sum |J| M_n b^2) over each
+// block's element range; every entry is written.
 int l2_partials(edge_el_solver* s, const void* a, int prec_a, const void* b, int prec_b, double* part, unsigned nblk,
                 hipStream_t st);
 

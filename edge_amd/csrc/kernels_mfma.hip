@@ -656,12 +656,60 @@ __device__ __forceinline__ void load_proj(const float* __restrict__ p, v4f (&w)[
   for (int j = 0; j < 3; j++) w[j] = reinterpret_cast<const v4f*>(p + j * 256)[lane];
 }
 
+// The neighbouring surface integral of one element from the face
+// projections its neighbours wrote (proj, [slot][3][256] lane-major):
+// acc_q += sum_fa U_fa (F^T_fa^T w_fa), acc in the D layout.  At order 5
+// every face's projection is requested up front and each face's lift
+// operands one face ahead; below, the registers this takes would halve the
+// occupancy.
+// init(acc) fills the accumulators (the element's DOFs) once the projections
+// are requested.
+template <int O, typename Init>
+__device__ __forceinline__ void neigh_lifts(const SolverDev<float>& s, const float* __restrict__ proj, int64_t el,
+                                            v4f (&acc)[9][Dims<O>::NT], int lane, Init init) {
+  constexpr int NT = Dims<O>::NT;
+  constexpr bool AHEAD = O >= 5;
+  const v4f* plft = reinterpret_cast<const v4f*>(s.dg.pk + pk::Layout{O}.lft());
+  int32_t rcv[4];
+  v4f w[AHEAD ? 4 : 1][3];
+#pragma unroll
+  for (int fa = 0; fa < 4; fa++) {
+    rcv[fa] = as_const(s.rcv)[el * 4 + fa];
+    if (AHEAD && rcv[fa] >= 0) load_proj(proj + (int64_t)rcv[fa] * kProjFace, w[AHEAD ? fa : 0], lane);
+  }
+  v4f at[NT];
+  load_lift<O>(plft, at, lane);
+  init(acc);
+#pragma unroll
+  for (int fa = 0; fa < 4; fa++) {
+    v4f atn[NT];
+    if (AHEAD && fa < 3) load_lift<O>(plft + (fa + 1) * NT * 64, atn, lane);
+    if (!AHEAD && fa > 0) load_lift<O>(plft + fa * NT * 64, at, lane);
+    if (rcv[fa] >= 0) {  // else outflow: no projection (uniform across the wave)
+      if (!AHEAD) load_proj(proj + (int64_t)rcv[fa] * kProjFace, w[0], lane);
+      lift3<O>(at, as_const(s.fsf + ((s.n_el + el) * 4 + fa) * 54), w[AHEAD ? fa : 0], acc);
+    }
+    if (AHEAD && fa < 3) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) at[t] = atn[t];
+    }
+  }
+}
+
 // WP: also write the face projections for the neighbour step (s.proj,
-// rank-3 solvers only)
-template <int O, bool SP, bool R3, bool WP = false>
+// rank-3 solvers only).
+// NB: first the previous time step's neighbouring surface integral from the
+// projections in proj_in (written by the previous launch; s.proj receives
+// this step's, so the two alternate between two buffers).  The updated DOFs
+// go to LDS for the CK and back to global memory, from where the output
+// stage reads them as it reads Q otherwise (and to s.tap for the elements
+// s.tap_on marks).  One launch then carries the neighbour step of step
+// k - 1 and the local step of step k.
+template <int O, bool SP, bool R3, bool WP = false, bool NB = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __restrict__ dofs,
-             float* __restrict__ t_int) {
+             float* __restrict__ t_int, const float* __restrict__ proj_in) {
+  static_assert(!NB || (R3 && WP), "the fused step exchanges face projections");
   using D = Dims<O>;
   constexpr int N = D::N, NF = D::NF, NT = D::NT;
   constexpr int NV = n_modes(O - 1);  // rows of tInt the volume integral reads
@@ -671,9 +719,36 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
   if (off >= n) return;
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-  const float* q0 = dofs + el * D::SZ;
+  float* q0 = dofs + el * D::SZ;
   const cfloat* star = as_const(s.star + el * 243);
-  tile_to_lds<D::CHUNKS>(q0, l_ti, D::NVEC, lane);
+  if constexpr (NB) {
+    {
+      v4f nq[9][NT];
+      neigh_lifts<O>(s, proj_in, el, nq, lane, [&](v4f (&a)[9][NT]) {
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+#pragma unroll
+          for (int t = 0; t < NT; t++) a[q][t] = load_d(q0 + q * N * R, N, t, g, c);
+      });
+      const int ln = opaque(lane), go = ln >> 4, co = ln & 15;
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) l_ti[lds_idx<O>(q, 16 * t + 4 * go + v, N, co)] = nq[q][t][v];
+    }
+    __syncthreads();
+    float4* gq = reinterpret_cast<float4*>(q0);
+    const float4* lq = reinterpret_cast<const float4*>(l_ti);
+    for (int i = lane; i < D::NVEC; i += 64) gq[i] = lq[i];
+    if (s.tap && as_const(s.tap_on)[el]) {  // uniform across the wave
+      float4* tq = reinterpret_cast<float4*>(s.tap + el * D::SZ);
+      for (int i = lane; i < D::NVEC; i += 64) tq[i] = lq[i];
+    }
+  } else {
+    tile_to_lds<D::CHUNKS>(q0, l_ti, D::NVEC, lane);
+  }
 
   // ---- CK: tInt = dt Q + sum_de dt^{de+1}/(de+1)! der_de, tile in LDS
   {
@@ -970,47 +1045,20 @@ template <int O>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 void k_neigh_proj(SolverDev<float> s, int64_t first, int64_t n, float* __restrict__ dofs) {
   using D = Dims<O>;
-  constexpr int N = D::N, NT = D::NT, NF = D::NF;
+  constexpr int N = D::N, NT = D::NT;
   __shared__ __align__(16) float l_out[D::SZ + kLdsPad];
   const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   float* q0 = dofs + el * D::SZ;
-  (void)NF;
-  // At order 5 (two waves per SIMD either way) every face's projection is
-  // requested before the DOFs and each face's lift operands one face ahead;
-  // below, the registers this takes would halve the occupancy.
-  constexpr bool AHEAD = O >= 5;
-  const v4f* plft = reinterpret_cast<const v4f*>(s.dg.pk + pk::Layout{O}.lft());
-  int32_t rcv[4];
-  v4f w[AHEAD ? 4 : 1][3];
-#pragma unroll
-  for (int fa = 0; fa < 4; fa++) {
-    rcv[fa] = as_const(s.rcv)[el * 4 + fa];
-    if (AHEAD && rcv[fa] >= 0) load_proj(s.proj + (int64_t)rcv[fa] * kProjFace, w[AHEAD ? fa : 0], lane);
-  }
-  v4f at[NT];
-  load_lift<O>(plft, at, lane);
   v4f acc[9][NT];
+  neigh_lifts<O>(s, s.proj, el, acc, lane, [&](v4f (&a)[9][NT]) {
 #pragma unroll
-  for (int q = 0; q < 9; q++)
+    for (int q = 0; q < 9; q++)
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[q][t] = load_d(q0 + q * N * R, N, t, g, c);
-#pragma unroll
-  for (int fa = 0; fa < 4; fa++) {
-    v4f atn[NT];
-    if (AHEAD && fa < 3) load_lift<O>(plft + (fa + 1) * NT * 64, atn, lane);
-    if (!AHEAD && fa > 0) load_lift<O>(plft + fa * NT * 64, at, lane);
-    if (rcv[fa] >= 0) {  // else outflow: no projection (uniform across the wave)
-      if (!AHEAD) load_proj(s.proj + (int64_t)rcv[fa] * kProjFace, w[0], lane);
-      lift3<O>(at, as_const(s.fsf + ((s.n_el + el) * 4 + fa) * 54), w[AHEAD ? fa : 0], acc);
-    }
-    if (AHEAD && fa < 3) {
-#pragma unroll
-      for (int t = 0; t < NT; t++) at[t] = atn[t];
-    }
-  }
+      for (int t = 0; t < NT; t++) a[q][t] = load_d(q0 + q * N * R, N, t, g, c);
+  });
   {
     const int ln = opaque(lane), go = ln >> 4, co = ln & 15;
 #pragma unroll
@@ -1032,18 +1080,33 @@ template <int O>
 void launch_local_o(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs, float* t_int,
                     hipStream_t st) {
   const dim3 grid(grid_of(n)), block(64);
+  const float* np = nullptr;
   if (s.sparse_star && s.fs_rank3 && s.proj)
-    hipLaunchKernelGGL((k_local<O, true, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local<O, true, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int, np);
   else if (s.sparse_star && s.fs_rank3)
-    hipLaunchKernelGGL((k_local<O, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local<O, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int, np);
   else if (s.fs_rank3 && s.proj)
-    hipLaunchKernelGGL((k_local<O, false, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local<O, false, true, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int, np);
   else if (s.sparse_star)
-    hipLaunchKernelGGL((k_local<O, true, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local<O, true, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int, np);
   else if (s.fs_rank3)
-    hipLaunchKernelGGL((k_local<O, false, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local<O, false, true>), grid, block, 0, st, s, first, n, dt, dofs, t_int, np);
   else
-    hipLaunchKernelGGL((k_local<O, false, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int);
+    hipLaunchKernelGGL((k_local<O, false, false>), grid, block, 0, st, s, first, n, dt, dofs, t_int, np);
+}
+
+// The neighbour step of the previous time step (projections proj_in) fused
+// with this step's local step (projections to s.proj), over all elements.
+template <int O>
+void launch_fused_o(const SolverDev<float>& s, float dt, const float* proj_in, float* dofs, float* t_int,
+                    hipStream_t st) {
+  const dim3 grid(grid_of(s.n_el)), block(64);
+  if (s.sparse_star)
+    hipLaunchKernelGGL((k_local<O, true, true, true, true>), grid, block, 0, st, s, (int64_t)0, s.n_el, dt, dofs,
+                       t_int, proj_in);
+  else
+    hipLaunchKernelGGL((k_local<O, false, true, true, true>), grid, block, 0, st, s, (int64_t)0, s.n_el, dt, dofs,
+                       t_int, proj_in);
 }
 
 template <int O>
@@ -1145,6 +1208,22 @@ int launch_neigh_mfma<float>(const SolverDev<float>& s, int64_t first, int64_t n
     case 3: launch_neigh_o<3>(s, first, n, t_int, dofs, st); break;
     case 4: launch_neigh_o<4>(s, first, n, t_int, dofs, st); break;
     case 5: launch_neigh_o<5>(s, first, n, t_int, dofs, st); break;
+    default: return fail(EDGE_ERR_UNSUPPORTED, "order");
+  }
+  EDGE_HIP_TRY(hipGetLastError());
+  return EDGE_OK;
+}
+
+int launch_fused_mfma(const SolverDev<float>& s, float dt, const float* proj_in, float* dofs, float* t_int,
+                      hipStream_t st) {
+  if (!s.sparse_star && !s.fs_rank3) return fail(EDGE_ERR_UNSUPPORTED, "fused step: rank-3 flux solvers");
+  if (!s.fs_rank3 || !s.proj || !proj_in) return fail(EDGE_ERR_UNSUPPORTED, "fused step: face projections");
+  if (s.n_el <= 0) return EDGE_OK;
+  switch (s.order) {
+    case 2: launch_fused_o<2>(s, dt, proj_in, dofs, t_int, st); break;
+    case 3: launch_fused_o<3>(s, dt, proj_in, dofs, t_int, st); break;
+    case 4: launch_fused_o<4>(s, dt, proj_in, dofs, t_int, st); break;
+    case 5: launch_fused_o<5>(s, dt, proj_in, dofs, t_int, st); break;
     default: return fail(EDGE_ERR_UNSUPPORTED, "order");
   }
   EDGE_HIP_TRY(hipGetLastError());

@@ -42,6 +42,10 @@ struct edge_el_solver {
   bool xcd_map = true;       // MFMA kernels: contiguous element ranges per XCD (the autotuner may turn this off)
   bool proj_tables = false;  // build the projection slot tables whatever the kernel (MFMA viscoelastic)
   void* proj = nullptr;      // [nEl][4][768]
+  void* proj_alt = nullptr;  // the second projection buffer of the fused step (allocated on first use)
+  bool use_fused = true;     // multi-step calls fuse neighbour step k - 1 with local step k (the autotuner may turn this off)
+  uint8_t* tap_on = nullptr; // [nEl] elements whose step-start DOFs the fused launches also write to tap
+  void* tap = nullptr;       // caller's array in the DOF layout (edge_el_set_taps), null: none
   int32_t* snd_v = nullptr;  // [nEl][4]
   uint8_t* snd_fm = nullptr; // [nEl][4]
   int32_t* rcv = nullptr;    // [nEl][4]
@@ -78,6 +82,7 @@ struct edge_el_solver {
   void* graph_tint = nullptr;
   double graph_dt = 0;
   hipStream_t graph_stream = nullptr;
+  int64_t graph_steps = 0;  // > 0: the graph holds a whole fused sequence of this many steps
 };
 
 namespace {
@@ -108,6 +113,8 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.snd_v = s->snd_v;
   d.snd_fm = s->snd_fm;
   d.rcv = s->rcv;
+  d.tap_on = s->tap ? s->tap_on : nullptr;
+  d.tap = static_cast<T*>(s->tap);
   d.n_rms = s->n_rms;
   d.rfs = static_cast<const T*>(s->rfs);
   d.star_a = static_cast<const T*>(s->star_a);
@@ -504,11 +511,44 @@ int one_step(edge_el_solver* s, double dt, void* dofs, void* dofs_a, void* t_int
                                 : neigh_t<float>(s, 0, s->n_el, t_int, dofs, st);
 }
 
+// Multi-step calls of FP32 MFMA elastic solvers that exchange face
+// projections fuse the neighbour step of step k - 1 with the local step of
+// step k (one launch per time step; the projections alternate between two
+// buffers, since a launch reads the previous step's while it writes its
+// own).  Any fault phase runs between the launches, where it runs between
+// local and neighbour step otherwise.
+bool fused_active(const edge_el_solver* s) {
+  return s->use_fused && s->n_rms == 0 && s->cfg.precision == 32 && proj_active(s);
+}
+
+int fused_steps(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* t_int, hipStream_t st) {
+  if (!s->proj_alt) EDGE_HIP_TRY(hipMalloc(&s->proj_alt, sizeof(float) * 4 * 768 * s->n_el));
+  float* P[2] = {static_cast<float*>(s->proj), static_cast<float*>(s->proj_alt)};
+  auto d = dev_view<float>(s);
+  const float fdt = (float)dt;
+  // the projection buffers no longer stand for a tInt array a separate
+  // neighbour step could be handed
+  s->proj_tint = nullptr;
+  s->proj_fresh = false;
+  d.proj = P[0];
+  int rc = edge::launch_local_mfma<float>(d, 0, s->n_el, fdt, (float*)dofs, (float*)t_int, st);
+  if (rc == EDGE_OK && s->fault) rc = edge::fault_apply(s->fault, dt, t_int, dofs, st);
+  for (int64_t k = 1; k < n_steps && rc == EDGE_OK; k++) {
+    d.proj = P[k % 2];
+    rc = edge::launch_fused_mfma(d, fdt, P[(k - 1) % 2], (float*)dofs, (float*)t_int, st);
+    if (rc == EDGE_OK && s->fault) rc = edge::fault_apply(s->fault, dt, t_int, dofs, st);
+  }
+  d.proj = P[(n_steps - 1) % 2];
+  if (rc == EDGE_OK) rc = edge::launch_neigh_mfma<float>(d, 0, s->n_el, (const float*)t_int, (float*)dofs, st);
+  return rc;
+}
+
 void drop_graph(edge_el_solver* s) {
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   s->graph_exec = nullptr;
   s->graph = nullptr;
+  s->graph_steps = 0;
 }
 
 }  // namespace
@@ -630,6 +670,8 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   (void)hipFree(s->geo);
   (void)hipFree(s->fsf);
   (void)hipFree(s->proj);
+  (void)hipFree(s->proj_alt);
+  (void)hipFree(s->tap_on);
   (void)hipFree(s->snd_v);
   (void)hipFree(s->snd_fm);
   (void)hipFree(s->rcv);
@@ -885,7 +927,7 @@ extern "C" int edge_el_kernel_info(const edge_el_solver* s, int32_t* kernel, int
   if (flags) {
     const bool r3 = s->kernel == 2 && s->fs_rank3 && s->fsf;
     *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (r3 && s->proj && s->use_proj ? 4 : 0) |
-             (s->n_rms > 0 ? 8 : 0);
+             (s->n_rms > 0 ? 8 : 0) | (fused_active(s) ? 16 : 0);
   }
   return EDGE_OK;
 }
@@ -895,6 +937,7 @@ struct TuneCand {
   int kernel;
   bool proj;
   bool xcd = true;
+  bool fused = false;  // multi-step calls fuse neighbour step k - 1 with local step k
 };
 
 // The kernel variants a solver can switch between with the data it holds.
@@ -904,7 +947,9 @@ std::vector<TuneCand> tune_candidates(const edge_el_solver* s) {
   const bool gen = s->n_rms > 0 ? edge::visco_supported(s->cfg.order, s->cfg.n_runs, prec_bytes, s->n_rms)
                                 : edge::generic_supported(s->cfg.order, s->cfg.n_runs, prec_bytes);
   if (s->kernel == 2) {
+    const bool fusable = s->proj && s->n_rms == 0 && s->cfg.precision == 32;
     for (int x = 1; x >= 0; x--) {  // XCD-aware element schedule, then the linear one
+      if (fusable) c.push_back({2, true, x == 1, true});
       if (s->proj) c.push_back({2, true, x == 1});
       c.push_back({2, false, x == 1});
     }
@@ -927,7 +972,7 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
   void *dofs = nullptr, *t_int = nullptr, *dofs_a = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const int old_kernel = s->kernel;
-  const bool old_proj = s->use_proj, old_xcd = s->xcd_map;
+  const bool old_proj = s->use_proj, old_xcd = s->xcd_map, old_fused = s->use_fused;
   std::vector<double> t(cand.size(), 0.0);
   int rc = EDGE_OK;
   auto hip = [&](hipError_t e, const char* what) {
@@ -946,9 +991,13 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
       s->kernel = cand[i].kernel;
       s->use_proj = cand[i].proj;
       s->xcd_map = cand[i].xcd;
+      s->use_fused = cand[i].fused;
       rc = one_step(s, dt, dofs, dofs_a, t_int, st);  // warm-up
       if (rc == EDGE_OK) hip(hipEventRecord(e0, st), "event");
-      for (int r = 0; r < n_reps && rc == EDGE_OK; r++) rc = one_step(s, dt, dofs, dofs_a, t_int, st);
+      if (cand[i].fused)  // one fused sequence of n_reps steps (n_reps + 1 launches)
+        rc = n_reps >= 2 ? fused_steps(s, n_reps, dt, dofs, t_int, st) : one_step(s, dt, dofs, dofs_a, t_int, st);
+      else
+        for (int r = 0; r < n_reps && rc == EDGE_OK; r++) rc = one_step(s, dt, dofs, dofs_a, t_int, st);
       if (rc == EDGE_OK && hip(hipEventRecord(e1, st), "event") && hip(hipEventSynchronize(e1), "sync")) {
         float f = 0.f;
         if (hip(hipEventElapsedTime(&f, e0, e1), "event")) t[i] = f / n_reps;
@@ -968,6 +1017,7 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
     s->kernel = old_kernel;
     s->use_proj = old_proj;
     s->xcd_map = old_xcd;
+    s->use_fused = old_fused;
     return rc;
   }
   size_t best = 0;
@@ -976,12 +1026,13 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
   s->kernel = cand[best].kernel;
   s->use_proj = cand[best].proj;
   s->xcd_map = cand[best].xcd;
+  s->use_fused = cand[best].fused;
   if (n_cands) {
     const int32_t cap = *n_cands;
     *n_cands = (int32_t)cand.size();
     for (int32_t i = 0; i < (int32_t)cand.size() && i < cap; i++) {
       if (kernels) kernels[i] = cand[i].kernel;
-      if (flags) flags[i] = (cand[i].proj ? 1 : 0) | (cand[i].xcd ? 2 : 0);
+      if (flags) flags[i] = (cand[i].proj ? 1 : 0) | (cand[i].xcd ? 2 : 0) | (cand[i].fused ? 4 : 0);
       if (ms) ms[i] = t[i];
     }
   }
@@ -1001,11 +1052,14 @@ extern "C" int edge_el_set_variant(edge_el_solver* s, int32_t kernel, int32_t fl
   if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
   for (const TuneCand& c : tune_candidates(s)) {
     if (c.kernel != kernel) continue;
-    if (c.kernel != 1 && (c.proj != ((flags & 1) != 0) || c.xcd != ((flags & 2) != 0))) continue;
+    if (c.kernel != 1 &&
+        (c.proj != ((flags & 1) != 0) || c.xcd != ((flags & 2) != 0) || c.fused != ((flags & 4) != 0)))
+      continue;
     drop_graph(s);
     s->kernel = c.kernel;
     s->use_proj = c.proj;
     s->xcd_map = c.xcd;
+    s->use_fused = c.fused;
     s->proj_tint = nullptr;
     s->proj_fresh = false;
     return EDGE_OK;
@@ -1147,6 +1201,10 @@ extern "C" int edge_el_neigh_lts(edge_el_solver* s, int64_t first, int64_t n_els
 namespace {
 int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* dofs_a, void* t_int, int use_graph,
               hipStream_t st) {
+  if (n_steps == 0) return EDGE_OK;
+  const bool fuse = n_steps >= 2 && !dofs_a && fused_active(s);
+  if (fuse && !s->proj_alt) EDGE_HIP_TRY(hipMalloc(&s->proj_alt, sizeof(float) * 4 * 768 * s->n_el));
+  if (!use_graph && fuse) return fused_steps(s, n_steps, dt, dofs, t_int, st);
   if (!use_graph) {
     for (int64_t i = 0; i < n_steps; i++) {
       int rc = one_step(s, dt, dofs, dofs_a, t_int, st);
@@ -1154,8 +1212,11 @@ int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* d
     }
     return EDGE_OK;
   }
+  // a fused sequence is captured whole (one replay per call), a single step
+  // otherwise (one replay per step)
+  const int64_t gsteps = fuse ? n_steps : 0;
   if (s->graph_exec && (s->graph_dofs != dofs || s->graph_dofs_a != dofs_a || s->graph_tint != t_int ||
-                        s->graph_dt != dt || s->graph_stream != st))
+                        s->graph_dt != dt || s->graph_stream != st || s->graph_steps != gsteps))
     drop_graph(s);
   if (!s->graph_exec) {
     // capture needs a non-default stream
@@ -1170,7 +1231,7 @@ int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* d
       if (own) (void)hipStreamDestroy(cap);
       return edge::fail(EDGE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(eb));
     }
-    int rc = one_step(s, dt, dofs, dofs_a, t_int, cap);
+    int rc = fuse ? fused_steps(s, n_steps, dt, dofs, t_int, cap) : one_step(s, dt, dofs, dofs_a, t_int, cap);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cap, &g);
     if (rc == EDGE_OK && e != hipSuccess) rc = edge::fail(EDGE_ERR_HIP, hipGetErrorString(e));
@@ -1189,6 +1250,13 @@ int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* d
     s->graph_tint = t_int;
     s->graph_dt = dt;
     s->graph_stream = st;
+    s->graph_steps = gsteps;
+  }
+  if (gsteps) {
+    s->proj_tint = nullptr;  // as after an eager fused sequence
+    s->proj_fresh = false;
+    EDGE_HIP_TRY(hipGraphLaunch(s->graph_exec, st));
+    return EDGE_OK;
   }
   for (int64_t i = 0; i < n_steps; i++) EDGE_HIP_TRY(hipGraphLaunch(s->graph_exec, st));
   return EDGE_OK;
@@ -1209,6 +1277,44 @@ extern "C" int edge_el_step_a(edge_el_solver* s, int64_t n_steps, double dt, voi
   if (s->n_rms == 0) return edge_el_step(s, n_steps, dt, dofs, t_int, use_graph, stream);
   if (!dofs || !dofs_a || !t_int || n_steps < 0) return edge::fail(EDGE_ERR_ARG, "step arguments");
   return step_impl(s, n_steps, dt, dofs, dofs_a, t_int, use_graph, (hipStream_t)stream);
+}
+
+extern "C" int edge_el_neigh_local(edge_el_solver* s, double dt, void* dofs, void* t_int, void* stream) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (!dofs || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
+  if (!fused_active(s))
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "fused neighbour and local step: FP32 MFMA elastic solvers that exchange "
+                                            "face projections (edge_el_kernel_info flag bit 4)");
+  if (!s->proj_fresh)
+    return edge::fail(EDGE_ERR_ARG, "fused neighbour and local step: no local step is pending (run edge_el_local "
+                                    "first, or its neighbour step has run)");
+  int rc = check_neigh(s, t_int);
+  if (rc) return rc;
+  if (!s->proj_alt) EDGE_HIP_TRY(hipMalloc(&s->proj_alt, sizeof(float) * 4 * 768 * s->n_el));
+  auto d = dev_view<float>(s);
+  d.proj = static_cast<float*>(s->proj_alt);
+  rc = edge::launch_fused_mfma(d, (float)dt, static_cast<const float*>(s->proj), (float*)dofs, (float*)t_int,
+                               (hipStream_t)stream);
+  if (rc) return rc;
+  std::swap(s->proj, s->proj_alt);
+  note_local(s, t_int);
+  return EDGE_OK;
+}
+
+extern "C" int edge_el_set_taps(edge_el_solver* s, int64_t n, const int64_t* el, void* taps) {
+  if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
+  if (n < 0 || (n > 0 && (!el || !taps))) return edge::fail(EDGE_ERR_ARG, "taps: elements or array missing");
+  for (int64_t i = 0; i < n; i++)
+    if (el[i] < 0 || el[i] >= s->n_el) return edge::fail(EDGE_ERR_ARG, "taps: element out of range");
+  drop_graph(s);
+  s->tap = nullptr;
+  if (n == 0) return EDGE_OK;
+  std::vector<uint8_t> on(s->n_el, 0);
+  for (int64_t i = 0; i < n; i++) on[el[i]] = 1;
+  if (!s->tap_on) EDGE_HIP_TRY(hipMalloc(&s->tap_on, s->n_el));
+  EDGE_HIP_TRY(hipMemcpy(s->tap_on, on.data(), s->n_el, hipMemcpyHostToDevice));
+  s->tap = taps;
+  return EDGE_OK;
 }
 
 extern "C" int edge_el_info(const edge_el_solver* s, int64_t* n_el, int32_t* n_modes, int32_t* n_runs,
@@ -1281,7 +1387,7 @@ extern "C" int edge_el_l2_norms(edge_el_solver* s, const void* a, int prec_a, co
     return edge::fail(EDGE_ERR_ARG, "norm arguments");
   hipStream_t st = (hipStream_t)stream;
   const int nqr = 9 * s->cfg.n_runs;
-  const unsigned nblk = 1024;
+  const unsigned nblk = edge::kL2Blocks;
   double* d_part = nullptr;
   EDGE_HIP_TRY(hipMalloc(&d_part, sizeof(double) * 2 * nqr * nblk));
   int rc = edge::l2_partials(s, a, prec_a, b, prec_b, d_part, nblk, st);

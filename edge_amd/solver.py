@@ -13,7 +13,7 @@ from . import _lib
 from .setups import n_modes
 
 KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES, KERNEL_MFMA64 = 0, 1, 2, 3, 4
-PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS, PATH_VISCO = 1, 2, 4, 8
+PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS, PATH_VISCO, PATH_FUSED = 1, 2, 4, 8, 16
 
 
 def _stream_ptr(stream):
@@ -88,17 +88,19 @@ class ElasticSolver:
     def autotune(self, dt, n_reps=3, stream=None):
         """Times the kernel variants and schedules this solver can switch to on
         its mesh and keeps the fastest; returns [(kernel, projections,
-        xcd_schedule, ms per step)]."""
+        xcd_schedule, ms per step, fused)] (fused: multi-step calls run the
+        neighbour step of step k - 1 and the local step of step k as one
+        launch)."""
         cap = 16
         n = C.c_int32(cap)
         k, f, ms = (C.c_int32 * cap)(), (C.c_int32 * cap)(), (C.c_double * cap)()
         _lib.check(_lib.lib().edge_el_autotune(self._h, float(dt), int(n_reps), _stream_ptr(stream), C.byref(n), k,
                                                f, ms))
-        return [(k[i], bool(f[i] & 1), bool(f[i] & 2), ms[i]) for i in range(min(n.value, cap))]
+        return [(k[i], bool(f[i] & 1), bool(f[i] & 2), ms[i], bool(f[i] & 4)) for i in range(min(n.value, cap))]
 
-    def set_variant(self, kernel, projections=False, xcd_schedule=True):
+    def set_variant(self, kernel, projections=False, xcd_schedule=True, fused=False):
         """Switches to one of the autotuner's candidates without timing it."""
-        flags = (1 if projections else 0) | (2 if xcd_schedule else 0)
+        flags = (1 if projections else 0) | (2 if xcd_schedule else 0) | (4 if fused else 0)
         _lib.check(_lib.lib().edge_el_set_variant(self._h, int(kernel), flags))
 
     def close(self):
@@ -153,8 +155,29 @@ class ElasticSolver:
         _lib.check(_lib.lib().edge_el_neigh(self._h, first, n, t_int.data_ptr(), dofs.data_ptr(),
                                             _stream_ptr(stream)))
 
+    def neigh_local(self, dofs, t_int, dt, stream=None):
+        """The pending step's neigh and the next step's local as one launch
+        (solvers with PATH_FUSED)."""
+        self._check_arr(dofs)
+        self._check_arr(t_int)
+        _lib.check(_lib.lib().edge_el_neigh_local(self._h, float(dt), dofs.data_ptr(), t_int.data_ptr(),
+                                                  _stream_ptr(stream)))
+
+    def set_taps(self, elements, taps=None):
+        """Fused launches also write the step-start DOFs of `elements` to the
+        device array `taps` (DOF layout); elements=None switches them off."""
+        if elements is None or len(elements) == 0:
+            _lib.check(_lib.lib().edge_el_set_taps(self._h, 0, None, None))
+            return
+        self._check_arr(taps)
+        el = np.ascontiguousarray(elements, dtype=np.int64)
+        self._taps = taps  # keeps the array alive while the solver writes it
+        _lib.check(_lib.lib().edge_el_set_taps(self._h, len(el), el.ctypes.data, taps.data_ptr()))
+
     def step(self, dofs, t_int, dt, n_steps=1, graph=False, stream=None):
-        """n_steps global time steps (local, then neigh)."""
+        """n_steps global time steps (local, then neigh); with graph=True the
+        step (or a fused sequence, PATH_FUSED) is captured into a HIP graph on
+        first use and replayed."""
         self._check_arr(dofs)
         self._check_arr(t_int)
         _lib.check(_lib.lib().edge_el_step(self._h, int(n_steps), float(dt), dofs.data_ptr(), t_int.data_ptr(),

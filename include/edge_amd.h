@@ -211,9 +211,41 @@ int edge_el_neigh(edge_el_solver *s, int64_t first, int64_t n_els, const void *t
 
 /* n_steps full time steps (local then neigh over all elements) with time
  * step dt.  use_graph != 0 captures one step into a HIP graph on first use
- * and replays it. */
+ * and replays it.  FP32 MFMA elastic solvers that exchange face projections
+ * (edge_el_kernel_info flag bit 4) run n_steps >= 2 as one launch per time
+ * step: the local step of step 0, then n_steps - 1 launches that each carry
+ * the neighbour step of step k - 1 and the local step of step k (the
+ * projections alternate between two solver-held buffers), then the
+ * neighbour step of the last step; with use_graph != 0 the whole sequence
+ * is one graph (recaptured when n_steps changes).  An attached fault's phase
+ * runs between the launches.  The DOFs are those of n_steps full steps
+ * either way; after the call t_int holds the last step's tInt and no
+ * separate edge_el_neigh may follow. */
 int edge_el_step(edge_el_solver *s, int64_t n_steps, double dt, void *dofs, void *t_int, int use_graph,
                  void *stream);
+
+/* AderDg::neigh of step k followed by AderDg::local of step k + 1 (time
+ * step dt) over all elements, as one launch: equals edge_el_neigh(t_int,
+ * dofs) then edge_el_local(dt, dofs, t_int), so a caller's time loop
+ * (steps.inc:24-77) becomes local, then neigh_local per further step (with
+ * point sources or a fault phase between the calls, where they run between
+ * local and neigh), then a last neigh.  For FP32 MFMA elastic solvers that
+ * exchange face projections (edge_el_kernel_info flag bit 4;
+ * EDGE_ERR_UNSUPPORTED otherwise).  t_int must be the array the pending
+ * local step wrote, as edge_el_neigh checks; EDGE_ERR_ARG when no local step
+ * is pending.  No counterpart in the reference (its neigh and local are
+ * separate element loops, AderDg.hpp:338-534, 553-722). */
+int edge_el_neigh_local(edge_el_solver *s, double dt, void *dofs, void *t_int, void *stream);
+
+/* The DOFs each fused launch's local step starts from (after the
+ * neighbour step) exist only inside that launch; edge_el_neigh_local and the
+ * fused edge_el_step also write them to taps (a device array in the DOF
+ * layout and size, the caller's) for the n elements el[n], and leave its
+ * other elements untouched: receivers sampled at the step start
+ * (AderDg.hpp:475-496) read them there (edge_recv_eval on taps).  n = 0
+ * switches the taps off.  Drops a captured step graph.  No counterpart in
+ * the reference. */
+int edge_el_set_taps(edge_el_solver *s, int64_t n, const int64_t *el, void *taps);
 
 /* Number of elements / modes / runs / bytes of one DOF array of a solver. */
 int edge_el_info(const edge_el_solver *s, int64_t *n_el, int32_t *n_modes, int32_t *n_runs,
@@ -463,20 +495,23 @@ int edge_el_neigh_lts_a(edge_el_solver *s, int64_t first, int64_t n_els, int32_t
  * (FP64 single run on FP64 MFMA);
  * *flags bit 0: sparse (elastic-pattern) star matrices, bit 1: rank-3
  * factored flux solvers, bit 2: the neighbour step reads face projections,
- * bit 3: viscoelastic (anelastic DOFs).
+ * bit 3: viscoelastic (anelastic DOFs), bit 4: multi-step calls fuse the
+ * neighbour step of step k - 1 with the local step of step k (edge_el_step).
  * Diagnostic with no counterpart in the reference (for tests and tools). */
 int edge_el_kernel_info(const edge_el_solver *s, int32_t *kernel, int32_t *flags);
 
 /* Autotuner: times every kernel variant and schedule the solver can switch
- * to (the FP32 MFMA kernels with and without the face-projection exchange
- * and the FP64 MFMA kernels, each with the XCD-aware element schedule or
- * the linear one; the generic kernels; all parity-tested against the
- * oracle) over
- * n_reps full steps of time step dt on scratch DOFs on its own mesh, and
+ * to (the FP32 MFMA kernels with and without the face-projection exchange,
+ * for elastic solvers with it also the fused multi-step form, and the FP64
+ * MFMA kernels, each with the XCD-aware element schedule or the linear one;
+ * the generic kernels; all parity-tested against the oracle) over
+ * n_reps full steps of time step dt on scratch DOFs on its own mesh (a fused
+ * candidate: one fused sequence of n_reps steps), and
  * keeps the fastest for the solver's later calls.  In: *n_cands = capacity
  * of the arrays (may be NULL); out: *n_cands = number of candidates,
  * kernels[i] (as edge_el_kernel_info), flags[i] (bit 0: projections
- * exchanged, bit 1: XCD-aware schedule) and ms[i] (per step).  Drops a
+ * exchanged, bit 1: XCD-aware schedule, bit 2: fused multi-step form) and
+ * ms[i] (per step).  Drops a
  * captured step graph.  Its steps overwrite the face projections: a
  * neighbour step pending from before the call returns EDGE_ERR_ARG.  No counterpart in the reference (its back-end is a
  * compile-time choice, Kernels.hpp:30-44). */
@@ -499,6 +534,7 @@ int edge_el_set_variant(edge_el_solver *s, int32_t kernel, int32_t flags);
  * reduction of src/setups/InitialDofs.hpp:277-370 in modal form. */
 int edge_el_l2_norms(edge_el_solver *s, const void *a, int prec_a, const void *b, int prec_b, double *diff_sq,
                      double *ref_sq, void *stream);
+
 
 /* Elastic energy per run of a solver's device DOFs (its precision), FP64 on
  * the GPU: kinetic 1/2 rho |v|^2 and strain 1/2 sigma : C^-1 : sigma with the

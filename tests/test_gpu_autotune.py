@@ -10,8 +10,8 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, PATH_PROJECTIONS, ElasticSolver,
-                             ViscoSolver)
+from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, PATH_FUSED, PATH_PROJECTIONS,
+                             ElasticSolver, ViscoSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -35,10 +35,11 @@ def test_autotune_elastic(gpu, prec, runs, expect):
     s = ElasticSolver(m, order, runs, prec, rho, lam, mu)
     dt = S.dt_cfl(m, rho, lam, mu, order)
     cands = s.autotune(dt, n_reps=2)
-    assert expect <= {(k, p) for k, p, _, _ in cands}
+    assert expect <= {(k, p) for k, p, *_ in cands}
     assert all(ms > 0 for *_, ms in cands)
-    k, p, _, _ = min(cands, key=lambda c: c[3])
+    k, p, _, _, f = min(cands, key=lambda c: c[3])
     kernel, flags = s.kernel_info()
+    assert bool(flags & PATH_FUSED) == f
     assert kernel == k and bool(flags & PATH_PROJECTIONS) == p
     rng = np.random.default_rng(1)
     d0 = rng.normal(size=(m.n_el, 9, S.n_modes(order), runs))
@@ -60,9 +61,9 @@ def test_autotune_visco(gpu):
     visco = orc.setup_mesh_visco(m, rho, lam, mu, qp, qp / 2, nr, 20.0, 10.0)
     dt = S.dt_cfl(m, rho, visco["lam_e"], visco["mu_e"], order)
     cands = s.autotune(dt, n_reps=2)
-    assert {(KERNEL_MFMA, True), (KERNEL_MFMA, False), (KERNEL_GENERIC, False)} <= {(k, p) for k, p, _, _ in cands}
-    assert {x for k, _, x, _ in cands if k == KERNEL_MFMA} == {True, False}  # both element schedules
-    k, p, _, _ = min(cands, key=lambda c: c[3])
+    assert {(KERNEL_MFMA, True), (KERNEL_MFMA, False), (KERNEL_GENERIC, False)} <= {(k, p) for k, p, *_ in cands}
+    assert {x for k, _, x, *_ in cands if k == KERNEL_MFMA} == {True, False}  # both element schedules
+    k, *_ = min(cands, key=lambda c: c[3])
     assert s.kernel_info()[0] == k
     rng = np.random.default_rng(2)
     n = S.n_modes(order)
@@ -103,23 +104,29 @@ def test_every_candidate_matches_oracle(gpu, kind, prec, runs):
         s = ElasticSolver(m, order, runs, prec, rho, lam, mu)
         dt = S.dt_cfl(m, rho, lam, mu, order)
         mo = orc.MeshOracle(m, order, runs, rho=rho, lam=lam, mu=mu)
+    # three steps: a fused candidate runs local, two fused launches (both
+    # projection buffers read and written) and the last neighbour step
     d, t = d0.copy(), np.zeros_like(d0)
     a = a0.copy() if a0 is not None else None
-    mo.step(d, t, dt, 2, dofs_a=a) if a is not None else mo.step(d, t, dt, 2)
+    mo.step(d, t, dt, 3, dofs_a=a) if a is not None else mo.step(d, t, dt, 3)
     cands = s.autotune(dt, n_reps=1)
     assert len(cands) >= 2
+    if kind == "elastic" and prec == 32:
+        assert any(f for *_, f in cands)
     tol = 1e-12 if prec == 64 else 1e-5
-    for k, p, x, _ in cands:
-        s.set_variant(k, p, x)
+    for k, p, x, _, f in cands:
+        s.set_variant(k, p, x, f)
         assert s.kernel_info()[0] == k
-        gd, gt = _dev(d0, prec), _dev(np.zeros_like(d0), prec)
-        if a is not None:
-            ga = _dev(a0, prec)
-            s.step(gd, gt, dt, 2, ga)
-            assert rel(ga.cpu().numpy(), a) <= tol, (k, p, x)
-        else:
-            s.step(gd, gt, dt, 2)
-        assert rel(gd.cpu().numpy(), d) <= tol, (k, p, x)
+        assert bool(s.kernel_info()[1] & PATH_FUSED) == f
+        for graph in (False, True):
+            gd, gt = _dev(d0, prec), _dev(np.zeros_like(d0), prec)
+            if a is not None:
+                ga = _dev(a0, prec)
+                s.step(gd, gt, dt, 3, ga, graph=graph)
+                assert rel(ga.cpu().numpy(), a) <= tol, (k, p, x, f, graph)
+            else:
+                s.step(gd, gt, dt, 3, graph=graph)
+            assert rel(gd.cpu().numpy(), d) <= tol, (k, p, x, f, graph)
     with pytest.raises(Exception):
         s.set_variant(7)
     s.close()
