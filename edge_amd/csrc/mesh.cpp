@@ -327,18 +327,129 @@ int tets_check(const edge_tets_in* in) {
   return EDGE_OK;
 }
 
+// Periodic boundaries (EDGE-V's setPeriodicBnds, tools/edge_v/src/mesh/
+// Mesh.cpp:157-288): every triangle tagged EDGE_FA_PERIODIC lies on a plane
+// of one constant coordinate, and pairs with the periodic triangle on another
+// plane of the same axis whose vertices agree in the other two coordinates
+// (absolute tolerance 1e-6, the reference's).  The paired vertices are
+// identified (union-find over vertex ids), so the two triangles' tets match
+// through canonical vertex ids as the box generator's periodic faces do.
+// canon[v] = the smallest vertex id identified with v.
+constexpr double kPeriodicTol = 1e-6;
+
+int periodic_canon(const edge_tets_in* in, std::vector<int64_t>* canon) {
+  const int64_t nv = in->n_ve;
+  canon->resize(nv);
+  for (int64_t v = 0; v < nv; v++) (*canon)[v] = v;
+  const double* x = in->ve_crds;
+  std::vector<int64_t> tri;  // periodic triangles (indices into bf_ve)
+  for (int64_t i = 0; i < in->n_bf; i++)
+    if (in->bf_type[i] == EDGE_FA_PERIODIC) tri.push_back(i);
+  if (tri.empty()) return EDGE_OK;
+  // constant axis and in-plane centroid of each triangle
+  std::vector<int> axis(tri.size());
+  std::vector<double> cen(2 * tri.size()), pln(tri.size());
+  for (size_t t = 0; t < tri.size(); t++) {
+    const int64_t* v = in->bf_ve + 3 * tri[t];
+    int ax = -1;
+    for (int d = 0; d < 3; d++) {
+      const double a = x[3 * v[0] + d], b = x[3 * v[1] + d], c = x[3 * v[2] + d];
+      if (std::fabs(a - b) < kPeriodicTol && std::fabs(a - c) < kPeriodicTol && std::fabs(b - c) < kPeriodicTol) {
+        if (ax >= 0) return edge::fail(EDGE_ERR_MESH, "periodic triangle is degenerate");
+        ax = d;
+      }
+    }
+    if (ax < 0) return edge::fail(EDGE_ERR_MESH, "periodic triangle not on a plane of constant coordinate");
+    axis[t] = ax;
+    pln[t] = x[3 * v[0] + ax];
+    for (int k = 0, d = 0; d < 3; d++) {
+      if (d == ax) continue;
+      cen[2 * t + k++] = (x[3 * v[0] + d] + x[3 * v[1] + d] + x[3 * v[2] + d]) / 3.0;
+    }
+  }
+  // candidates through a hash of the centroid on a grid of cell 4 tol
+  // (partners' centroids differ by less than tol: the 3 x 3 neighbourhood)
+  const double cell = 4 * kPeriodicTol;
+  struct CellKey {
+    int ax;
+    int64_t i, j;
+    bool operator==(const CellKey& o) const { return ax == o.ax && i == o.i && j == o.j; }
+  };
+  struct CellHash {
+    size_t operator()(const CellKey& k) const {
+      return (size_t)((uint64_t)k.i * 0x9E3779B97F4A7C15ull ^ (uint64_t)k.j * 0xC2B2AE3D27D4EB4Full ^ (uint64_t)k.ax);
+    }
+  };
+  std::unordered_map<CellKey, std::vector<size_t>, CellHash> grid;
+  auto key_of = [&](size_t t, int di, int dj) {
+    return CellKey{axis[t], (int64_t)std::floor(cen[2 * t] / cell) + di, (int64_t)std::floor(cen[2 * t + 1] / cell) + dj};
+  };
+  for (size_t t = 0; t < tri.size(); t++) grid[key_of(t, 0, 0)].push_back(t);
+  // vertex b of triangle u matching vertex a (same coordinates off the axis)
+  auto match = [&](int64_t a, const int64_t* u, int ax) -> int64_t {
+    for (int s = 0; s < 3; s++) {
+      bool eq = true;
+      for (int d = 0; d < 3; d++)
+        if (d != ax && std::fabs(x[3 * a + d] - x[3 * u[s] + d]) >= kPeriodicTol) eq = false;
+      if (eq) return u[s];
+    }
+    return -1;
+  };
+  std::vector<int64_t> partner(tri.size(), -1);
+  for (size_t t = 0; t < tri.size(); t++) {
+    const int64_t* v = in->bf_ve + 3 * tri[t];
+    for (int di = -1; di <= 1; di++)
+      for (int dj = -1; dj <= 1; dj++) {
+        const auto it = grid.find(key_of(t, di, dj));
+        if (it == grid.end()) continue;
+        for (size_t u : it->second) {
+          if (u == t || std::fabs(pln[u] - pln[t]) < kPeriodicTol) continue;  // same plane
+          const int64_t* w = in->bf_ve + 3 * tri[u];
+          if (match(v[0], w, axis[t]) < 0 || match(v[1], w, axis[t]) < 0 || match(v[2], w, axis[t]) < 0) continue;
+          if (partner[t] >= 0 && partner[t] != (int64_t)u)
+            return edge::fail(EDGE_ERR_MESH, "periodic triangle with more than one partner");
+          partner[t] = (int64_t)u;
+        }
+      }
+    if (partner[t] < 0) return edge::fail(EDGE_ERR_MESH, "periodic triangle without a partner on the opposite side");
+  }
+  // union-find; the root of a set is its smallest vertex id
+  std::vector<int64_t>& par = *canon;
+  auto find = [&](int64_t v) {
+    while (par[v] != v) {
+      par[v] = par[par[v]];
+      v = par[v];
+    }
+    return v;
+  };
+  for (size_t t = 0; t < tri.size(); t++) {
+    if ((size_t)partner[partner[t]] != t) return edge::fail(EDGE_ERR_MESH, "periodic triangles do not pair up");
+    const int64_t* v = in->bf_ve + 3 * tri[t];
+    const int64_t* w = in->bf_ve + 3 * tri[partner[t]];
+    for (int s = 0; s < 3; s++) {
+      const int64_t a = find(v[s]), b = find(match(v[s], w, axis[t]));
+      if (a < b) par[b] = a;
+      if (b < a) par[a] = b;
+    }
+  }
+  for (int64_t v = 0; v < nv; v++) par[v] = find(v);
+  return EDGE_OK;
+}
+
 }  // namespace
 
 extern "C" int edge_mesh_tets_sizes(const edge_tets_in* in, int64_t* n_fa) {
   int st = tets_check(in);
   if (st != EDGE_OK) return st;
   if (!n_fa) return edge::fail(EDGE_ERR_ARG, "null argument");
+  std::vector<int64_t> canon;
+  if ((st = periodic_canon(in, &canon)) != EDGE_OK) return st;
   std::unordered_map<TriKey, int, TriHash> faces;
   faces.reserve((size_t)in->n_el * 3);
   for (int64_t e = 0; e < in->n_el; e++)
     for (int lf = 0; lf < 4; lf++) {
       const int64_t* v = in->el_ve + e * 4;
-      const int c = ++faces[tri_key(v[kFaVe[lf][0]], v[kFaVe[lf][1]], v[kFaVe[lf][2]])];
+      const int c = ++faces[tri_key(canon[v[kFaVe[lf][0]]], canon[v[kFaVe[lf][1]]], canon[v[kFaVe[lf][2]]])];
       if (c > 2) return edge::fail(EDGE_ERR_MESH, "face shared by more than two tets");
     }
   *n_fa = (int64_t)faces.size();
@@ -371,16 +482,26 @@ extern "C" int edge_mesh_tets(const edge_tets_in* in, edge_mesh_out* m) {
   std::unordered_map<TriKey, int32_t, TriHash> tags;
   for (int64_t i = 0; i < in->n_bf; i++)
     tags[tri_key(in->bf_ve[3 * i], in->bf_ve[3 * i + 1], in->bf_ve[3 * i + 2])] = in->bf_type[i];
+  std::vector<int64_t> canon;
+  if ((st = periodic_canon(in, &canon)) != EDGE_OK) return st;
   auto face_type = [&](int64_t eL, int lf, int64_t eR, int32_t* ty) {
     const int64_t* v = m->el_ve + eL * 4;
     const auto it = tags.find(tri_key(v[kFaVe[lf][0]], v[kFaVe[lf][1]], v[kFaVe[lf][2]]));
     *ty = it != tags.end() ? it->second : (eR < 0 ? in->default_bc : 0);
     if (eR < 0 && (*ty == 0 || *ty == EDGE_FA_PERIODIC))
       return edge::fail(EDGE_ERR_MESH, "a boundary face needs a boundary type (free surface or outflow)");
+    if (eR >= 0 && *ty == EDGE_FA_PERIODIC) {  // a periodic pair: both triangles tagged
+      const int64_t* w = m->el_ve + eR * 4;
+      int nf = -1;
+      for (int s = 0; s < 4; s++)
+        if (m->el_fa[eR * 4 + s] == m->el_fa[eL * 4 + lf]) nf = s;
+      if (nf < 0 || !tags.count(tri_key(w[kFaVe[nf][0]], w[kFaVe[nf][1]], w[kFaVe[nf][2]])))
+        return edge::fail(EDGE_ERR_MESH, "periodic face tagged on one side only");
+    }
     return EDGE_OK;
   };
   int64_t built = 0;
-  st = connect(m, in->n_el, [](int64_t v) { return v; }, face_type, nfa, &built);
+  st = connect(m, in->n_el, [&](int64_t v) { return canon[v]; }, face_type, nfa, &built);
   if (st != EDGE_OK) return st;
   return built == nfa ? EDGE_OK : edge::fail(EDGE_ERR_MESH, "face count mismatch");
 }

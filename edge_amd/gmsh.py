@@ -4,6 +4,10 @@ tetrahedra are the elements, and the triangles of a physical group of
 surfaces carry the group's tag as their face type (101 free surface,
 105 outflow, or an internal-boundary type).
 
+Periodic boundaries are triangles of a physical group tagged 106 on both
+sides; the library pairs them by their coordinates as EDGE-V does
+(tools/edge_v/src/mesh/Mesh.cpp:157-288; edge_mesh_tets).
+
 Materials and time groups, which EDGE-V keeps in HDF5 annotations, are the
 caller's (no HDF5 library here)."""
 import numpy as np
@@ -110,17 +114,19 @@ def read(path, default_bc=M.FA_OUTFLOW):
 
 
 def write(path, mesh, version=4.1):
-    """Writes a mesh's tetrahedra and its typed faces (free surface, outflow
-    and other non-zero, non-periodic types) as one physical group per type."""
+    """Writes a mesh's tetrahedra and its typed faces (free surface, outflow,
+    periodic and other non-zero types) as one physical group per type; a
+    periodic face becomes its two triangles, one on each side of the box."""
     ve = mesh.ve_crds
     ty = np.asarray(mesh.fa_type)
-    faces = [f for f in range(mesh.n_fa) if ty[f] not in (0, M.FA_PERIODIC)]
+    faces = [f for f in range(mesh.n_fa) if ty[f] != 0]
     kf = [[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]]
     tri = {}
     for f in faces:
-        e = mesh.fa_el[f][0]
-        lf = list(mesh.el_fa[e]).index(f)
-        tri.setdefault(int(ty[f]), []).append([int(mesh.el_ve[e][s]) for s in kf[lf]])
+        for side in ((0, 1) if ty[f] == M.FA_PERIODIC else (0,)):
+            e = mesh.fa_el[f][side]
+            lf = list(mesh.el_fa[e]).index(f)
+            tri.setdefault(int(ty[f]), []).append([int(mesh.el_ve[e][s]) for s in kf[lf]])
     types = sorted(tri)
     lines = []
     if version >= 4:

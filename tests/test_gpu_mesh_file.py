@@ -1,6 +1,9 @@
 """A mesh read from a Gmsh file drives the GPU solver: two FP64 steps of a
 jittered box with free-surface and outflow sides, written to MSH 4.1 and
-read back, against the FP64 oracle on the same mesh (relative L2 <= 1e-12)."""
+read back, against the FP64 oracle on the same mesh (relative L2 <= 1e-12);
+a periodic box read back from a file (its sides paired by coordinates)
+carries a P plane wave as the generated box does, bit for bit, and as the
+oracle does (FP64 1e-12, FP32 fused-16 1e-5)."""
 import numpy as np
 import pytest
 
@@ -31,4 +34,40 @@ def test_solver_on_a_gmsh_mesh(gpu, tmp_path):
     o.step(ref, tr, dt, 2)
     err = np.linalg.norm(d.cpu().numpy() - ref) / np.linalg.norm(ref)
     assert err <= 1e-12, err
+    s.close()
+
+
+def test_periodic_plane_wave_on_a_gmsh_mesh(gpu, tmp_path):
+    import torch
+    box = M.box(4, jitter=0.1, seed=22)  # periodic on every axis
+    p = tmp_path / "periodic.msh"
+    gmsh.write(str(p), box, 2.2)
+    m = gmsh.read(str(p))
+    assert (m.fa_type == M.FA_PERIODIC).sum() > 0
+    rho, lam, mu = S.random_materials(m.n_el, seed=22)
+    order = 4
+    pw = S.PlaneWave(k=(2 * np.pi,) * 3)
+    q0 = S.project(m, order, pw)
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    o = orc.MeshOracle(m, order, rho=rho, lam=lam, mu=mu)
+    ref, tr = q0.copy(), np.zeros_like(q0)
+    o.step(ref, tr, dt, 3)
+    out = {}
+    for name, mesh in (("file", m), ("box", box)):
+        s = ElasticSolver(mesh, order, 1, 64, rho, lam, mu)
+        d = torch.tensor(q0, device="cuda")
+        t = torch.zeros_like(d)
+        s.step(d, t, dt, 3)
+        out[name] = d.cpu().numpy()
+        s.close()
+    np.testing.assert_array_equal(out["file"], out["box"])
+    assert np.linalg.norm(out["file"] - ref) / np.linalg.norm(ref) <= 1e-12
+    # FP32 fused-16 on the file's mesh
+    s = ElasticSolver(m, order, 16, 32, rho, lam, mu)
+    d = torch.tensor(np.repeat(q0, 16, axis=3), dtype=torch.float32, device="cuda")
+    t = torch.zeros_like(d)
+    s.step(d, t, dt, 3)
+    got = d.cpu().double().numpy()
+    ref16 = np.repeat(ref, 16, axis=3)
+    assert np.linalg.norm(got - ref16) / np.linalg.norm(ref16) <= 1e-5
     s.close()
