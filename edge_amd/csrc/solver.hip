@@ -21,6 +21,7 @@ struct edge_el_solver {
   int64_t n_el = 0;
   int n_modes = 0, n_face_modes = 0;
   int kernel = 1;  // 1 generic, 2 MFMA-tiled
+  int tune_kernel = 0;  // the kernel set-up chose (0: not recorded yet): the autotuner's candidates derive from it
   size_t real_bytes = 8;
   // device allocations
   void* mats = nullptr;  // stiffTn | stiffV | fluxLN | fluxT  (precision T); FP64: then edge::PadLayout
@@ -499,6 +500,10 @@ int neigh_lts_a_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, con
 }
 
 int one_step(edge_el_solver* s, double dt, void* dofs, void* dofs_a, void* t_int, hipStream_t st) {
+  // a whole step writes the projections its neighbour step reads: a local
+  // step pending from before is superseded
+  s->proj_fresh = false;
+  s->proj_mixed = false;
   if (s->n_rms > 0) {
     const int rc = edge_el_local_a(s, 0, s->n_el, dt, dofs, dofs_a, t_int, st);
     return rc ? rc : edge_el_neigh_a(s, 0, s->n_el, t_int, dofs, dofs_a, st);
@@ -941,19 +946,21 @@ struct TuneCand {
 };
 
 // The kernel variants a solver can switch between with the data it holds.
-std::vector<TuneCand> tune_candidates(const edge_el_solver* s) {
+std::vector<TuneCand> tune_candidates(edge_el_solver* s) {
   std::vector<TuneCand> c;
+  if (!s->tune_kernel) s->tune_kernel = s->kernel;  // before the autotuner or a variant switch first changes it
+  const int base = s->tune_kernel;
   const int prec_bytes = (int)s->real_bytes;
   const bool gen = s->n_rms > 0 ? edge::visco_supported(s->cfg.order, s->cfg.n_runs, prec_bytes, s->n_rms)
                                 : edge::generic_supported(s->cfg.order, s->cfg.n_runs, prec_bytes);
-  if (s->kernel == 2) {
+  if (base == 2) {
     const bool fusable = s->proj && s->n_rms == 0 && s->cfg.precision == 32;
     for (int x = 1; x >= 0; x--) {  // XCD-aware element schedule, then the linear one
       if (fusable) c.push_back({2, true, x == 1, true});
       if (s->proj) c.push_back({2, true, x == 1});
       c.push_back({2, false, x == 1});
     }
-  } else if (s->kernel == 4) {
+  } else if (base == 4) {
     for (int x = 1; x >= 0; x--) c.push_back({4, false, x == 1});  // XCD-aware schedule, then the linear one
   }
   if (gen || c.empty()) c.push_back({1, false});

@@ -83,6 +83,7 @@ class Receivers:
 
     def __init__(self, solver, points, freq, t0=0.0):
         self.solver = solver
+        self.points = np.asarray(points, dtype=np.float64).reshape(-1, 3)
         self.el, self.xi = locate(solver.mesh, points)
         self.n = len(self.el)
         self.freq = float(freq)
@@ -142,15 +143,32 @@ class Receivers:
         """(times [k], values [k][9][R]) of receiver i."""
         return np.array(self.times[i]), np.array(self.values[i]).reshape(-1, 9, self.solver.n_runs)
 
-    def write_csv(self, path_prefix, names=None):
-        """One CSV per receiver: time, then the 9 quantities of every run."""
+    def write_csv(self, out_dir, names=None):
+        """One CSV per receiver, in the layout of io::Receivers
+        (src/io/Receivers.cpp:178-220, 292-318): comment lines with the
+        receiver's name and coordinates (the given point, which is also where
+        it is evaluated: it lies in its element), the column names
+        time,Q0_C0,Q0_C1,...,Q8_C{R-1} (quantity-major, run innermost), then
+        one row per output time: the time with six decimals, the values in
+        scientific notation.  Files are out_dir/<name>.csv; returns their
+        paths."""
+        import os
         r = self.solver.n_runs
-        head = ",".join(["time"] + [f"{q}_{c}" for q in QUANTITIES for c in range(r)])
+        head = ",".join(["time"] + [f"Q{q}_C{c}" for q in range(9) for c in range(r)])
+        paths = []
         for i in range(self.n):
             t, v = self.seismogram(i)
             name = names[i] if names is not None else f"receiver_{i}"
-            np.savetxt(f"{path_prefix}{name}.csv", np.column_stack([t, v.reshape(len(t), -1)]), delimiter=",",
-                       header=head, comments="")
+            x = " ".join(repr(float(c)) for c in self.points[i])
+            path = os.path.join(out_dir, f"{name}.csv")
+            with open(path, "w") as f:
+                f.write("# EDGE\n# code version: edge_amd\n")
+                f.write(f"# receiver name: {name}\n# specified coordinates: {x}\n# projected coordinates: {x}\n")
+                f.write(head + "\n")
+                for k in range(len(t)):
+                    f.write(f"{t[k]:.6f}," + ",".join(f"{y:e}" for y in v[k].reshape(-1)) + "\n")
+            paths.append(path)
+        return paths
 
 
 class PointSources:
@@ -160,6 +178,7 @@ class PointSources:
 
     def __init__(self, solver, points, runs, scas, t0, dts, series):
         self.solver = solver
+        self.points = np.asarray(points, dtype=np.float64).reshape(-1, 3)
         self.el, self.xi = locate(solver.mesh, points)
         n = len(self.el)
         self.n = n

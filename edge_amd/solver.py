@@ -169,10 +169,12 @@ class ElasticSolver:
         if elements is None or len(elements) == 0:
             _lib.check(_lib.lib().edge_el_set_taps(self._h, 0, None, None))
             return
-        self._check_arr(taps)
+        if taps is not None:  # (none: the C ABI refuses it)
+            self._check_arr(taps)
         el = np.ascontiguousarray(elements, dtype=np.int64)
         self._taps = taps  # keeps the array alive while the solver writes it
-        _lib.check(_lib.lib().edge_el_set_taps(self._h, len(el), el.ctypes.data, taps.data_ptr()))
+        _lib.check(_lib.lib().edge_el_set_taps(self._h, len(el), el.ctypes.data,
+                                               None if taps is None else taps.data_ptr()))
 
     def step(self, dofs, t_int, dt, n_steps=1, graph=False, stream=None):
         """n_steps global time steps (local, then neigh); with graph=True the

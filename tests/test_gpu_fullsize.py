@@ -223,7 +223,8 @@ def test_ensemble_fused8_matches_fp64_over_300_steps(gpu):
             setup = s.setup_a()
             dt = S.dt_cfl(u, rho, setup["lam_e"], setup["mu_e"], order)
             q0 = torch.from_numpy(S.project(u, order, S.PlaneWave(k=(2 * np.pi,) * 3), chunk=16384))
-        d = (q0.cuda() * scale[:runs].cuda()).to(s.dtype).contiguous()
+        # FP32 run r starts from scale[r] q0, the FP64 reference from q0
+        d = (q0.cuda() * (scale[:runs].cuda() if prec == 32 else 1.0)).to(s.dtype).contiguous()
         t, a = torch.zeros_like(d), s.alloc_a()
         s.step(d, t, dt, n_steps, a, graph=True)
         torch.cuda.synchronize()

@@ -10,8 +10,8 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, PATH_PROJECTIONS,
-                             PATH_RANK3_FS, PATH_SPARSE_STAR, ElasticSolver)
+from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, PATH_FUSED,
+                             PATH_PROJECTIONS, PATH_RANK3_FS, PATH_SPARSE_STAR, ElasticSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -252,7 +252,8 @@ def test_mfma_physical_solvers_match_oracle(gpu, order, kernel):
     neighbour passes separately, then steps, against the FP64 oracle."""
     m, rho, lam, mu = _problem(n=3, bc=(FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC), seed=10 + order)
     s = ElasticSolver(m, order, 16, 32, rho, lam, mu, kernel=kernel)
-    want = PATH_SPARSE_STAR | PATH_RANK3_FS | (PATH_PROJECTIONS if kernel == KERNEL_MFMA else 0)
+    # exchanged projections: multi-step calls fuse neighbour step k - 1 with local step k
+    want = PATH_SPARSE_STAR | PATH_RANK3_FS | (PATH_PROJECTIONS | PATH_FUSED if kernel == KERNEL_MFMA else 0)
     assert s.kernel_info() == (2, want)
     mo = orc.MeshOracle(m, order, 16, rho=rho, lam=lam, mu=mu)
     dofs0 = _ic(m, order, 16, seed=order)
@@ -281,7 +282,7 @@ def test_mfma_raw_physical_solvers_are_factored(gpu):
     star, fs, _ = orc.setup_mesh(m, rho, lam, mu)
     order = 5
     s = ElasticSolver(m, order, 16, 32, star=star, fs=fs)
-    assert s.kernel_info() == (2, PATH_SPARSE_STAR | PATH_RANK3_FS | PATH_PROJECTIONS)
+    assert s.kernel_info() == (2, PATH_SPARSE_STAR | PATH_RANK3_FS | PATH_PROJECTIONS | PATH_FUSED)
     mo = orc.MeshOracle(m, order, 16, star=star, fs=fs)
     dofs0 = _ic(m, order, 16, seed=1)
     dt = S.dt_cfl(m, rho, lam, mu, order)

@@ -60,3 +60,31 @@ def test_points_c_abi_checks_arguments():
     assert L.edge_src_apply(None, None, 0.0, 1.0, None, None) == _lib.EDGE_ERR_ARG
     assert L.edge_el_point_basis(None, 1, el.ctypes.data, z.ctypes.data, 0, z.ctypes.data) == _lib.EDGE_ERR_ARG
     assert L.edge_points_destroy(None) == 0
+
+
+def test_receiver_csv_layout(tmp_path):
+    """io::Receivers' file layout (src/io/Receivers.cpp:178-220, 292-318):
+    comment header with name and coordinates, columns time,Q<q>_C<run>
+    quantity-major, one row per output time; values read back exactly to the
+    printed precision."""
+    from types import SimpleNamespace
+
+    from edge_amd.points import Receivers
+    r = Receivers.__new__(Receivers)
+    r.solver = SimpleNamespace(n_runs=2)
+    r.n = 2
+    r.points = np.array([[0.25, 0.5, 0.75], [0.1, 0.2, 0.3]])
+    rng = np.random.default_rng(1)
+    r.times = [[0.0, 0.125, 0.25], [0.5]]
+    r.values = [[rng.normal(size=(9, 2)) for _ in range(3)], [rng.normal(size=(9, 2))]]
+    paths = r.write_csv(str(tmp_path), names=["a", "b"])
+    lines = open(paths[0]).read().splitlines()
+    assert lines[0] == "# EDGE"
+    assert "# receiver name: a" in lines
+    assert "# specified coordinates: 0.25 0.5 0.75" in lines
+    cols = [ln for ln in lines if not ln.startswith("#")][0].split(",")
+    assert cols[:4] == ["time", "Q0_C0", "Q0_C1", "Q1_C0"] and cols[-1] == "Q8_C1" and len(cols) == 19
+    rows = [ln for ln in lines if not ln.startswith("#")][1:]
+    assert len(rows) == 3 and rows[1].split(",")[0] == "0.125000"
+    got = np.array([[float(x) for x in row.split(",")[1:]] for row in rows]).reshape(3, 9, 2)
+    np.testing.assert_allclose(got, np.array(r.values[0]), rtol=1e-6)
