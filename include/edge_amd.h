@@ -535,6 +535,19 @@ int edge_el_set_variant(edge_el_solver *s, int32_t kernel, int32_t flags);
 int edge_el_l2_norms(edge_el_solver *s, const void *a, int prec_a, const void *b, int prec_b, double *diff_sq,
                      double *ref_sq, void *stream);
 
+/* edge_el_l2_norms without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  No counterpart in the reference.  Same arguments and results,
+ * bit for bit: the per-block partial sums live in a static device array of
+ * the library (1024 blocks x 2 x 9 x 16 doubles, 2.25 MiB, held on every
+ * GPU the library runs on until the process ends) and are summed on the
+ * device in the same order.  Solvers of at most 16 runs (EDGE_ERR_ARG
+ * beyond).  Calls on one GPU share the array: calls that overlap on one GPU
+ * (other streams or threads) race.  Synchronises the stream before it
+ * returns (the results are on the host). */
+int edge_el_l2_norms_noalloc(edge_el_solver *s, const void *a, int prec_a, const void *b, int prec_b,
+                             double *diff_sq, double *ref_sq, void *stream);
+
 
 /* Elastic energy per run of a solver's device DOFs (its precision), FP64 on
  * the GPU: kinetic 1/2 rho |v|^2 and strain 1/2 sigma : C^-1 : sigma with the
