@@ -140,7 +140,9 @@ def build_problem(n_hex, order, seed=3343):
     m = M.box(n_hex, jitter=0.1, seed=seed, block=4)
     rho, lam, mu = S.random_materials(m.n_el, seed=seed)
     dt = S.dt_cfl(m, rho, lam, mu, order)
-    q0 = S.project(m, order, S.PlaneWave(k=(2 * np.pi,) * 3), chunk=4096)[..., 0]  # [el][9][N]
+    # a P plane wave of each element's medium: wave vector 2 pi (1, 1, 1)
+    q0 = S.project(m, order, S.PlaneWave(k=(2 * np.pi,) * 3), chunk=4096,
+                   materials=(rho, lam, mu))[..., 0]  # [el][9][N]
     return m, rho, lam, mu, dt, q0
 
 

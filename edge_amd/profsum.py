@@ -24,11 +24,28 @@ import sys
 N_CU = 256
 
 
+def short_name(name):
+    """The kernel's template name without namespaces, return type and
+    arguments: `void edge::(anonymous namespace)::k_local<5, true>(...)` ->
+    `k_local<5, true>`."""
+    n = name[5:] if name.startswith("void ") else name
+    n = n.replace("(anonymous namespace)::", "").replace("edge::", "")
+    depth = 0
+    for i, ch in enumerate(n):  # cut at the argument list: the first '(' outside template brackets
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return n[:i]
+    return n
+
+
 def read_stats(path):
     rows = []
     with open(path, newline="") as f:
         for r in csv.DictReader(f):
-            rows.append({"name": r["Name"], "calls": int(r["Calls"]), "avg_ms": float(r["AverageNs"]) * 1e-6,
+            rows.append({"name": short_name(r["Name"]), "calls": int(r["Calls"]), "avg_ms": float(r["AverageNs"]) * 1e-6,
                          "min_ms": float(r["MinNs"]) * 1e-6, "max_ms": float(r["MaxNs"]) * 1e-6,
                          "pct": float(r["Percentage"])})
     rows.sort(key=lambda r: -r["pct"])
@@ -42,7 +59,7 @@ def read_counters(path, match=None):
     pat = re.compile(match) if match else None
     with open(path, newline="") as f:
         for r in csv.DictReader(f):
-            name = r["Kernel_Name"]
+            name = short_name(r["Kernel_Name"])
             if pat and not pat.search(name):
                 continue
             did = int(r["Dispatch_Id"])

@@ -99,6 +99,18 @@ class PlaneWave:
         ph = np.sin(x @ self.k - self.omega * t)
         return ph[..., None] * self.r
 
+    def polarisations(self, rho, lam, mu):
+        """r_P of this wave's direction in each of the materials (rho, lam,
+        mu) [nEl], scaled like self.r (velocity amplitude amp): [nEl][9]."""
+        n = self.k / np.linalg.norm(self.k)
+        amp = np.linalg.norm(self.r[6:])
+        lam, mu = np.asarray(lam, np.float64), np.asarray(mu, np.float64)
+        cp = np.sqrt((lam + 2 * mu) / np.asarray(rho, np.float64))
+        s = -(lam[:, None, None] * np.eye(3) + 2 * mu[:, None, None] * np.outer(n, n)) / cp[:, None, None]
+        r = np.stack([s[:, 0, 0], s[:, 1, 1], s[:, 2, 2], s[:, 0, 1], s[:, 1, 2], s[:, 0, 2]] +
+                     [np.full(len(cp), n[i]) for i in range(3)], axis=1)
+        return amp * r
+
 
 def map_points(mesh, ref_pts, els=None):
     """Physical coordinates of reference points per element: [nEl][nPts][3]."""
@@ -107,10 +119,13 @@ def map_points(mesh, ref_pts, els=None):
     return ve[:, None, 0, :] + np.matmul(ref_pts[None], d)
 
 
-def project(mesh, order, fun, n_runs=1, chunk=8192):
+def project(mesh, order, fun, n_runs=1, chunk=8192, materials=None):
     """L2 projection of fun(x [..., 3]) -> [..., 9] (or [..., 9, R]) onto the
     modal basis with (O+1)^3 collapsed Gauss points: dofs [nEl][9][N][R]
-    (src/setups/InitialDofs.hpp:172-251, src/dg/Basis.cpp:136-161)."""
+    (src/setups/InitialDofs.hpp:172-251, src/dg/Basis.cpp:136-161).
+    materials = (rho, lam, mu) per element, for a PlaneWave only: each
+    element takes the wave's P polarisation in its own material (a P wave
+    of the element's medium) instead of the wave's own."""
     pts, w = quad_tet(order + 1)
     b = eval_basis(order, pts)                 # [qp][N]
     mass = dg_matrices(order)["mass"]
@@ -118,12 +133,16 @@ def project(mesh, order, fun, n_runs=1, chunk=8192):
     n = n_modes(order)
     out = np.zeros((mesh.n_el, N_QTS, n, n_runs))
     wave = isinstance(fun, PlaneWave)
+    if materials is not None and not wave:
+        raise ValueError("per-element polarisations need a PlaneWave")
+    r_el = fun.polarisations(*materials) if materials is not None else None
     for e0 in range(0, mesh.n_el, chunk):
         els = np.arange(e0, min(mesh.n_el, e0 + chunk))
         x = map_points(mesh, pts, els)
         if wave:  # q = r sin(k.x): project the scalar phase once, one matrix product
             coef = np.sin(x @ fun.k) @ proj  # [e][N]
-            out[els] = coef[:, None, :, None] * fun.r[None, :, None, None]
+            r = fun.r[None, :] if r_el is None else r_el[els]
+            out[els] = coef[:, None, :, None] * r[:, :, None, None]
             continue
         f = fun(x)
         if f.ndim == 3:

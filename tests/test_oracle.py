@@ -309,3 +309,27 @@ def test_visco_flux_solvers_rotate_the_middle_state():
     l, r = orc.fs_a(*args, n, s, t)
     np.testing.assert_allclose(l, T[:6, :6] @ ml @ Ti, rtol=1e-13, atol=1e-14)
     np.testing.assert_allclose(r, T[:6, :6] @ mr @ Ti, rtol=1e-13, atol=1e-14)
+
+
+def test_plane_wave_polarised_per_element():
+    """project(..., materials=...) gives each element the P eigenvector of its
+    own material: the same DOFs as a homogeneous wave of that material, and
+    the eigenvector of the normal Jacobian A_n for c_p."""
+    m = M.box(3, jitter=0.1, seed=4)
+    rho, lam, mu = S.random_materials(m.n_el, seed=4)
+    pw = S.PlaneWave(k=(2 * np.pi, 0.0, 2 * np.pi))
+    q = S.project(m, 3, pw, materials=(rho, lam, mu))
+    for e in (0, 7, m.n_el - 1):
+        qe = S.project(m, 3, S.PlaneWave(rho[e], lam[e], mu[e], k=pw.k))[e]
+        np.testing.assert_allclose(q[e], qe, rtol=1e-14, atol=1e-14)
+    r = pw.polarisations(rho, lam, mu)
+    n = pw.k / np.linalg.norm(pw.k)
+    for e in (0, 5):
+        cp = np.sqrt((lam[e] + 2 * mu[e]) / rho[e])
+        a = sum(n[d] * orc.star(rho[e], lam[e], mu[e], np.eye(3))[d].reshape(9, 9) for d in range(3))
+        w = np.linalg.eigvals(a)
+        assert np.min(np.abs(np.abs(w) - cp)) < 1e-9
+        # r_P is an eigenvector of A_n for an eigenvalue of magnitude c_p
+        lam_r = (a @ r[e]) @ r[e] / (r[e] @ r[e])
+        np.testing.assert_allclose(a @ r[e], lam_r * r[e], rtol=1e-12, atol=1e-10)
+        assert abs(abs(lam_r) - cp) < 1e-9
