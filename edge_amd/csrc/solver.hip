@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "ec64_pack.hpp"
 #include "kernels_mfma.hpp"
 #include "mfma_pack.hpp"
 #include "setup.hpp"
@@ -56,6 +57,13 @@ struct edge_el_solver {
   const void* proj_tint = nullptr;  // tInt of the local steps since the last neighbour step (null: none valid)
   bool proj_mixed = false;          // those local steps wrote different tInt arrays
   bool proj_fresh = false;          // a local step ran since the last neighbour step
+  // FP64 element-batched kernels (kernel 5): packed operands, face-orientation
+  // maps and their face projections [nEl][4][3][16]
+  bool want_ec = false;  // set-up builds their data when the solver qualifies
+  bool ec_ok = false;
+  double* ec_pk = nullptr;
+  double* ec_rot = nullptr;
+  double* ec_proj = nullptr;
   // viscoelastic solvers
   int n_rms = 0;
   void* rfs = nullptr;       // [n_rms]
@@ -171,7 +179,7 @@ int check_cfg(const edge_el_config* c) {
   if (c->n_runs != 1 && c->n_runs != 8 && c->n_runs != 16)
     return edge::fail(EDGE_ERR_UNSUPPORTED, "n_runs must be 1, 8 or 16");
   if (c->precision != 64 && c->precision != 32) return edge::fail(EDGE_ERR_UNSUPPORTED, "precision must be 32 or 64");
-  if (c->kernel < 0 || c->kernel > 4) return edge::fail(EDGE_ERR_ARG, "kernel selector");
+  if (c->kernel < 0 || c->kernel > 5) return edge::fail(EDGE_ERR_ARG, "kernel selector");
   return EDGE_OK;
 }
 
@@ -219,7 +227,7 @@ int build_neigh(edge_el_solver* s, const edge_mesh_in* m) {
   EDGE_HIP_TRY(hipMalloc(&s->fmat, 4 * ne));
   EDGE_HIP_TRY(hipMemcpy(s->ne, nb.data(), sizeof(int32_t) * 4 * ne, hipMemcpyHostToDevice));
   EDGE_HIP_TRY(hipMemcpy(s->fmat, fm.data(), 4 * ne, hipMemcpyHostToDevice));
-  if (s->kernel != 2 && !s->proj_tables) return EDGE_OK;
+  if (s->kernel != 2 && !s->proj_tables && !s->want_ec) return EDGE_OK;
   // Face projections: element el's face fa receives the projection that its
   // neighbour n computes through the shared face (n's face fId, or fa itself
   // on a free surface) with el's flux matrix and neighbour-side solver.
@@ -294,11 +302,51 @@ int finish_mfma(edge_el_solver* s) {
   return EDGE_OK;
 }
 
+// The element-batched FP64 kernels' data, once the flux solvers and their
+// factors exist: they need the elastic star pattern, rank-3 solvers, one
+// projection slot per element face and face-orientation maps.
+int finish_ec(edge_el_solver* s) {
+  if (!s->want_ec) return EDGE_OK;
+  edge::DgTables t;
+  std::vector<double> rot;
+  const bool ok = s->sparse_star && s->fs_rank3 && s->fsf && s->proj_ok && edge::dg_tables(s->cfg.order, &t) &&
+                  edge::ec64_rotations(t, &rot);
+  if (!ok) {
+    if (s->kernel == 5)
+      return edge::fail(EDGE_ERR_UNSUPPORTED, "element-batched FP64 kernels: they need the elastic star pattern and "
+                                              "rank-3 flux solvers");
+    return EDGE_OK;
+  }
+  const std::vector<double> pk = edge::ec64_pack_operands(t);
+  EDGE_HIP_TRY(hipMalloc(&s->ec_pk, sizeof(double) * pk.size()));
+  EDGE_HIP_TRY(hipMemcpy(s->ec_pk, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
+  EDGE_HIP_TRY(hipMalloc(&s->ec_rot, sizeof(double) * rot.size()));
+  EDGE_HIP_TRY(hipMemcpy(s->ec_rot, rot.data(), sizeof(double) * rot.size(), hipMemcpyHostToDevice));
+  EDGE_HIP_TRY(hipMalloc(&s->ec_proj, sizeof(double) * 4 * 48 * s->n_el));
+  s->ec_ok = true;
+  return EDGE_OK;
+}
+
+edge::Ec64Dev ec_view(const edge_el_solver* s) {
+  edge::Ec64Dev x;
+  x.pk = s->ec_pk;
+  x.rot = s->ec_rot;
+  x.proj = s->ec_proj;
+  return x;
+}
+
 int pick_kernel(edge_el_solver* s) {
   const bool mfma_ok = edge::mfma_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision);
   const bool gen_ok = edge::generic_supported(s->cfg.order, s->cfg.n_runs, (int)s->real_bytes);
   const bool p64_ok = edge::plain64_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8);
-  if (s->cfg.kernel == 2 || s->cfg.kernel == 3) {
+  const bool ec_ok = edge::ec64_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8);
+  // the element-batched FP64 kernels' data is built whenever they may run
+  // (requested, or as an autotuner candidate beside the FP64 MFMA kernels)
+  s->want_ec = ec_ok && (s->cfg.kernel == 0 || s->cfg.kernel == 4 || s->cfg.kernel == 5);
+  if (s->cfg.kernel == 5) {
+    if (!ec_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "element-batched FP64 kernels: FP64, one run, orders 2-5");
+    s->kernel = 5;
+  } else if (s->cfg.kernel == 2 || s->cfg.kernel == 3) {
     if (!mfma_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no MFMA-tiled kernel for this configuration");
     s->kernel = 2;
     s->want_proj = s->cfg.kernel == 2;
@@ -339,7 +387,7 @@ int create_common(const edge_el_config* cfg, const edge_mesh_in* mesh, edge_el_s
   if ((rc = build_neigh(s, mesh))) return rc;
   EDGE_HIP_TRY(hipMalloc(&s->star, s->real_bytes * 243 * s->n_el));
   EDGE_HIP_TRY(hipMalloc(&s->fs, s->real_bytes * 2 * 324 * s->n_el));
-  if (s->kernel == 2) EDGE_HIP_TRY(hipMalloc(&s->fsf, s->real_bytes * 2 * 4 * 54 * s->n_el));
+  if (s->kernel == 2 || s->want_ec) EDGE_HIP_TRY(hipMalloc(&s->fsf, s->real_bytes * 2 * 4 * 54 * s->n_el));
   EDGE_HIP_TRY(hipMalloc(&s->mass, sizeof(double) * t.n));
   EDGE_HIP_TRY(hipMemcpy(s->mass, t.mass, sizeof(double) * t.n, hipMemcpyHostToDevice));
   return EDGE_OK;
@@ -348,7 +396,7 @@ int create_common(const edge_el_config* cfg, const edge_mesh_in* mesh, edge_el_s
 // The MFMA kernels exchange face projections between the local and the
 // neighbour step: the neighbour step then reads them instead of tInt.
 bool proj_active(const edge_el_solver* s) {
-  return s->kernel == 2 && s->fs_rank3 && s->fsf && s->use_proj && s->proj;
+  return (s->kernel == 2 && s->fs_rank3 && s->fsf && s->use_proj && s->proj) || (s->kernel == 5 && s->ec_ok);
 }
 
 // A local step that writes projections records the tInt array it wrote.
@@ -385,8 +433,11 @@ int local_t(edge_el_solver* s, int64_t first, int64_t n, double dt, void* dofs, 
   const auto d = dev_view<T>(s);
   note_local(s, t_int);
   if (s->kernel == 2) return edge::launch_local_mfma<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
-  if constexpr (sizeof(T) == 8)
+  if constexpr (sizeof(T) == 8) {
     if (s->kernel == 4) return edge::launch_local_plain64(d, first, n, dt, (double*)dofs, (double*)t_int, st);
+    if (s->kernel == 5)
+      return edge::launch_local_ec64(d, ec_view(s), first, n, dt, (double*)dofs, (double*)t_int, st);
+  }
   return edge::launch_local_generic<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
 }
 
@@ -396,8 +447,10 @@ int neigh_t(edge_el_solver* s, int64_t first, int64_t n, const void* t_int, void
   const auto d = dev_view<T>(s);
   if (int rc = check_neigh(s, t_int)) return rc;
   if (s->kernel == 2) return edge::launch_neigh_mfma<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
-  if constexpr (sizeof(T) == 8)
+  if constexpr (sizeof(T) == 8) {
     if (s->kernel == 4) return edge::launch_neigh_plain64(d, first, n, (const double*)t_int, (double*)dofs, st);
+    if (s->kernel == 5) return edge::launch_neigh_ec64(d, ec_view(s), first, n, (double*)dofs, st);
+  }
   return edge::launch_neigh_generic<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
 }
 
@@ -450,8 +503,9 @@ int neigh_lts_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, const
   if (s->kernel == 2)
     rc = edge::launch_neigh_mfma<T>(d, first, n, (const T*)t0, (T*)dofs, st);
   else if constexpr (sizeof(T) == 8)
-    rc = s->kernel == 4 ? edge::launch_neigh_plain64(d, first, n, (const double*)t0, (double*)dofs, st)
-                        : edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
+    rc = s->kernel == 4   ? edge::launch_neigh_plain64(d, first, n, (const double*)t0, (double*)dofs, st)
+         : s->kernel == 5 ? edge::launch_neigh_ec64(d, ec_view(s), first, n, (double*)dofs, st)
+                          : edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
   else
     rc = edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
   if (rc) return rc;
@@ -611,6 +665,7 @@ extern "C" int edge_el_create(const edge_el_config* cfg, const edge_mesh_in* mes
                                           &s->fs_rank3, nullptr, nullptr, nullptr, nullptr);
     if (rc == EDGE_OK) rc = zero_fault_faces(s);
     if (rc == EDGE_OK) rc = finish_mfma(s);
+    if (rc == EDGE_OK) rc = finish_ec(s);
     if (rc == EDGE_OK) rc = upload_element_data(s, mesh, rho, lam, mu);
   }
   if (rc != EDGE_OK) {
@@ -651,6 +706,7 @@ extern "C" int edge_el_create_raw(const edge_el_config* cfg, const edge_mesh_in*
                                 : edge::factor_flux_solvers<float>(fs, 8 * s->n_el, (float*)s->fsf, &s->fs_rank3);
     if (rc == EDGE_OK) rc = zero_fault_faces(s);
     if (rc == EDGE_OK) rc = finish_mfma(s);
+    if (rc == EDGE_OK) rc = finish_ec(s);
   }
   if (rc != EDGE_OK) {
     if (s) edge_el_destroy(s);
@@ -685,6 +741,9 @@ extern "C" int edge_el_destroy(edge_el_solver* s) {
   (void)hipFree(s->src_a);
   (void)hipFree(s->fs_a);
   (void)hipFree(s->fsfa);
+  (void)hipFree(s->ec_pk);
+  (void)hipFree(s->ec_rot);
+  (void)hipFree(s->ec_proj);
   drop_lts(s);
   delete s;
   return EDGE_OK;
@@ -930,8 +989,8 @@ extern "C" int edge_el_kernel_info(const edge_el_solver* s, int32_t* kernel, int
   if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
   if (kernel) *kernel = s->kernel;
   if (flags) {
-    const bool r3 = s->kernel == 2 && s->fs_rank3 && s->fsf;
-    *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (r3 && s->proj && s->use_proj ? 4 : 0) |
+    const bool r3 = (s->kernel == 2 || s->kernel == 5) && s->fs_rank3 && s->fsf;
+    *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (proj_active(s) ? 4 : 0) |
              (s->n_rms > 0 ? 8 : 0) | (fused_active(s) ? 16 : 0);
   }
   return EDGE_OK;
@@ -960,8 +1019,11 @@ std::vector<TuneCand> tune_candidates(edge_el_solver* s) {
       if (s->proj) c.push_back({2, true, x == 1});
       c.push_back({2, false, x == 1});
     }
-  } else if (base == 4) {
-    for (int x = 1; x >= 0; x--) c.push_back({4, false, x == 1});  // XCD-aware schedule, then the linear one
+  } else if (base == 4 || base == 5) {
+    for (int x = 1; x >= 0; x--) {  // XCD-aware schedule, then the linear one
+      c.push_back({4, false, x == 1});
+      if (s->ec_ok) c.push_back({5, true, x == 1});  // element-batched: projections always exchanged
+    }
   }
   if (gen || c.empty()) c.push_back({1, false});
   return c;

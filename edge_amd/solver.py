@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib
 from .setups import n_modes
 
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES, KERNEL_MFMA64 = 0, 1, 2, 3, 4
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES, KERNEL_MFMA64, KERNEL_EC64 = 0, 1, 2, 3, 4, 5
 PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS, PATH_VISCO, PATH_FUSED = 1, 2, 4, 8, 16
 
 

@@ -14,7 +14,7 @@ from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd._lib import EdgeError
 from edge_amd.mesh import FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, ElasticSolver
+from edge_amd.solver import KERNEL_EC64, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, ElasticSolver
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +45,8 @@ def _problem(order, R, mus=0.677, seed=3, n=4):
     return m, rho, lam, mu, frame, glob, st0
 
 
-CASES = [(64, 1, KERNEL_MFMA64, 1e-12), (64, 1, KERNEL_GENERIC, 1e-12), (32, 16, KERNEL_MFMA, 1e-5),
+CASES = [(64, 1, KERNEL_MFMA64, 1e-12), (64, 1, KERNEL_GENERIC, 1e-12), (64, 1, KERNEL_EC64, 1e-12),
+         (32, 16, KERNEL_MFMA, 1e-5),
          (32, 16, KERNEL_MFMA_TILES, 1e-5)]
 
 

@@ -11,8 +11,8 @@ from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.lts import Rate2Lts
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, PATH_PROJECTIONS,
-                             ElasticSolver)
+from edge_amd.solver import (KERNEL_AUTO, KERNEL_EC64, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES,
+                             PATH_PROJECTIONS, ElasticSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -35,12 +35,14 @@ def _run_oracle(o, dofs, ranges, dt0, cycles):
         step(len(ranges) - 1)
 
 
-@pytest.mark.parametrize("prec,order,runs,n_groups", [(64, 4, 1, 2), (64, 3, 1, 3), (32, 5, 16, 2), (32, 4, 16, 3)])
-def test_lts_matches_oracle(gpu, prec, order, runs, n_groups):
+@pytest.mark.parametrize("prec,order,runs,n_groups,kernel", [(64, 4, 1, 2, KERNEL_AUTO), (64, 3, 1, 3, KERNEL_AUTO),
+                                                             (32, 5, 16, 2, KERNEL_AUTO), (32, 4, 16, 3, KERNEL_AUTO),
+                                                             (64, 5, 1, 2, KERNEL_EC64), (64, 4, 1, 3, KERNEL_EC64)])
+def test_lts_matches_oracle(gpu, prec, order, runs, n_groups, kernel):
     import torch
     m = M.box(4, jitter=0.1, seed=17)
     rho, lam, mu = S.random_materials(m.n_el, seed=17)
-    s = ElasticSolver(m, order, runs, prec, rho, lam, mu)
+    s = ElasticSolver(m, order, runs, prec, rho, lam, mu, kernel=kernel)
     if prec == 32:
         assert s.kernel_info()[1] & PATH_PROJECTIONS
     cut = np.linspace(0, m.n_el, n_groups + 1).astype(int)

@@ -10,8 +10,8 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, PATH_FUSED,
-                             PATH_PROJECTIONS, PATH_RANK3_FS, PATH_SPARSE_STAR, ElasticSolver)
+from edge_amd.solver import (KERNEL_EC64, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES,
+                             PATH_FUSED, PATH_PROJECTIONS, PATH_RANK3_FS, PATH_SPARSE_STAR, ElasticSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -58,7 +58,7 @@ def test_setup_matches_oracle(gpu, bc):
     assert rel_l2(fs, ofs) <= 1e-13
 
 
-@pytest.mark.parametrize("kernel", [KERNEL_GENERIC, KERNEL_MFMA64])
+@pytest.mark.parametrize("kernel", [KERNEL_GENERIC, KERNEL_MFMA64, KERNEL_EC64])
 @pytest.mark.parametrize("order", [2, 3, 4, 5])
 def test_fp64_local_neigh_match_oracle(gpu, order, kernel):
     m, rho, lam, mu = _problem()
@@ -92,6 +92,33 @@ def test_fp64_steps_match_oracle(gpu, order, bc):
     mo.step(d, t, dt, n)
     gd, gt = _to_dev(dofs0, 64), _to_dev(np.zeros_like(dofs0), 64)
     s.step(gd, gt, dt, n)
+    assert rel_l2(gd.cpu().numpy(), d) <= FP64_TOL
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+@pytest.mark.parametrize("bc", [(FA_PERIODIC,) * 3, (FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC)])
+def test_ec64_matches_oracle(gpu, order, bc):
+    """The FP64 element-batched kernels (16 elements per wave on FP64 MFMA):
+    162 elements, so the last wave is partly filled; free-surface, outflow
+    and periodic faces (every face orientation); tInt and DOFs of the local
+    step, the neighbour step from the face projections, then steps."""
+    m, rho, lam, mu = _problem(n=3, bc=bc, seed=30 + order)
+    s = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=KERNEL_EC64)
+    assert s.kernel_info() == (KERNEL_EC64, PATH_SPARSE_STAR | PATH_RANK3_FS | PATH_PROJECTIONS)
+    mo = orc.MeshOracle(m, order, rho=rho, lam=lam, mu=mu)
+    dofs0 = _ic(m, order, 1, seed=order)
+    dt = S.dt_cfl(m, rho, lam, mu, order)
+    d, t = dofs0.copy(), np.zeros_like(dofs0)
+    mo.local(d, t, dt)
+    gd, gt = _to_dev(dofs0, 64), _to_dev(np.zeros_like(dofs0), 64)
+    s.local(gd, gt, dt)
+    assert rel_l2(gt.cpu().numpy(), t) <= FP64_TOL
+    assert rel_l2(gd.cpu().numpy(), d) <= FP64_TOL
+    mo.neigh(d, t)
+    s.neigh(gd, gt)
+    assert rel_l2(gd.cpu().numpy(), d) <= FP64_TOL
+    mo.step(d, t, dt, 5)
+    s.step(gd, gt, dt, 5)
     assert rel_l2(gd.cpu().numpy(), d) <= FP64_TOL
 
 

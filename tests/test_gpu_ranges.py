@@ -9,8 +9,8 @@ import pytest
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES, ElasticSolver,
-                             ViscoSolver)
+from edge_amd.solver import (KERNEL_EC64, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, KERNEL_MFMA_TILES,
+                             ElasticSolver, ViscoSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,7 @@ def _ranges(cuts, n_el):
 
 
 @pytest.mark.parametrize("prec,runs,kernel", [(64, 1, KERNEL_MFMA64), (64, 1, KERNEL_GENERIC), (32, 16, KERNEL_MFMA),
-                                              (32, 16, KERNEL_MFMA_TILES)])
+                                              (32, 16, KERNEL_MFMA_TILES), (64, 1, KERNEL_EC64)])
 def test_elastic_ranges_equal_full(gpu, prec, runs, kernel):
     import torch
     m, rho, lam, mu = _mesh()
