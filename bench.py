@@ -4,9 +4,9 @@ Workload (the headline): 32^3 hexes x 6 = 196,608 tets, interior vertices
 jittered by +-10% (seeded), per-element rho ~ U[2,3], vp ~ U[4,6],
 vs = vp / sqrt(3), degree 4 (order 5, 35 modes), 16 fused runs in FP32,
 global time stepping at the CFL time step.  Beside it the plain FP64
-single-run update on the same mesh is timed (the drop-in), and the FP64
-oracle (the vanilla-semantics CPU restatement) on a bounded sample of the
-same workload on the host cores.
+single-run update on the same mesh is timed (the drop-in), the FP32
+single-run update, and the FP64 oracle (the vanilla-semantics CPU
+restatement) on a bounded sample of the same workload on the host cores.
 
 A step of the fused-16 run is the reference's (src/impl/seismic/steps.inc):
 the receivers' time prediction at the step start (64 receivers), one local
@@ -600,7 +600,7 @@ def main():
         rows(args)
         return
     import torch
-    from edge_amd.solver import KERNEL_MFMA, PATH_FUSED, PATH_PROJECTIONS, ElasticSolver
+    from edge_amd.solver import KERNEL_EC64, KERNEL_MFMA, PATH_FUSED, PATH_PROJECTIONS, ElasticSolver
 
     order, runs = args.order, 16
     print(f"[bench] building the {6 * args.hex ** 3}-tet mesh (order {order})", file=sys.stderr, flush=True)
@@ -644,8 +644,22 @@ def main():
         print("[bench] fp64 plain timing", file=sys.stderr, flush=True)
         w64, k64 = time_solver(s64, d64, t64, dt, max(1, args.steps // 2), args.warmup)
         res["fp64"] = dict(wall=w64, k=k64, steps=max(1, args.steps // 2))
+        res["fp64"]["kernel"] = s64.kernel_info()[0]
         del d64, t64
         s64.close()
+        # ---- FP32 single run (config 2's FP32 row), beside the FP64 one
+        s1 = ElasticSolver(m, order, 1, 32, rho, lam, mu)
+        if not args.no_autotune:
+            tune["fp32_single_run"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "ms_per_step": ms}
+                                       for k, p, x, ms, _ in s1.autotune(dt, 2)]
+        d1 = torch.from_numpy(q0[..., None].copy()).to(torch.float32).cuda().contiguous()
+        t1 = torch.zeros_like(d1)
+        print("[bench] fp32 single-run timing", file=sys.stderr, flush=True)
+        w1, k1 = time_solver(s1, d1, t1, dt, max(1, args.steps // 2), args.warmup)
+        res["fp32_r1"] = dict(wall=w1, k=k1, steps=max(1, args.steps // 2), kernel=s1.kernel_info()[0],
+                              finite=bool(torch.isfinite(d1).all()))
+        del d1, t1
+        s1.close()
 
     dofs_per_step = m.n_el * 9 * n
     value = dofs_per_step * runs * args.steps / res["fp32"]["wall"]
@@ -700,10 +714,22 @@ def main():
         if fl:
             a64 = (fl["local"] + fl["neigh"]) * m.n_el / (k64["local"] + k64["neigh"])
             out["fp64_plain"]["roofline_frac"] = a64 / PEAK["fp64"]
-            u64 = read_mfma_util(MFMA_PROFILE, m.n_el, order, 1, f"k_local_p64<{order}, true>")
+            name64 = (f"k_local_ec<double, {order}>" if res["fp64"]["kernel"] == KERNEL_EC64
+                      else f"k_local_p64<{order}, true>")
+            u64 = read_mfma_util(MFMA_PROFILE, m.n_el, order, 1, name64)
+            out["fp64_plain"]["kernel"] = name64
             if u64 is not None:
                 out["fp64_plain"]["mfma_busy_local"] = u64
         out["speedup_fp32_fused16_vs_fp64_plain"] = value / v64
+    if "fp32_r1" in res:
+        k1 = res["fp32_r1"]["k"]
+        out["fp32_single_run"] = {"value": dofs_per_step * res["fp32_r1"]["steps"] / res["fp32_r1"]["wall"],
+                                  "unit": "DOF-updates/s", "ms_per_step": 1e3 * res["fp32_r1"]["wall"] /
+                                  res["fp32_r1"]["steps"], "local_ms": 1e3 * k1["local"], "neigh_ms": 1e3 * k1["neigh"],
+                                  "kernel": res["fp32_r1"]["kernel"], "finite": res["fp32_r1"]["finite"]}
+        if fl:
+            a1 = (fl["local"] + fl["neigh"]) * m.n_el / (k1["local"] + k1["neigh"])
+            out["fp32_single_run"]["roofline_frac"] = a1 / PEAK["fp32_mfma"]
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(m, order, runs, rho, lam, mu, dt, q0, args.cpu_seconds)

@@ -1,5 +1,5 @@
-// Host side of the FP64 element-batched kernels: packed A operands and the
-// face-orientation maps (ec64_pack.hpp).
+// Host side of the element-batched single-run kernels: packed A operands and
+// the face-orientation maps (ec64_pack.hpp).
 #include "ec64_pack.hpp"
 
 #include <algorithm>
@@ -9,15 +9,19 @@ namespace edge {
 namespace {
 
 // Appends the packed A operand of Out[m][e] = sum_{k < nk} M[k][m] X_e[k],
-// m < nn, for M [rows][ld] (row-major).
-void pack_product(const std::vector<double>& m, int ld, int nk, int nn, std::vector<double>* out) {
-  const int nt = ec::tiles(nn), np = ec::pairs(ec::ksteps(nk));
+// m < nn, for M [rows][ld] (row-major), over ks k-steps with the row map of
+// B operands from LDS (chain = false) or from D tiles (chain = true, which
+// differs for FP32).
+void pack_product(const std::vector<double>& m, int ld, int nk, int nn, int ks, bool chain, bool f32,
+                  std::vector<double>* out) {
+  const int nt = ec::tiles(nn), np = ec::pairs(ks);
   for (int t = 0; t < nt; t++)
     for (int p = 0; p < np; p++)
       for (int lane = 0; lane < 64; lane++)
         for (int i = 0; i < 2; i++) {
-          const int g = lane >> 4, c = lane & 15, k = 4 * (2 * p + i) + g, col = 16 * t + c;
-          out->push_back(k < nk && col < nn ? m[(size_t)k * ld + col] : 0.0);
+          const int g = lane >> 4, c = lane & 15, s = 2 * p + i, col = 16 * t + c;
+          const int k = chain && f32 ? 16 * (s / 4) + 4 * g + s % 4 : 4 * s + g;
+          out->push_back(s < ks && k < nk && col < nn ? m[(size_t)k * ld + col] : 0.0);
         }
 }
 
@@ -57,34 +61,42 @@ void lstsq(int n, int k, std::vector<double> a, std::vector<double> b, std::vect
 
 }  // namespace
 
-std::vector<double> ec64_pack_operands(const DgTables& t) {
+std::vector<double> ec_pack_operands(const DgTables& t, bool f32) {
   const int O = t.order, N = t.n, NF = t.nf;
   std::vector<double> out;
-  const ec::Layout L{O};
+  const ec::Layout L{O, f32};
   out.reserve((size_t)L.size());
   std::vector<double> m;
   for (int de = 1; de < O; de++)
     for (int d = 0; d < 3; d++) {
       m.assign(t.stiffT + (size_t)d * N * N, t.stiffT + (size_t)(d + 1) * N * N);
       for (double& x : m) x = -x;  // -(K_d^T M^-1), as src/dg/TimePred.hpp:86 stores it
-      pack_product(m, N, n_modes(O - de + 1), n_modes(O - de), &out);
+      const int nk = n_modes(O - de + 1);
+      if (de == 1)
+        pack_product(m, N, nk, n_modes(O - de), ec::ksteps(nk), false, f32, &out);
+      else
+        pack_product(m, N, nk, n_modes(O - de), L.chain(nk), true, f32, &out);
     }
   for (int d = 0; d < 3; d++) {
     m.assign(t.stiffV + (size_t)d * N * N, t.stiffV + (size_t)(d + 1) * N * N);
-    pack_product(m, N, n_modes(O - 1), N, &out);
+    pack_product(m, N, n_modes(O - 1), N, ec::ksteps(n_modes(O - 1)), false, f32, &out);
   }
   for (int f = 0; f < 4; f++) {
     m.assign(t.fluxL + (size_t)f * N * NF, t.fluxL + (size_t)(f + 1) * N * NF);
-    pack_product(m, NF, N, NF, &out);
+    pack_product(m, NF, N, NF, ec::ksteps(N), false, f32, &out);
   }
   for (int f = 0; f < 4; f++) {
     m.assign(t.fluxT + (size_t)f * NF * N, t.fluxT + (size_t)(f + 1) * NF * N);
-    pack_product(m, N, NF, N, &out);
+    pack_product(m, N, NF, N, L.chain(NF), true, f32, &out);
+  }
+  for (int f = 0; f < 4; f++) {
+    m.assign(t.fluxT + (size_t)f * NF * N, t.fluxT + (size_t)(f + 1) * NF * N);
+    pack_product(m, N, NF, N, ec::ksteps(NF), false, f32, &out);
   }
   return out;
 }
 
-bool ec64_rotations(const DgTables& t, std::vector<double>* rot) {
+bool ec_rotations(const DgTables& t, std::vector<double>* rot) {
   const int N = t.n, NF = t.nf;
   if (NF > 16) return false;
   rot->assign(ec::kRotSize, 0.0);

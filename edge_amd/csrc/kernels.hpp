@@ -117,20 +117,22 @@ int launch_local_plain64(const SolverDev<double>& s, int64_t first, int64_t n, d
                          double* t_int, hipStream_t st);
 int launch_neigh_plain64(const SolverDev<double>& s, int64_t first, int64_t n, const double* t_int, double* dofs,
                          hipStream_t st);
-// FP64 single-run kernels with 16 elements per wave on FP64 MFMA
+// Single-run kernels with 16 elements per wave on FP64 / FP32 MFMA
 // (kernels_ec64.hip): elastic star pattern and rank-3 factored flux solvers;
-// the local step writes face projections [nEl][4][3][16] (doubles) that the
-// neighbour step lifts after the face-orientation map.
-struct Ec64Dev {
-  const double* pk = nullptr;   // packed A operands (ec64_pack.hpp, ec::Layout)
-  const double* rot = nullptr;  // face-orientation maps [4][16][16]
-  double* proj = nullptr;       // face projections [nEl][4][3][16]
+// the local step writes face projections [nEl][4][3][16] that the neighbour
+// step lifts after the face-orientation map.
+template <typename T>
+struct EcDev {
+  const T* pk = nullptr;   // packed A operands (ec64_pack.hpp, ec::Layout)
+  const T* rot = nullptr;  // face-orientation maps [4][16][16]
+  T* proj = nullptr;       // face projections [nEl][4][3][16]
 };
-bool ec64_supported(int order, int n_runs, int prec_bytes);
-int launch_local_ec64(const SolverDev<double>& s, const Ec64Dev& x, int64_t first, int64_t n, double dt,
-                      double* dofs, double* t_int, hipStream_t st);
-int launch_neigh_ec64(const SolverDev<double>& s, const Ec64Dev& x, int64_t first, int64_t n, double* dofs,
-                      hipStream_t st);
+bool ec_supported(int order, int n_runs);
+template <typename T>
+int launch_local_ec(const SolverDev<T>& s, const EcDev<T>& x, int64_t first, int64_t n, T dt, T* dofs, T* t_int,
+                    hipStream_t st);
+template <typename T>
+int launch_neigh_ec(const SolverDev<T>& s, const EcDev<T>& x, int64_t first, int64_t n, T* dofs, hipStream_t st);
 // ... and their viscoelastic form (1 to 3 mechanisms; anelastic rows 9..14 of the tiles)
 bool plain64_visco_supported(int order, int n_runs, int prec_bytes, int n_rms);
 int launch_local_plain64_visco(const SolverDev<double>& s, int64_t first, int64_t n, double dt, double* dofs,

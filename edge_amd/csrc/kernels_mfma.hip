@@ -29,6 +29,7 @@
 // applied before the stiffness product); results agree to rounding.
 #include <hip/hip_runtime.h>
 
+#include "dg_masks.hpp"
 #include "kernels_mfma.hpp"
 #include "mfma_pack.hpp"
 
@@ -385,15 +386,19 @@ __device__ __forceinline__ void ck_first_el(const float* __restrict__ pk, const 
     for (int p = 0; p < 9; p++) l_ti[lds_idx<O>(p, k, NK, c)] = dt * b[p];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
+      // A blocks that are zero (the stiffness matrices' sparsity) are skipped
+      constexpr masks::Tab NZ = masks::Nz<O, true>::tab;
 #pragma unroll
       for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int t = 0; t < TN; t++) pv[d][i][t] = mfma(a4[d][t][s % 4], b[6 + i], pv[d][i][t]);
+        for (int t = 0; t < TN; t++)
+          if ((NZ.ck[1][d][t] >> s) & 1) pv[d][i][t] = mfma(a4[d][t][s % 4], b[6 + i], pv[d][i][t]);
 #pragma unroll
       for (int r = 0; r < 3; r++) {
         const float z = sv[d].row(r, b);
 #pragma unroll
-        for (int t = 0; t < TN; t++) der[6 + r][t] = mfma(a4[d][t][s % 4], z, der[6 + r][t]);
+        for (int t = 0; t < TN; t++)
+          if ((NZ.ck[1][d][t] >> s) & 1) der[6 + r][t] = mfma(a4[d][t][s % 4], z, der[6 + r][t]);
       }
     }
     if (s % 4 == 3) {
@@ -439,13 +444,15 @@ __device__ __forceinline__ void ck_next_el(const float* __restrict__ pk, const c
       }
       __builtin_amdgcn_sched_barrier(0);
       const float a = a4[v];
-      float b[9];
+      if ((masks::Nz<O, true>::tab.ck[DE][d][0] >> j) & 1) {  // a non-zero A block
+        float b[9];
 #pragma unroll
-      for (int p = 0; p < 9; p++) b[p] = prev[p][u][v];
+        for (int p = 0; p < 9; p++) b[p] = prev[p][u][v];
 #pragma unroll
-      for (int q = 0; q < 3; q++) pv[q][0] = mfma(a, b[6 + q], pv[q][0]);
+        for (int q = 0; q < 3; q++) pv[q][0] = mfma(a, b[6 + q], pv[q][0]);
 #pragma unroll
-      for (int r = 0; r < 3; r++) cur[6 + r][0] = mfma(a, sv.row(r, b), cur[6 + r][0]);
+        for (int r = 0; r < 3; r++) cur[6 + r][0] = mfma(a, sv.row(r, b), cur[6 + r][0]);
+      }
       if (v == 3 || j == PER_D - 1) a4 = a4n;
     }
     add_stress<1>(star + d * 81, pv, cur, d == 0);
@@ -839,15 +846,18 @@ void k_local(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __re
         else if (d + 1 < 3)
           load_b(0, bn);
         __builtin_amdgcn_sched_barrier(0);
+        constexpr masks::Tab NZ = masks::Nz<O, true>::tab;  // zero A blocks are skipped
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-          for (int t = 0; t < NT; t++) pv[i][t] = mfma(a4[t][ks % 4], b[6 + i], pv[i][t]);
+          for (int t = 0; t < NT; t++)
+            if ((NZ.vol[d][t] >> ks) & 1) pv[i][t] = mfma(a4[t][ks % 4], b[6 + i], pv[i][t]);
 #pragma unroll
         for (int r = 0; r < 3; r++) {
           const float z = sv.row(r, b);
 #pragma unroll
-          for (int t = 0; t < NT; t++) acc[6 + r][t] = mfma(a4[t][ks % 4], z, acc[6 + r][t]);
+          for (int t = 0; t < NT; t++)
+            if ((NZ.vol[d][t] >> ks) & 1) acc[6 + r][t] = mfma(a4[t][ks % 4], z, acc[6 + r][t]);
         }
         if (ks % 4 == 3 || ks == KV - 1) {
 #pragma unroll
@@ -1187,6 +1197,7 @@ bool mfma_supported(int order, int n_runs, int precision) {
 template <>
 int launch_local_mfma<float>(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs,
                              float* t_int, hipStream_t st) {
+  if (s.n_runs != 16) return fail(EDGE_ERR_UNSUPPORTED, "MFMA-tiled kernels: 16 runs");
   if (n <= 0) return EDGE_OK;
   switch (s.order) {
     case 2: launch_local_o<2>(s, first, n, dt, dofs, t_int, st); break;
@@ -1202,6 +1213,7 @@ int launch_local_mfma<float>(const SolverDev<float>& s, int64_t first, int64_t n
 template <>
 int launch_neigh_mfma<float>(const SolverDev<float>& s, int64_t first, int64_t n, const float* t_int, float* dofs,
                              hipStream_t st) {
+  if (s.n_runs != 16) return fail(EDGE_ERR_UNSUPPORTED, "MFMA-tiled kernels: 16 runs");
   if (n <= 0) return EDGE_OK;
   switch (s.order) {
     case 2: launch_neigh_o<2>(s, first, n, t_int, dofs, st); break;
@@ -1216,6 +1228,7 @@ int launch_neigh_mfma<float>(const SolverDev<float>& s, int64_t first, int64_t n
 
 int launch_fused_mfma(const SolverDev<float>& s, float dt, const float* proj_in, float* dofs, float* t_int,
                       hipStream_t st) {
+  if (s.n_runs != 16) return fail(EDGE_ERR_UNSUPPORTED, "fused step: 16 runs");
   if (!s.sparse_star && !s.fs_rank3) return fail(EDGE_ERR_UNSUPPORTED, "fused step: rank-3 flux solvers");
   if (!s.fs_rank3 || !s.proj || !proj_in) return fail(EDGE_ERR_UNSUPPORTED, "fused step: face projections");
   if (s.n_el <= 0) return EDGE_OK;

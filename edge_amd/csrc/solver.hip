@@ -57,13 +57,14 @@ struct edge_el_solver {
   const void* proj_tint = nullptr;  // tInt of the local steps since the last neighbour step (null: none valid)
   bool proj_mixed = false;          // those local steps wrote different tInt arrays
   bool proj_fresh = false;          // a local step ran since the last neighbour step
-  // FP64 element-batched kernels (kernel 5): packed operands, face-orientation
-  // maps and their face projections [nEl][4][3][16]
+  // element-batched single-run kernels (kernel 5): packed operands,
+  // face-orientation maps and their face projections [nEl][4][3][16]
+  // (solver precision)
   bool want_ec = false;  // set-up builds their data when the solver qualifies
   bool ec_ok = false;
-  double* ec_pk = nullptr;
-  double* ec_rot = nullptr;
-  double* ec_proj = nullptr;
+  void* ec_pk = nullptr;
+  void* ec_rot = nullptr;
+  void* ec_proj = nullptr;
   // viscoelastic solvers
   int n_rms = 0;
   void* rfs = nullptr;       // [n_rms]
@@ -302,36 +303,48 @@ int finish_mfma(edge_el_solver* s) {
   return EDGE_OK;
 }
 
-// The element-batched FP64 kernels' data, once the flux solvers and their
-// factors exist: they need the elastic star pattern, rank-3 solvers, one
-// projection slot per element face and face-orientation maps.
+// Uploads FP64 host values as the solver's precision T.
+template <typename T>
+int upload_as(const std::vector<double>& h, void** dst) {
+  const std::vector<T> v(h.begin(), h.end());
+  EDGE_HIP_TRY(hipMalloc(dst, sizeof(T) * v.size()));
+  EDGE_HIP_TRY(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return EDGE_OK;
+}
+
+// The element-batched kernels' data, once the flux solvers and their factors
+// exist: they need the elastic star pattern, rank-3 solvers, one projection
+// slot per element face and face-orientation maps.  Solvers that may run
+// them and were not given another kernel take them by default.
 int finish_ec(edge_el_solver* s) {
   if (!s->want_ec) return EDGE_OK;
   edge::DgTables t;
   std::vector<double> rot;
   const bool ok = s->sparse_star && s->fs_rank3 && s->fsf && s->proj_ok && edge::dg_tables(s->cfg.order, &t) &&
-                  edge::ec64_rotations(t, &rot);
+                  edge::ec_rotations(t, &rot);
   if (!ok) {
     if (s->kernel == 5)
-      return edge::fail(EDGE_ERR_UNSUPPORTED, "element-batched FP64 kernels: they need the elastic star pattern and "
+      return edge::fail(EDGE_ERR_UNSUPPORTED, "element-batched kernels: they need the elastic star pattern and "
                                               "rank-3 flux solvers");
     return EDGE_OK;
   }
-  const std::vector<double> pk = edge::ec64_pack_operands(t);
-  EDGE_HIP_TRY(hipMalloc(&s->ec_pk, sizeof(double) * pk.size()));
-  EDGE_HIP_TRY(hipMemcpy(s->ec_pk, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice));
-  EDGE_HIP_TRY(hipMalloc(&s->ec_rot, sizeof(double) * rot.size()));
-  EDGE_HIP_TRY(hipMemcpy(s->ec_rot, rot.data(), sizeof(double) * rot.size(), hipMemcpyHostToDevice));
-  EDGE_HIP_TRY(hipMalloc(&s->ec_proj, sizeof(double) * 4 * 48 * s->n_el));
+  const bool f32 = s->cfg.precision == 32;
+  const std::vector<double> pk = edge::ec_pack_operands(t, f32);
+  int rc = f32 ? upload_as<float>(pk, &s->ec_pk) : upload_as<double>(pk, &s->ec_pk);
+  if (rc == EDGE_OK) rc = f32 ? upload_as<float>(rot, &s->ec_rot) : upload_as<double>(rot, &s->ec_rot);
+  if (rc) return rc;
+  EDGE_HIP_TRY(hipMalloc(&s->ec_proj, s->real_bytes * 4 * 48 * s->n_el));
   s->ec_ok = true;
+  if (s->cfg.kernel == 0) s->kernel = 5;
   return EDGE_OK;
 }
 
-edge::Ec64Dev ec_view(const edge_el_solver* s) {
-  edge::Ec64Dev x;
-  x.pk = s->ec_pk;
-  x.rot = s->ec_rot;
-  x.proj = s->ec_proj;
+template <typename T>
+edge::EcDev<T> ec_view(const edge_el_solver* s) {
+  edge::EcDev<T> x;
+  x.pk = static_cast<const T*>(s->ec_pk);
+  x.rot = static_cast<const T*>(s->ec_rot);
+  x.proj = static_cast<T*>(s->ec_proj);
   return x;
 }
 
@@ -339,12 +352,12 @@ int pick_kernel(edge_el_solver* s) {
   const bool mfma_ok = edge::mfma_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision);
   const bool gen_ok = edge::generic_supported(s->cfg.order, s->cfg.n_runs, (int)s->real_bytes);
   const bool p64_ok = edge::plain64_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8);
-  const bool ec_ok = edge::ec64_supported(s->cfg.order, s->cfg.n_runs, s->cfg.precision / 8);
-  // the element-batched FP64 kernels' data is built whenever they may run
-  // (requested, or as an autotuner candidate beside the FP64 MFMA kernels)
+  const bool ec_ok = edge::ec_supported(s->cfg.order, s->cfg.n_runs);
+  // the element-batched kernels' data is built whenever they may run
+  // (requested, the default, or an autotuner candidate)
   s->want_ec = ec_ok && (s->cfg.kernel == 0 || s->cfg.kernel == 4 || s->cfg.kernel == 5);
   if (s->cfg.kernel == 5) {
-    if (!ec_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "element-batched FP64 kernels: FP64, one run, orders 2-5");
+    if (!ec_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "element-batched kernels: one run, orders 2-5");
     s->kernel = 5;
   } else if (s->cfg.kernel == 2 || s->cfg.kernel == 3) {
     if (!mfma_ok) return edge::fail(EDGE_ERR_UNSUPPORTED, "no MFMA-tiled kernel for this configuration");
@@ -433,11 +446,9 @@ int local_t(edge_el_solver* s, int64_t first, int64_t n, double dt, void* dofs, 
   const auto d = dev_view<T>(s);
   note_local(s, t_int);
   if (s->kernel == 2) return edge::launch_local_mfma<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
-  if constexpr (sizeof(T) == 8) {
+  if (s->kernel == 5) return edge::launch_local_ec<T>(d, ec_view<T>(s), first, n, (T)dt, (T*)dofs, (T*)t_int, st);
+  if constexpr (sizeof(T) == 8)
     if (s->kernel == 4) return edge::launch_local_plain64(d, first, n, dt, (double*)dofs, (double*)t_int, st);
-    if (s->kernel == 5)
-      return edge::launch_local_ec64(d, ec_view(s), first, n, dt, (double*)dofs, (double*)t_int, st);
-  }
   return edge::launch_local_generic<T>(d, first, n, (T)dt, (T*)dofs, (T*)t_int, st);
 }
 
@@ -447,10 +458,9 @@ int neigh_t(edge_el_solver* s, int64_t first, int64_t n, const void* t_int, void
   const auto d = dev_view<T>(s);
   if (int rc = check_neigh(s, t_int)) return rc;
   if (s->kernel == 2) return edge::launch_neigh_mfma<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
-  if constexpr (sizeof(T) == 8) {
+  if (s->kernel == 5) return edge::launch_neigh_ec<T>(d, ec_view<T>(s), first, n, (T*)dofs, st);
+  if constexpr (sizeof(T) == 8)
     if (s->kernel == 4) return edge::launch_neigh_plain64(d, first, n, (const double*)t_int, (double*)dofs, st);
-    if (s->kernel == 5) return edge::launch_neigh_ec64(d, ec_view(s), first, n, (double*)dofs, st);
-  }
   return edge::launch_neigh_generic<T>(d, first, n, (const T*)t_int, (T*)dofs, st);
 }
 
@@ -502,10 +512,11 @@ int neigh_lts_t(edge_el_solver* s, int64_t first, int64_t n, int first_ts, const
   if (rc) return rc;
   if (s->kernel == 2)
     rc = edge::launch_neigh_mfma<T>(d, first, n, (const T*)t0, (T*)dofs, st);
+  else if (s->kernel == 5)
+    rc = edge::launch_neigh_ec<T>(d, ec_view<T>(s), first, n, (T*)dofs, st);
   else if constexpr (sizeof(T) == 8)
-    rc = s->kernel == 4   ? edge::launch_neigh_plain64(d, first, n, (const double*)t0, (double*)dofs, st)
-         : s->kernel == 5 ? edge::launch_neigh_ec64(d, ec_view(s), first, n, (double*)dofs, st)
-                          : edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
+    rc = s->kernel == 4 ? edge::launch_neigh_plain64(d, first, n, (const double*)t0, (double*)dofs, st)
+                        : edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
   else
     rc = edge::launch_neigh_generic<T>(d, first, n, (const T*)t0, (T*)dofs, st);
   if (rc) return rc;
@@ -577,10 +588,13 @@ int one_step(edge_el_solver* s, double dt, void* dofs, void* dofs_a, void* t_int
 // own).  Any fault phase runs between the launches, where it runs between
 // local and neighbour step otherwise.
 bool fused_active(const edge_el_solver* s) {
-  return s->use_fused && s->n_rms == 0 && s->cfg.precision == 32 && proj_active(s);
+  return s->kernel == 2 && s->use_fused && s->n_rms == 0 && s->cfg.precision == 32 && proj_active(s);
 }
 
 int fused_steps(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* t_int, hipStream_t st) {
+  // the fused kernels are the 16-run FP32 MFMA kernels: nothing else may get here
+  if (!fused_active(s) || s->cfg.n_runs != 16)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "fused steps: FP32 MFMA elastic solvers of 16 runs");
   if (!s->proj_alt) EDGE_HIP_TRY(hipMalloc(&s->proj_alt, sizeof(float) * 4 * 768 * s->n_el));
   float* P[2] = {static_cast<float*>(s->proj), static_cast<float*>(s->proj_alt)};
   auto d = dev_view<float>(s);
@@ -1020,8 +1034,9 @@ std::vector<TuneCand> tune_candidates(edge_el_solver* s) {
       c.push_back({2, false, x == 1});
     }
   } else if (base == 4 || base == 5) {
+    const bool p64 = edge::plain64_supported(s->cfg.order, s->cfg.n_runs, prec_bytes);
     for (int x = 1; x >= 0; x--) {  // XCD-aware schedule, then the linear one
-      c.push_back({4, false, x == 1});
+      if (p64) c.push_back({4, false, x == 1});
       if (s->ec_ok) c.push_back({5, true, x == 1});  // element-batched: projections always exchanged
     }
   }

@@ -372,7 +372,7 @@ def _emit_array(name, data, per_line=4):
     lines = []
     for i in range(0, len(vals), per_line):
         lines.append("  " + ", ".join(vals[i:i + per_line]) + ",")
-    return f"static const double {name}[{len(vals)}] = {{\n" + "\n".join(lines) + "\n};\n"
+    return f"static constexpr double {name}[{len(vals)}] = {{\n" + "\n".join(lines) + "\n};\n"
 
 
 def _emit_basis(name, polys, nvar):

@@ -80,7 +80,7 @@ def test_autotune_visco(gpu):
 
 
 def _cases():
-    return [("elastic", 32, 16), ("elastic", 64, 1), ("visco", 32, 8), ("visco", 64, 1)]
+    return [("elastic", 32, 16), ("elastic", 64, 1), ("elastic", 32, 1), ("visco", 32, 8), ("visco", 64, 1)]
 
 
 @pytest.mark.parametrize("kind,prec,runs", _cases())
@@ -111,7 +111,7 @@ def test_every_candidate_matches_oracle(gpu, kind, prec, runs):
     mo.step(d, t, dt, 3, dofs_a=a) if a is not None else mo.step(d, t, dt, 3)
     cands = s.autotune(dt, n_reps=1)
     assert len(cands) >= 2
-    if kind == "elastic" and prec == 32:
+    if kind == "elastic" and prec == 32 and runs == 16:
         assert any(f for *_, f in cands)
     tol = 1e-12 if prec == 64 else 1e-5
     for k, p, x, _, f in cands:

@@ -83,7 +83,7 @@ def test_fp64_local_neigh_match_oracle(gpu, order, kernel):
 def test_fp64_steps_match_oracle(gpu, order, bc):
     m, rho, lam, mu = _problem(bc=bc)
     s = ElasticSolver(m, order, 1, 64, rho, lam, mu)
-    assert s.kernel_info()[0] == KERNEL_MFMA64  # the FP64 single-run default
+    assert s.kernel_info()[0] == KERNEL_EC64  # the single-run default
     mo = orc.MeshOracle(m, order, rho=rho, lam=lam, mu=mu)
     dofs0 = _ic(m, order, 1)
     dt = S.dt_cfl(m, rho, lam, mu, order)
@@ -95,31 +95,34 @@ def test_fp64_steps_match_oracle(gpu, order, bc):
     assert rel_l2(gd.cpu().numpy(), d) <= FP64_TOL
 
 
+@pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("order", [2, 3, 4, 5])
 @pytest.mark.parametrize("bc", [(FA_PERIODIC,) * 3, (FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC)])
-def test_ec64_matches_oracle(gpu, order, bc):
-    """The FP64 element-batched kernels (16 elements per wave on FP64 MFMA):
-    162 elements, so the last wave is partly filled; free-surface, outflow
-    and periodic faces (every face orientation); tInt and DOFs of the local
-    step, the neighbour step from the face projections, then steps."""
+def test_ec64_matches_oracle(gpu, order, bc, prec):
+    """The element-batched single-run kernels (16 elements per wave on FP64 /
+    FP32 MFMA, the single-run default): 162 elements, so the last wave is
+    partly filled; free-surface, outflow and periodic faces (every face
+    orientation); tInt and DOFs of the local step, the neighbour step from
+    the face projections, then steps.  FP64 1e-12, FP32 1e-5."""
     m, rho, lam, mu = _problem(n=3, bc=bc, seed=30 + order)
-    s = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=KERNEL_EC64)
+    s = ElasticSolver(m, order, 1, prec, rho, lam, mu)
     assert s.kernel_info() == (KERNEL_EC64, PATH_SPARSE_STAR | PATH_RANK3_FS | PATH_PROJECTIONS)
     mo = orc.MeshOracle(m, order, rho=rho, lam=lam, mu=mu)
     dofs0 = _ic(m, order, 1, seed=order)
     dt = S.dt_cfl(m, rho, lam, mu, order)
+    tol = FP64_TOL if prec == 64 else FP32_TOL
     d, t = dofs0.copy(), np.zeros_like(dofs0)
     mo.local(d, t, dt)
-    gd, gt = _to_dev(dofs0, 64), _to_dev(np.zeros_like(dofs0), 64)
+    gd, gt = _to_dev(dofs0, prec), _to_dev(np.zeros_like(dofs0), prec)
     s.local(gd, gt, dt)
-    assert rel_l2(gt.cpu().numpy(), t) <= FP64_TOL
-    assert rel_l2(gd.cpu().numpy(), d) <= FP64_TOL
+    assert rel_l2(gt.cpu().numpy(), t) <= tol
+    assert rel_l2(gd.cpu().numpy(), d) <= tol
     mo.neigh(d, t)
     s.neigh(gd, gt)
-    assert rel_l2(gd.cpu().numpy(), d) <= FP64_TOL
+    assert rel_l2(gd.cpu().numpy(), d) <= tol
     mo.step(d, t, dt, 5)
     s.step(gd, gt, dt, 5)
-    assert rel_l2(gd.cpu().numpy(), d) <= FP64_TOL
+    assert rel_l2(gd.cpu().numpy(), d) <= tol
 
 
 def test_random_star_and_flux_solvers_match_oracle(gpu):

@@ -29,7 +29,7 @@ def _ranges(cuts, n_el):
 
 
 @pytest.mark.parametrize("prec,runs,kernel", [(64, 1, KERNEL_MFMA64), (64, 1, KERNEL_GENERIC), (32, 16, KERNEL_MFMA),
-                                              (32, 16, KERNEL_MFMA_TILES), (64, 1, KERNEL_EC64)])
+                                              (32, 16, KERNEL_MFMA_TILES), (64, 1, KERNEL_EC64), (32, 1, KERNEL_EC64)])
 def test_elastic_ranges_equal_full(gpu, prec, runs, kernel):
     import torch
     m, rho, lam, mu = _mesh()
