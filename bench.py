@@ -600,7 +600,7 @@ def main():
         rows(args)
         return
     import torch
-    from edge_amd.solver import KERNEL_EC64, KERNEL_MFMA, PATH_FUSED, PATH_PROJECTIONS, ElasticSolver
+    from edge_amd.solver import KERNEL_EC64, KERNEL_MFMA, KERNEL_MFMA64, PATH_FUSED, PATH_PROJECTIONS, ElasticSolver
 
     order, runs = args.order, 16
     print(f"[bench] building the {6 * args.hex ** 3}-tet mesh (order {order})", file=sys.stderr, flush=True)
@@ -645,6 +645,14 @@ def main():
         w64, k64 = time_solver(s64, d64, t64, dt, max(1, args.steps // 2), args.warmup)
         res["fp64"] = dict(wall=w64, k=k64, steps=max(1, args.steps // 2))
         res["fp64"]["kernel"] = s64.kernel_info()[0]
+        if res["fp64"]["kernel"] == KERNEL_EC64:
+            # beside it the per-element FP64 MFMA kernels (one element per wave)
+            s64.set_variant(KERNEL_MFMA64, False, True)
+            d64.copy_(torch.from_numpy(q0[..., None].copy()))
+            t64.zero_()
+            print("[bench] fp64 per-element timing", file=sys.stderr, flush=True)
+            wpe, kpe = time_solver(s64, d64, t64, dt, max(1, args.steps // 2), args.warmup)
+            res["fp64_pe"] = dict(wall=wpe, k=kpe, steps=max(1, args.steps // 2))
         del d64, t64
         s64.close()
         # ---- FP32 single run (config 2's FP32 row), beside the FP64 one
@@ -721,6 +729,16 @@ def main():
             if u64 is not None:
                 out["fp64_plain"]["mfma_busy_local"] = u64
         out["speedup_fp32_fused16_vs_fp64_plain"] = value / v64
+    if "fp64_pe" in res:
+        kpe = res["fp64_pe"]["k"]
+        vpe = dofs_per_step * res["fp64_pe"]["steps"] / res["fp64_pe"]["wall"]
+        out["fp64_per_element"] = {"value": vpe, "unit": "DOF-updates/s", "ms_per_step": 1e3 * res["fp64_pe"]["wall"]
+                                   / res["fp64_pe"]["steps"], "local_ms": 1e3 * kpe["local"],
+                                   "neigh_ms": 1e3 * kpe["neigh"], "kernel": f"k_local_p64<{order}, true>"}
+        if fl:
+            out["fp64_per_element"]["roofline_frac"] = ((fl["local"] + fl["neigh"]) * m.n_el /
+                                                        (kpe["local"] + kpe["neigh"]) / PEAK["fp64"])
+        out["speedup_fp32_fused16_vs_fp64_per_element"] = value / vpe
     if "fp32_r1" in res:
         k1 = res["fp32_r1"]["k"]
         out["fp32_single_run"] = {"value": dofs_per_step * res["fp32_r1"]["steps"] / res["fp32_r1"]["wall"],
