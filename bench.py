@@ -375,6 +375,7 @@ def ensemble_row(args, timed, flops_el_run):
         s = ViscoSolver(u, args.order, runs, prec, rho, lam, mu, qp, qp / 2, 3, 1.0, 10.0)
         t_setup = time.time() - t0
         dt = S.dt_cfl(u, rho, s.setup_a()["lam_e"], s.setup_a()["mu_e"], args.order)
+        tune = [] if args.no_autotune else s.autotune(dt, 2)
         d = torch.randn(s.dof_shape(), dtype=s.dtype, device="cuda")
         a = 0.01 * torch.randn(s.dof_a_shape(), dtype=s.dtype, device="cuda")
         t = torch.zeros_like(d)
@@ -383,7 +384,10 @@ def ensemble_row(args, timed, flops_el_run):
         n = s.n_modes
         print(json.dumps({"row": "batched viscoelastic ensemble", "meshes": len(sizes), "hexes_per_axis": sizes,
                           "elements": u.n_el, "order": args.order, "runs": runs, "precision": prec, "n_rms": 3,
-                          "kernel": s.kernel_info()[0], "graph": True, "setup_s": round(t_setup, 1),
+                          "kernel": s.kernel_info()[0], "hi_occ": bool(s.kernel_info()[1] & 32),
+                          "autotune": [{"kernel": k, "projections": p, "xcd_schedule": x, "occupancy": o,
+                                        "ms_per_step": c} for k, p, x, c, _, o in tune],
+                          "graph": True, "setup_s": round(t_setup, 1),
                           "ms_per_step": ms * 1e3, "dof_updates_per_s": u.n_el * 27 * n * runs / ms,
                           "roofline_frac": flops_el_run * runs * u.n_el / ms / peak}), flush=True)
         s.close()
@@ -437,7 +441,9 @@ def fault_row(args, timed):
     d = torch.zeros(s.dof_shape(), dtype=torch.float32, device="cuda")
     t = torch.zeros_like(d)
     n_steps = 100
-    ms = timed(lambda st: s.step(d, t, dt, 1, graph=True, stream=st), n_steps)
+    # one multi-step call: fused neighbour + local launches with the fault
+    # phase between them, captured as one graph
+    ms = timed(lambda st: s.step(d, t, dt, n_steps, graph=True, stream=st), 1) / n_steps
     st, slip = f.get()
     fault_ms = timed(lambda st: f.apply(d, t, dt, stream=st), 5)
     fl = FLOPS[order]
@@ -445,7 +451,8 @@ def fault_row(args, timed):
     step_flop = (fl["local"] + fl["neigh"]) * runs * m.n_el
     print(json.dumps({"row": "config 4: elastic + dynamic rupture (coupled step)", "elements": m.n_el,
                       "fault_faces": f.n_faces, "sub_faces": f.n_sf, "order": order, "runs": runs, "precision": 32,
-                      "graph": True, "steps": n_steps + 1, "t_end": (n_steps + 1) * dt, "ms_per_step": ms * 1e3,
+                      "graph": True, "fused": bool(s.kernel_info()[1] & 16), "steps": 2 * n_steps,
+                      "t_end": 2 * n_steps * dt, "ms_per_step": ms * 1e3,
                       "dof_updates_per_s": m.n_el * 9 * n * runs / ms, "fault_phase_ms": fault_ms * 1e3,
                       "step_tflops": step_flop / ms / 1e12, "step_frac": step_flop / ms / PEAK["fp32_mfma"],
                       "slipping_sub_faces": float(slip.mean()), "max_slip": float(st[..., 11].max()),
@@ -492,6 +499,7 @@ def rows(args):
     fl, fn_ = visco_flops(args.order, 3)
     for runs, prec in (((8, 32), (1, 64)) if "visco" in want else ()):
         s = ViscoSolver(m, args.order, runs, prec, rho, lam, mu, qp, qp / 2, n_rms=3, freq_cen=1.0, freq_rat=10.0)
+        tune = [] if args.no_autotune else s.autotune(dt, 2)
         d = torch.from_numpy(q0)[..., None].repeat(1, 1, 1, runs).to(s.dtype).cuda().contiguous()
         t, a = torch.zeros_like(d), s.alloc_a()
         loc = timed(lambda st: s.local(d, t, dt, dofs_a=a, stream=st), 2)
@@ -500,6 +508,9 @@ def rows(args):
         kern = {1: "generic (LDS tiles, VALU)", 2: "MFMA (paired products, elementwise anelastic terms)",
                 4: "FP64 MFMA (anelastic rows in the quantity tiles)"}
         print(json.dumps({"row": "a12 viscoelastic update", "kernel": kern.get(s.kernel_info()[0], "?"),
+                          "hi_occ": bool(s.kernel_info()[1] & 32),
+                          "autotune": [{"kernel": k, "projections": p, "xcd_schedule": x, "occupancy": o,
+                                        "ms_per_step": c} for k, p, x, c, _, o in tune],
                           "order": args.order,
                           "runs": runs, "precision": prec, "n_rms": 3, "elements": m.n_el,
                           "local_ms": loc * 1e3, "neigh_ms": nei * 1e3,
@@ -616,7 +627,7 @@ def main():
         s32.set_variant(KERNEL_MFMA, *FP32_VARIANTS[args.fp32_variant])
     elif not args.no_autotune:  # untimed set-up: the fastest kernel variant on this mesh
         tune["fp32"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "fused": f, "ms_per_step": ms}
-                        for k, p, x, ms, f in s32.autotune(dt, 3)]
+                        for k, p, x, ms, f, _ in s32.autotune(dt, 3)]
     flags = s32.kernel_info()[1]
     proj, fused = bool(flags & PATH_PROJECTIONS), bool(flags & PATH_FUSED)
     variant = "fused" if fused else ("proj" if proj else "tiles")
@@ -638,7 +649,7 @@ def main():
         s64 = ElasticSolver(m, order, 1, 64, rho, lam, mu, kernel=args.kernel)
         if not args.no_autotune:
             tune["fp64"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "ms_per_step": ms}
-                            for k, p, x, ms, _ in s64.autotune(dt, 2)]
+                            for k, p, x, ms, _, _ in s64.autotune(dt, 2)]
         d64 = torch.from_numpy(q0[..., None].copy()).cuda().contiguous()
         t64 = torch.zeros_like(d64)
         print("[bench] fp64 plain timing", file=sys.stderr, flush=True)
@@ -659,7 +670,7 @@ def main():
         s1 = ElasticSolver(m, order, 1, 32, rho, lam, mu)
         if not args.no_autotune:
             tune["fp32_single_run"] = [{"kernel": k, "projections": p, "xcd_schedule": x, "ms_per_step": ms}
-                                       for k, p, x, ms, _ in s1.autotune(dt, 2)]
+                                       for k, p, x, ms, _, _ in s1.autotune(dt, 2)]
         d1 = torch.from_numpy(q0[..., None].copy()).to(torch.float32).cuda().contiguous()
         t1 = torch.zeros_like(d1)
         print("[bench] fp32 single-run timing", file=sys.stderr, flush=True)

@@ -53,6 +53,9 @@ struct SolverDev {
   // MFMA kernels' workgroup -> element schedule: contiguous element ranges
   // per XCD (true) or workgroup i -> element i (false)
   bool xcd_map = true;
+  // kernels with an occupancy variant: the one that keeps more waves per
+  // SIMD resident (state moved from registers to LDS)
+  bool hi_occ = false;
   // rank-3 factors of the flux solvers [2][nEl][4][54] (U [9][3], then V^T
   // [3][9]); null unless the MFMA-tiled kernels run
   const T* fsf = nullptr;

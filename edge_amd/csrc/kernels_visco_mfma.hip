@@ -317,15 +317,19 @@ __device__ __forceinline__ void add_source(const cfloat* __restrict__ e, const f
   for (int q = 3; q < 6; q++) y[q] += e[q * 6 + q] * x[q];
 }
 
-template <int O, int NRM>
-__global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt,
-                                                            float* __restrict__ dofs, float* __restrict__ dofs_a,
-                                                            float* __restrict__ t_int) {
+// TL: tIntA in LDS (each thread's own positions, no barriers) instead of
+// registers, so that five waves per SIMD fit the register file (the
+// occupancy variant; LDS then allows three workgroups per CU at O5).
+template <int O, int NRM, bool TL>
+__global__ __launch_bounds__(kLocalThreads) __attribute__((amdgpu_waves_per_eu(TL ? 5 : 1)))
+void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __restrict__ dofs,
+                float* __restrict__ dofs_a, float* __restrict__ t_int) {
   using D = VDims<O>;
   constexpr int N = D::N, NV = D::NV, NP = D::NP, SZ = D::SZ, SZA = D::SZA;
   constexpr int PJ = (NP + kLocalThreads - 1) / kLocalThreads;  // positions per thread
   __shared__ __align__(16) float l_x[SZ];      // Q, then der_{de-1}, then tInt
   __shared__ __align__(16) float l_p[18 * NP];  // raw products [3][6][N][R]; lifted tiles [4][3][N][R]
+  __shared__ __align__(16) float l_ta[TL ? NRM * 6 * NP : 1];  // tIntA [rm][6][N][R] (TL)
   const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
@@ -346,8 +350,14 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
     float4* lq = reinterpret_cast<float4*>(l_x);
     for (int i = tid; i < SZ / 4; i += kLocalThreads) lq[i] = gq[i];
   }
-  // anelastic state per position: derA (CK), tIntA
-  float da[NRM][6][PJ], ta[NRM][6][PJ], ti[9][PJ];
+  // anelastic state per position: derA (CK), tIntA (registers, or l_ta)
+  float da[NRM][6][PJ], ta_r[TL ? 1 : NRM][6][PJ], ti[9][PJ];
+  auto ta = [&](int rm, int q, int j) -> float& {
+    if constexpr (TL)
+      return l_ta[(rm * 6 + q) * NP + tid + j * kLocalThreads];
+    else
+      return ta_r[rm][q][j];
+  };
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
     const int p = tid + j * kLocalThreads;
@@ -358,7 +368,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
         for (int q = 0; q < 6; q++) {
           const float v = qa0[(rm * 6 + q) * NP + p];
           da[rm][q][j] = v;
-          ta[rm][q][j] = dt * v;
+          ta(rm, q, j) = dt * v;
         }
     }
   }
@@ -410,7 +420,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
           for (int q = 0; q < 6; q++) {
             const float v = w * (a[q] + x[q]);
             da[rm][q][j] = v;
-            ta[rm][q][j] += sca * v;
+            ta(rm, q, j) += sca * v;
           }
         }
 #pragma unroll
@@ -477,7 +487,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
       for (int rm = 0; rm < NRM; rm++) {
         float x[6];
 #pragma unroll
-        for (int q = 0; q < 6; q++) x[q] = ta[rm][q][j];
+        for (int q = 0; q < 6; q++) x[q] = ta(rm, q, j);
         add_source(src + rm * 36, x, y);
       }
     } else {
@@ -546,7 +556,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local_vm(SolverDev<float> s, 
       for (int rm = 0; rm < NRM; rm++) {
         const float w = rfs[rm];
 #pragma unroll
-        for (int q = 0; q < 6; q++) qa0[(rm * 6 + q) * NP + p] = qain[rm][q][j] + w * (ua[q] - ta[rm][q][j]);
+        for (int q = 0; q < 6; q++) qa0[(rm * 6 + q) * NP + p] = qain[rm][q][j] + w * (ua[q] - ta(rm, q, j));
       }
     }
   }
@@ -649,9 +659,12 @@ unsigned grid_of(int64_t n) { return (unsigned)(8 * ((n + 7) / 8)); }
 template <int O, int NRM>
 void launch_o(const SolverDev<float>& s, int64_t first, int64_t n, float dt, float* dofs, float* dofs_a,
               float* t_int, const float* tin, bool local, hipStream_t st) {
-  if (local)
-    hipLaunchKernelGGL((k_local_vm<O, NRM>), dim3(grid_of(n)), dim3(kLocalThreads), 0, st, s, first, n, dt, dofs,
-                       dofs_a, t_int);
+  if (local && s.hi_occ)
+    hipLaunchKernelGGL((k_local_vm<O, NRM, true>), dim3(grid_of(n)), dim3(kLocalThreads), 0, st, s, first, n, dt,
+                       dofs, dofs_a, t_int);
+  else if (local)
+    hipLaunchKernelGGL((k_local_vm<O, NRM, false>), dim3(grid_of(n)), dim3(kLocalThreads), 0, st, s, first, n, dt,
+                       dofs, dofs_a, t_int);
   else if (s.proj)
     hipLaunchKernelGGL((k_neigh_vm<O, NRM, true>), dim3(grid_of(n)), dim3(kNeighThreads), 0, st, s, first, n, tin,
                        dofs, dofs_a);

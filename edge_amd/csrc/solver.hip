@@ -42,6 +42,7 @@ struct edge_el_solver {
   bool want_proj = false;    // kernel selector 2: exchange face projections when possible
   bool use_proj = true;      // the projections are exchanged when they exist (the autotuner may turn this off)
   bool xcd_map = true;       // MFMA kernels: contiguous element ranges per XCD (the autotuner may turn this off)
+  bool hi_occ = false;       // kernels with an occupancy variant: the higher-occupancy one (an autotuner choice)
   bool proj_tables = false;  // build the projection slot tables whatever the kernel (MFMA viscoelastic)
   void* proj = nullptr;      // [nEl][4][768]
   void* proj_alt = nullptr;  // the second projection buffer of the fused step (allocated on first use)
@@ -117,6 +118,7 @@ edge::SolverDev<T> dev_view(const edge_el_solver* s) {
   d.fs = static_cast<const T*>(s->fs);
   d.sparse_star = s->sparse_star;
   d.xcd_map = s->xcd_map;
+  d.hi_occ = s->hi_occ;
   d.fsf = static_cast<const T*>(s->fsf);
   d.fs_rank3 = s->fs_rank3 && s->fsf;
   d.proj = d.fs_rank3 && s->use_proj ? static_cast<T*>(s->proj) : nullptr;
@@ -1005,7 +1007,7 @@ extern "C" int edge_el_kernel_info(const edge_el_solver* s, int32_t* kernel, int
   if (flags) {
     const bool r3 = (s->kernel == 2 || s->kernel == 5) && s->fs_rank3 && s->fsf;
     *flags = (s->sparse_star ? 1 : 0) | (r3 ? 2 : 0) | (proj_active(s) ? 4 : 0) |
-             (s->n_rms > 0 ? 8 : 0) | (fused_active(s) ? 16 : 0);
+             (s->n_rms > 0 ? 8 : 0) | (fused_active(s) ? 16 : 0) | (s->hi_occ ? 32 : 0);
   }
   return EDGE_OK;
 }
@@ -1016,6 +1018,7 @@ struct TuneCand {
   bool proj;
   bool xcd = true;
   bool fused = false;  // multi-step calls fuse neighbour step k - 1 with local step k
+  bool occ = false;    // the higher-occupancy variant of the kernels (hi_occ)
 };
 
 // The kernel variants a solver can switch between with the data it holds.
@@ -1028,11 +1031,14 @@ std::vector<TuneCand> tune_candidates(edge_el_solver* s) {
                                 : edge::generic_supported(s->cfg.order, s->cfg.n_runs, prec_bytes);
   if (base == 2) {
     const bool fusable = s->proj && s->n_rms == 0 && s->cfg.precision == 32;
-    for (int x = 1; x >= 0; x--) {  // XCD-aware element schedule, then the linear one
-      if (fusable) c.push_back({2, true, x == 1, true});
-      if (s->proj) c.push_back({2, true, x == 1});
-      c.push_back({2, false, x == 1});
-    }
+    // the viscoelastic local kernel also comes with tIntA in LDS, five waves per SIMD
+    const int n_occ = s->n_rms > 0 ? 2 : 1;
+    for (int x = 1; x >= 0; x--)  // XCD-aware element schedule, then the linear one
+      for (int o = 0; o < n_occ; o++) {
+        if (fusable) c.push_back({2, true, x == 1, true});
+        if (s->proj) c.push_back({2, true, x == 1, false, o == 1});
+        c.push_back({2, false, x == 1, false, o == 1});
+      }
   } else if (base == 4 || base == 5) {
     const bool p64 = edge::plain64_supported(s->cfg.order, s->cfg.n_runs, prec_bytes);
     for (int x = 1; x >= 0; x--) {  // XCD-aware schedule, then the linear one
@@ -1056,7 +1062,7 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
   void *dofs = nullptr, *t_int = nullptr, *dofs_a = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const int old_kernel = s->kernel;
-  const bool old_proj = s->use_proj, old_xcd = s->xcd_map, old_fused = s->use_fused;
+  const bool old_proj = s->use_proj, old_xcd = s->xcd_map, old_fused = s->use_fused, old_occ = s->hi_occ;
   std::vector<double> t(cand.size(), 0.0);
   int rc = EDGE_OK;
   auto hip = [&](hipError_t e, const char* what) {
@@ -1076,6 +1082,7 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
       s->use_proj = cand[i].proj;
       s->xcd_map = cand[i].xcd;
       s->use_fused = cand[i].fused;
+      s->hi_occ = cand[i].occ;
       rc = one_step(s, dt, dofs, dofs_a, t_int, st);  // warm-up
       if (rc == EDGE_OK) hip(hipEventRecord(e0, st), "event");
       if (cand[i].fused)  // one fused sequence of n_reps steps (n_reps + 1 launches)
@@ -1102,6 +1109,7 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
     s->use_proj = old_proj;
     s->xcd_map = old_xcd;
     s->use_fused = old_fused;
+    s->hi_occ = old_occ;
     return rc;
   }
   size_t best = 0;
@@ -1111,12 +1119,14 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
   s->use_proj = cand[best].proj;
   s->xcd_map = cand[best].xcd;
   s->use_fused = cand[best].fused;
+  s->hi_occ = cand[best].occ;
   if (n_cands) {
     const int32_t cap = *n_cands;
     *n_cands = (int32_t)cand.size();
     for (int32_t i = 0; i < (int32_t)cand.size() && i < cap; i++) {
       if (kernels) kernels[i] = cand[i].kernel;
-      if (flags) flags[i] = (cand[i].proj ? 1 : 0) | (cand[i].xcd ? 2 : 0) | (cand[i].fused ? 4 : 0);
+      if (flags)
+        flags[i] = (cand[i].proj ? 1 : 0) | (cand[i].xcd ? 2 : 0) | (cand[i].fused ? 4 : 0) | (cand[i].occ ? 8 : 0);
       if (ms) ms[i] = t[i];
     }
   }
@@ -1137,13 +1147,15 @@ extern "C" int edge_el_set_variant(edge_el_solver* s, int32_t kernel, int32_t fl
   for (const TuneCand& c : tune_candidates(s)) {
     if (c.kernel != kernel) continue;
     if (c.kernel != 1 &&
-        (c.proj != ((flags & 1) != 0) || c.xcd != ((flags & 2) != 0) || c.fused != ((flags & 4) != 0)))
+        (c.proj != ((flags & 1) != 0) || c.xcd != ((flags & 2) != 0) || c.fused != ((flags & 4) != 0) ||
+         c.occ != ((flags & 8) != 0)))
       continue;
     drop_graph(s);
     s->kernel = c.kernel;
     s->use_proj = c.proj;
     s->xcd_map = c.xcd;
     s->use_fused = c.fused;
+    s->hi_occ = c.occ;
     s->proj_tint = nullptr;
     s->proj_fresh = false;
     return EDGE_OK;

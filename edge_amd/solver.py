@@ -13,7 +13,7 @@ from . import _lib
 from .setups import n_modes
 
 KERNEL_AUTO, KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA_TILES, KERNEL_MFMA64, KERNEL_EC64 = 0, 1, 2, 3, 4, 5
-PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS, PATH_VISCO, PATH_FUSED = 1, 2, 4, 8, 16
+PATH_SPARSE_STAR, PATH_RANK3_FS, PATH_PROJECTIONS, PATH_VISCO, PATH_FUSED, PATH_HI_OCC = 1, 2, 4, 8, 16, 32
 
 
 def _stream_ptr(stream):
@@ -53,7 +53,7 @@ class ElasticSolver:
 
     # ------------------------------------------------------------------
     def kernel_info(self):
-        """(kernel, flags): 1 generic / 2 MFMA-tiled / 4 FP64 MFMA; flags PATH_* bits."""
+        """(kernel, flags): 1 generic / 2 MFMA-tiled / 4 FP64 MFMA / 5 element-batched; flags PATH_* bits."""
         k, f = C.c_int32(), C.c_int32()
         _lib.check(_lib.lib().edge_el_kernel_info(self._h, C.byref(k), C.byref(f)))
         return k.value, f.value
@@ -88,19 +88,20 @@ class ElasticSolver:
     def autotune(self, dt, n_reps=3, stream=None):
         """Times the kernel variants and schedules this solver can switch to on
         its mesh and keeps the fastest; returns [(kernel, projections,
-        xcd_schedule, ms per step, fused)] (fused: multi-step calls run the
-        neighbour step of step k - 1 and the local step of step k as one
-        launch)."""
-        cap = 16
+        xcd_schedule, ms per step, fused, occupancy)] (fused: multi-step calls
+        run the neighbour step of step k - 1 and the local step of step k as
+        one launch; occupancy: the higher-occupancy variant of the kernels)."""
+        cap = 32
         n = C.c_int32(cap)
         k, f, ms = (C.c_int32 * cap)(), (C.c_int32 * cap)(), (C.c_double * cap)()
         _lib.check(_lib.lib().edge_el_autotune(self._h, float(dt), int(n_reps), _stream_ptr(stream), C.byref(n), k,
                                                f, ms))
-        return [(k[i], bool(f[i] & 1), bool(f[i] & 2), ms[i], bool(f[i] & 4)) for i in range(min(n.value, cap))]
+        return [(k[i], bool(f[i] & 1), bool(f[i] & 2), ms[i], bool(f[i] & 4), bool(f[i] & 8))
+                for i in range(min(n.value, cap))]
 
-    def set_variant(self, kernel, projections=False, xcd_schedule=True, fused=False):
+    def set_variant(self, kernel, projections=False, xcd_schedule=True, fused=False, occupancy=False):
         """Switches to one of the autotuner's candidates without timing it."""
-        flags = (1 if projections else 0) | (2 if xcd_schedule else 0) | (4 if fused else 0)
+        flags = (1 if projections else 0) | (2 if xcd_schedule else 0) | (4 if fused else 0) | (8 if occupancy else 0)
         _lib.check(_lib.lib().edge_el_set_variant(self._h, int(kernel), flags))
 
     def close(self):

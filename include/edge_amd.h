@@ -491,26 +491,32 @@ int edge_el_local_lts_a(edge_el_solver *s, int64_t first, int64_t n_els, double 
 int edge_el_neigh_lts_a(edge_el_solver *s, int64_t first, int64_t n_els, int32_t first_ts, const void *t_int0,
                         const void *t_int1, const void *t_int2, void *dofs, void *dofs_a, void *stream);
 
-/* Which kernels a solver runs: *kernel = 1 (generic), 2 (MFMA-tiled) or 4
- * (FP64 single run on FP64 MFMA);
+/* Which kernels a solver runs: *kernel = 1 (generic), 2 (MFMA-tiled, runs
+ * on the MFMA N dimension), 4 (FP64 single run on FP64 MFMA, quantities on
+ * the M dimension) or 5 (single run, 16 elements on the N dimension);
  * *flags bit 0: sparse (elastic-pattern) star matrices, bit 1: rank-3
  * factored flux solvers, bit 2: the neighbour step reads face projections,
  * bit 3: viscoelastic (anelastic DOFs), bit 4: multi-step calls fuse the
- * neighbour step of step k - 1 with the local step of step k (edge_el_step).
+ * neighbour step of step k - 1 with the local step of step k (edge_el_step),
+ * bit 5: the higher-occupancy variant of the kernels (FP32 viscoelastic MFMA
+ * local kernel: tIntA in LDS, five waves per SIMD).
  * Diagnostic with no counterpart in the reference (for tests and tools). */
 int edge_el_kernel_info(const edge_el_solver *s, int32_t *kernel, int32_t *flags);
 
 /* Autotuner: times every kernel variant and schedule the solver can switch
- * to (the FP32 MFMA kernels with and without the face-projection exchange,
- * for elastic solvers with it also the fused multi-step form, and the FP64
- * MFMA kernels, each with the XCD-aware element schedule or the linear one;
- * the generic kernels; all parity-tested against the oracle) over
+ * to (tile shapes: runs on the MFMA N dimension, quantities on the M
+ * dimension or 16 elements on the N dimension; the FP32 MFMA kernels with
+ * and without the face-projection exchange, for elastic solvers with it
+ * also the fused multi-step form, for viscoelastic ones each in two
+ * occupancy variants; each with the XCD-aware element schedule or the
+ * linear one; the generic kernels; all parity-tested against the oracle) over
  * n_reps full steps of time step dt on scratch DOFs on its own mesh (a fused
  * candidate: one fused sequence of n_reps steps), and
  * keeps the fastest for the solver's later calls.  In: *n_cands = capacity
  * of the arrays (may be NULL); out: *n_cands = number of candidates,
  * kernels[i] (as edge_el_kernel_info), flags[i] (bit 0: projections
- * exchanged, bit 1: XCD-aware schedule, bit 2: fused multi-step form) and
+ * exchanged, bit 1: XCD-aware schedule, bit 2: fused multi-step form, bit 3:
+ * the higher-occupancy variant) and
  * ms[i] (per step).  Drops a
  * captured step graph.  Its steps overwrite the face projections: a
  * neighbour step pending from before the call returns EDGE_ERR_ARG.  No counterpart in the reference (its back-end is a

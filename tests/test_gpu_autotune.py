@@ -10,8 +10,8 @@ import oracle as orc
 from edge_amd import mesh as M
 from edge_amd import setups as S
 from edge_amd.mesh import FA_FREE_SURFACE, FA_OUTFLOW, FA_PERIODIC
-from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, PATH_FUSED, PATH_PROJECTIONS,
-                             ElasticSolver, ViscoSolver)
+from edge_amd.solver import (KERNEL_GENERIC, KERNEL_MFMA, KERNEL_MFMA64, PATH_FUSED, PATH_HI_OCC,
+                             PATH_PROJECTIONS, ElasticSolver, ViscoSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +37,7 @@ def test_autotune_elastic(gpu, prec, runs, expect):
     cands = s.autotune(dt, n_reps=2)
     assert expect <= {(k, p) for k, p, *_ in cands}
     assert all(c[3] > 0 for c in cands)
-    k, p, _, _, f = min(cands, key=lambda c: c[3])
+    k, p, _, _, f, _ = min(cands, key=lambda c: c[3])
     kernel, flags = s.kernel_info()
     assert bool(flags & PATH_FUSED) == f
     assert kernel == k and bool(flags & PATH_PROJECTIONS) == p
@@ -63,8 +63,9 @@ def test_autotune_visco(gpu):
     cands = s.autotune(dt, n_reps=2)
     assert {(KERNEL_MFMA, True), (KERNEL_MFMA, False), (KERNEL_GENERIC, False)} <= {(k, p) for k, p, *_ in cands}
     assert {x for k, _, x, *_ in cands if k == KERNEL_MFMA} == {True, False}  # both element schedules
-    k, *_ = min(cands, key=lambda c: c[3])
-    assert s.kernel_info()[0] == k
+    assert {c[5] for c in cands if c[0] == KERNEL_MFMA} == {True, False}  # both occupancy variants
+    k, *_, o = min(cands, key=lambda c: c[3])
+    assert s.kernel_info()[0] == k and bool(s.kernel_info()[1] & PATH_HI_OCC) == o
     rng = np.random.default_rng(2)
     n = S.n_modes(order)
     d0 = rng.normal(size=(m.n_el, 9, n, runs))
@@ -112,21 +113,22 @@ def test_every_candidate_matches_oracle(gpu, kind, prec, runs):
     cands = s.autotune(dt, n_reps=1)
     assert len(cands) >= 2
     if kind == "elastic" and prec == 32 and runs == 16:
-        assert any(f for *_, f in cands)
+        assert any(c[4] for c in cands)  # a fused candidate
     tol = 1e-12 if prec == 64 else 1e-5
-    for k, p, x, _, f in cands:
-        s.set_variant(k, p, x, f)
+    for k, p, x, _, f, o in cands:
+        s.set_variant(k, p, x, f, o)
         assert s.kernel_info()[0] == k
         assert bool(s.kernel_info()[1] & PATH_FUSED) == f
+        assert bool(s.kernel_info()[1] & PATH_HI_OCC) == o
         for graph in (False, True):
             gd, gt = _dev(d0, prec), _dev(np.zeros_like(d0), prec)
             if a is not None:
                 ga = _dev(a0, prec)
                 s.step(gd, gt, dt, 3, ga, graph=graph)
-                assert rel(ga.cpu().numpy(), a) <= tol, (k, p, x, f, graph)
+                assert rel(ga.cpu().numpy(), a) <= tol, (k, p, x, f, o, graph)
             else:
                 s.step(gd, gt, dt, 3, graph=graph)
-            assert rel(gd.cpu().numpy(), d) <= tol, (k, p, x, f, graph)
+            assert rel(gd.cpu().numpy(), d) <= tol, (k, p, x, f, o, graph)
     with pytest.raises(Exception):
         s.set_variant(7)
     s.close()
