@@ -132,13 +132,25 @@ __device__ __forceinline__ void raw_products(const float* __restrict__ m, const 
   for (int t = 0; t < TT; t++)
 #pragma unroll
     for (int p = 0; p < 3; p++) acc[t][p] = zero4();
+  // without apre every k-step's A operands are requested up front: one
+  // memory round trip instead of one per k-step
+  float aall[apre ? 1 : KS][TT];
+  if (!apre) {
+#pragma unroll
+    for (int s = 0; s < KS; s++)
+#pragma unroll
+      for (int t = 0; t < TT; t++) {
+        const int k = 4 * s + g;
+        aall[s][t] = ldg(m, k * N + 16 * (t0 + t) + c, k < NK && 16 * (t0 + t) + c < NN);
+      }
+  }
   auto load = [&](int s, float (&a)[TT], float (&b)[9]) {
     const int k = 4 * s + g;
     if (apre) {
       a[0] = apre[s];
     } else {
 #pragma unroll
-      for (int t = 0; t < TT; t++) a[t] = ldg(m, k * N + 16 * (t0 + t) + c, k < NK && 16 * (t0 + t) + c < NN);
+      for (int t = 0; t < TT; t++) a[t] = aall[s][t];
     }
 #pragma unroll
     for (int q = 0; q < 9; q++) {
@@ -200,9 +212,16 @@ __device__ __forceinline__ void face_project(const float* __restrict__ fl, const
     ch[p] = h ? v1 : v0;
     c2[p] = vt[18 + p];
   }
+  // every k-step's A operand requested up front (one memory round trip)
+  float aall[KS];
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    const int k = 4 * s + g;
+    aall[s] = ldg(fl, k * NF + c, k < N && c < NF);
+  }
   auto load = [&](int s, float& a, float (&b)[9]) {
     const int k = 4 * s + g;
-    a = ldg(fl, k * NF + c, k < N && c < NF);
+    a = aall[s];
 #pragma unroll
     for (int q = 0; q < 9; q++) {
       if (4 * s + 3 < N)  // constant after unrolling
