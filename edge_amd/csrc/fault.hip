@@ -47,6 +47,9 @@ struct edge_fault {
   std::vector<int64_t> col;   // colour ranges in list (no element twice in one range)
   double* scat = nullptr;     // [16][N][n_sf]: element modes -> sub-face means of the trace
   double* lift = nullptr;     // [16][n_sf][N]: sub-face net updates -> DOF updates
+  float* scat32 = nullptr;    // FP32 copies for the MFMA phase (FP32, 16 runs)
+  float* lift32 = nullptr;
+  size_t lds_mfma = 0;
   std::vector<int64_t> faces; // mesh face ids of the fault faces, in fault order
   std::vector<int32_t> h_fel;
   std::vector<uint8_t> h_ftab;
@@ -125,6 +128,148 @@ __global__ __launch_bounds__(256) void k_fault(const int32_t* __restrict__ list,
 
 size_t fault_lds(int N, int NSF, int R, int prec_bytes) { return (size_t)prec_bytes * 2 * 9 * R * (N + NSF); }
 
+// FP32 with 16 runs: the sub-face projections and the lifts are products of
+// the face's tables with the element tiles, as v_mfma_f32_16x16x4f32 with
+// the runs on the N dimension (A lane (g, c): table row 16 t + c, k-step
+// row 4 s + g; B lane (g, c): tile row 4 s + g, run c; D register v: row
+// 16 t + 4 g + v, run c); the friction in FP64 per (sub-face, run) as above.
+// LDS (dynamic): both tInt tiles [2][9][N][16], then both sides' sub-face
+// traces, overwritten by their net updates, [2][9][n_sf][16]; the last
+// k-step of a product reads rows past the tile as zeros (at order 4, 79,488
+// bytes: two workgroups per CU).
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr size_t fault_lds_mfma(int N, int NSF) { return sizeof(float) * 2 * 9 * 16 * (N + NSF); }
+
+// B operand of k-step row k of a [rows][16] tile (zero past the tile)
+template <int ROWS>
+__device__ __forceinline__ float tile_b(const float* x, int s, int k, int c) {
+  if (4 * s + 3 < ROWS) return x[k * 16 + c];  // constant after unrolling
+  const float v = x[(k < ROWS ? k : 0) * 16 + c];
+  return k < ROWS ? v : 0.f;
+}
+
+template <int O>
+__global__ __launch_bounds__(256) void k_fault_mfma(const int32_t* __restrict__ list, int64_t first, double dt,
+                                                     const int32_t* __restrict__ fel,
+                                                     const uint8_t* __restrict__ ftab,
+                                                     const float* __restrict__ scat32,
+                                                     const float* __restrict__ lift32, RuptureDev rd,
+                                                     const float* __restrict__ t_int, float* __restrict__ dofs) {
+  constexpr int N = n_modes(O), NSF = n_sub_faces(O), R = 16;
+  constexpr int KN = (N + 3) / 4, KSF = (NSF + 3) / 4, RN = N, RSF = NSF;
+  constexpr int TN = (N + 15) / 16, TSF = (NSF + 15) / 16;
+  extern __shared__ __align__(16) unsigned char lds[];
+  float* sX = reinterpret_cast<float*>(lds);  // [2][9][N][16]
+  float* nX = sX + 2 * 9 * RN * 16;           // [2][9][NSF][16]
+  const int64_t fi = list[first + blockIdx.x];
+  const int64_t e[2] = {fel[2 * fi], fel[2 * fi + 1]};
+  const int tb[2] = {ftab[2 * fi], ftab[2 * fi + 1]};
+  const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 2 * 9 * N * R; i += 256) {
+    const int side = i / (9 * N * R), j = i % (9 * N * R);
+    sX[i] = t_int[e[side] * 9 * N * R + j];
+  }
+  __syncthreads();
+  // sub-face traces (time averages): D[sf][r] = sum_n scat[n][sf] tInt[q][n][r] / dt
+  const float idt = (float)(1.0 / dt);
+  for (int task = wv; task < 2 * 9 * TSF; task += 4) {
+    const int side = task / (9 * TSF), q = (task / TSF) % 9, t = task % TSF;
+    const float* sc = scat32 + (size_t)tb[side] * N * NSF;
+    const float* x = sX + (side * 9 + q) * RN * R;
+    const int m = 16 * t + c;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KN; s++) {
+      const int k = 4 * s + g;
+      const bool in = k < N && m < NSF;
+      const float a = sc[in ? k * NSF + m : 0];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? a : 0.f, tile_b<N>(x, s, k, c), acc, 0, 0, 0);
+    }
+    float* o = nX + (side * 9 + q) * RSF * R;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int sf = 16 * t + 4 * g + v;
+      if (sf < NSF) o[sf * R + c] = acc[v] * idt;
+    }
+  }
+  __syncthreads();
+  const double* s = rd.solv + fi * 324;
+  const double* face = rd.face + fi * 3;
+  for (int i = tid; i < NSF * R; i += 256) {
+    const int sf = i / R, r = i % R;
+    double qL[9], qR[9], a[9], b[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      qL[q] = nX[(q * RSF + sf) * R + r];
+      qR[q] = nX[((9 + q) * RSF + sf) * R + r];
+    }
+    double* st = rd.state + (fi * kRuptureStateVals * NSF + sf) * R + r;
+    const bool failing = lsw_net_updates(s, face, rd.glob[r], rd.glob[R + r], rd.glob[2 * R + r], dt, qL, qR, st,
+                                         (int64_t)NSF * R, a, b);
+    rd.per[(fi * NSF + sf) * R + r] = failing ? 1 : 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      nX[(q * RSF + sf) * R + r] = (float)a[q];
+      nX[((9 + q) * RSF + sf) * R + r] = (float)b[q];
+    }
+  }
+  __syncthreads();
+  // DOF updates: D[n][r] = sum_sf lift[sf][n] net[q][sf][r]
+  for (int task = wv; task < 2 * 9 * TN; task += 4) {
+    const int side = task / (9 * TN), q = (task / TN) % 9, t = task % TN;
+    const float* lf = lift32 + (size_t)tb[side] * NSF * N;
+    const float* x = nX + (side * 9 + q) * RSF * R;
+    const int m = 16 * t + c;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KSF; s++) {
+      const int k = 4 * s + g;
+      const bool in = k < NSF && m < N;
+      const float a = lf[in ? k * N + m : 0];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(in ? a : 0.f, tile_b<NSF>(x, s, k, c), acc, 0, 0, 0);
+    }
+    float* d = dofs + (e[side] * 9 + q) * N * R;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int n = 16 * t + 4 * g + v;
+      if (n < N) d[n * R + c] += acc[v];
+    }
+  }
+}
+
+template <int O>
+void launch_fault_mfma_o(const edge_fault* f, int64_t a, int64_t n, double dt, const RuptureDev& rd,
+                         const float* t_int, float* dofs, hipStream_t st) {
+  hipLaunchKernelGGL((k_fault_mfma<O>), dim3((unsigned)n), dim3(256), f->lds_mfma, st, f->list, a, dt, f->fel,
+                     f->ftab, f->scat32, f->lift32, rd, t_int, dofs);
+}
+
+int launch_fault_mfma(const edge_fault* f, int64_t a, int64_t n, double dt, const RuptureDev& rd,
+                      const float* t_int, float* dofs, hipStream_t st) {
+  switch (f->order) {
+    case 2: launch_fault_mfma_o<2>(f, a, n, dt, rd, t_int, dofs, st); break;
+    case 3: launch_fault_mfma_o<3>(f, a, n, dt, rd, t_int, dofs, st); break;
+    case 4: launch_fault_mfma_o<4>(f, a, n, dt, rd, t_int, dofs, st); break;
+    case 5: launch_fault_mfma_o<5>(f, a, n, dt, rd, t_int, dofs, st); break;
+    default: return fail(EDGE_ERR_UNSUPPORTED, "fault: order");
+  }
+  EDGE_HIP_TRY(hipGetLastError());
+  return EDGE_OK;
+}
+
+hipError_t fault_mfma_allow_lds(int order, int lds) {
+  const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
+  switch (order) {
+    case 2: return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fault_mfma<2>), at, lds);
+    case 3: return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fault_mfma<3>), at, lds);
+    case 4: return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fault_mfma<4>), at, lds);
+    case 5: return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fault_mfma<5>), at, lds);
+  }
+  return hipErrorInvalidValue;
+}
+
 // scat[t] = F_t M_F^-1 S^T and lift[t] = c M^-1 F_t G with F_t the face
 // trace integrals of table t (fluxL for t < 4, fluxN[t - 4] after), S the
 // sub-face means of the face basis and G its least-squares inverse.
@@ -158,6 +303,11 @@ int fault_apply(edge_fault* f, double dt, const void* t_int, void* dofs, hipStre
   for (size_t c = 0; c + 1 < f->col.size(); c++) {
     const int64_t a = f->col[c], n = f->col[c + 1] - a;
     if (n <= 0) continue;
+    if (f->scat32) {
+      rc = launch_fault_mfma(f, a, n, dt, rd, (const float*)t_int, (float*)dofs, st);
+      if (rc) return rc;
+      continue;
+    }
     if (f->precision == 64)
       hipLaunchKernelGGL(k_fault<double>, dim3((unsigned)n), dim3(256), f->lds, st, f->list, a, N, f->n_sf, R, dt,
                          f->fel, f->ftab, f->scat, f->lift, rd, (const double*)t_int, (double*)dofs);
@@ -318,6 +468,13 @@ extern "C" int edge_fault_create(const edge_el_solver* s, const edge_mesh_in* m,
   up((void**)&F->list, list.data(), sizeof(int32_t) * list.size());
   up((void**)&F->scat, scat.data(), sizeof(double) * scat.size());
   up((void**)&F->lift, lift.data(), sizeof(double) * lift.size());
+  if (F->precision == 32 && R == 16) {  // the MFMA fault phase
+    const std::vector<float> s32(scat.begin(), scat.end()), l32(lift.begin(), lift.end());
+    up((void**)&F->scat32, s32.data(), sizeof(float) * s32.size());
+    up((void**)&F->lift32, l32.data(), sizeof(float) * l32.size());
+    F->lds_mfma = edge::fault_lds_mfma(N, nsf);
+    if (e == hipSuccess && F->lds_mfma > 64 * 1024) e = edge::fault_mfma_allow_lds(F->order, (int)F->lds_mfma);
+  }
   if (e == hipSuccess && lds > 64 * 1024) {
     e = F->precision == 64 ? hipFuncSetAttribute(reinterpret_cast<const void*>(&edge::k_fault<double>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
@@ -335,7 +492,9 @@ extern "C" int edge_fault_create(const edge_el_solver* s, const edge_mesh_in* m,
 extern "C" int edge_fault_destroy(edge_fault* f) {
   if (!f) return EDGE_OK;
   edge_rupture_destroy(f->rup);
-  for (void* p : {(void*)f->fel, (void*)f->ftab, (void*)f->list, (void*)f->scat, (void*)f->lift}) (void)hipFree(p);
+  for (void* p : {(void*)f->fel, (void*)f->ftab, (void*)f->list, (void*)f->scat, (void*)f->lift, (void*)f->scat32,
+                  (void*)f->lift32})
+    (void)hipFree(p);
   delete f;
   return EDGE_OK;
 }
