@@ -379,15 +379,18 @@ def ensemble_row(args, timed, flops_el_run):
         d = torch.randn(s.dof_shape(), dtype=s.dtype, device="cuda")
         a = 0.01 * torch.randn(s.dof_a_shape(), dtype=s.dtype, device="cuda")
         t = torch.zeros_like(d)
-        ms = timed(lambda st: s.step(d, t, dt, 1, a, graph=True, stream=st), 3)
+        # one graph-captured multi-step call (fused launches where the solver has them)
+        n_steps = 6
+        ms = timed(lambda st: s.step(d, t, dt, n_steps, a, graph=True, stream=st), 1) / n_steps
         peak = PEAK["fp32_mfma"] if prec == 32 else PEAK["fp64"]
         n = s.n_modes
         print(json.dumps({"row": "batched viscoelastic ensemble", "meshes": len(sizes), "hexes_per_axis": sizes,
                           "elements": u.n_el, "order": args.order, "runs": runs, "precision": prec, "n_rms": 3,
                           "kernel": s.kernel_info()[0], "hi_occ": bool(s.kernel_info()[1] & 32),
-                          "autotune": [{"kernel": k, "projections": p, "xcd_schedule": x, "occupancy": o,
-                                        "ms_per_step": c} for k, p, x, c, _, o in tune],
-                          "graph": True, "setup_s": round(t_setup, 1),
+                          "fused": bool(s.kernel_info()[1] & 16),
+                          "autotune": [{"kernel": k, "projections": p, "xcd_schedule": x, "fused": f, "occupancy": o,
+                                        "ms_per_step": c} for k, p, x, c, f, o in tune],
+                          "graph": True, "steps_per_call": n_steps, "setup_s": round(t_setup, 1),
                           "ms_per_step": ms * 1e3, "dof_updates_per_s": u.n_el * 27 * n * runs / ms,
                           "roofline_frac": flops_el_run * runs * u.n_el / ms / peak}), flush=True)
         s.close()
@@ -504,19 +507,22 @@ def rows(args):
         t, a = torch.zeros_like(d), s.alloc_a()
         loc = timed(lambda st: s.local(d, t, dt, dofs_a=a, stream=st), 2)
         nei = timed(lambda st: s.neigh(d, t, dofs_a=a, stream=st), 2)
+        fused = bool(s.kernel_info()[1] & 16)
+        # whole steps: one multi-step call (fused launches where the solver has them)
+        step = timed(lambda st: s.step(d, t, dt, 6, a, stream=st), 1) / 6
         peak = PEAK["fp32_mfma"] if prec == 32 else PEAK["fp64"]
         kern = {1: "generic (LDS tiles, VALU)", 2: "MFMA (paired products, elementwise anelastic terms)",
                 4: "FP64 MFMA (anelastic rows in the quantity tiles)"}
         print(json.dumps({"row": "a12 viscoelastic update", "kernel": kern.get(s.kernel_info()[0], "?"),
                           "hi_occ": bool(s.kernel_info()[1] & 32),
-                          "autotune": [{"kernel": k, "projections": p, "xcd_schedule": x, "occupancy": o,
-                                        "ms_per_step": c} for k, p, x, c, _, o in tune],
+                          "autotune": [{"kernel": k, "projections": p, "xcd_schedule": x, "fused": f, "occupancy": o,
+                                        "ms_per_step": c} for k, p, x, c, f, o in tune],
                           "order": args.order,
                           "runs": runs, "precision": prec, "n_rms": 3, "elements": m.n_el,
-                          "local_ms": loc * 1e3, "neigh_ms": nei * 1e3,
-                          "dof_updates_per_s": m.n_el * 27 * n * runs / (loc + nei),
-                          "tflops": (fl + fn_) * runs * m.n_el / (loc + nei) / 1e12,
-                          "roofline_frac": (fl + fn_) * runs * m.n_el / (loc + nei) / peak}), flush=True)
+                          "local_ms": loc * 1e3, "neigh_ms": nei * 1e3, "fused": fused,
+                          "step_ms": step * 1e3, "dof_updates_per_s": m.n_el * 27 * n * runs / step,
+                          "tflops": (fl + fn_) * runs * m.n_el / step / 1e12,
+                          "roofline_frac": (fl + fn_) * runs * m.n_el / step / peak}), flush=True)
         s.close()
         del d, t, a
     # ---- batched viscoelastic ensemble: 24 meshes of seeded sizes (6 n^3 tets,

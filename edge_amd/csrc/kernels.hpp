@@ -113,6 +113,11 @@ int launch_local_visco_mfma(const SolverDev<float>& s, int64_t first, int64_t n,
                             float* dofs_a, float* t_int, hipStream_t st);
 int launch_neigh_visco_mfma(const SolverDev<float>& s, int64_t first, int64_t n, const float* t_int, float* dofs,
                             float* dofs_a, hipStream_t st);
+// The neighbour step of step k - 1 (from proj_in, the previous launch's
+// projections) and the local step of step k (projections to s.proj) as one
+// launch over all elements (FP32, 8 runs, rank-3 factored solvers).
+int launch_fused_visco_mfma(const SolverDev<float>& s, float dt, const float* proj_in, float* dofs, float* dofs_a,
+                            float* t_int, hipStream_t st);
 
 // FP64 single-run kernels on FP64 MFMA (kernels_plain.hip).
 bool plain64_supported(int order, int n_runs, int prec_bytes);

@@ -339,10 +339,17 @@ __device__ __forceinline__ void add_source(const cfloat* __restrict__ e, const f
 // TL: tIntA in LDS (each thread's own positions, no barriers) instead of
 // registers, so that five waves per SIMD fit the register file (the
 // occupancy variant; LDS then allows three workgroups per CU at O5).
-template <int O, int NRM, bool TL>
-__global__ __launch_bounds__(kLocalThreads) __attribute__((amdgpu_waves_per_eu(TL ? 5 : 1)))
+// NB: first the previous time step's neighbouring surface integral from the
+// projections in proj_in (written by the previous launch; s.proj receives
+// this step's, so the two alternate between two buffers): its lifted tiles
+// update Q and QA before the CK, and the output adds them to the DOFs read
+// at the start, so the neighbour step of step k - 1 and the local step of
+// step k run as one launch that reads and writes the DOFs once.
+template <int O, int NRM, bool TL, bool NB = false>
+__global__ __launch_bounds__(kLocalThreads) __attribute__((amdgpu_waves_per_eu(TL ? 5 : (NB ? 4 : 1))))
 void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __restrict__ dofs,
-                float* __restrict__ dofs_a, float* __restrict__ t_int) {
+                float* __restrict__ dofs_a, float* __restrict__ t_int, const float* __restrict__ proj_in) {
+  static_assert(!(NB && TL), "the fused step keeps tIntA in registers");
   using D = VDims<O>;
   constexpr int N = D::N, NV = D::NV, NP = D::NP, SZ = D::SZ, SZA = D::SZA;
   constexpr int PJ = (NP + kLocalThreads - 1) / kLocalThreads;  // positions per thread
@@ -369,6 +376,16 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
     float4* lq = reinterpret_cast<float4*>(l_x);
     for (int i = tid; i < SZ / 4; i += kLocalThreads) lq[i] = gq[i];
   }
+  // the previous step's neighbour lifts, face wv < 4 by wave wv (NB)
+  if constexpr (NB) {
+    if (wv < 4) {
+      const int32_t slot = ((const cint32*)s.rcv)[el * 4 + wv];
+      if (slot >= 0) {  // else outflow (uniform)
+        const v4f* pw = reinterpret_cast<const v4f*>(proj_in + (int64_t)slot * kProjFace);
+        face_lift<O>(s.dg.fluxT + wv * D::NF * N, pw[lane], pw[64 + lane], l_p + wv * 3 * NP, g, c);
+      }
+    }
+  }
   // anelastic state per position: derA (CK), tIntA (registers, or l_ta)
   float da[NRM][6][PJ], ta_r[TL ? 1 : NRM][6][PJ], ti[9][PJ];
   auto ta = [&](int rm, int q, int j) -> float& {
@@ -384,14 +401,58 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++)
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-          const float v = qa0[(rm * 6 + q) * NP + p];
-          da[rm][q][j] = v;
-          ta(rm, q, j) = dt * v;
-        }
+        for (int q = 0; q < 6; q++) da[rm][q][j] = qa0[(rm * 6 + q) * NP + p];
     }
   }
   __syncthreads();
+  // NB: Q += sum_f U_f L_f, QA_rm += w_rm sum_f U_A,f L_f (the neighbour-side
+  // factors); nq, nqa are added again to the DOFs the output reads
+  float nq[NB ? 9 : 1][PJ], nqa[NB ? 6 : 1][PJ];
+  if constexpr (NB) {
+    const cint32* rcv = (const cint32*)(s.rcv + el * 4);
+    const cfloat* fsn = as_const(s.fsf + (s.n_el + el) * 4 * 54);
+    const cfloat* fsna = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
+#pragma unroll
+    for (int j = 0; j < PJ; j++) {
+      const int p = tid + j * kLocalThreads;
+#pragma unroll
+      for (int q = 0; q < 9; q++) nq[q][j] = 0.f;
+#pragma unroll
+      for (int q = 0; q < 6; q++) nqa[q][j] = 0.f;
+      if (p < NP) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+          if (rcv[f] < 0) continue;  // outflow (uniform)
+          const float l0 = l_p[(f * 3) * NP + p], l1 = l_p[(f * 3 + 1) * NP + p], l2 = l_p[(f * 3 + 2) * NP + p];
+          const cfloat* u = fsn + f * 54;
+          const cfloat* uan = fsna + f * 18;
+#pragma unroll
+          for (int q = 0; q < 9; q++) nq[q][j] += u[q * 3] * l0 + u[q * 3 + 1] * l1 + u[q * 3 + 2] * l2;
+#pragma unroll
+          for (int q = 0; q < 6; q++) nqa[q][j] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) l_x[q * NP + p] += nq[q][j];
+#pragma unroll
+        for (int rm = 0; rm < NRM; rm++) {
+          const float w = rfs[rm];
+#pragma unroll
+          for (int q = 0; q < 6; q++) da[rm][q][j] += w * nqa[q][j];
+        }
+      }
+    }
+    __syncthreads();  // Q complete (and the lifted tiles read) before the CK
+  }
+#pragma unroll
+  for (int j = 0; j < PJ; j++) {
+    const int p = tid + j * kLocalThreads;
+    if (p < NP) {
+#pragma unroll
+      for (int rm = 0; rm < NRM; rm++)
+#pragma unroll
+        for (int q = 0; q < 6; q++) ta(rm, q, j) = dt * da[rm][q][j];
+    }
+  }
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
     const int p = tid + j * kLocalThreads;
@@ -569,6 +630,12 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 #pragma unroll
         for (int q = 0; q < 6; q++) ua[q] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
       }
+      if constexpr (NB) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) y[q] += nq[q][j];
+#pragma unroll
+        for (int q = 0; q < 6; q++) ua[q] += nqa[q][j];
+      }
 #pragma unroll
       for (int q = 0; q < 9; q++) gq[q * NP + p] = qin[q][j] + y[q];
 #pragma unroll
@@ -680,10 +747,10 @@ void launch_o(const SolverDev<float>& s, int64_t first, int64_t n, float dt, flo
               float* t_int, const float* tin, bool local, hipStream_t st) {
   if (local && s.hi_occ)
     hipLaunchKernelGGL((k_local_vm<O, NRM, true>), dim3(grid_of(n)), dim3(kLocalThreads), 0, st, s, first, n, dt,
-                       dofs, dofs_a, t_int);
+                       dofs, dofs_a, t_int, nullptr);
   else if (local)
     hipLaunchKernelGGL((k_local_vm<O, NRM, false>), dim3(grid_of(n)), dim3(kLocalThreads), 0, st, s, first, n, dt,
-                       dofs, dofs_a, t_int);
+                       dofs, dofs_a, t_int, nullptr);
   else if (s.proj)
     hipLaunchKernelGGL((k_neigh_vm<O, NRM, true>), dim3(grid_of(n)), dim3(kNeighThreads), 0, st, s, first, n, tin,
                        dofs, dofs_a);
@@ -718,7 +785,42 @@ int launch_vm(const SolverDev<float>& s, int64_t first, int64_t n, float dt, flo
   return fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: orders 3 to 5");
 }
 
+// The fused launch (k_local_vm with NB) over all elements.
+template <int O, int NRM>
+void launch_fused_o(const SolverDev<float>& s, float dt, const float* proj_in, float* dofs, float* dofs_a,
+                    float* t_int, hipStream_t st) {
+  hipLaunchKernelGGL((k_local_vm<O, NRM, false, true>), dim3(grid_of(s.n_el)), dim3(kLocalThreads), 0, st, s,
+                     (int64_t)0, s.n_el, dt, dofs, dofs_a, t_int, proj_in);
+}
+
+template <int O>
+int launch_fused_rm(const SolverDev<float>& s, float dt, const float* proj_in, float* dofs, float* dofs_a,
+                    float* t_int, hipStream_t st) {
+  switch (s.n_rms) {
+    case 1: launch_fused_o<O, 1>(s, dt, proj_in, dofs, dofs_a, t_int, st); break;
+    case 2: launch_fused_o<O, 2>(s, dt, proj_in, dofs, dofs_a, t_int, st); break;
+    case 3: launch_fused_o<O, 3>(s, dt, proj_in, dofs, dofs_a, t_int, st); break;
+    default: return fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: 1 to 3 mechanisms");
+  }
+  EDGE_HIP_TRY(hipGetLastError());
+  return EDGE_OK;
+}
+
 }  // namespace
+
+int launch_fused_visco_mfma(const SolverDev<float>& s, float dt, const float* proj_in, float* dofs, float* dofs_a,
+                            float* t_int, hipStream_t st) {
+  if (!s.fsf || !s.fsfa || !s.fs_rank3 || !s.sparse_star || !s.proj || !proj_in || !s.rcv)
+    return fail(EDGE_ERR_UNSUPPORTED, "fused viscoelastic step: factored flux solvers and face projections");
+  if (s.n_runs != R) return fail(EDGE_ERR_UNSUPPORTED, "fused viscoelastic step: 8 runs");
+  if (s.n_el <= 0) return EDGE_OK;
+  switch (s.order) {
+    case 3: return launch_fused_rm<3>(s, dt, proj_in, dofs, dofs_a, t_int, st);
+    case 4: return launch_fused_rm<4>(s, dt, proj_in, dofs, dofs_a, t_int, st);
+    case 5: return launch_fused_rm<5>(s, dt, proj_in, dofs, dofs_a, t_int, st);
+  }
+  return fail(EDGE_ERR_UNSUPPORTED, "MFMA viscoelastic kernels: orders 3 to 5");
+}
 
 bool visco_mfma_supported(int order, int n_runs, int precision, int n_rms) {
   return precision == 32 && n_runs == R && order >= 3 && order <= 5 && n_rms >= 1 && n_rms <= 3;

@@ -590,21 +590,44 @@ int one_step(edge_el_solver* s, double dt, void* dofs, void* dofs_a, void* t_int
 // own).  Any fault phase runs between the launches, where it runs between
 // local and neighbour step otherwise.
 bool fused_active(const edge_el_solver* s) {
-  return s->kernel == 2 && s->use_fused && s->n_rms == 0 && s->cfg.precision == 32 && proj_active(s);
+  return s->kernel == 2 && s->use_fused && s->cfg.precision == 32 && proj_active(s) &&
+         s->cfg.n_runs == (s->n_rms == 0 ? 16 : 8);
 }
 
-int fused_steps(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* t_int, hipStream_t st) {
-  // the fused kernels are the 16-run FP32 MFMA kernels: nothing else may get here
-  if (!fused_active(s) || s->cfg.n_runs != 16)
-    return edge::fail(EDGE_ERR_UNSUPPORTED, "fused steps: FP32 MFMA elastic solvers of 16 runs");
-  if (!s->proj_alt) EDGE_HIP_TRY(hipMalloc(&s->proj_alt, sizeof(float) * 4 * 768 * s->n_el));
+// The viscoelastic form of fused_steps (FP32, 8 runs): the same launch
+// sequence with the anelastic DOFs.
+int fused_steps_a(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* dofs_a, void* t_int,
+                  hipStream_t st) {
   float* P[2] = {static_cast<float*>(s->proj), static_cast<float*>(s->proj_alt)};
   auto d = dev_view<float>(s);
   const float fdt = (float)dt;
+  float *q = (float*)dofs, *qa = (float*)dofs_a, *ti = (float*)t_int;
+  d.proj = P[0];
+  int rc = edge::launch_local_visco_mfma(d, 0, s->n_el, fdt, q, qa, ti, st);
+  for (int64_t k = 1; k < n_steps && rc == EDGE_OK; k++) {
+    d.proj = P[k % 2];
+    rc = edge::launch_fused_visco_mfma(d, fdt, P[(k - 1) % 2], q, qa, ti, st);
+  }
+  d.proj = P[(n_steps - 1) % 2];
+  if (rc == EDGE_OK) rc = edge::launch_neigh_visco_mfma(d, 0, s->n_el, ti, q, qa, st);
+  return rc;
+}
+
+int fused_steps(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* dofs_a, void* t_int,
+                hipStream_t st) {
+  // the fused kernels are the FP32 MFMA kernels of 16 elastic or 8
+  // viscoelastic runs: nothing else may get here
+  if (!fused_active(s) || (s->n_rms > 0) != (dofs_a != nullptr))
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "fused steps: FP32 MFMA solvers, 16 elastic or 8 viscoelastic runs");
+  if (!s->proj_alt) EDGE_HIP_TRY(hipMalloc(&s->proj_alt, sizeof(float) * 4 * 768 * s->n_el));
   // the projection buffers no longer stand for a tInt array a separate
   // neighbour step could be handed
   s->proj_tint = nullptr;
   s->proj_fresh = false;
+  if (s->n_rms > 0) return fused_steps_a(s, n_steps, dt, dofs, dofs_a, t_int, st);
+  float* P[2] = {static_cast<float*>(s->proj), static_cast<float*>(s->proj_alt)};
+  auto d = dev_view<float>(s);
+  const float fdt = (float)dt;
   d.proj = P[0];
   int rc = edge::launch_local_mfma<float>(d, 0, s->n_el, fdt, (float*)dofs, (float*)t_int, st);
   if (rc == EDGE_OK && s->fault) rc = edge::fault_apply(s->fault, dt, t_int, dofs, st);
@@ -1030,12 +1053,12 @@ std::vector<TuneCand> tune_candidates(edge_el_solver* s) {
   const bool gen = s->n_rms > 0 ? edge::visco_supported(s->cfg.order, s->cfg.n_runs, prec_bytes, s->n_rms)
                                 : edge::generic_supported(s->cfg.order, s->cfg.n_runs, prec_bytes);
   if (base == 2) {
-    const bool fusable = s->proj && s->n_rms == 0 && s->cfg.precision == 32;
+    const bool fusable = s->proj && s->cfg.precision == 32;
     // the viscoelastic local kernel also comes with tIntA in LDS, five waves per SIMD
     const int n_occ = s->n_rms > 0 ? 2 : 1;
     for (int x = 1; x >= 0; x--)  // XCD-aware element schedule, then the linear one
       for (int o = 0; o < n_occ; o++) {
-        if (fusable) c.push_back({2, true, x == 1, true});
+        if (fusable && o == 0) c.push_back({2, true, x == 1, true});
         if (s->proj) c.push_back({2, true, x == 1, false, o == 1});
         c.push_back({2, false, x == 1, false, o == 1});
       }
@@ -1086,7 +1109,8 @@ extern "C" int edge_el_autotune(edge_el_solver* s, double dt, int32_t n_reps, vo
       rc = one_step(s, dt, dofs, dofs_a, t_int, st);  // warm-up
       if (rc == EDGE_OK) hip(hipEventRecord(e0, st), "event");
       if (cand[i].fused)  // one fused sequence of n_reps steps (n_reps + 1 launches)
-        rc = n_reps >= 2 ? fused_steps(s, n_reps, dt, dofs, t_int, st) : one_step(s, dt, dofs, dofs_a, t_int, st);
+        rc = n_reps >= 2 ? fused_steps(s, n_reps, dt, dofs, dofs_a, t_int, st)
+                         : one_step(s, dt, dofs, dofs_a, t_int, st);
       else
         for (int r = 0; r < n_reps && rc == EDGE_OK; r++) rc = one_step(s, dt, dofs, dofs_a, t_int, st);
       if (rc == EDGE_OK && hip(hipEventRecord(e1, st), "event") && hip(hipEventSynchronize(e1), "sync")) {
@@ -1298,9 +1322,9 @@ namespace {
 int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* dofs_a, void* t_int, int use_graph,
               hipStream_t st) {
   if (n_steps == 0) return EDGE_OK;
-  const bool fuse = n_steps >= 2 && !dofs_a && fused_active(s);
+  const bool fuse = n_steps >= 2 && fused_active(s) && (s->n_rms > 0) == (dofs_a != nullptr);
   if (fuse && !s->proj_alt) EDGE_HIP_TRY(hipMalloc(&s->proj_alt, sizeof(float) * 4 * 768 * s->n_el));
-  if (!use_graph && fuse) return fused_steps(s, n_steps, dt, dofs, t_int, st);
+  if (!use_graph && fuse) return fused_steps(s, n_steps, dt, dofs, dofs_a, t_int, st);
   if (!use_graph) {
     for (int64_t i = 0; i < n_steps; i++) {
       int rc = one_step(s, dt, dofs, dofs_a, t_int, st);
@@ -1327,7 +1351,7 @@ int step_impl(edge_el_solver* s, int64_t n_steps, double dt, void* dofs, void* d
       if (own) (void)hipStreamDestroy(cap);
       return edge::fail(EDGE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(eb));
     }
-    int rc = fuse ? fused_steps(s, n_steps, dt, dofs, t_int, cap) : one_step(s, dt, dofs, dofs_a, t_int, cap);
+    int rc = fuse ? fused_steps(s, n_steps, dt, dofs, dofs_a, t_int, cap) : one_step(s, dt, dofs, dofs_a, t_int, cap);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cap, &g);
     if (rc == EDGE_OK && e != hipSuccess) rc = edge::fail(EDGE_ERR_HIP, hipGetErrorString(e));
@@ -1378,7 +1402,7 @@ extern "C" int edge_el_step_a(edge_el_solver* s, int64_t n_steps, double dt, voi
 extern "C" int edge_el_neigh_local(edge_el_solver* s, double dt, void* dofs, void* t_int, void* stream) {
   if (!s) return edge::fail(EDGE_ERR_ARG, "solver handle missing");
   if (!dofs || !t_int) return edge::fail(EDGE_ERR_ARG, "device arrays missing");
-  if (!fused_active(s))
+  if (!fused_active(s) || s->n_rms > 0)
     return edge::fail(EDGE_ERR_UNSUPPORTED, "fused neighbour and local step: FP32 MFMA elastic solvers that exchange "
                                             "face projections (edge_el_kernel_info flag bit 4)");
   if (!s->proj_fresh)
@@ -1402,6 +1426,8 @@ extern "C" int edge_el_set_taps(edge_el_solver* s, int64_t n, const int64_t* el,
   if (n < 0 || (n > 0 && (!el || !taps))) return edge::fail(EDGE_ERR_ARG, "taps: elements or array missing");
   for (int64_t i = 0; i < n; i++)
     if (el[i] < 0 || el[i] >= s->n_el) return edge::fail(EDGE_ERR_ARG, "taps: element out of range");
+  if (n > 0 && s->n_rms > 0)
+    return edge::fail(EDGE_ERR_UNSUPPORTED, "taps: elastic solvers (the fused viscoelastic launch writes none)");
   drop_graph(s);
   s->tap = nullptr;
   if (n == 0) return EDGE_OK;

@@ -243,8 +243,9 @@ int edge_el_neigh_local(edge_el_solver *s, double dt, void *dofs, void *t_int, v
  * layout and size, the caller's) for the n elements el[n], and leave its
  * other elements untouched: receivers sampled at the step start
  * (AderDg.hpp:475-496) read them there (edge_recv_eval on taps).  n = 0
- * switches the taps off.  Drops a captured step graph.  No counterpart in
- * the reference. */
+ * switches the taps off.  Elastic solvers only (EDGE_ERR_UNSUPPORTED for
+ * viscoelastic ones, whose fused launches write no taps).  Drops a captured
+ * step graph.  No counterpart in the reference. */
 int edge_el_set_taps(edge_el_solver *s, int64_t n, const int64_t *el, void *taps);
 
 /* Number of elements / modes / runs / bytes of one DOF array of a solver. */
@@ -291,7 +292,10 @@ int edge_el_neigh_a(edge_el_solver *s, int64_t first, int64_t n_els, const void 
 /* edge_el_step with the anelastic DOFs (time loop of the reference's
  * TimeGroupStatic::computeStep over AderDg::local / neigh with TL_N_RMS > 0);
  * use_graph != 0 captures one step into a HIP graph on first use and
- * replays it.  As edge_el_step for an elastic solver (dofs_a ignored). */
+ * replays it.  FP32 MFMA viscoelastic solvers of 8 runs that exchange face
+ * projections with flag bit 4 run n_steps >= 2 as fused launches, as
+ * edge_el_step describes (after the call no separate edge_el_neigh_a may
+ * follow).  As edge_el_step for an elastic solver (dofs_a ignored). */
 int edge_el_step_a(edge_el_solver *s, int64_t n_steps, double dt, void *dofs, void *dofs_a, void *t_int,
                    int use_graph, void *stream);
 

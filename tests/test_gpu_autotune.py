@@ -112,7 +112,7 @@ def test_every_candidate_matches_oracle(gpu, kind, prec, runs):
     mo.step(d, t, dt, 3, dofs_a=a) if a is not None else mo.step(d, t, dt, 3)
     cands = s.autotune(dt, n_reps=1)
     assert len(cands) >= 2
-    if kind == "elastic" and prec == 32 and runs == 16:
+    if prec == 32 and runs in (16, 8):
         assert any(c[4] for c in cands)  # a fused candidate
     tol = 1e-12 if prec == 64 else 1e-5
     for k, p, x, _, f, o in cands:
