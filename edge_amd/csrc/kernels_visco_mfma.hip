@@ -56,7 +56,11 @@ typedef const __attribute__((address_space(4))) float cfloat;
 typedef const __attribute__((address_space(4))) int32_t cint32;
 typedef const __attribute__((address_space(4))) uint8_t cuint8;
 constexpr int R = 8;
-constexpr int kLocalThreads = 384, kNeighThreads = 256;
+// Local kernel: eight waves, so that two workgroups fill a CU's four SIMDs
+// evenly (two waves of each per SIMD): waves are placed round robin from the
+// first SIMD, and six-wave workgroups left two SIMDs with twice the waves of
+// the other two.
+constexpr int kLocalThreads = 512, kNeighThreads = 256;
 
 __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -352,7 +356,10 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   static_assert(!(NB && TL), "the fused step keeps tIntA in registers");
   using D = VDims<O>;
   constexpr int N = D::N, NV = D::NV, NP = D::NP, SZ = D::SZ, SZA = D::SZA;
-  constexpr int PJ = (NP + kLocalThreads - 1) / kLocalThreads;  // positions per thread
+  // elementwise phases: (mode, run) position tid (compact: fewer waves
+  // issue the phase's instructions than with the positions spread out)
+  constexpr int PJ = 1;
+  static_assert(NP <= kLocalThreads, "one position per thread");
   __shared__ __align__(16) float l_x[SZ];      // Q, then der_{de-1}, then tInt
   __shared__ __align__(16) float l_p[18 * NP];  // raw products [3][6][N][R]; lifted tiles [4][3][N][R]
   __shared__ __align__(16) float l_ta[TL ? NRM * 6 * NP : 1];  // tIntA [rm][6][N][R] (TL)
@@ -361,7 +368,16 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   const int64_t el = first + off;
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
-  const int dir = wv % 3, t_ck = wv / 3;  // CK / volume products: direction, half of the row tiles
+  const int pos = tid;  // this thread's position (none from NP on)
+  // The six product tasks (three directions x two halves of the row tiles)
+  // run on logical waves 0..5: waves 0..5 in even workgroups and waves
+  // 0..3, 6, 7 in odd ones (by blockIdx / 256, which alternates between the
+  // workgroups of one CU in the first dispatch round), so that the two
+  // workgroups of a CU put three task waves on every SIMD; logical waves
+  // 6.. have no task.
+  const int odd = (int)((blockIdx.x >> 8) & 1);
+  const int lw = (odd && wv >= 4) ? (wv >= 6 ? wv - 2 : wv + 4) : wv;
+  const int dir = lw % 3, t_ck = lw / 3;  // CK / volume products: direction, half of the row tiles
   const cfloat* star = as_const(s.star + el * 243);
   const cfloat* star_a = as_const(s.star_a + el * 54);
   const cfloat* src = as_const(s.src_a + el * NRM * 36);
@@ -378,7 +394,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   }
   // the previous step's neighbour lifts, face wv < 4 by wave wv (NB)
   if constexpr (NB) {
-    if (wv < 4) {
+    if (lw < 4) {
       const int32_t slot = ((const cint32*)s.rcv)[el * 4 + wv];
       if (slot >= 0) {  // else outflow (uniform)
         const v4f* pw = reinterpret_cast<const v4f*>(proj_in + (int64_t)slot * kProjFace);
@@ -390,13 +406,13 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   float da[NRM][6][PJ], ta_r[TL ? 1 : NRM][6][PJ], ti[9][PJ];
   auto ta = [&](int rm, int q, int j) -> float& {
     if constexpr (TL)
-      return l_ta[(rm * 6 + q) * NP + tid + j * kLocalThreads];
+      return l_ta[(rm * 6 + q) * NP + pos + j];
     else
       return ta_r[rm][q][j];
   };
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
-    const int p = tid + j * kLocalThreads;
+    const int p = pos + j;
     if (p < NP) {
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++)
@@ -414,7 +430,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
     const cfloat* fsna = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
 #pragma unroll
     for (int j = 0; j < PJ; j++) {
-      const int p = tid + j * kLocalThreads;
+      const int p = pos + j;
 #pragma unroll
       for (int q = 0; q < 9; q++) nq[q][j] = 0.f;
 #pragma unroll
@@ -445,7 +461,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   }
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
-    const int p = tid + j * kLocalThreads;
+    const int p = pos + j;
     if (p < NP) {
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++)
@@ -455,7 +471,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   }
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
-    const int p = tid + j * kLocalThreads;
+    const int p = pos + j;
 #pragma unroll
     for (int q = 0; q < 9; q++) ti[q][j] = p < NP ? dt * l_x[q * NP + p] : 0.f;
   }
@@ -478,7 +494,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
     sca *= dt / float(de + 1);
 #pragma unroll
     for (int j = 0; j < PJ; j++) {
-      const int p = tid + j * kLocalThreads;
+      const int p = pos + j;
       if (p < NP) {
         float y[9], a[6];
         if (p < NV * R) {
@@ -517,7 +533,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   float* gti = t_int + el * SZ;
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
-    const int p = tid + j * kLocalThreads;
+    const int p = pos + j;
     if (p < NP) {
 #pragma unroll
       for (int q = 0; q < 9; q++) {
@@ -548,7 +564,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
     constexpr int TN = tiles(N), TV0 = (TN + 1) / 2;
     if (t_ck == 0) {
       raw_products<O, NV, N, TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, 0, g, c);
-    } else {
+    } else if (t_ck == 1) {
       if constexpr (TN > TV0)
         raw_products<O, NV, N, TN - TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, TV0,
                                          g, c);
@@ -559,7 +575,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   float aq[9][PJ], aa[6][PJ];
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
-    const int p = tid + j * kLocalThreads;
+    const int p = pos + j;
     float y[9], a[6];
     if (p < NP) {
       combine_raw<NP>(l_p, star, star_a, p, y, a);
@@ -588,7 +604,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   float qin[9][PJ], qain[NRM][6][PJ];
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
-    const int p = tid + j * kLocalThreads;
+    const int p = pos + j;
     if (p < NP) {
 #pragma unroll
       for (int q = 0; q < 9; q++) qin[q][j] = gq[q * NP + p];
@@ -601,11 +617,11 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 
   // ---- local surface: waves 0..3 the lifted tiles of face wv in
   // l_p[wv][3][N][R]; wave 4 the neighbours' projection through face 3
-  if (wv < 4) {
+  if (lw < 4) {
     v4f w01, w2;
     face_project<O>(s.dg.fluxLN + wv * N * D::NF, fsf + wv * 54 + 27, l_x, w01, w2, g, c);
     face_lift<O>(s.dg.fluxT + wv * D::NF * N, w01, w2, l_p + wv * 3 * NP, g, c);
-  } else if (wv == 4 && s.proj) {
+  } else if (lw == 4 && s.proj) {
     nb_proj(3);
   }
   __syncthreads();
@@ -613,7 +629,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   // ---- Q += volume + source + surface; QA_rm += w_rm (A - tIntA_rm + surface)
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
-    const int p = tid + j * kLocalThreads;
+    const int p = pos + j;
     if (p < NP) {
       float y[9], ua[6];
 #pragma unroll
