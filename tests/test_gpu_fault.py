@@ -45,14 +45,16 @@ def _problem(order, R, mus=0.677, seed=3, n=4):
     return m, rho, lam, mu, frame, glob, st0
 
 
-CASES = [(64, 1, KERNEL_MFMA64, 1e-12), (64, 1, KERNEL_GENERIC, 1e-12), (64, 1, KERNEL_EC64, 1e-12),
-         (32, 16, KERNEL_MFMA, 1e-5),
-         (32, 16, KERNEL_MFMA_TILES, 1e-5)]
+CASES = [(64, 1, KERNEL_MFMA64, 1e-12, 4), (64, 1, KERNEL_GENERIC, 1e-12, 4), (64, 1, KERNEL_EC64, 1e-12, 4),
+         (32, 16, KERNEL_MFMA, 1e-5, 4), (32, 16, KERNEL_MFMA_TILES, 1e-5, 4),
+         # the MFMA fault phase of FP32 16-run solvers at the other orders
+         # (order 5: 133 KB of LDS per workgroup)
+         (32, 16, KERNEL_MFMA, 1e-5, 2), (32, 16, KERNEL_MFMA, 1e-5, 3), (32, 16, KERNEL_MFMA, 1e-5, 5)]
 
 
-@pytest.mark.parametrize("prec,R,kernel,tol", CASES)
-def test_coupled_step_matches_oracle(gpu, prec, R, kernel, tol):
-    order, n_steps = 4, 3
+@pytest.mark.parametrize("prec,R,kernel,tol,order", CASES)
+def test_coupled_step_matches_oracle(gpu, prec, R, kernel, tol, order):
+    n_steps = 3
     m, rho, lam, mu, frame, glob, st0 = _problem(order, R)
     dt = S.dt_cfl(m, rho, lam, mu, order)
     s = ElasticSolver(m, order, R, prec, rho, lam, mu, kernel=kernel)
