@@ -241,6 +241,7 @@ def test_gpu_plane_wave_convergence(gpu, order):
         _, l2, _ = S.error_norms(m, order, g.cpu().numpy(), lambda x: pw(x, T), n1d=order + 2)
         errs.append(np.sqrt(l2.sum()))
     rate = np.log2(errs[0] / errs[1])
+    print(f"order {order}: L2 errors {errs[0]:.3e} (4^3 hexes), {errs[1]:.3e} (8^3), rate {rate:.2f}")
     assert rate > order - 0.3, (errs, rate)
 
 
