@@ -37,7 +37,8 @@ def _run_oracle(o, dofs, ranges, dt0, cycles):
 
 @pytest.mark.parametrize("prec,order,runs,n_groups,kernel", [(64, 4, 1, 2, KERNEL_AUTO), (64, 3, 1, 3, KERNEL_AUTO),
                                                              (32, 5, 16, 2, KERNEL_AUTO), (32, 4, 16, 3, KERNEL_AUTO),
-                                                             (64, 5, 1, 2, KERNEL_EC64), (64, 4, 1, 3, KERNEL_EC64)])
+                                                             (64, 5, 1, 2, KERNEL_EC64), (64, 4, 1, 3, KERNEL_EC64),
+                                                             (32, 5, 1, 2, KERNEL_AUTO), (32, 3, 1, 3, KERNEL_EC64)])
 def test_lts_matches_oracle(gpu, prec, order, runs, n_groups, kernel):
     import torch
     m = M.box(4, jitter=0.1, seed=17)
