@@ -46,7 +46,7 @@ def _problem(order, R, mus=0.677, seed=3, n=4):
 
 
 CASES = [(64, 1, KERNEL_MFMA64, 1e-12, 4), (64, 1, KERNEL_GENERIC, 1e-12, 4), (64, 1, KERNEL_EC64, 1e-12, 4),
-         (32, 16, KERNEL_MFMA, 1e-5, 4), (32, 16, KERNEL_MFMA_TILES, 1e-5, 4),
+         (32, 16, KERNEL_MFMA, 1e-5, 4), (32, 16, KERNEL_MFMA_TILES, 1e-5, 4), (32, 1, KERNEL_EC64, 1e-5, 4),
          # the MFMA fault phase of FP32 16-run solvers at the other orders
          # (order 5: 133 KB of LDS per workgroup)
          (32, 16, KERNEL_MFMA, 1e-5, 2), (32, 16, KERNEL_MFMA, 1e-5, 3), (32, 16, KERNEL_MFMA, 1e-5, 5)]
