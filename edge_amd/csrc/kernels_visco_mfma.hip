@@ -111,18 +111,118 @@ struct VDims {
   static constexpr int SZ = 9 * NP, SZA = 6 * NP;
 };
 
-// Raw products of one direction: rows k < NK of the tile x (LDS, [9][N][R])
-// with the matrix m ([N][N], rows k, columns n < NN), output row tiles t0 ..
-// t0 + TT - 1, written to out (LDS, [6][N][R]) as P_0, P_1, P_2 (velocity
-// tiles) and V_0, V_1, V_2 (velocity rows of the star-combined tile).
-// Column-half pairs: (P_0 | P_1), (P_2 | V_0), (V_1 | V_2), so product
-// 2 p + h sits in half h of pair p.
+// Velocity rows of the star-combined tile of direction d at one position:
+// z_{d,i} = sum_j S_d[6 + i][kVelCols[i][j]] y[kVelCols[i][j]].
+__device__ __forceinline__ float vel_row(const cfloat* __restrict__ sd, int i, const float (&y)[9]) {
+  return sd[(6 + i) * 9 + kVelCols[i][0]] * y[kVelCols[i][0]] + sd[(6 + i) * 9 + kVelCols[i][1]] * y[kVelCols[i][1]] +
+         sd[(6 + i) * 9 + kVelCols[i][2]] * y[kVelCols[i][2]];
+}
+
+// The B rows of the products at position p of an LDS tile x ([18][N][R]):
+// the velocity rows 6..8 of y, the stress rows 0..5 when `stress`, and the
+// star-combined velocity rows z_{d,i} of the three directions in rows
+// 9 + 3 d + i.  One thread per position forms each z once; formed inside
+// the products, every k-step of both waves of a direction would form them
+// again (four times per position).
+__device__ __forceinline__ void store_rows(float* __restrict__ x, const cfloat* __restrict__ star,
+                                           const float (&y)[9], int np, int p, bool stress) {
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    if (q >= 6 || stress) x[q * np + p] = y[q];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) x[(9 + 3 * d + i) * np + p] = vel_row(star + d * 81, i, y);
+}
+
+// raw_products (below) with the B rows ready-made by store_rows: three LDS
+// reads per k-step and lane, no VALU work.
+template <int O, int NK, int NN, int TT>
+__device__ __forceinline__ void raw_products_zr(const float* __restrict__ m, const float* __restrict__ x,
+                                                float* __restrict__ out, int dir, int t0, int g, int c,
+                                                const float* apre) {
+  using D = VDims<O>;
+  constexpr int N = D::N, NP = D::NP, KS = ksteps(NK);
+  const int h = c >> 3, r = c & 7;
+  // this lane's B rows: (P_0 | P_1) 6 | 7, (P_2 | V_0) 8 | z_0, (V_1 | V_2) z_1 | z_2
+  const float* xb[3] = {x + (h ? 7 : 6) * NP + r, x + (h ? 9 + 3 * dir : 8) * NP + r,
+                        x + (10 + 3 * dir + h) * NP + r};
+  v4f acc[TT][3];
+#pragma unroll
+  for (int t = 0; t < TT; t++)
+#pragma unroll
+    for (int p = 0; p < 3; p++) acc[t][p] = zero4();
+  float aall[apre ? 1 : KS][TT];
+  if (!apre) {
+#pragma unroll
+    for (int s = 0; s < KS; s++)
+#pragma unroll
+      for (int t = 0; t < TT; t++) {
+        const int k = 4 * s + g;
+        aall[s][t] = ldg(m, k * N + 16 * (t0 + t) + c, k < NK && 16 * (t0 + t) + c < NN);
+      }
+  }
+  auto load = [&](int s, float (&a)[TT], float (&b)[3]) {
+    const int k = 4 * s + g;
+    if (apre) {
+      a[0] = apre[s];
+    } else {
+#pragma unroll
+      for (int t = 0; t < TT; t++) a[t] = aall[s][t];
+    }
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      if (4 * s + 3 < NK)  // constant after unrolling
+        b[p] = xb[p][k * R];
+      else
+        b[p] = k < NK ? xb[p][(k < NK ? k : 0) * R] : 0.f;
+    }
+  };
+  float an[TT], bn[3];
+  load(0, an, bn);
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    float a[TT], b[3];
+#pragma unroll
+    for (int t = 0; t < TT; t++) a[t] = an[t];
+#pragma unroll
+    for (int p = 0; p < 3; p++) b[p] = bn[p];
+    if (s + 1 < KS) load(s + 1, an, bn);
+    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+#pragma unroll
+    for (int t = 0; t < TT; t++)
+#pragma unroll
+      for (int p = 0; p < 3; p++) acc[t][p] = mfma(a[t], b[p], acc[t][p]);
+  }
+#pragma unroll
+  for (int t = 0; t < TT; t++)
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int n = 16 * (t0 + t) + 4 * g + v;
+        if (n < NN) out[(2 * p + h) * NP + n * R + r] = acc[t][p][v];
+      }
+}
+
+// Raw products of one direction d: rows k < NK of the tile x (LDS,
+// [9][N][R], and with ZR the star-combined rows of store_rows) with the
+// matrix m ([N][N], rows k, columns n < NN), output row tiles t0 .. t0 +
+// TT - 1, written to out (LDS, [6][N][R]) as P_0, P_1, P_2 (velocity tiles)
+// and V_0, V_1, V_2 (velocity rows of the star-combined tile).  Column-half
+// pairs: (P_0 | P_1), (P_2 | V_0), (V_1 | V_2), so product 2 p + h sits in
+// half h of pair p.  Without ZR each lane combines the star rows from the
+// nine quantities per k-step.
 // apre (optional): the A operands of every k-step of one tile (TT = 1),
 // loaded by the caller ahead of time.
-template <int O, int NK, int NN, int TT>
+template <int O, int NK, int NN, int TT, bool ZR>
 __device__ __forceinline__ void raw_products(const float* __restrict__ m, const cfloat* __restrict__ sd,
-                                             const float* __restrict__ x, float* __restrict__ out, int t0, int g,
-                                             int c, const float* apre = nullptr) {
+                                             const float* __restrict__ x, float* __restrict__ out, int dir, int t0,
+                                             int g, int c, const float* apre = nullptr) {
+  if constexpr (ZR) {
+    raw_products_zr<O, NK, NN, TT>(m, x, out, dir, t0, g, c, apre);
+    return;
+  }
   using D = VDims<O>;
   constexpr int N = D::N, NP = D::NP, KS = ksteps(NK);
   const int h = c >> 3, r = c & 7;
@@ -360,7 +460,10 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   // issue the phase's instructions than with the positions spread out)
   constexpr int PJ = 1;
   static_assert(NP <= kLocalThreads, "one position per thread");
-  __shared__ __align__(16) float l_x[SZ];      // Q, then der_{de-1}, then tInt
+  // ZR: the products' star-combined B rows formed per position (store_rows);
+  // the occupancy variant keeps the smaller tile (three workgroups per CU)
+  constexpr bool ZR = !TL;
+  __shared__ __align__(16) float l_x[ZR ? 2 * SZ : SZ];  // Q, then der_{de-1}, then tInt (+ z rows)
   __shared__ __align__(16) float l_p[18 * NP];  // raw products [3][6][N][R]; lifted tiles [4][3][N][R]
   __shared__ __align__(16) float l_ta[TL ? NRM * 6 * NP : 1];  // tIntA [rm][6][N][R] (TL)
   const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
@@ -387,7 +490,16 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   const cfloat* fsf = as_const(s.fsf + el * 4 * 54);
   const cfloat* fsfa = as_const(s.fsfa + el * 4 * 18);
 
-  {
+  // Q: per position into registers (ZR), or copied to LDS as a whole
+  float qv[ZR ? 9 : 1][PJ];
+  if constexpr (ZR) {
+#pragma unroll
+    for (int j = 0; j < PJ; j++) {
+      const int p = pos + j;
+#pragma unroll
+      for (int q = 0; q < 9; q++) qv[q][j] = p < NP ? q0[q * NP + p] : 0.f;
+    }
+  } else {
     const float4* gq = reinterpret_cast<const float4*>(q0);
     float4* lq = reinterpret_cast<float4*>(l_x);
     for (int i = tid; i < SZ / 4; i += kLocalThreads) lq[i] = gq[i];
@@ -420,7 +532,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
         for (int q = 0; q < 6; q++) da[rm][q][j] = qa0[(rm * 6 + q) * NP + p];
     }
   }
-  __syncthreads();
+  if constexpr (NB || !ZR) __syncthreads();  // the lifted tiles, or Q in LDS
   // NB: Q += sum_f U_f L_f, QA_rm += w_rm sum_f U_A,f L_f (the neighbour-side
   // factors); nq, nqa are added again to the DOFs the output reads
   float nq[NB ? 9 : 1][PJ], nqa[NB ? 6 : 1][PJ];
@@ -448,7 +560,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
           for (int q = 0; q < 6; q++) nqa[q][j] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
         }
 #pragma unroll
-        for (int q = 0; q < 9; q++) l_x[q * NP + p] += nq[q][j];
+        for (int q = 0; q < 9; q++) qv[q][j] += nq[q][j];
 #pragma unroll
         for (int rm = 0; rm < NRM; rm++) {
           const float w = rfs[rm];
@@ -457,7 +569,6 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
         }
       }
     }
-    __syncthreads();  // Q complete (and the lifted tiles read) before the CK
   }
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
@@ -472,9 +583,20 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 #pragma unroll
   for (int j = 0; j < PJ; j++) {
     const int p = pos + j;
+    if constexpr (ZR) {
+      float y[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) ti[q][j] = p < NP ? dt * l_x[q * NP + p] : 0.f;
+      for (int q = 0; q < 9; q++) {
+        y[q] = qv[q][j];
+        ti[q][j] = dt * y[q];
+      }
+      if (p < NP) store_rows(l_x, star, y, NP, p, false);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) ti[q][j] = p < NP ? dt * l_x[q * NP + p] : 0.f;
+    }
   }
+  if constexpr (ZR) __syncthreads();  // the CK's first B rows (and, NB, the lifted tiles read)
 
   // ---- CK; the wave's stiffness A operands (one row tile, every k-step)
   // are the same in every stage
@@ -488,8 +610,8 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 #pragma unroll 1
   for (int de = 1; de < O; de++) {
     if (t_ck < tiles(NV))
-      raw_products<O, N, NV, 1>(s.dg.stiffTn + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, t_ck, g, c,
-                                a_ck);
+      raw_products<O, N, NV, 1, ZR>(s.dg.stiffTn + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, dir,
+                                    t_ck, g, c, a_ck);
     __syncthreads();
     sca *= dt / float(de + 1);
 #pragma unroll
@@ -520,9 +642,12 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
           }
         }
 #pragma unroll
-        for (int q = 0; q < 9; q++) {
-          l_x[q * NP + p] = y[q];
-          ti[q][j] += sca * y[q];
+        for (int q = 0; q < 9; q++) ti[q][j] += sca * y[q];
+        if constexpr (ZR) {
+          store_rows(l_x, star, y, NP, p, false);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 9; q++) l_x[q * NP + p] = y[q];
         }
       }
     }
@@ -535,11 +660,14 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   for (int j = 0; j < PJ; j++) {
     const int p = pos + j;
     if (p < NP) {
+      float y[9];
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        l_x[q * NP + p] = ti[q][j];
-        gti[q * NP + p] = ti[q][j];
+        y[q] = ti[q][j];
+        l_x[q * NP + p] = y[q];
+        gti[q * NP + p] = y[q];
       }
+      if constexpr (ZR) store_rows(l_x, star, y, NP, p, false);
     }
   }
   __syncthreads();
@@ -563,11 +691,12 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
     // output tiles: the first half of the waves [0, TV0), the second [TV0, tiles(N))
     constexpr int TN = tiles(N), TV0 = (TN + 1) / 2;
     if (t_ck == 0) {
-      raw_products<O, NV, N, TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, 0, g, c);
+      raw_products<O, NV, N, TV0, ZR>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, dir, 0,
+                                      g, c);
     } else if (t_ck == 1) {
       if constexpr (TN > TV0)
-        raw_products<O, NV, N, TN - TV0>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP, TV0,
-                                         g, c);
+        raw_products<O, NV, N, TN - TV0, ZR>(s.dg.stiffV + dir * N * N, star + dir * 81, l_x, l_p + dir * 6 * NP,
+                                             dir, TV0, g, c);
       if (s.proj) nb_proj(dir);
     }
   }
