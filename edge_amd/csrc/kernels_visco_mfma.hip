@@ -15,9 +15,9 @@
 // workgroup, with the anelastic state in registers, and the raw products
 // move between the MFMA and the elementwise phases through LDS.
 //
-// Local step, six waves (one position per thread at O5); wave w takes
-// direction w % 3 and half w / 3 of the output row tiles in the CK and volume
-// products, and waves 0..3 a face each in the surface integral
+// Local step, eight waves (one position per thread at O5); six of them take
+// a direction and a half of the output row tiles each in the CK and volume
+// products, and four a face each in the surface integral
 // (src/impl/seismic/solvers/AderDg.hpp:362-533):
 //  CK (kernels/TimePredVanilla.hpp:200-306; with attenuation every
 //  derivative keeps all N modes): per stage de the raw products of
@@ -39,8 +39,8 @@
 // Neighbour step (kernels/SurfIntVanilla.hpp:274-357), four waves, wave f
 // lifts face f's projection of the neighbour's tInt (F^N and the
 // neighbour-side factors): the projection the neighbour's local step wrote
-// (waves 4 and 5 of the local kernel compute them during the surface
-// integral), or one made here from the neighbour's tInt tile; the lifted
+// (the local kernel's waves with spare issue make them beside the volume and
+// surface products), or one made here from the neighbour's tInt tile; the lifted
 // tiles of the four faces are combined per position.
 // Summation order differs from the reference's loops; results agree to
 // rounding.
@@ -449,15 +449,18 @@ __device__ __forceinline__ void add_source(const cfloat* __restrict__ e, const f
 // update Q and QA before the CK, and the output adds them to the DOFs read
 // at the start, so the neighbour step of step k - 1 and the local step of
 // step k run as one launch that reads and writes the DOFs once.
+// Registers: at most 128 VGPRs (four waves per SIMD), so that two eight-wave
+// workgroups share a CU; at 129 only one fits and the launch takes 1.5x.
 template <int O, int NRM, bool TL, bool NB = false>
-__global__ __launch_bounds__(kLocalThreads) __attribute__((amdgpu_waves_per_eu(TL ? 5 : (NB ? 4 : 1))))
+__global__ __launch_bounds__(kLocalThreads) __attribute__((amdgpu_waves_per_eu(TL ? 5 : 4)))
 void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* __restrict__ dofs,
                 float* __restrict__ dofs_a, float* __restrict__ t_int, const float* __restrict__ proj_in) {
   static_assert(!(NB && TL), "the fused step keeps tIntA in registers");
   using D = VDims<O>;
   constexpr int N = D::N, NV = D::NV, NP = D::NP, SZ = D::SZ, SZA = D::SZA;
-  // elementwise phases: (mode, run) position tid (compact: fewer waves
-  // issue the phase's instructions than with the positions spread out)
+  // elementwise phases: one (mode, run) position per thread, compact in the
+  // first waves (fewer waves issue the phase's instructions than with the
+  // positions spread out)
   constexpr int PJ = 1;
   static_assert(NP <= kLocalThreads, "one position per thread");
   // ZR: the products' star-combined B rows formed per position (store_rows);
@@ -471,7 +474,6 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   const int64_t el = first + off;
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c = lane & 15;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
-  const int pos = tid;  // this thread's position (none from NP on)
   // The six product tasks (three directions x two halves of the row tiles)
   // run on logical waves 0..5: waves 0..5 in even workgroups and waves
   // 0..3, 6, 7 in odd ones (by blockIdx / 256, which alternates between the
@@ -480,6 +482,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   // 6.. have no task.
   const int odd = (int)((blockIdx.x >> 8) & 1);
   const int lw = (odd && wv >= 4) ? (wv >= 6 ? wv - 2 : wv + 4) : wv;
+  const int pos = tid;  // this thread's position (none from NP on)
   const int dir = lw % 3, t_ck = lw / 3;  // CK / volume products: direction, half of the row tiles
   const cfloat* star = as_const(s.star + el * 243);
   const cfloat* star_a = as_const(s.star_a + el * 54);
