@@ -26,6 +26,12 @@ namespace edge {
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+// Per-element tables that no kernel writes (star matrices, flux solvers,
+// source matrices) are read through the constant address space: wave-uniform
+// values by scalar loads, which the scalar cache serves with a short latency
+// and which land in scalar registers, not in vector ones.
+typedef const __attribute__((address_space(4))) double cdouble;
+__device__ __forceinline__ const cdouble* as_const(const double* p) { return (const cdouble*)p; }
 
 __device__ __forceinline__ v4d mfma64(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -33,6 +39,15 @@ __device__ __forceinline__ v4d mfma64(double a, double b, v4d c) {
 
 constexpr int tiles16(int n) { return (n + 15) / 16; }
 constexpr int ksteps4(int n) { return (n + 3) / 4; }
+
+// LDS row stride of an A-operand tile whose k-steps cover P4(n) columns.
+// Lanes (g, c) of an MFMA read row c, column 4 s + g; with a stride that is
+// a multiple of 4 doubles, rows c and c + 8 (c + 4, ...) fall on the same
+// banks, so the tiles are padded past P4(n).
+#ifndef EDGE_LDS_PAD
+#define EDGE_LDS_PAD 1
+#endif
+constexpr int lds_ld(int n) { return PadLayout::p4(n) + EDGE_LDS_PAD; }
 
 __device__ __forceinline__ v4d zero4d() { return v4d{0.0, 0.0, 0.0, 0.0}; }
 
@@ -168,7 +183,7 @@ __device__ __forceinline__ void mixf4(const double* __restrict__ f, double* __re
 // lane per k): the elastic star's non-zeros (SP) or all 81 entries; S
 // wave-uniform; X and Y leading dimensions at least P4(NK)
 template <bool SP, int NK>
-__device__ __forceinline__ void mixp(const double* __restrict__ sd, const double* __restrict__ x, int ldx,
+__device__ __forceinline__ void mixp(const cdouble* __restrict__ sd, const double* __restrict__ x, int ldx,
                                      double* __restrict__ y, int ldy, int lane) {
   for (int k = lane; k < PadLayout::p4(NK); k += 64) {
     const bool in = k < NK;
@@ -211,7 +226,7 @@ __device__ __forceinline__ void store_der(double* __restrict__ x, v4d (&ti)[NTN]
 // The directions' A operands alternate between two LDS buffers, so that one
 // direction's mix runs while the previous direction's products are in flight.
 template <int O, int DE, bool SP, int NTN>
-__device__ __forceinline__ void ck_stage(const double* __restrict__ stiff, const double* __restrict__ star,
+__device__ __forceinline__ void ck_stage(const double* __restrict__ stiff, const cdouble* __restrict__ star,
                                          double dt, double sca, double* l_der, double* l_a0, double* l_a1,
                                          v4d (&ti)[NTN], int lane, int g, int c) {
   if constexpr (DE < O) {
@@ -254,7 +269,7 @@ void k_local_p64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, do
   if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-  const double* star = s.star + el * 243;
+  const cdouble* star = as_const(s.star + el * 243);
   const double* pad = s.dg.pad;
   {
     constexpr int IT = (SZ + 63) / 64;
@@ -366,7 +381,7 @@ __global__ __launch_bounds__(64) void k_neigh_p64(SolverDev<double> s, int64_t f
     mmg<N, LQF>(t_int + (int64_t)ne * SZ, pad + P::flux_ln(N) + fm * LD * LQF, s0[0], g, c);
     store_tiles<1>(l_s0, LDF, NF, s0, g, c);
     __syncthreads();
-    mixp<false, NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, l_s0, LDF, l_s1, LDF, lane);
+    mixp<false, NF>(as_const(s.fs + ((s.n_el + el) * 4 + fa) * 81), l_s0, LDF, l_s1, LDF, lane);
     __syncthreads();
     mmp<NT, ksteps4(NF), LDF, LQ>(l_s1, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
     __syncthreads();
@@ -400,7 +415,7 @@ __global__ __launch_bounds__(64) void k_neigh_p64(SolverDev<double> s, int64_t f
 // Y rows 0..8 = S X (elastic star pattern or all 81 entries), rows 9..14 =
 // S^A X_vel, for columns k < NK, and 0 for NK <= k < P4(NK) (one lane per k)
 template <bool SP, int NK>
-__device__ __forceinline__ void mix15(const double* __restrict__ sd, const double* __restrict__ sa,
+__device__ __forceinline__ void mix15(const cdouble* __restrict__ sd, const cdouble* __restrict__ sa,
                                       const double* __restrict__ x, int ldx, double* __restrict__ y, int ldy,
                                       int lane) {
   for (int k = lane; k < PadLayout::p4(NK); k += 64) {
@@ -429,7 +444,6 @@ __device__ __forceinline__ void mix15(const double* __restrict__ sd, const doubl
 // anelastic part F_A (6 x 9), columns k < NK, and 0 for NK <= k < P4(NK)
 // (coefficients row by row through scalar loads: all 135 held at once
 // would not fit the registers)
-typedef const __attribute__((address_space(4))) double cdouble;
 template <int NK>
 __device__ __forceinline__ void mixf15(const double* __restrict__ f, const double* __restrict__ fa,
                                        const double* __restrict__ x, int ldx, double* __restrict__ y, int ldy,
@@ -461,7 +475,7 @@ __device__ __forceinline__ int opaque_u(int x) {
 // y[0..5] += E x for an anelastic source matrix E (6 x 6) of the source
 // pattern (src/impl/seismic/setups/ViscoElasticity.h:379-458): the
 // normal-stress block rows 0..2 x columns 0..2 and the shear diagonal
-__device__ __forceinline__ void add_src(const double* __restrict__ e, const double (&x)[6], double (&y)[9]) {
+__device__ __forceinline__ void add_src(const cdouble* __restrict__ e, const double (&x)[6], double (&y)[9]) {
 #pragma unroll
   for (int q = 0; q < 3; q++) y[q] += e[q * 6] * x[0] + e[q * 6 + 1] * x[1] + e[q * 6 + 2] * x[2];
 #pragma unroll
@@ -477,20 +491,21 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ, SZA = 6 * N;
   constexpr int NV = n_modes(O - 1), TV = tiles16(NV);
   constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
+  constexpr int SL = lds_ld(N), SLF = lds_ld(NF);  // LDS row strides
   static_assert(N <= 64, "one lane per mode");
   // l_x: der (rows 0..8) and the anelastic combination (rows 9..14); l_a: A
   // operands; l_ta: tIntA [rm][6][mode]
   // l_x and l_a also hold two faces' projections and mixed tiles
-  constexpr int LXX = 15 * LD > 2 * 9 * LDF ? 15 * LD : 2 * 9 * LDF;
-  constexpr int LXA = 15 * LD > 2 * 15 * LDF ? 15 * LD : 2 * 15 * LDF;
-  __shared__ double l_ti[9 * LD], l_x[LXX], l_a[LXA], l_ta[NRM * 6 * N];
+  constexpr int LXX = 15 * SL > 2 * 9 * SLF ? 15 * SL : 2 * 9 * SLF;
+  constexpr int LXA = 15 * SL > 2 * 15 * SLF ? 15 * SL : 2 * 15 * SLF;
+  __shared__ double l_ti[9 * SL], l_x[LXX], l_a[LXA], l_ta[NRM * 6 * N];
   const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
   if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
   const int64_t el = first + off;
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-  const double* star = s.star + el * 243;
-  const double* star_a = s.star_a + el * 54;
-  const double* src = s.src_a + el * NRM * 36;
+  const cdouble* star = as_const(s.star + el * 243);
+  const cdouble* star_a = as_const(s.star_a + el * 54);
+  const cdouble* src = as_const(s.src_a + el * NRM * 36);
   const double* pad = s.dg.pad;
   double* qa = dofs_a + el * NRM * SZA;
   {
@@ -502,13 +517,13 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
     for (int j = 0; j < IT; j++) {
       const int i = lane + 64 * j, q = i / N, n = i - q * N;
       if (i < SZ) {
-        l_x[q * LD + n] = v[j];
-        l_ti[q * LD + n] = dt * v[j];
+        l_x[q * SL + n] = v[j];
+        l_ti[q * SL + n] = dt * v[j];
       }
     }
     // tInt's k-step tail: the face projections read it as an A operand
     if constexpr (LD > N)
-      for (int i = lane; i < 9 * (LD - N); i += 64) l_ti[(i / (LD - N)) * LD + N + i % (LD - N)] = 0.0;
+      for (int i = lane; i < 9 * (LD - N); i += 64) l_ti[(i / (LD - N)) * SL + N + i % (LD - N)] = 0.0;
   }
   // anelastic state of mode n = lane: derA in registers, tIntA in LDS
   double da[NRM][6];
@@ -530,25 +545,25 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
     for (int t = 0; t < TV; t++) acc[t] = zero4d();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      mix15<SP, N>(star + opaque_u(d * 81), star_a + opaque_u(d * 18), l_x, LD, l_a, LD, lane);
+      mix15<SP, N>(star + opaque_u(d * 81), star_a + opaque_u(d * 18), l_x, SL, l_a, SL, lane);
       __syncthreads();
-      mmp<TV, ksteps4(N), LD, LQ, 15>(l_a, pad + d * LD * LQ, acc, g, c);
+      mmp<TV, ksteps4(N), SL, LQ, 15>(l_a, pad + d * LD * LQ, acc, g, c);
       __syncthreads();
     }
-    store_tiles<TV, 15>(l_x, LD, NV, acc, g, c);
+    store_tiles<TV, 15>(l_x, SL, NV, acc, g, c);
     __syncthreads();
     sca *= dt / double(de + 1);
     if (lane < N) {
       const int n = lane;
       double y[9], a[6];
 #pragma unroll
-      for (int q = 0; q < 9; q++) y[q] = n < NV ? l_x[q * LD + n] : 0.0;
+      for (int q = 0; q < 9; q++) y[q] = n < NV ? l_x[q * SL + n] : 0.0;
 #pragma unroll
-      for (int j = 0; j < 6; j++) a[j] = n < NV ? l_x[(9 + j) * LD + n] : 0.0;
+      for (int j = 0; j < 6; j++) a[j] = n < NV ? l_x[(9 + j) * SL + n] : 0.0;
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++) {
         add_src(src + opaque_u(rm * 36), da[rm], y);
-        const double w = s.rfs[rm];
+        const double w = as_const(s.rfs)[rm];
 #pragma unroll
         for (int j = 0; j < 6; j++) {
           da[rm][j] = w * (a[j] + da[rm][j]);
@@ -557,13 +572,13 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        l_x[q * LD + n] = y[q];
-        l_ti[q * LD + n] += sca * y[q];
+        l_x[q * SL + n] = y[q];
+        l_ti[q * SL + n] += sca * y[q];
       }
     }
     __syncthreads();
   }
-  for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[(i / N) * LD + i % N];
+  for (int i = lane; i < SZ; i += 64) t_int[el * SZ + i] = l_ti[(i / N) * SL + i % N];
 
   // ---- volume: rows 0..8 from Q, rows 9..14 the anelastic combination A
   v4d acc[NT];
@@ -575,9 +590,9 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
       acc[t][v] = ld(dofs + el * SZ, q * N + n, q < 9 && n < N);
     }
   for (int d = 0; d < 3; d++) {
-    mix15<SP, NV>(star + d * 81, star_a + d * 18, l_ti, LD, l_a, LD, lane);
+    mix15<SP, NV>(star + d * 81, star_a + d * 18, l_ti, SL, l_a, SL, lane);
     __syncthreads();
-    mmp<NT, ksteps4(NV), LD, LQ, 15>(l_a, pad + P::stiff_v(N) + d * LD * LQ, acc, g, c);
+    mmp<NT, ksteps4(NV), SL, LQ, 15>(l_a, pad + P::stiff_v(N) + d * LD * LQ, acc, g, c);
     __syncthreads();
   }
   // sum_rm E_rm tIntA_rm into the stress rows (through LDS)
@@ -591,7 +606,7 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
       add_src(src + rm * 36, ta, y);
     }
 #pragma unroll
-    for (int j = 0; j < 6; j++) l_x[j * LD + lane] = y[j];
+    for (int j = 0; j < 6; j++) l_x[j * SL + lane] = y[j];
   }
   __syncthreads();
 #pragma unroll
@@ -599,7 +614,7 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int q = g + 4 * v, n = 16 * t + c;
-      if (q < 6 && n < N) acc[t][v] += l_x[q * LD + n];
+      if (q < 6 && n < N) acc[t][v] += l_x[q * SL + n];
     }
   __syncthreads();
   // ---- local surface
@@ -610,15 +625,15 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
 #pragma unroll 1
   for (int fp = 0; fp < 4; fp += 2) {
     v4d s0[2] = {zero4d(), zero4d()};
-    mmp<2, ksteps4(N), LD, LQF, 9, LD * LQF>(l_ti, pad + P::flux_ln(N) + fp * LD * LQF, s0, g, c);
-    store_tile(l_x, LDF, NF, s0[0], g, c);
-    store_tile(l_x + 9 * LDF, LDF, NF, s0[1], g, c);
+    mmp<2, ksteps4(N), SL, LQF, 9, LD * LQF>(l_ti, pad + P::flux_ln(N) + fp * LD * LQF, s0, g, c);
+    store_tile(l_x, SLF, NF, s0[0], g, c);
+    store_tile(l_x + 9 * SLF, SLF, NF, s0[1], g, c);
     __syncthreads();
-    mixf15<NF>(fs0 + fp * 81, fsa0 + fp * 54, l_x, LDF, l_a, LDF, lane);
-    mixf15<NF>(fs0 + (fp + 1) * 81, fsa0 + (fp + 1) * 54, l_x + 9 * LDF, LDF, l_a + 15 * LDF, LDF, lane);
+    mixf15<NF>(fs0 + fp * 81, fsa0 + fp * 54, l_x, SLF, l_a, SLF, lane);
+    mixf15<NF>(fs0 + (fp + 1) * 81, fsa0 + (fp + 1) * 54, l_x + 9 * SLF, SLF, l_a + 15 * SLF, SLF, lane);
     __syncthreads();
-    mmp<NT, ksteps4(NF), LDF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fp * LDF * LQ, acc, g, c);
-    mmp<NT, ksteps4(NF), LDF, LQ, 15>(l_a + 15 * LDF, pad + P::flux_t(N, NF) + (fp + 1) * LDF * LQ, acc, g, c);
+    mmp<NT, ksteps4(NF), SLF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fp * LDF * LQ, acc, g, c);
+    mmp<NT, ksteps4(NF), SLF, LQ, 15>(l_a + 15 * SLF, pad + P::flux_t(N, NF) + (fp + 1) * LDF * LQ, acc, g, c);
     __syncthreads();
   }
   // ---- DOFs out; rows 9..14 (A + upA) through LDS to the anelastic update
@@ -629,15 +644,15 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
       const int q = g + 4 * v, n = 16 * t + c;
       if (q < 9 && n < N) dofs[el * SZ + q * N + n] = acc[t][v];
     }
-  store_tiles<NT, 15, 9>(l_x, LD, N, acc, g, c);
+  store_tiles<NT, 15, 9>(l_x, SL, N, acc, g, c);
   __syncthreads();
   if (lane < N) {
 #pragma unroll
     for (int rm = 0; rm < NRM; rm++) {
-      const double w = s.rfs[rm];
+      const double w = as_const(s.rfs)[rm];
 #pragma unroll
       for (int j = 0; j < 6; j++)
-        qa[(rm * 6 + j) * N + lane] += w * (l_x[(9 + j) * LD + lane] - l_ta[(rm * 6 + j) * N + lane]);
+        qa[(rm * 6 + j) * N + lane] += w * (l_x[(9 + j) * SL + lane] - l_ta[(rm * 6 + j) * N + lane]);
     }
   }
 }
@@ -650,8 +665,9 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
   using P = PadLayout;
   constexpr int N = D::N, NF = D::NF, NT = D::NT, SZ = D::SZ, SZA = 6 * N;
   constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
-  static_assert(LDF <= N, "l_a holds the [15][LDF] operand");
-  __shared__ double l_s0[9 * LDF], l_a[15 * N];
+  constexpr int SLF = lds_ld(NF);  // LDS row stride of the face tiles
+  // l_a: the mixed face operand [15][SLF], then the output rows [15][N]
+  __shared__ double l_s0[9 * SLF], l_a[15 * (SLF > N ? SLF : N)];
   const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
   if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
   const int64_t el = first + off;
@@ -671,12 +687,12 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
     const int fm = s.nb.fmat[el * 4 + fa];
     v4d s0[1] = {zero4d()};
     mmg<N, LQF>(t_int + (int64_t)ne * SZ, pad + P::flux_ln(N) + fm * LD * LQF, s0[0], g, c);
-    store_tiles<1>(l_s0, LDF, NF, s0, g, c);
+    store_tiles<1>(l_s0, SLF, NF, s0, g, c);
     __syncthreads();
-    mixf15<NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, s.fs_a + ((s.n_el + el) * 4 + fa) * 54, l_s0, LDF, l_a,
-               LDF, lane);
+    mixf15<NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, s.fs_a + ((s.n_el + el) * 4 + fa) * 54, l_s0, SLF, l_a,
+               SLF, lane);
     __syncthreads();
-    mmp<NT, ksteps4(NF), LDF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
+    mmp<NT, ksteps4(NF), SLF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
     __syncthreads();
   }
 #pragma unroll
@@ -692,7 +708,7 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
     double* qa = dofs_a + el * NRM * SZA;
 #pragma unroll
     for (int rm = 0; rm < NRM; rm++) {
-      const double w = s.rfs[rm];
+      const double w = as_const(s.rfs)[rm];
 #pragma unroll
       for (int j = 0; j < 6; j++) qa[(rm * 6 + j) * N + lane] += w * l_a[(9 + j) * N + lane];
     }
