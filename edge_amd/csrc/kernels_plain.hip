@@ -130,27 +130,51 @@ __device__ __forceinline__ void mmp(const double* __restrict__ a, const double* 
   }
 }
 
+// The B operands of mmp for every k-step at once, bq[s][t] = B_t[4 s + g][c]:
+// requested before the A tile's producer runs, their latency (the shared
+// matrices come from L2) overlaps it instead of stalling each k-step.
+template <int NT, int KS, int LDB, int TS = 16>
+__device__ __forceinline__ void load_b(const double* __restrict__ b, double (&bq)[KS][NT], int g, int c) {
+  const double* bp = b + g * LDB + c;
+#pragma unroll
+  for (int s = 0; s < KS; s++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) bq[s][t] = bp[4 * s * LDB + TS * t];
+}
+
+// mmp with the B operands of load_b
+template <int NT, int KS, int LDA, int MR = 9>
+__device__ __forceinline__ void mmq(const double* __restrict__ a, const double (&bq)[KS][NT], v4d (&acc)[NT], int g,
+                                    int c) {
+  const double* ap = a + (c < MR ? c : MR - 1) * LDA + g;
+  double an = ap[0];
+#pragma unroll
+  for (int s = 0; s < KS; s++) {
+    const double av = an;
+    if (s + 1 < KS) an = ap[4 * (s + 1)];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = mfma64(av, bq[s][t], acc[t]);
+  }
+}
+
 // acc += A B for A [9][NK] in global memory (the reference layout, leading
-// dimension NK) and B [P4(NK)][16] (leading dimension LDB, padded copy); only
-// the last k-step's A load is guarded
+// dimension NK) and B [P4(NK)][16] (leading dimension LDB, padded copy); every
+// k-step's operands requested at once (one memory round trip); only the last
+// k-step's A load is guarded
 template <int NK, int LDB>
 __device__ __forceinline__ void mmg(const double* __restrict__ a, const double* __restrict__ b, v4d& acc, int g,
                                     int c) {
   constexpr int KS = ksteps4(NK);
   const double* ap = a + (c < 9 ? c : 8) * NK + g;
   const double* bp = b + g * LDB + c;
-  double an, bn;
-  auto load = [&](int s) {
-    an = (4 * s + 3 < NK) ? ap[4 * s] : ld(ap, 4 * s, 4 * s + g < NK);
-    bn = bp[4 * s * LDB];
-  };
-  load(0);
+  double aq[KS], bq[KS];
 #pragma unroll
   for (int s = 0; s < KS; s++) {
-    const double av = an, bv = bn;
-    if (s + 1 < KS) load(s + 1);
-    acc = mfma64(av, bv, acc);
+    aq[s] = (4 * s + 3 < NK) ? ap[4 * s] : ld(ap, 4 * s, 4 * s + g < NK);
+    bq[s] = bp[4 * s * LDB];
   }
+#pragma unroll
+  for (int s = 0; s < KS; s++) acc = mfma64(aq[s], bq[s], acc);
 }
 
 // Y_f = F_f X_f in place for the four faces' tiles X_f [9][P4(NF)] at
@@ -545,9 +569,11 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
     for (int t = 0; t < TV; t++) acc[t] = zero4d();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
+      double bq[ksteps4(N)][TV];
+      load_b<TV, ksteps4(N), LQ>(pad + d * LD * LQ, bq, g, c);
       mix15<SP, N>(star + opaque_u(d * 81), star_a + opaque_u(d * 18), l_x, SL, l_a, SL, lane);
       __syncthreads();
-      mmp<TV, ksteps4(N), SL, LQ, 15>(l_a, pad + d * LD * LQ, acc, g, c);
+      mmq<TV, ksteps4(N), SL, 15>(l_a, bq, acc, g, c);
       __syncthreads();
     }
     store_tiles<TV, 15>(l_x, SL, NV, acc, g, c);
@@ -590,9 +616,11 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
       acc[t][v] = ld(dofs + el * SZ, q * N + n, q < 9 && n < N);
     }
   for (int d = 0; d < 3; d++) {
+    double bq[ksteps4(NV)][NT];
+    load_b<NT, ksteps4(NV), LQ>(pad + P::stiff_v(N) + d * LD * LQ, bq, g, c);
     mix15<SP, NV>(star + d * 81, star_a + d * 18, l_ti, SL, l_a, SL, lane);
     __syncthreads();
-    mmp<NT, ksteps4(NV), SL, LQ, 15>(l_a, pad + P::stiff_v(N) + d * LD * LQ, acc, g, c);
+    mmq<NT, ksteps4(NV), SL, 15>(l_a, bq, acc, g, c);
     __syncthreads();
   }
   // sum_rm E_rm tIntA_rm into the stress rows (through LDS)
@@ -626,14 +654,17 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
   for (int fp = 0; fp < 4; fp += 2) {
     v4d s0[2] = {zero4d(), zero4d()};
     mmp<2, ksteps4(N), SL, LQF, 9, LD * LQF>(l_ti, pad + P::flux_ln(N) + fp * LD * LQF, s0, g, c);
+    double bq[2][ksteps4(NF)][NT];
+#pragma unroll
+    for (int f = 0; f < 2; f++) load_b<NT, ksteps4(NF), LQ>(pad + P::flux_t(N, NF) + (fp + f) * LDF * LQ, bq[f], g, c);
     store_tile(l_x, SLF, NF, s0[0], g, c);
     store_tile(l_x + 9 * SLF, SLF, NF, s0[1], g, c);
     __syncthreads();
     mixf15<NF>(fs0 + fp * 81, fsa0 + fp * 54, l_x, SLF, l_a, SLF, lane);
     mixf15<NF>(fs0 + (fp + 1) * 81, fsa0 + (fp + 1) * 54, l_x + 9 * SLF, SLF, l_a + 15 * SLF, SLF, lane);
     __syncthreads();
-    mmp<NT, ksteps4(NF), SLF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fp * LDF * LQ, acc, g, c);
-    mmp<NT, ksteps4(NF), SLF, LQ, 15>(l_a + 15 * SLF, pad + P::flux_t(N, NF) + (fp + 1) * LDF * LQ, acc, g, c);
+    mmq<NT, ksteps4(NF), SLF, 15>(l_a, bq[0], acc, g, c);
+    mmq<NT, ksteps4(NF), SLF, 15>(l_a + 15 * SLF, bq[1], acc, g, c);
     __syncthreads();
   }
   // ---- DOFs out; rows 9..14 (A + upA) through LDS to the anelastic update
@@ -687,12 +718,14 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
     const int fm = s.nb.fmat[el * 4 + fa];
     v4d s0[1] = {zero4d()};
     mmg<N, LQF>(t_int + (int64_t)ne * SZ, pad + P::flux_ln(N) + fm * LD * LQF, s0[0], g, c);
+    double bq[ksteps4(NF)][NT];
+    load_b<NT, ksteps4(NF), LQ>(pad + P::flux_t(N, NF) + fa * LDF * LQ, bq, g, c);
     store_tiles<1>(l_s0, SLF, NF, s0, g, c);
     __syncthreads();
     mixf15<NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, s.fs_a + ((s.n_el + el) * 4 + fa) * 54, l_s0, SLF, l_a,
                SLF, lane);
     __syncthreads();
-    mmp<NT, ksteps4(NF), SLF, LQ, 15>(l_a, pad + P::flux_t(N, NF) + fa * LDF * LQ, acc, g, c);
+    mmq<NT, ksteps4(NF), SLF, 15>(l_a, bq, acc, g, c);
     __syncthreads();
   }
 #pragma unroll
