@@ -819,20 +819,19 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
   const cfloat* fsfa = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
   float* gq = dofs + el * SZ;
   float* gqa = dofs_a + el * NRM * SZA;
-  // this element's Q and QA, requested before the lifts
-  float qin[9][PJ], qain[NRM][6][PJ];
+  // this element's Q and QA at the thread's first position, requested
+  // before the lifts; the further positions (a few threads at O5) are read
+  // after them, so that fewer registers are held across the lifts
+  float qin[9], qain[NRM][6];
+  auto load_q = [&](int p) {
 #pragma unroll
-  for (int j = 0; j < PJ; j++) {
-    const int p = tid + j * kNeighThreads;
-    if (p < NP) {
+    for (int q = 0; q < 9; q++) qin[q] = gq[q * NP + p];
 #pragma unroll
-      for (int q = 0; q < 9; q++) qin[q][j] = gq[q * NP + p];
+    for (int rm = 0; rm < NRM; rm++)
 #pragma unroll
-      for (int rm = 0; rm < NRM; rm++)
-#pragma unroll
-        for (int q = 0; q < 6; q++) qain[rm][q][j] = gqa[(rm * 6 + q) * NP + p];
-    }
-  }
+      for (int q = 0; q < 6; q++) qain[rm][q] = gqa[(rm * 6 + q) * NP + p];
+  };
+  if (tid < NP) load_q(tid);
   if constexpr (PROJ) {
     const int32_t slot = rcv[wv];
     if (slot >= 0) {  // else outflow (uniform)
@@ -876,13 +875,14 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
 #pragma unroll
         for (int q = 0; q < 6; q++) ua[q] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
       }
+      if (j > 0) load_q(p);
 #pragma unroll
-      for (int q = 0; q < 9; q++) gq[q * NP + p] = qin[q][j] + y[q];
+      for (int q = 0; q < 9; q++) gq[q * NP + p] = qin[q] + y[q];
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++) {
         const float w = rfs[rm];
 #pragma unroll
-        for (int q = 0; q < 6; q++) gqa[(rm * 6 + q) * NP + p] = qain[rm][q][j] + w * ua[q];
+        for (int q = 0; q < 6; q++) gqa[(rm * 6 + q) * NP + p] = qain[rm][q] + w * ua[q];
       }
     }
   }
