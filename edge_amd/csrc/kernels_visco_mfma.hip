@@ -446,9 +446,9 @@ __device__ __forceinline__ void add_source(const cfloat* __restrict__ e, const f
 // NB: first the previous time step's neighbouring surface integral from the
 // projections in proj_in (written by the previous launch; s.proj receives
 // this step's, so the two alternate between two buffers): its lifted tiles
-// update Q and QA before the CK, and the output adds them to the DOFs read
-// at the start, so the neighbour step of step k - 1 and the local step of
-// step k run as one launch that reads and writes the DOFs once.
+// update Q and QA before the CK and go back to the DOFs at once, where the
+// output stage reads them, so the neighbour step of step k - 1 and the
+// local step of step k run as one launch.
 // Registers: at most 128 VGPRs (four waves per SIMD), so that two eight-wave
 // workgroups share a CU; at 129 only one fits and the launch takes 1.5x.
 template <int O, int NRM, bool TL, bool NB = false>
@@ -537,9 +537,11 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   }
   if constexpr (NB || !ZR) __syncthreads();  // the lifted tiles, or Q in LDS
   // NB: Q += sum_f U_f L_f, QA_rm += w_rm sum_f U_A,f L_f (the neighbour-side
-  // factors); nq, nqa are added again to the DOFs the output reads
-  float nq[NB ? 9 : 1][PJ], nqa[NB ? 6 : 1][PJ];
+  // factors), written back to the DOFs at once: the output stage then reads
+  // them as it reads Q and QA otherwise, and no register holds the lifted
+  // sums across the CK
   if constexpr (NB) {
+    float nq[9][PJ], nqa[6][PJ];
     const cint32* rcv = (const cint32*)(s.rcv + el * 4);
     const cfloat* fsn = as_const(s.fsf + (s.n_el + el) * 4 * 54);
     const cfloat* fsna = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
@@ -562,13 +564,20 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 #pragma unroll
           for (int q = 0; q < 6; q++) nqa[q][j] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
         }
+        float* gw = dofs + el * SZ;
 #pragma unroll
-        for (int q = 0; q < 9; q++) qv[q][j] += nq[q][j];
+        for (int q = 0; q < 9; q++) {
+          qv[q][j] += nq[q][j];
+          gw[q * NP + p] = qv[q][j];
+        }
 #pragma unroll
         for (int rm = 0; rm < NRM; rm++) {
           const float w = rfs[rm];
 #pragma unroll
-          for (int q = 0; q < 6; q++) da[rm][q][j] += w * nqa[q][j];
+          for (int q = 0; q < 6; q++) {
+            da[rm][q][j] += w * nqa[q][j];
+            qa0[(rm * 6 + q) * NP + p] = da[rm][q][j];
+          }
         }
       }
     }
@@ -777,12 +786,6 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
         for (int q = 0; q < 9; q++) y[q] += u[q * 3] * l0 + u[q * 3 + 1] * l1 + u[q * 3 + 2] * l2;
 #pragma unroll
         for (int q = 0; q < 6; q++) ua[q] += uan[q * 3] * l0 + uan[q * 3 + 1] * l1 + uan[q * 3 + 2] * l2;
-      }
-      if constexpr (NB) {
-#pragma unroll
-        for (int q = 0; q < 9; q++) y[q] += nq[q][j];
-#pragma unroll
-        for (int q = 0; q < 6; q++) ua[q] += nqa[q][j];
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) gq[q * NP + p] = qin[q][j] + y[q];
