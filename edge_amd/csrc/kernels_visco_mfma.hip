@@ -46,6 +46,8 @@
 // rounding.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace edge {
@@ -66,6 +68,27 @@ __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ v4f zero4() { return v4f{0.f, 0.f, 0.f, 0.f}; }
+
+// Row tiles with at most 4 rows (the last tile of 20 or 35 modes) run on
+// v_mfma_f32_4x4x1f32 (16 blocks of 4 x 4 x 1, half the cycles of the
+// 16 x 16 x 4 tile it replaces): block 4 g + c / 4 takes the lane group's k
+// (4 s + g), rows 0..3 and columns 4 (c / 4) .. + 3, so the A operand of
+// lane (g, c) is M[4 s + g][16 t + c % 4] and the B operand is unchanged;
+// the four k-groups are summed across g at the end, after which lane
+// (g, c), register v holds row 16 t + v, column c.
+__device__ __forceinline__ v4f mfma44(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+constexpr bool narrow_tile(int nn, int t) { return nn - 16 * t <= 4; }
+constexpr int a_col(int nn, int t, int c) { return narrow_tile(nn, t) ? (c & 3) : c; }
+__device__ __forceinline__ v4f sum_groups(v4f x) {
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    x[v] += __shfl_xor(x[v], 16);
+    x[v] += __shfl_xor(x[v], 32);
+  }
+  return x;
+}
 constexpr int tiles(int rows) { return (rows + 15) / 16; }
 constexpr int ksteps(int rows) { return (rows + 3) / 4; }
 
@@ -159,7 +182,8 @@ __device__ __forceinline__ void raw_products_zr(const float* __restrict__ m, con
 #pragma unroll
       for (int t = 0; t < TT; t++) {
         const int k = 4 * s + g;
-        aall[s][t] = ldg(m, k * N + 16 * (t0 + t) + c, k < NK && 16 * (t0 + t) + c < NN);
+        const int col = 16 * (t0 + t) + (TT == 1 ? a_col(NN, t0, c) : c);
+        aall[s][t] = ldg(m, k * N + col, k < NK && col < NN);
       }
   }
   auto load = [&](int s, float (&a)[TT], float (&b)[3]) {
@@ -178,31 +202,43 @@ __device__ __forceinline__ void raw_products_zr(const float* __restrict__ m, con
         b[p] = k < NK ? xb[p][(k < NK ? k : 0) * R] : 0.f;
     }
   };
-  float an[TT], bn[3];
-  load(0, an, bn);
+  // NAR: one narrow tile (TT = 1, at most 4 rows), on the 4 x 4 blocks
+  auto run = [&](auto nar) {
+    constexpr bool NAR = decltype(nar)::value;
+    float an[TT], bn[3];
+    load(0, an, bn);
 #pragma unroll
-  for (int s = 0; s < KS; s++) {
-    float a[TT], b[3];
+    for (int s = 0; s < KS; s++) {
+      float a[TT], b[3];
 #pragma unroll
-    for (int t = 0; t < TT; t++) a[t] = an[t];
+      for (int t = 0; t < TT; t++) a[t] = an[t];
 #pragma unroll
-    for (int p = 0; p < 3; p++) b[p] = bn[p];
-    if (s + 1 < KS) load(s + 1, an, bn);
-    __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+      for (int p = 0; p < 3; p++) b[p] = bn[p];
+      if (s + 1 < KS) load(s + 1, an, bn);
+      __builtin_amdgcn_sched_barrier(0);  // bounds register pressure: no hoisting across k-steps
+#pragma unroll
+      for (int t = 0; t < TT; t++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) acc[t][p] = NAR ? mfma44(a[t], b[p], acc[t][p]) : mfma(a[t], b[p], acc[t][p]);
+    }
+    if constexpr (NAR) {
+#pragma unroll
+      for (int p = 0; p < 3; p++) acc[0][p] = sum_groups(acc[0][p]);
+    }
 #pragma unroll
     for (int t = 0; t < TT; t++)
 #pragma unroll
-      for (int p = 0; p < 3; p++) acc[t][p] = mfma(a[t], b[p], acc[t][p]);
-  }
+      for (int p = 0; p < 3; p++)
 #pragma unroll
-  for (int t = 0; t < TT; t++)
-#pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int n = 16 * (t0 + t) + 4 * g + v;
-        if (n < NN) out[(2 * p + h) * NP + n * R + r] = acc[t][p][v];
-      }
+        for (int v = 0; v < 4; v++) {
+          const int n = 16 * (t0 + t) + (NAR ? 0 : 4 * g) + v;
+          if (n < NN && (!NAR || g == 0)) out[(2 * p + h) * NP + n * R + r] = acc[t][p][v];
+        }
+  };
+  if (TT == 1 && narrow_tile(NN, t0))
+    run(std::true_type{});
+  else
+    run(std::false_type{});
 }
 
 // Raw products of one direction d: rows k < NK of the tile x (LDS,
@@ -376,21 +412,33 @@ __device__ __forceinline__ void face_lift(const float* __restrict__ ft, v4f w01,
     const int fm = 4 * g + v;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      const float at = ldg(ft, fm * N + 16 * t + c, fm < NF && 16 * t + c < N);
-      l01[t] = mfma(at, w01[v], l01[t]);
-      l2[t] = mfma(at, w2[v], l2[t]);
+      const int col = 16 * t + a_col(N, t, c);
+      const float at = ldg(ft, fm * N + col, fm < NF && col < N);
+      if (narrow_tile(N, t)) {  // constant after unrolling
+        l01[t] = mfma44(at, w01[v], l01[t]);
+        l2[t] = mfma44(at, w2[v], l2[t]);
+      } else {
+        l01[t] = mfma(at, w01[v], l01[t]);
+        l2[t] = mfma(at, w2[v], l2[t]);
+      }
     }
   }
 #pragma unroll
-  for (int t = 0; t < NT; t++)
+  for (int t = 0; t < NT; t++) {
+    const bool NAR = narrow_tile(N, t);  // constant after unrolling
+    if (NAR) {
+      l01[t] = sum_groups(l01[t]);
+      l2[t] = sum_groups(l2[t]);
+    }
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const int n = 16 * t + 4 * g + v;
-      if (n < N) {
+      const int n = 16 * t + (NAR ? 0 : 4 * g) + v;
+      if (n < N && (!NAR || g == 0)) {
         out[h * NP + n * R + r] = l01[t][v];
         if (h == 0) out[2 * NP + n * R + r] = l2[t][v];
       }
     }
+  }
 }
 
 // Face projections exchanged between the local and the neighbour step: per
@@ -616,7 +664,8 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 #pragma unroll
   for (int st = 0; st < ksteps(N); st++) {
     const int k = 4 * st + g;
-    a_ck[st] = ldg(s.dg.stiffTn + dir * N * N, k * N + 16 * t_ck + c, k < N && 16 * t_ck + c < NV);
+    const int col = 16 * t_ck + (ZR ? a_col(NV, t_ck, c) : c);
+    a_ck[st] = ldg(s.dg.stiffTn + dir * N * N, k * N + col, k < N && col < NV);
   }
   float sca = dt;
 #pragma unroll 1
