@@ -134,11 +134,55 @@ struct VDims {
   static constexpr int SZ = 9 * NP, SZA = 6 * NP;
 };
 
+// The per-element coefficients the elementwise phases use, gathered in LDS
+// in the order they are used (each workgroup copies its element's at the
+// start).  Read from the star, anelastic star and source matrices where
+// they lie, the pattern's scattered entries became a chain of small scalar
+// loads, each waited for in full before the next FMAs; from LDS every
+// thread reads the same address (a broadcast) and the reads pipeline.
+//   [kCS + 15 d ..)  stress rows: S_d[q][6 + kStrVel[q][j]]   (15 per direction)
+//   [kCA + 9 d ..)   anelastic star: S^A_d[q][kAnCol[q][j]]   (9 per direction)
+//   [kCZ + 9 d ..)   star-combined velocity rows: S_d[6 + i][kVelCols[i][j]]
+//   [kCE + 12 rm ..) source matrix E_rm: normal block [3][3], then the diagonal 3..5
+constexpr int kCS = 0, kCA = 45, kCZ = 72, kCE = 99;
+constexpr int n_coef(int nrm) { return kCE + 12 * nrm; }
+constexpr int kStrOff[6] = {0, 3, 6, 9, 11, 13}, kAnOff[6] = {0, 1, 2, 3, 5, 7};
+struct CoefTab {
+  // index into [star 243 | anelastic star 54 | source matrices 36 n_rms]
+  short src[n_coef(3)] = {};
+  constexpr CoefTab() {
+    int i = 0;
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < 6; q++)
+        for (int j = 0; j < kStrNnz[q]; j++) src[i++] = (short)(d * 81 + q * 9 + 6 + kStrVel[q][j]);
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < 6; q++)
+        for (int j = 0; j < kAnNnz[q]; j++) src[i++] = (short)(243 + d * 18 + q * 3 + kAnCol[q][j]);
+    for (int d = 0; d < 3; d++)
+      for (int r = 0; r < 3; r++)
+        for (int j = 0; j < 3; j++) src[i++] = (short)(d * 81 + (6 + r) * 9 + kVelCols[r][j]);
+    for (int rm = 0; rm < 3; rm++) {
+      for (int q = 0; q < 3; q++)
+        for (int j = 0; j < 3; j++) src[i++] = (short)(297 + rm * 36 + q * 6 + j);
+      for (int q = 3; q < 6; q++) src[i++] = (short)(297 + rm * 36 + q * 6 + q);
+    }
+  }
+};
+constexpr CoefTab kCoef{};
+
+// The coefficients' LDS base, opaque to the optimiser: read where they are
+// used instead of hoisted out of the time-derivative loop into registers.
+__device__ __forceinline__ const float* coef_base(const float* l_c) {
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  return l_c + z;
+}
+
 // Velocity rows of the star-combined tile of direction d at one position:
 // z_{d,i} = sum_j S_d[6 + i][kVelCols[i][j]] y[kVelCols[i][j]].
-__device__ __forceinline__ float vel_row(const cfloat* __restrict__ sd, int i, const float (&y)[9]) {
-  return sd[(6 + i) * 9 + kVelCols[i][0]] * y[kVelCols[i][0]] + sd[(6 + i) * 9 + kVelCols[i][1]] * y[kVelCols[i][1]] +
-         sd[(6 + i) * 9 + kVelCols[i][2]] * y[kVelCols[i][2]];
+__device__ __forceinline__ float vel_row(const float* __restrict__ lc, int d, int i, const float (&y)[9]) {
+  const float* z = lc + kCZ + 9 * d + 3 * i;
+  return z[0] * y[kVelCols[i][0]] + z[1] * y[kVelCols[i][1]] + z[2] * y[kVelCols[i][2]];
 }
 
 // The B rows of the products at position p of an LDS tile x ([18][N][R]):
@@ -147,7 +191,7 @@ __device__ __forceinline__ float vel_row(const cfloat* __restrict__ sd, int i, c
 // 9 + 3 d + i.  One thread per position forms each z once; formed inside
 // the products, every k-step of both waves of a direction would form them
 // again (four times per position).
-__device__ __forceinline__ void store_rows(float* __restrict__ x, const cfloat* __restrict__ star,
+__device__ __forceinline__ void store_rows(float* __restrict__ x, const float* __restrict__ lc,
                                            const float (&y)[9], int np, int p, bool stress) {
 #pragma unroll
   for (int q = 0; q < 9; q++)
@@ -155,7 +199,7 @@ __device__ __forceinline__ void store_rows(float* __restrict__ x, const cfloat* 
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) x[(9 + 3 * d + i) * np + p] = vel_row(star + d * 81, i, y);
+    for (int i = 0; i < 3; i++) x[(9 + 3 * d + i) * np + p] = vel_row(lc, d, i, y);
 }
 
 // raw_products (below) with the B rows ready-made by store_rows: three LDS
@@ -455,8 +499,8 @@ __device__ __forceinline__ void store_proj(float* __restrict__ p, v4f w01, v4f w
 // combination A (6) from the raw products of the three directions
 // (l_p: [3][6][N][R]).
 template <int NP>
-__device__ __forceinline__ void combine_raw(const float* __restrict__ l_p, const cfloat* __restrict__ star,
-                                            const cfloat* __restrict__ star_a, int p, float (&y)[9], float (&a)[6]) {
+__device__ __forceinline__ void combine_raw(const float* __restrict__ l_p, const float* __restrict__ lc, int p,
+                                            float (&y)[9], float (&a)[6]) {
 #pragma unroll
   for (int q = 0; q < 9; q++) y[q] = 0.f;
 #pragma unroll
@@ -468,24 +512,25 @@ __device__ __forceinline__ void combine_raw(const float* __restrict__ l_p, const
     for (int i = 0; i < 3; i++) pv[i] = l_p[(d * 6 + i) * NP + p];
 #pragma unroll
     for (int r = 0; r < 3; r++) y[6 + r] += l_p[(d * 6 + 3 + r) * NP + p];
-    const cfloat* sd = star + d * 81;
-    const cfloat* sa = star_a + d * 18;
+    const float* sd = lc + kCS + 15 * d;
+    const float* sa = lc + kCA + 9 * d;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
 #pragma unroll
-      for (int j = 0; j < kStrNnz[q]; j++) y[q] += sd[q * 9 + 6 + kStrVel[q][j]] * pv[kStrVel[q][j]];
+      for (int j = 0; j < kStrNnz[q]; j++) y[q] += sd[kStrOff[q] + j] * pv[kStrVel[q][j]];
 #pragma unroll
-      for (int j = 0; j < kAnNnz[q]; j++) a[q] += sa[q * 3 + kAnCol[q][j]] * pv[kAnCol[q][j]];
+      for (int j = 0; j < kAnNnz[q]; j++) a[q] += sa[kAnOff[q] + j] * pv[kAnCol[q][j]];
     }
   }
 }
 
-// y[0..5] += E x for an anelastic source matrix E [6][6] of the source pattern
-__device__ __forceinline__ void add_source(const cfloat* __restrict__ e, const float (&x)[6], float (&y)[9]) {
+// y[0..5] += E x for an anelastic source matrix E of the source pattern
+// (its 12 entries as gathered in LDS at lc + kCE + 12 rm)
+__device__ __forceinline__ void add_source(const float* __restrict__ e, const float (&x)[6], float (&y)[9]) {
 #pragma unroll
-  for (int q = 0; q < 3; q++) y[q] += e[q * 6] * x[0] + e[q * 6 + 1] * x[1] + e[q * 6 + 2] * x[2];
+  for (int q = 0; q < 3; q++) y[q] += e[q * 3] * x[0] + e[q * 3 + 1] * x[1] + e[q * 3 + 2] * x[2];
 #pragma unroll
-  for (int q = 3; q < 6; q++) y[q] += e[q * 6 + q] * x[q];
+  for (int q = 3; q < 6; q++) y[q] += e[6 + q] * x[q];
 }
 
 // TL: tIntA in LDS (each thread's own positions, no barriers) instead of
@@ -517,6 +562,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   __shared__ __align__(16) float l_x[ZR ? 2 * SZ : SZ];  // Q, then der_{de-1}, then tInt (+ z rows)
   __shared__ __align__(16) float l_p[18 * NP];  // raw products [3][6][N][R]; lifted tiles [4][3][N][R]
   __shared__ __align__(16) float l_ta[TL ? NRM * 6 * NP : 1];  // tIntA [rm][6][N][R] (TL)
+  __shared__ __align__(16) float l_c[n_coef(NRM)];              // the element's coefficients (CoefTab order)
   const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
@@ -532,9 +578,12 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   const int lw = (odd && wv >= 4) ? (wv >= 6 ? wv - 2 : wv + 4) : wv;
   const int pos = tid;  // this thread's position (none from NP on)
   const int dir = lw % 3, t_ck = lw / 3;  // CK / volume products: direction, half of the row tiles
-  const cfloat* star = as_const(s.star + el * 243);
-  const cfloat* star_a = as_const(s.star_a + el * 54);
-  const cfloat* src = as_const(s.src_a + el * NRM * 36);
+  const cfloat* star = as_const(s.star + el * 243);  // the products' own star rows (without ZR)
+  if (tid < n_coef(NRM)) {
+    const int v = kCoef.src[tid];
+    l_c[tid] = v < 243 ? s.star[el * 243 + v]
+                       : (v < 297 ? s.star_a[el * 54 + v - 243] : s.src_a[el * NRM * 36 + v - 297]);
+  }
   const cfloat* rfs = as_const(s.rfs);
   const float* q0 = dofs + el * SZ;
   float* qa0 = dofs_a + el * NRM * SZA;
@@ -583,7 +632,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
         for (int q = 0; q < 6; q++) da[rm][q][j] = qa0[(rm * 6 + q) * NP + p];
     }
   }
-  if constexpr (NB || !ZR) __syncthreads();  // the lifted tiles, or Q in LDS
+  __syncthreads();  // the coefficients (and the lifted tiles, or Q in LDS)
   // NB: Q += sum_f U_f L_f, QA_rm += w_rm sum_f U_A,f L_f (the neighbour-side
   // factors), written back to the DOFs at once: the output stage then reads
   // them as it reads Q and QA otherwise, and no register holds the lifted
@@ -650,7 +699,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
         y[q] = qv[q][j];
         ti[q][j] = dt * y[q];
       }
-      if (p < NP) store_rows(l_x, star, y, NP, p, false);
+      if (p < NP) store_rows(l_x, l_c, y, NP, p, false);
     } else {
 #pragma unroll
       for (int q = 0; q < 9; q++) ti[q][j] = p < NP ? dt * l_x[q * NP + p] : 0.f;
@@ -675,13 +724,14 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
                                     t_ck, g, c, a_ck);
     __syncthreads();
     sca *= dt / float(de + 1);
+    const float* lc = coef_base(l_c);
 #pragma unroll
     for (int j = 0; j < PJ; j++) {
       const int p = pos + j;
       if (p < NP) {
         float y[9], a[6];
         if (p < NV * R) {
-          combine_raw<NP>(l_p, star, star_a, p, y, a);
+          combine_raw<NP>(l_p, lc, p, y, a);
         } else {  // top-degree modes: no stiffness contribution
 #pragma unroll
           for (int q = 0; q < 9; q++) y[q] = 0.f;
@@ -693,7 +743,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
           float x[6];
 #pragma unroll
           for (int q = 0; q < 6; q++) x[q] = da[rm][q][j];
-          add_source(src + rm * 36, x, y);
+          add_source(lc + kCE + 12 * rm, x, y);
           const float w = rfs[rm];
 #pragma unroll
           for (int q = 0; q < 6; q++) {
@@ -705,7 +755,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 #pragma unroll
         for (int q = 0; q < 9; q++) ti[q][j] += sca * y[q];
         if constexpr (ZR) {
-          store_rows(l_x, star, y, NP, p, false);
+          store_rows(l_x, lc, y, NP, p, false);
         } else {
 #pragma unroll
           for (int q = 0; q < 9; q++) l_x[q * NP + p] = y[q];
@@ -728,7 +778,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
         l_x[q * NP + p] = y[q];
         gti[q * NP + p] = y[q];
       }
-      if constexpr (ZR) store_rows(l_x, star, y, NP, p, false);
+      if constexpr (ZR) store_rows(l_x, l_c, y, NP, p, false);
     }
   }
   __syncthreads();
@@ -768,13 +818,13 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
     const int p = pos + j;
     float y[9], a[6];
     if (p < NP) {
-      combine_raw<NP>(l_p, star, star_a, p, y, a);
+      combine_raw<NP>(l_p, l_c, p, y, a);
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++) {
         float x[6];
 #pragma unroll
         for (int q = 0; q < 6; q++) x[q] = ta(rm, q, j);
-        add_source(src + rm * 36, x, y);
+        add_source(l_c + kCE + 12 * rm, x, y);
       }
     } else {
 #pragma unroll
