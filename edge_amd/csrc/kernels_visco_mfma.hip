@@ -170,6 +170,15 @@ struct CoefTab {
 };
 constexpr CoefTab kCoef{};
 
+// A face's rank-3 solver factors for the surface terms, gathered in LDS the
+// same way: U (9 x 3) then U_A (6 x 3), 45 values per face.
+constexpr int kUF = 45;
+__device__ __forceinline__ void gather_u(float* __restrict__ lu, const float* __restrict__ fs,
+                                         const float* __restrict__ fsa, int i) {
+  const int f = i / kUF, k = i - f * kUF;
+  lu[i] = k < 27 ? fs[f * 54 + k] : fsa[f * 18 + k - 27];
+}
+
 // The coefficients' LDS base, opaque to the optimiser: read where they are
 // used instead of hoisted out of the time-derivative loop into registers.
 __device__ __forceinline__ const float* coef_base(const float* l_c) {
@@ -563,6 +572,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   __shared__ __align__(16) float l_p[18 * NP];  // raw products [3][6][N][R]; lifted tiles [4][3][N][R]
   __shared__ __align__(16) float l_ta[TL ? NRM * 6 * NP : 1];  // tIntA [rm][6][N][R] (TL)
   __shared__ __align__(16) float l_c[n_coef(NRM)];              // the element's coefficients (CoefTab order)
+  __shared__ __align__(16) float l_u[(NB ? 8 : 4) * kUF];        // U, U_A per face: local side, (NB) neighbour side
   const int64_t off = s.xcd_map ? xcd_element(n) : (int64_t)blockIdx.x;
   if (off >= n) return;
   const int64_t el = first + off;
@@ -584,11 +594,13 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
     l_c[tid] = v < 243 ? s.star[el * 243 + v]
                        : (v < 297 ? s.star_a[el * 54 + v - 243] : s.src_a[el * NRM * 36 + v - 297]);
   }
+  if (tid < 4 * kUF) gather_u(l_u, s.fsf + el * 4 * 54, s.fsfa + el * 4 * 18, tid);
+  if (NB && tid >= 4 * kUF && tid < 8 * kUF)
+    gather_u(l_u + 4 * kUF, s.fsf + (s.n_el + el) * 4 * 54, s.fsfa + (s.n_el + el) * 4 * 18, tid - 4 * kUF);
   const cfloat* rfs = as_const(s.rfs);
   const float* q0 = dofs + el * SZ;
   float* qa0 = dofs_a + el * NRM * SZA;
   const cfloat* fsf = as_const(s.fsf + el * 4 * 54);
-  const cfloat* fsfa = as_const(s.fsfa + el * 4 * 18);
 
   // Q: per position into registers (ZR), or copied to LDS as a whole
   float qv[ZR ? 9 : 1][PJ];
@@ -640,8 +652,7 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
   if constexpr (NB) {
     float nq[9][PJ], nqa[6][PJ];
     const cint32* rcv = (const cint32*)(s.rcv + el * 4);
-    const cfloat* fsn = as_const(s.fsf + (s.n_el + el) * 4 * 54);
-    const cfloat* fsna = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
+    const float* lun = coef_base(l_u) + 4 * kUF;
 #pragma unroll
     for (int j = 0; j < PJ; j++) {
       const int p = pos + j;
@@ -654,8 +665,8 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
         for (int f = 0; f < 4; f++) {
           if (rcv[f] < 0) continue;  // outflow (uniform)
           const float l0 = l_p[(f * 3) * NP + p], l1 = l_p[(f * 3 + 1) * NP + p], l2 = l_p[(f * 3 + 2) * NP + p];
-          const cfloat* u = fsn + f * 54;
-          const cfloat* uan = fsna + f * 18;
+          const float* u = lun + f * kUF;
+          const float* uan = u + 27;
 #pragma unroll
           for (int q = 0; q < 9; q++) nq[q][j] += u[q * 3] * l0 + u[q * 3 + 1] * l1 + u[q * 3 + 2] * l2;
 #pragma unroll
@@ -879,8 +890,8 @@ void k_local_vm(SolverDev<float> s, int64_t first, int64_t n, float dt, float* _
 #pragma unroll
       for (int f = 0; f < 4; f++) {
         const float l0 = l_p[(f * 3) * NP + p], l1 = l_p[(f * 3 + 1) * NP + p], l2 = l_p[(f * 3 + 2) * NP + p];
-        const cfloat* u = fsf + f * 54;
-        const cfloat* uan = fsfa + f * 18;
+        const float* u = l_u + f * kUF;
+        const float* uan = u + 27;
 #pragma unroll
         for (int q = 0; q < 9; q++) y[q] += u[q * 3] * l0 + u[q * 3 + 1] * l1 + u[q * 3 + 2] * l2;
 #pragma unroll
@@ -918,7 +929,9 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
   const cint32* nes = (const cint32*)(s.nb.ne + el * 4);
   const cint32* rcv = (const cint32*)(s.rcv + el * 4);
   const cfloat* fsf = as_const(s.fsf + (s.n_el + el) * 4 * 54);
-  const cfloat* fsfa = as_const(s.fsfa + (s.n_el + el) * 4 * 18);
+  // the neighbour-side U, U_A per face, gathered in LDS (read by broadcast)
+  __shared__ __align__(16) float l_u[4 * kUF];
+  if (tid < 4 * kUF) gather_u(l_u, s.fsf + (s.n_el + el) * 4 * 54, s.fsfa + (s.n_el + el) * 4 * 18, tid);
   float* gq = dofs + el * SZ;
   float* gqa = dofs_a + el * NRM * SZA;
   // this element's Q and QA at the thread's first position, requested
@@ -970,8 +983,8 @@ __global__ __launch_bounds__(kNeighThreads) void k_neigh_vm(SolverDev<float> s, 
       for (int f = 0; f < 4; f++) {
         if ((PROJ ? rcv[f] : nes[f]) < 0) continue;  // outflow (uniform)
         const float l0 = l_n[f][p], l1 = l_n[f][NP + p], l2 = l_n[f][2 * NP + p];
-        const cfloat* u = fsf + f * 54;
-        const cfloat* uan = fsfa + f * 18;
+        const float* u = l_u + f * kUF;
+        const float* uan = u + 27;
 #pragma unroll
         for (int q = 0; q < 9; q++) y[q] += u[q * 3] * l0 + u[q * 3 + 1] * l1 + u[q * 3 + 2] * l2;
 #pragma unroll
