@@ -496,6 +496,67 @@ __device__ __forceinline__ int opaque_u(int x) {
   return x;
 }
 
+// With the elastic star pattern the per-element coefficients of the
+// elementwise work are gathered in LDS once per element, in the order they
+// are used: per direction the star's 24 non-zeros (rows in order, columns
+// kStarCol) and the anelastic star (6 x 3), then per mechanism the source
+// matrix's 12 pattern entries (normal block, then the shear diagonal).
+// Through scalar loads the scattered entries were a chain of small loads,
+// each waited for in full; from LDS every lane reads the same address (a
+// broadcast) and the reads pipeline.
+constexpr int kCfDir = 42, kCfSrc = 3 * kCfDir;
+struct StarNnz {
+  int idx[24] = {};
+  constexpr StarNnz() {
+    int i = 0;
+    for (int q = 0; q < 9; q++)
+      for (int j = 0; j < kStarNnzRow[q]; j++) idx[i++] = q * 9 + kStarCol[q][j];
+  }
+};
+constexpr StarNnz kStarNnz{};
+__device__ __forceinline__ double gather_cf(const double* __restrict__ star, const double* __restrict__ star_a,
+                                            const double* __restrict__ src, int i) {
+  if (i < kCfSrc) {
+    const int d = i / kCfDir, k = i - d * kCfDir;
+    return k < 24 ? star[d * 81 + kStarNnz.idx[k]] : star_a[d * 18 + k - 24];
+  }
+  const int rm = (i - kCfSrc) / 12, k = (i - kCfSrc) - 12 * rm;
+  return src[rm * 36 + (k < 9 ? (k / 3) * 6 + k % 3 : 7 * (k - 6))];
+}
+
+// mix15 for the elastic star pattern with the direction's gathered
+// coefficients cf (24 star non-zeros, then the anelastic star)
+template <int NK>
+__device__ __forceinline__ void mix15s(const double* __restrict__ cf, const double* __restrict__ x, int ldx,
+                                       double* __restrict__ y, int ldy, int lane) {
+  for (int k = lane; k < PadLayout::p4(NK); k += 64) {
+    const bool in = k < NK;
+    double xv[9];
+#pragma unroll
+    for (int p = 0; p < 9; p++) xv[p] = in ? x[p * ldx + k] : 0.0;
+    int o = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      double z = 0.0;
+#pragma unroll
+      for (int j = 0; j < kStarNnzRow[q]; j++) z += cf[o + j] * xv[kStarCol[q][j]];
+      o += kStarNnzRow[q];
+      y[q * ldy + k] = z;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; j++)
+      y[(9 + j) * ldy + k] = cf[24 + j * 3] * xv[6] + cf[24 + j * 3 + 1] * xv[7] + cf[24 + j * 3 + 2] * xv[8];
+  }
+}
+
+// add_src with the 12 gathered entries of E
+__device__ __forceinline__ void add_src_s(const double* __restrict__ e, const double (&x)[6], double (&y)[9]) {
+#pragma unroll
+  for (int q = 0; q < 3; q++) y[q] += e[q * 3] * x[0] + e[q * 3 + 1] * x[1] + e[q * 3 + 2] * x[2];
+#pragma unroll
+  for (int q = 3; q < 6; q++) y[q] += e[6 + q] * x[q];
+}
+
 // y[0..5] += E x for an anelastic source matrix E (6 x 6) of the source
 // pattern (src/impl/seismic/setups/ViscoElasticity.h:379-458): the
 // normal-stress block rows 0..2 x columns 0..2 and the shear diagonal
@@ -523,6 +584,7 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
   constexpr int LXX = 15 * SL > 2 * 9 * SLF ? 15 * SL : 2 * 9 * SLF;
   constexpr int LXA = 15 * SL > 2 * 15 * SLF ? 15 * SL : 2 * 15 * SLF;
   __shared__ double l_ti[9 * SL], l_x[LXX], l_a[LXA], l_ta[NRM * 6 * N];
+  __shared__ double l_cf[SP ? kCfSrc + 12 * NRM : 1];  // gathered coefficients (SP)
   const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
   if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
   const int64_t el = first + off;
@@ -532,6 +594,9 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
   const cdouble* src = as_const(s.src_a + el * NRM * 36);
   const double* pad = s.dg.pad;
   double* qa = dofs_a + el * NRM * SZA;
+  if constexpr (SP)
+    for (int i = lane; i < kCfSrc + 12 * NRM; i += 64)
+      l_cf[i] = gather_cf(s.star + el * 243, s.star_a + el * 54, s.src_a + el * NRM * 36, i);
   {
     constexpr int IT = (SZ + 63) / 64;
     double v[IT];
@@ -571,7 +636,10 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
     for (int d = 0; d < 3; d++) {
       double bq[ksteps4(N)][TV];
       load_b<TV, ksteps4(N), LQ>(pad + d * LD * LQ, bq, g, c);
-      mix15<SP, N>(star + opaque_u(d * 81), star_a + opaque_u(d * 18), l_x, SL, l_a, SL, lane);
+      if constexpr (SP)
+        mix15s<N>(l_cf + opaque_u(d * kCfDir), l_x, SL, l_a, SL, lane);
+      else
+        mix15<SP, N>(star + opaque_u(d * 81), star_a + opaque_u(d * 18), l_x, SL, l_a, SL, lane);
       __syncthreads();
       mmq<TV, ksteps4(N), SL, 15>(l_a, bq, acc, g, c);
       __syncthreads();
@@ -588,7 +656,10 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
       for (int j = 0; j < 6; j++) a[j] = n < NV ? l_x[(9 + j) * SL + n] : 0.0;
 #pragma unroll
       for (int rm = 0; rm < NRM; rm++) {
-        add_src(src + opaque_u(rm * 36), da[rm], y);
+        if constexpr (SP)
+          add_src_s(l_cf + opaque_u(kCfSrc + rm * 12), da[rm], y);
+        else
+          add_src(src + opaque_u(rm * 36), da[rm], y);
         const double w = as_const(s.rfs)[rm];
 #pragma unroll
         for (int j = 0; j < 6; j++) {
@@ -618,7 +689,10 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
   for (int d = 0; d < 3; d++) {
     double bq[ksteps4(NV)][NT];
     load_b<NT, ksteps4(NV), LQ>(pad + P::stiff_v(N) + d * LD * LQ, bq, g, c);
-    mix15<SP, NV>(star + d * 81, star_a + d * 18, l_ti, SL, l_a, SL, lane);
+    if constexpr (SP)
+      mix15s<NV>(l_cf + d * kCfDir, l_ti, SL, l_a, SL, lane);
+    else
+      mix15<SP, NV>(star + d * 81, star_a + d * 18, l_ti, SL, l_a, SL, lane);
     __syncthreads();
     mmq<NT, ksteps4(NV), SL, 15>(l_a, bq, acc, g, c);
     __syncthreads();
@@ -631,7 +705,10 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
       double ta[6];
 #pragma unroll
       for (int j = 0; j < 6; j++) ta[j] = l_ta[(rm * 6 + j) * N + lane];
-      add_src(src + rm * 36, ta, y);
+      if constexpr (SP)
+        add_src_s(l_cf + kCfSrc + rm * 12, ta, y);
+      else
+        add_src(src + rm * 36, ta, y);
     }
 #pragma unroll
     for (int j = 0; j < 6; j++) l_x[j * SL + lane] = y[j];
