@@ -488,6 +488,32 @@ __device__ __forceinline__ void mixf15(const double* __restrict__ f, const doubl
   }
 }
 
+// mixf15 with the coefficients gathered in LDS: cf [15][9], the flux
+// solver's 9 rows then its anelastic part's 6 (gather_fs)
+template <int NK>
+__device__ __forceinline__ void mixf15l(const double* __restrict__ cf, const double* __restrict__ x, int ldx,
+                                        double* __restrict__ y, int ldy, int lane) {
+  for (int k = lane; k < PadLayout::p4(NK); k += 64) {
+    const bool in = k < NK;
+    double xv[9];
+#pragma unroll
+    for (int p = 0; p < 9; p++) xv[p] = in ? x[p * ldx + k] : 0.0;
+#pragma unroll 1
+    for (int q = 0; q < 15; q++) {
+      const double* row = cf + q * 9;
+      double z = 0.0;
+#pragma unroll
+      for (int p = 0; p < 9; p++) z += row[p] * xv[p];
+      y[q * ldy + k] = z;
+    }
+  }
+}
+constexpr int kFsCoef = 135;
+__device__ __forceinline__ void gather_fs(double* __restrict__ cf, const double* __restrict__ f,
+                                          const double* __restrict__ fa, int lane) {
+  for (int i = lane; i < kFsCoef; i += 64) cf[i] = i < 81 ? f[i] : fa[i - 81];
+}
+
 // The wave-uniform value x, opaque to the optimiser: per-element
 // coefficients addressed with it are loaded where they are used instead of
 // being hoisted out of the time-derivative loop (and held in registers).
@@ -584,7 +610,9 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
   constexpr int LXX = 15 * SL > 2 * 9 * SLF ? 15 * SL : 2 * 9 * SLF;
   constexpr int LXA = 15 * SL > 2 * 15 * SLF ? 15 * SL : 2 * 15 * SLF;
   __shared__ double l_ti[9 * SL], l_x[LXX], l_a[LXA], l_ta[NRM * 6 * N];
-  __shared__ double l_cf[SP ? kCfSrc + 12 * NRM : 1];  // gathered coefficients (SP)
+  // gathered coefficients (SP): the CK's and the volume's, then two faces'
+  // flux solvers in the surface integral
+  __shared__ double l_cf[SP ? (kCfSrc + 12 * NRM > 2 * kFsCoef ? kCfSrc + 12 * NRM : 2 * kFsCoef) : 1];
   const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
   if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
   const int64_t el = first + off;
@@ -730,6 +758,10 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
 #pragma unroll 1
   for (int fp = 0; fp < 4; fp += 2) {
     v4d s0[2] = {zero4d(), zero4d()};
+    if constexpr (SP) {  // l_cf's last reads are behind the previous barrier
+      gather_fs(l_cf, fs0 + fp * 81, fsa0 + fp * 54, lane);
+      gather_fs(l_cf + kFsCoef, fs0 + (fp + 1) * 81, fsa0 + (fp + 1) * 54, lane);
+    }
     mmp<2, ksteps4(N), SL, LQF, 9, LD * LQF>(l_ti, pad + P::flux_ln(N) + fp * LD * LQF, s0, g, c);
     double bq[2][ksteps4(NF)][NT];
 #pragma unroll
@@ -737,8 +769,13 @@ void k_local_pv64(SolverDev<double> s, int64_t first, int64_t n_el, double dt, d
     store_tile(l_x, SLF, NF, s0[0], g, c);
     store_tile(l_x + 9 * SLF, SLF, NF, s0[1], g, c);
     __syncthreads();
-    mixf15<NF>(fs0 + fp * 81, fsa0 + fp * 54, l_x, SLF, l_a, SLF, lane);
-    mixf15<NF>(fs0 + (fp + 1) * 81, fsa0 + (fp + 1) * 54, l_x + 9 * SLF, SLF, l_a + 15 * SLF, SLF, lane);
+    if constexpr (SP) {
+      mixf15l<NF>(l_cf, l_x, SLF, l_a, SLF, lane);
+      mixf15l<NF>(l_cf + kFsCoef, l_x + 9 * SLF, SLF, l_a + 15 * SLF, SLF, lane);
+    } else {
+      mixf15<NF>(fs0 + fp * 81, fsa0 + fp * 54, l_x, SLF, l_a, SLF, lane);
+      mixf15<NF>(fs0 + (fp + 1) * 81, fsa0 + (fp + 1) * 54, l_x + 9 * SLF, SLF, l_a + 15 * SLF, SLF, lane);
+    }
     __syncthreads();
     mmq<NT, ksteps4(NF), SLF, 15>(l_a, bq[0], acc, g, c);
     mmq<NT, ksteps4(NF), SLF, 15>(l_a + 15 * SLF, bq[1], acc, g, c);
@@ -775,7 +812,7 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
   constexpr int LD = P::p4(N), LQ = P::p16(N), LDF = P::p4(NF), LQF = P::p16(NF);
   constexpr int SLF = lds_ld(NF);  // LDS row stride of the face tiles
   // l_a: the mixed face operand [15][SLF], then the output rows [15][N]
-  __shared__ double l_s0[9 * SLF], l_a[15 * (SLF > N ? SLF : N)];
+  __shared__ double l_s0[9 * SLF], l_a[15 * (SLF > N ? SLF : N)], l_fc[kFsCoef];
   const int64_t off = s.xcd_map ? xcd_element(n_el) : (int64_t)blockIdx.x;
   if (off >= n_el) return;  // one wave per workgroup: no barrier is left waiting
   const int64_t el = first + off;
@@ -793,14 +830,15 @@ __global__ __launch_bounds__(64) void k_neigh_pv64(SolverDev<double> s, int64_t 
     const int32_t ne = s.nb.ne[el * 4 + fa];
     if (ne < 0) continue;  // outflow: no neighbouring contribution (uniform per wave)
     const int fm = s.nb.fmat[el * 4 + fa];
+    // the neighbour-side solver, gathered (l_fc's last reads are behind the previous barrier)
+    gather_fs(l_fc, s.fs + ((s.n_el + el) * 4 + fa) * 81, s.fs_a + ((s.n_el + el) * 4 + fa) * 54, lane);
     v4d s0[1] = {zero4d()};
     mmg<N, LQF>(t_int + (int64_t)ne * SZ, pad + P::flux_ln(N) + fm * LD * LQF, s0[0], g, c);
     double bq[ksteps4(NF)][NT];
     load_b<NT, ksteps4(NF), LQ>(pad + P::flux_t(N, NF) + fa * LDF * LQ, bq, g, c);
     store_tiles<1>(l_s0, SLF, NF, s0, g, c);
     __syncthreads();
-    mixf15<NF>(s.fs + ((s.n_el + el) * 4 + fa) * 81, s.fs_a + ((s.n_el + el) * 4 + fa) * 54, l_s0, SLF, l_a,
-               SLF, lane);
+    mixf15l<NF>(l_fc, l_s0, SLF, l_a, SLF, lane);
     __syncthreads();
     mmq<NT, ksteps4(NF), SLF, 15>(l_a, bq, acc, g, c);
     __syncthreads();
