@@ -498,7 +498,7 @@ __device__ __forceinline__ void mixf15l(const double* __restrict__ cf, const dou
     double xv[9];
 #pragma unroll
     for (int p = 0; p < 9; p++) xv[p] = in ? x[p * ldx + k] : 0.0;
-#pragma unroll 1
+#pragma unroll 3
     for (int q = 0; q < 15; q++) {
       const double* row = cf + q * 9;
       double z = 0.0;
